@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark of the sparse Krylov hot path on one MI355X.
+
+Step = one FP64 CSR SpMV y = A x on the 3-D 7-point Laplacian 128^3 (n = 2,097,152,
+nnz = 14,581,760), inputs resident in HBM.  Each timed SpMV is preceded by a 512 MB
+device write that evicts the 256 MB Infinity Cache, so `value` is an HBM figure; the
+back-to-back (cache-assisted) rate is reported as `warm_gbs`.  The JSON line also carries
+the roofline of the SpMV kernel, the CPU baseline (the reference's own OpenMP host backend,
+compiled from its sources into oracle/_ref), and time-to-solution of the FP64 solvers.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "viennacl-dev_amd"))
+
+METRIC = "CG time-to-1e-10 rel. residual (FP64 vs mixed, 1 GPU); SpMV GB/s vs HBM peak"
+HBM_PEAK_GBS = 8000.0
+FLUSH_BYTES = 512 << 20
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def csr_bytes(n, nnz, val_bytes=8, vec_bytes=8):
+    """Algorithmic bytes of y = A x: values + column indices + row pointers + x once + y written."""
+    return nnz * (val_bytes + 4) + 4 * (n + 1) + 2 * vec_bytes * n
+
+
+def sell_bytes(stored, slices, n):
+    return 12 * stored + 8 * slices + 16 * n
+
+
+class Events:
+    def __init__(self, vh, count):
+        self.vh = vh
+        self.ev = []
+        for _ in range(count):
+            h = C.c_void_p()
+            vh.call("vclhip_event_create", C.byref(h))
+            self.ev.append(h)
+
+    def record(self, i, stream):
+        self.vh.call("vclhip_event_record", self.ev[i], stream)
+
+    def ms(self, i, j):
+        t = C.c_float()
+        self.vh.call("vclhip_event_elapsed_ms", self.ev[i], self.ev[j], C.byref(t))
+        return t.value
+
+    def close(self):
+        for h in self.ev:
+            self.vh.call("vclhip_event_destroy", h)
+
+
+def time_spmv(vh, M, x, y, stream, steps, warmup, flush):
+    """Cold (flushed) per-launch times and warm back-to-back time of M.spmv, via HIP events on `stream`."""
+    for _ in range(warmup):
+        M.spmv(x, y, stream=stream.handle)
+    stream.synchronize()
+    ev = Events(vh, 2 * steps + 2)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        vh.call("vclhip_fill_zero", flush.ptr, C.c_size_t(flush.nbytes), stream.handle)
+        ev.record(2 * k, stream.handle)
+        M.spmv(x, y, stream=stream.handle)
+        ev.record(2 * k + 1, stream.handle)
+    stream.synchronize()
+    wall = time.perf_counter() - t0
+    cold = np.array([ev.ms(2 * k, 2 * k + 1) for k in range(steps)])
+    ev.record(2 * steps, stream.handle)
+    for _ in range(steps):
+        M.spmv(x, y, stream=stream.handle)
+    ev.record(2 * steps + 1, stream.handle)
+    stream.synchronize()
+    warm = ev.ms(2 * steps, 2 * steps + 1) / steps
+    ev.close()
+    return cold, warm, wall
+
+
+def true_relres(vh, A, b, x, n):
+    """||b - A x|| / ||b|| in FP64 on the device."""
+    r = vh.DeviceArray.from_numpy(np.zeros(n))
+    vh.call("vclhip_memcpy_d2d", r.ptr, b.ptr, C.c_size_t(8 * n), None)
+    A.spmv(x, r, -1.0, 1.0)
+    nr, nb = C.c_double(), C.c_double()
+    vh.call("vclhip_norm2_f64", n, r.ptr, 0, 1, C.byref(nr), None)
+    vh.call("vclhip_norm2_f64", n, b.ptr, 0, 1, C.byref(nb), None)
+    return nr.value / nb.value
+
+
+def timed_solve(vh, kind, A, b, n, precond=0, reps=2, **kw):
+    """Best-of-`reps` wall time of one solve (device-synchronized), with its FP64 true residual."""
+    x = vh.DeviceArray.zeros(n, np.float64)
+    best = None
+    res = None
+    for _ in range(reps):
+        vh.call("vclhip_device_synchronize")
+        t0 = time.perf_counter()
+        res = (vh.cg if kind == "cg" else vh.bicgstab)(A, b, x, precond=precond, **kw)
+        vh.call("vclhip_device_synchronize")
+        t = time.perf_counter() - t0
+        best = t if best is None else min(best, t)
+    rel = true_relres(vh, A, b, x, n)
+    return {"seconds": round(best, 6), "iters": int(res.iters), "error": float(res.error),
+            "true_relres": float(rel), "converged": bool(res.converged)}
+
+
+def cpu_baseline(rp, col, val, x, n, nnz, seconds):
+    """The reference host backend (OpenMP) timed on this machine's cores over a bounded sample."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as po
+    if not po.ref_available():
+        return None
+    R = po.ref()
+    y = np.zeros(n)
+    one = R.ref_bench_csr_spmv(n, po.p(rp), po.p(col), po.p(val), po.p(x), po.p(y), 1)
+    reps = max(3, int(seconds / max(one, 1e-6)))
+    t = R.ref_bench_csr_spmv(n, po.p(rp), po.p(col), po.p(val), po.p(x), po.p(y), reps)
+    gbs = csr_bytes(n, nnz) * reps / t / 1e9
+    return {"value": round(gbs, 2), "unit": "GB/s", "cores": int(R.ref_max_threads()), "kind": "reference",
+            "sample": f"{reps} FP64 CSR SpMVs y=Ax of the same 128^3 Laplacian (viennacl::linalg::prod, "
+                      f"host_based OpenMP backend), {t:.1f} s"}, y
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solvers", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import vclhip as vh
+    assert vh.device_count() > 0, "no HIP device"
+    vh.call("vclhip_set_device", 0)
+    name = C.create_string_buffer(64)
+    mem, cu, l2, llc = C.c_size_t(), C.c_int(), C.c_int(), C.c_int()
+    vh.call("vclhip_device_info", 0, name, 64, C.byref(mem), C.byref(cu), C.byref(l2), C.byref(llc))
+    log(f"device {name.value.decode()} CUs={cu.value} HBM={mem.value / 2**30:.0f} GiB L2={l2.value >> 20} MiB")
+
+    N = args.grid
+    t = time.perf_counter()
+    rp, col, val = vh.laplace_3d(N)
+    n, nnz = rp.size - 1, int(rp[-1])
+    log(f"generated 3-D 7-point Laplacian {N}^3: n={n} nnz={nnz} ({time.perf_counter() - t:.1f} s)")
+    stream = vh.Stream()
+    A = vh.CsrMatrix(rp, col, val)
+    S = vh.SellMatrix(rp, col, val, 64)
+    xh = vh.random_uniform(1, n)
+    x = vh.DeviceArray.from_numpy(xh)
+    y = vh.DeviceArray.zeros(n, np.float64)
+    flush = vh.DeviceArray(FLUSH_BYTES, np.uint8)
+
+    cold, warm, wall = time_spmv(vh, A, x, y, stream, args.steps, args.warmup, flush)
+    bytes_csr = csr_bytes(n, nnz)
+    ms = float(cold.mean())
+    gbs = bytes_csr / (ms * 1e-3) / 1e9
+    warm_gbs = bytes_csr / (warm * 1e-3) / 1e9
+    log(f"CSR SpMV cold {ms * 1e3:.1f} us ({gbs:.0f} GB/s), warm {warm * 1e3:.1f} us ({warm_gbs:.0f} GB/s)")
+    s_cold, s_warm, _ = time_spmv(vh, S, x, y, stream, args.steps, args.warmup, flush)
+    bytes_sell = sell_bytes(S.stored, (n + 63) // 64, n)
+    sell_gbs = bytes_sell / (s_cold.mean() * 1e-3) / 1e9
+    log(f"SELL-64 SpMV cold {s_cold.mean() * 1e3:.1f} us ({sell_gbs:.0f} GB/s), warm {s_warm * 1e3:.1f} us")
+    y_gpu = y.to_numpy()
+
+    traffic = None
+    tfile = ROOT / "profiles" / "csr_spmv_lap3d_128_traffic.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC, "value": round(gbs, 2), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 6), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"FP64 CSR SpMV y=Ax, 3-D 7-point Laplacian {N}^3 (n={n}, nnz={nnz}), "
+                               f"512 MB cache flush before each timed SpMV", "n": n, "nnz": nnz,
+                   "format": "CSR (uint32 indices)", "algorithmic_bytes_per_spmv": bytes_csr},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "warm_gbs": round(warm_gbs, 2),
+        "timed_region_wall_s": round(wall, 4),
+        "sell64": {"cold_gbs": round(sell_gbs, 2), "cold_ms": round(float(s_cold.mean()), 6),
+                   "warm_ms": round(s_warm, 6), "algorithmic_bytes": bytes_sell},
+    }
+
+    if not args.no_solvers:
+        b = vh.DeviceArray.from_numpy(vh.random_uniform(0, n))
+        sol = {}
+        sol["cg_fp64_csr"] = timed_solve(vh, "cg", A, b, n, tol=1e-10, max_iters=5000)
+        log("CG FP64 CSR", sol["cg_fp64_csr"])
+        sol["cg_fp64_sell64"] = timed_solve(vh, "cg", S, b, n, tol=1e-10, max_iters=5000)
+        sol["jacobi_cg_fp64_csr"] = timed_solve(vh, "cg", A, b, n, precond=1, tol=1e-10, max_iters=5000)
+        sol["bicgstab_fp64_csr"] = timed_solve(vh, "bicgstab", A, b, n, tol=1e-10, max_iters=5000)
+        sol["jacobi_bicgstab_fp64_csr"] = timed_solve(vh, "bicgstab", A, b, n, precond=1, tol=1e-10,
+                                                      max_iters=5000)
+        log("solvers 128^3", sol)
+        out["solvers_lap3d_128"] = sol
+
+    if not args.no_cpu:
+        log("CPU baseline (reference host backend) ...")
+        cb = cpu_baseline(rp, col, val, xh, n, nnz, args.cpu_seconds)
+        if cb is not None:
+            base, y_cpu = cb
+            out["cpu_baseline"] = base
+            out["cpu_vs_gpu_relinf"] = float(np.max(np.abs(y_cpu - y_gpu)) / np.max(np.abs(y_cpu)))
+            log("CPU baseline", base)
+        else:
+            out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,305 @@
+/*
+ * vclhip.h -- C ABI of the MI355X (gfx950) backend for ViennaCL's sparse Krylov path.
+ *
+ * This is the drop-in boundary: the HIP memory domain and the hot-path operations that
+ * ViennaCL's backend-agnostic dispatchers call for each memory domain.  Each entry point
+ * names the reference interface it stands in for (paths relative to the reference root,
+ * viennacl/viennacl-dev 1.7.1).  The C++ headers in viennacl-dev_amd/viennacl/ call only
+ * this ABI; so does the Python binding.
+ *
+ * Conventions
+ *   - Every function returns int: 0 (VCLHIP_SUCCESS), a negative VCLHIP_ERROR_* code, or
+ *     a positive hipError_t value.  Nothing throws across the ABI.  The text of the last
+ *     failure on the calling thread is returned by vclhip_last_error().
+ *     (reference: cuda_exception backend/cuda.hpp:38,59-68; VIENNACL_CUDA_LAST_ERROR_CHECK
+ *     cuda/common.hpp:30,165-175; ViennaCLStatus libviennacl/include/viennacl.hpp:98-102)
+ *   - Pointers named *_dev are device pointers (allocated by vclhip_malloc or any HIP
+ *     allocator on the current device); *_host are host pointers.
+ *   - `stream` is a hipStream_t passed as void*; NULL is the null stream.  Kernels are
+ *     enqueued asynchronously unless a function says it synchronizes.
+ *   - Indices are uint32_t like compressed_matrix<T> (compressed_matrix.hpp:57-58); vectors
+ *     carry (start, stride) like vector_base (detail/vector_def.hpp:104-349).
+ *   - x and y of an SpMV must not alias (the C++ layer copies through a temporary, as the
+ *     reference's op_executor does, compressed_matrix.hpp:1231-1246).
+ */
+#ifndef VCLHIP_H
+#define VCLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VCLHIP_SUCCESS 0
+#define VCLHIP_ERROR_INVALID_VALUE (-1)   /* bad size, null pointer, unsupported parameter */
+#define VCLHIP_ERROR_ZERO_DIAGONAL (-2)   /* zero_on_diagonal_exception, forwards.h:621-625 */
+#define VCLHIP_ERROR_NOT_SUPPORTED (-3)
+#define VCLHIP_ERROR_NO_DEVICE (-4)
+#define VCLHIP_ERROR_TOO_LARGE (-5)       /* size beyond a documented maximum */
+
+/* ------------------------------------------------------------------ runtime ---- */
+const char* vclhip_version(void);
+const char* vclhip_last_error(void);
+const char* vclhip_status_string(int status);
+int vclhip_device_count(int* count);
+int vclhip_set_device(int device);
+int vclhip_get_device(int* device);
+/* name (NUL-terminated, truncated to name_len), HBM bytes, CUs, L2 bytes, last-level cache bytes */
+int vclhip_device_info(int device, char* name, size_t name_len, size_t* total_mem, int* num_cu,
+                       int* l2_bytes, int* llc_bytes);
+
+/* ---------------------------------------------------------- memory domain ---- */
+/* memory_create (backend/memory.hpp:87-119 -> backend/cuda.hpp:103-117) */
+int vclhip_malloc(void** ptr_dev, size_t bytes);
+/* handle deleter (backend/cuda.hpp:85-93) */
+int vclhip_free(void* ptr_dev);
+/* memory_write (backend/memory.hpp:220-249 -> backend/cuda.hpp:154-177); returns when done */
+int vclhip_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes, void* stream);
+/* memory_read (backend/memory.hpp:261-292 -> backend/cuda.hpp:183-200); returns when done */
+int vclhip_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes, void* stream);
+/* memory_copy (backend/memory.hpp:140-171 -> backend/cuda.hpp:126-146); asynchronous */
+int vclhip_memcpy_d2d(void* dst_dev, const void* src_dev, size_t bytes, void* stream);
+/* zero fill with a kernel (safe inside captured graphs); asynchronous */
+int vclhip_fill_zero(void* dst_dev, size_t bytes, void* stream);
+/* backend::finish() (backend/memory.hpp:54-62) for one stream */
+int vclhip_stream_create(void** stream);
+int vclhip_stream_destroy(void* stream);
+int vclhip_stream_synchronize(void* stream);
+int vclhip_device_synchronize(void);
+/* events for timing on a given stream (no reference counterpart; tools/timer.hpp:90-115 is wall clock) */
+int vclhip_event_create(void** event);
+int vclhip_event_destroy(void* event);
+int vclhip_event_record(void* event, void* stream);
+int vclhip_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ------------------------------------------------------------ sparse formats ---- */
+/* CSR matrix on the device (compressed_matrix<T>: handle1 = row_ptr, handle2 = col, handle = values,
+ * handle3 = row blocks; compressed_matrix.hpp:1088-1104).  `blocks` is the kernel's row-block plan
+ * built by vclhip_csr_plan_blocks; `values` is double* or float* according to the function suffix. */
+typedef struct vclhip_csr {
+  uint32_t n_rows;
+  uint32_t n_cols;
+  uint32_t nnz;
+  uint32_t num_blocks;
+  const uint32_t* row_ptr;
+  const uint32_t* col_idx;
+  const void* values;
+  const uint32_t* blocks;
+} vclhip_csr;
+
+/* SELL-C-sigma (sigma = 1) matrix on the device (sliced_ell_matrix<T>: handle1 = columns_per_block,
+ * handle2 = column_indices, handle3 = block_start, handle = elements; sliced_ell_matrix.hpp:109-119).
+ * Slice s, entry k, row-in-slice i is at elements[block_start[s] + k*C + i]. */
+typedef struct vclhip_sell {
+  uint32_t n_rows;
+  uint32_t n_cols;
+  uint32_t C;
+  uint32_t num_slices;
+  const uint32_t* columns_per_block;
+  const uint32_t* column_indices;
+  const uint32_t* block_start;
+  const void* elements;
+} vclhip_sell;
+
+/* Row-block plan for the CSR kernels (host function).  Replaces the reference's 1024-nnz row
+ * blocks (compressed_matrix::generate_row_block_information, compressed_matrix.hpp:1152-1188) with
+ * blocks sized for 256-lane workgroups: each entry packs the first row (bits 0-27) and a mode
+ * (bits 28-31: log2 of lanes per row for LDS-staged blocks, 7 for one long row read directly).
+ * `blocks_host` must hold n_rows + 1 entries; *num_blocks receives the count (entry num_blocks is
+ * the end sentinel n_rows).  n_rows must be below 2^28. */
+int vclhip_csr_plan_blocks(const uint32_t* row_ptr_host, uint32_t n_rows, uint32_t* blocks_host,
+                           uint32_t* num_blocks);
+/* The reference's own row-block partition (compressed_matrix.hpp:1152-1188), for API parity. */
+int vclhip_csr_reference_row_blocks(const uint32_t* row_ptr_host, uint32_t n_rows, uint32_t* row_blocks_host,
+                                    uint32_t* num_blocks);
+/* CSR -> SELL-C conversion on the host (sliced_ell_matrix.hpp:140-214).  First call with
+ * NULL outputs to get num_slices and the stored entry count. */
+int vclhip_sell_from_csr_f64(uint32_t n_rows, uint32_t C, const uint32_t* row_ptr, const uint32_t* col,
+                             const double* val, uint32_t* num_slices, uint64_t* stored,
+                             uint32_t* columns_per_block, uint32_t* column_indices, uint32_t* block_start,
+                             double* elements);
+
+/* ---------------------------------------------------------------- SpMV ---- */
+/* y = alpha*A*x + beta*y.  linalg::prod_impl(compressed_matrix, x, alpha, y, beta)
+ * (linalg/sparse_matrix_operations.hpp:90-121; host_based/sparse_matrix_operations.hpp:145-186;
+ * cuda/sparse_matrix_operations.hpp:262-396).  beta is treated as zero when !(beta<0||beta>0). */
+int vclhip_csr_spmv_f64(const vclhip_csr* A, const double* x_dev, uint32_t x_start, uint32_t x_stride,
+                        double alpha, double* y_dev, uint32_t y_start, uint32_t y_stride, double beta,
+                        void* stream);
+int vclhip_csr_spmv_f32(const vclhip_csr* A, const float* x_dev, uint32_t x_start, uint32_t x_stride,
+                        float alpha, float* y_dev, uint32_t y_start, uint32_t y_stride, float beta,
+                        void* stream);
+/* linalg::prod_impl(sliced_ell_matrix, ...) (host_based/sparse_matrix_operations.hpp:1796-1868;
+ * cuda/sparse_matrix_operations.hpp:2247-2289).  Zero entries are skipped as in the reference. */
+int vclhip_sell_spmv_f64(const vclhip_sell* A, const double* x_dev, uint32_t x_start, uint32_t x_stride,
+                         double alpha, double* y_dev, uint32_t y_start, uint32_t y_stride, double beta,
+                         void* stream);
+int vclhip_sell_spmv_f32(const vclhip_sell* A, const float* x_dev, uint32_t x_start, uint32_t x_stride,
+                         float alpha, float* y_dev, uint32_t y_start, uint32_t y_stride, float beta,
+                         void* stream);
+/* linalg::detail::row_info (linalg/sparse_matrix_operations.hpp:48-74; host_based :52-98):
+ * info 0 = inf-norm, 1 = 1-norm, 2 = 2-norm, 3 = diagonal (0 when absent). */
+int vclhip_csr_row_info_f64(const vclhip_csr* A, double* out_dev, int info, void* stream);
+int vclhip_csr_row_info_f32(const vclhip_csr* A, float* out_dev, int info, void* stream);
+
+/* ----------------------------------------------------------- vector ops ---- */
+/* inner_prod_impl / inner_prod_cpu (linalg/vector_operations.hpp:500-590;
+ * host_based/vector_operations.hpp:556-575).  Result written to *result_host; synchronizes. */
+int vclhip_dot_f64(uint32_t n, const double* x_dev, uint32_t x_start, uint32_t x_stride,
+                   const double* y_dev, uint32_t y_start, uint32_t y_stride, double* result_host, void* stream);
+int vclhip_dot_f32(uint32_t n, const float* x_dev, uint32_t x_start, uint32_t x_stride,
+                   const float* y_dev, uint32_t y_start, uint32_t y_stride, float* result_host, void* stream);
+/* FP32 inputs, FP64 accumulation (no reference counterpart: the mixed-precision CG's inner dots) */
+int vclhip_dot_f32_acc64(uint32_t n, const float* x_dev, const float* y_dev, double* result_host, void* stream);
+/* norm_2_impl / norm_2_cpu (linalg/vector_operations.hpp:812-840; host_based :820-833) */
+int vclhip_norm2_f64(uint32_t n, const double* x_dev, uint32_t x_start, uint32_t x_stride,
+                     double* result_host, void* stream);
+int vclhip_norm2_f32(uint32_t n, const float* x_dev, uint32_t x_start, uint32_t x_stride,
+                     float* result_host, void* stream);
+/* z = a*x + b*y  (avbv, linalg/vector_operations.hpp:106-141; host_based :127-194).
+ * Contiguous vectors; z may alias x or y. */
+int vclhip_axpby_f64(uint32_t n, double a, const double* x_dev, double b, const double* y_dev, double* z_dev,
+                     void* stream);
+int vclhip_axpby_f32(uint32_t n, float a, const float* x_dev, float b, const float* y_dev, float* z_dev,
+                     void* stream);
+/* strided form: vec1 = vec2*alpha + vec3*beta with (start,stride) per vector */
+int vclhip_avbv_f64(uint32_t n, double* v1_dev, uint32_t s1, uint32_t i1,
+                    const double* v2_dev, uint32_t s2, uint32_t i2, double alpha,
+                    const double* v3_dev, uint32_t s3, uint32_t i3, double beta, void* stream);
+/* vec = alpha (vector_assign, linalg/vector_operations.hpp:208-240) */
+int vclhip_fill_f64(uint32_t n, double* v_dev, uint32_t start, uint32_t stride, double alpha, void* stream);
+int vclhip_fill_f32(uint32_t n, float* v_dev, uint32_t start, uint32_t stride, float alpha, void* stream);
+/* z = x / y element-wise (element_op with op_div, linalg/vector_operations.hpp:254-285) */
+int vclhip_element_div_f64(uint32_t n, const double* x_dev, const double* y_dev, double* z_dev, void* stream);
+int vclhip_element_div_f32(uint32_t n, const float* x_dev, const float* y_dev, float* z_dev, void* stream);
+/* convert (linalg/vector_operations.hpp:50-70; host_based :67-83) */
+int vclhip_convert_f64_to_f32(uint32_t n, const double* src_dev, float* dst_dev, void* stream);
+int vclhip_convert_f32_to_f64(uint32_t n, const float* src_dev, double* dst_dev, void* stream);
+
+/* -------------------------------------------------- pipelined operations ---- */
+/* Buffer convention of the reference (cg.hpp:144-172, bicgstab.hpp:114-189): inner products are
+ * left in chunks of `chunk` doubles; the host sums each chunk.  These entry points write each
+ * completed inner product into the first entry of its chunk and zero the rest of that chunk.
+ * All vectors contiguous, length A->n_rows. */
+/* linalg::pipelined_cg_vector_update (iterative_operations.hpp:58-87; host_based :378-418):
+ * x += alpha p; r -= alpha Ap; p = r + beta p; buf[0..chunk) <- <r,r> */
+int vclhip_pipelined_cg_vector_update_f64(uint32_t n, double* x_dev, double alpha, double* p_dev, double* r_dev,
+                                          const double* Ap_dev, double beta, double* buf_dev, uint32_t chunk,
+                                          void* stream);
+/* linalg::pipelined_cg_prod (iterative_operations.hpp:96-122; host_based :427-487):
+ * Ap = A p; chunk 1 <- <Ap,Ap>, chunk 2 <- <p,Ap> */
+int vclhip_pipelined_cg_prod_csr_f64(const vclhip_csr* A, const double* p_dev, double* Ap_dev, double* buf_dev,
+                                     uint32_t chunk, void* stream);
+int vclhip_pipelined_cg_prod_sell_f64(const vclhip_sell* A, const double* p_dev, double* Ap_dev, double* buf_dev,
+                                      uint32_t chunk, void* stream);
+/* linalg::pipelined_bicgstab_update_s (iterative_operations.hpp:133-161; host_based :518-563):
+ * alpha = sum(chunk 0)/sum(chunk 3) on the device; s = r - alpha Ap; buf[offset] <- <s,s> */
+int vclhip_pipelined_bicgstab_update_s_f64(uint32_t n, double* s_dev, const double* r_dev, const double* Ap_dev,
+                                           double* buf_dev, uint32_t chunk, uint32_t offset, void* stream);
+/* linalg::pipelined_bicgstab_vector_update (iterative_operations.hpp:170-198; host_based :572-621) */
+int vclhip_pipelined_bicgstab_vector_update_f64(uint32_t n, double* x_dev, double alpha, double* p_dev,
+                                                double omega, const double* s_dev, double* r_dev,
+                                                const double* As_dev, double beta, const double* Ap_dev,
+                                                const double* r0star_dev, double* buf_dev, uint32_t chunk,
+                                                void* stream);
+/* linalg::pipelined_bicgstab_prod (iterative_operations.hpp:207-236; host_based :629-701):
+ * Ap = A p; chunk 1 <- <Ap,Ap>, chunk 2 <- <p,Ap>, buf[offset..offset+chunk) <- <Ap,r0*> */
+int vclhip_pipelined_bicgstab_prod_csr_f64(const vclhip_csr* A, const double* p_dev, double* Ap_dev,
+                                           const double* r0star_dev, double* buf_dev, uint32_t chunk,
+                                           uint32_t offset, void* stream);
+int vclhip_pipelined_bicgstab_prod_sell_f64(const vclhip_sell* A, const double* p_dev, double* Ap_dev,
+                                            const double* r0star_dev, double* buf_dev, uint32_t chunk,
+                                            uint32_t offset, void* stream);
+
+/* ------------------------------------------------------------- solvers ---- */
+/* Solver parameters: the fields of cg_tag (cg.hpp:48-87), bicgstab_tag (bicgstab.hpp:47-90) and
+ * mixed_precision_cg_tag (mixed_precision_cg.hpp:47-83), plus launch controls. */
+typedef struct vclhip_solver_params {
+  double tol;                        /* relative tolerance */
+  double abs_tol;                    /* absolute tolerance (0 = off) */
+  uint32_t max_iters;
+  uint32_t max_iters_before_restart; /* BiCGStab with a preconditioner (bicgstab.hpp:366-370) */
+  float inner_tol;                   /* mixed CG: restart when inner <r,r> < inner_tol * initial (squared norms) */
+  uint32_t chunk;                    /* iterations enqueued between convergence polls (0 = default 16) */
+  int use_graph;                     /* capture one chunk as a hipGraph and replay it */
+  int mixed_mode;                    /* mixed CG: 0 = restart as the reference, 1 = reliable update (keeps p) */
+  double* history_host;              /* optional: per-iteration convergence quantity (may be NULL) */
+  uint32_t history_len;
+  /* optional monitor, called once per iteration with the device solution and the relative residual
+   * estimate; returning nonzero stops the solve (cg.hpp:174-175, cg_solver::set_monitor :420-424).
+   * A monitor forces one host synchronization per iteration. */
+  int (*monitor)(const double* x_dev, double rel_residual, void* user);
+  void* monitor_user;
+} vclhip_solver_params;
+
+typedef struct vclhip_solver_result {
+  uint32_t iters;     /* tag.iters() */
+  double error;       /* tag.error() */
+  uint32_t restarts;  /* mixed CG outer corrections / BiCGStab restarts */
+  int converged;
+} vclhip_solver_result;
+
+/* Fill params with the defaults of cg_tag() / bicgstab_tag() / mixed_precision_cg_tag(). */
+void vclhip_solver_params_default(vclhip_solver_params* params);
+
+#define VCLHIP_PRECOND_NONE 0
+#define VCLHIP_PRECOND_JACOBI 1   /* jacobi_precond<compressed_matrix> (jacobi_precond.hpp:103-130) */
+
+/* solve(A, b, cg_tag [, jacobi_precond]) (cg.hpp:338-375).  Without a preconditioner this is the
+ * pipelined Chronopoulos-Gear CG of cg.hpp:128-187 with its stopping rule; with Jacobi it is a
+ * pipelined preconditioned CG stopping on the M^-1 norm like cg.hpp:283,306-309.
+ * x_dev receives the solution (the solver starts from x = 0). */
+int vclhip_cg_solve_csr_f64(const vclhip_csr* A, const double* b_dev, double* x_dev, int precond,
+                            const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+int vclhip_cg_solve_sell_f64(const vclhip_sell* A, const double* b_dev, double* x_dev, int precond,
+                             const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+/* solve(A, b, bicgstab_tag [, jacobi_precond]) (bicgstab.hpp:495-533).  Pipelined BiCGStab of
+ * bicgstab.hpp:97-215; with Jacobi the left-preconditioned iteration of bicgstab.hpp:398-489
+ * (criterion ||M^-1 r|| / ||b||, restarts every max_iters_before_restart iterations or on
+ * breakdown).  Unlike the reference, the last x update is applied before returning, so x
+ * matches the residual estimate that stopped the iteration. */
+int vclhip_bicgstab_solve_csr_f64(const vclhip_csr* A, const double* b_dev, double* x_dev, int precond,
+                                  const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+int vclhip_bicgstab_solve_sell_f64(const vclhip_sell* A, const double* b_dev, double* x_dev, int precond,
+                                   const vclhip_solver_params* params, vclhip_solver_result* result,
+                                   void* stream);
+
+/* ---------------------------------------------------------------- tools ---- */
+/* Project RNG: std::mt19937_64(seed), u = (draw >> 11) * 2^-53; U[a,b] = a + (b-a) u; normals by
+ * Box-Muller on consecutive pairs (r = sqrt(-2 ln(1-u1)), z0 = r cos(2 pi u2), z1 = r sin(2 pi u2)).
+ * (tools/random.hpp:44-48 is the reference's generator; its sequence is not reproduced.) */
+int vclhip_random_uniform(uint64_t seed, uint64_t n, double a, double b, double* out_host);
+int vclhip_random_normal(uint64_t seed, uint64_t n, double mean, double sigma, double* out_host);
+/* tools::generate_fdm_laplace (tools/matrix_generation.hpp:47-86): 2-D 5-point, row = i + j*px,
+ * diagonal 4, neighbours -1.  Arrays: row_ptr n+1, col/val 5n; *nnz receives the count. */
+int vclhip_gen_fdm_laplace_2d(uint32_t px, uint32_t py, uint32_t* row_ptr, uint32_t* col, double* val,
+                              uint32_t* nnz);
+/* 3-D 7-point Laplacian, row = i + nx (j + ny k), diagonal 6, neighbours -1.  col/val 7n. */
+int vclhip_gen_laplace_3d(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t* row_ptr, uint32_t* col, double* val,
+                          uint32_t* nnz);
+/* Variable-coefficient diffusion -div(k grad u) on an nx*ny*nz (nz = 1: 2-D, 5-point) cell grid,
+ * unit spacing: face coefficient = 2 k_a k_b / (k_a + k_b); a Dirichlet boundary face adds the
+ * cell's k to the diagonal.  k_host holds one coefficient per cell (lexicographic). */
+int vclhip_gen_diffusion(uint32_t nx, uint32_t ny, uint32_t nz, const double* k_host, uint32_t* row_ptr,
+                         uint32_t* col, double* val, uint32_t* nnz);
+/* Random nonsymmetric matrix: per row L ~ P(k) ~ k^-alpha on [1, max_len] (inverse CDF) distinct
+ * off-diagonal columns drawn uniformly, values N(0,1), diagonal = diag_factor * sum|a_ij|; columns
+ * sorted.  First call with NULL arrays returns *nnz; arrays: row_ptr n+1, col/val nnz. */
+int vclhip_gen_powerlaw(uint32_t n, uint32_t max_len, double alpha, double diag_factor, uint64_t seed,
+                        uint32_t* row_ptr, uint32_t* col, double* val, uint32_t* nnz);
+/* Batch of graph Laplacians: system i has n_i ~ U{n_min..n_max} rows, round(n_i*avg_degree/2)
+ * undirected edges between distinct uniform endpoints (repeats merged by adding weights), weights
+ * U[w_lo, w_hi]; A_i = L_i + shift*I.  Block-diagonal CSR with global column indices; sys_ptr has
+ * num_systems+1 entries.  First call with NULL arrays returns *total_rows and *nnz. */
+int vclhip_gen_graph_laplacian_batch(uint32_t num_systems, uint32_t n_min, uint32_t n_max, double avg_degree,
+                                     double w_lo, double w_hi, double shift, uint64_t seed, uint32_t* sys_ptr,
+                                     uint32_t* row_ptr, uint32_t* col, double* val, uint32_t* total_rows,
+                                     uint32_t* nnz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VCLHIP_H */

@@ -1,0 +1,317 @@
+"""ctypes binding of the CPU oracle (liboracle.so) and of the reference build (_ref/libvclref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline, never by the product.  See vcl_oracle.h for what each function restates.
+"""
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+ORACLE_LIB = HERE / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libvclref.so"
+
+u32, u64, f32, f64, vp, sz, i32 = C.c_uint32, C.c_uint64, C.c_float, C.c_double, C.c_void_p, C.c_size_t, C.c_int
+P = C.POINTER
+
+
+class _Csr(C.Structure):
+    _fields_ = [("n_rows", u32), ("row_ptr", vp), ("col", vp), ("val", vp)]
+
+
+class _Sell(C.Structure):
+    _fields_ = [("n_rows", u32), ("C", u32), ("cpb", vp), ("ci", vp), ("bs", vp), ("el", vp)]
+
+
+class _Res(C.Structure):
+    _fields_ = [("iters", u32), ("error", f64)]
+
+
+_ORC_SIGS = {
+    "orc_set_num_threads": (None, [i32]),
+    "orc_get_num_threads": (i32, []),
+    "orc_csr_spmv_f64": (None, [u32, vp, vp, vp, vp, sz, sz, f64, vp, sz, sz, f64]),
+    "orc_csr_spmv_f32": (None, [u32, vp, vp, vp, vp, sz, sz, f32, vp, sz, sz, f32]),
+    "orc_sell_spmv_f64": (None, [u32, u32, vp, vp, vp, vp, vp, sz, sz, f64, vp, sz, sz, f64]),
+    "orc_sell_spmv_f32": (None, [u32, u32, vp, vp, vp, vp, vp, sz, sz, f32, vp, sz, sz, f32]),
+    "orc_sell_sizes": (None, [u32, u32, vp, P(u32), P(u64)]),
+    "orc_sell_from_csr_f64": (None, [u32, u32, vp, vp, vp, vp, vp, vp, vp]),
+    "orc_row_blocks": (u32, [u32, vp, vp]),
+    "orc_csr_row_info_f64": (None, [u32, vp, vp, vp, vp, i32]),
+    "orc_dot_f64": (f64, [sz, vp, vp]),
+    "orc_dot_f32": (f32, [sz, vp, vp]),
+    "orc_norm2_f64": (f64, [sz, vp]),
+    "orc_norm2_f32": (f32, [sz, vp]),
+    "orc_pipelined_prod_csr_f64": (None, [u32, vp, vp, vp, vp, vp, vp, vp, sz, sz]),
+    "orc_pipelined_prod_sell_f64": (None, [u32, u32, vp, vp, vp, vp, vp, vp, vp, vp, sz, sz]),
+    "orc_pipelined_cg_vector_update_f64": (None, [sz, vp, f64, vp, vp, vp, f64, vp]),
+    "orc_pipelined_bicgstab_update_s_f64": (None, [sz, vp, vp, vp, vp, sz, sz]),
+    "orc_pipelined_bicgstab_vector_update_f64": (None, [sz, vp, f64, vp, f64, vp, vp, vp, f64, vp, vp, vp]),
+    "orc_cg_pipelined_csr": (_Res, [P(_Csr), vp, vp, f64, f64, u32, vp]),
+    "orc_cg_pipelined_sell": (_Res, [P(_Sell), vp, vp, f64, f64, u32, vp]),
+    "orc_cg_classic_csr": (_Res, [P(_Csr), vp, vp, vp, f64, f64, u32, vp]),
+    "orc_bicgstab_pipelined_csr": (_Res, [P(_Csr), vp, vp, f64, f64, u32, vp]),
+    "orc_bicgstab_pipelined_sell": (_Res, [P(_Sell), vp, vp, f64, f64, u32, vp]),
+    "orc_bicgstab_classic_csr": (_Res, [P(_Csr), vp, vp, vp, f64, f64, u32, u32, vp]),
+    "orc_mixed_cg_csr": (_Res, [P(_Csr), vp, vp, f64, u32, f32, P(u32)]),
+    "orc_batched_cg_csr": (None, [u32, vp, P(_Csr), vp, vp, f64, u32, vp, vp]),
+    "orc_adjoint_grad_csr": (None, [u32, vp, vp, vp, vp, vp]),
+}
+
+_REF_SIGS = {
+    "ref_set_num_threads": (None, [i32]),
+    "ref_fdm_laplace": (u32, [u32, u32, vp, vp, vp]),
+    "ref_csr_spmv_f64": (None, [u32, vp, vp, vp, vp, f64, vp, f64]),
+    "ref_sell_spmv_f64": (None, [u32, u32, vp, vp, vp, vp, f64, vp, f64]),
+    "ref_sell_arrays": (None, [u32, u32, vp, vp, vp, vp, vp, vp, vp, u64]),
+    "ref_row_blocks": (u32, [u32, vp, vp, vp, vp]),
+    "ref_row_info_diag": (None, [u32, vp, vp, vp, vp]),
+    "ref_dot_f64": (f64, [u32, vp, vp]),
+    "ref_norm2_f64": (f64, [u32, vp]),
+    "ref_pipelined_cg_vector_update": (None, [u32, vp, f64, vp, vp, vp, f64, vp]),
+    "ref_pipelined_cg_prod_csr": (None, [u32, vp, vp, vp, vp, vp, vp]),
+    "ref_pipelined_bicgstab_prod_csr": (None, [u32, vp, vp, vp, vp, vp, vp, vp, u32]),
+    "ref_pipelined_bicgstab_update_s": (None, [u32, vp, vp, vp, vp, u32]),
+    "ref_pipelined_bicgstab_vector_update": (None, [u32, vp, f64, vp, f64, vp, vp, vp, f64, vp, vp, vp]),
+    "ref_cg_csr": (None, [u32, vp, vp, vp, vp, vp, f64, f64, u32, i32, P(u32), P(f64)]),
+    "ref_cg_sell": (None, [u32, u32, vp, vp, vp, vp, vp, f64, f64, u32, P(u32), P(f64)]),
+    "ref_bicgstab_csr": (None, [u32, vp, vp, vp, vp, vp, f64, f64, u32, u32, i32, P(u32), P(f64)]),
+    "ref_bicgstab_sell": (None, [u32, u32, vp, vp, vp, vp, vp, f64, f64, u32, P(u32), P(f64)]),
+    "ref_mixed_cg_csr": (None, [u32, vp, vp, vp, vp, vp, f64, u32, f32, P(u32), P(f64)]),
+    "ref_bench_csr_spmv": (f64, [u32, vp, vp, vp, vp, vp, u32]),
+    "ref_bench_solve": (f64, [u32, vp, vp, vp, vp, vp, f64, u32, i32, i32, P(u32), P(f64)]),
+    "ref_max_threads": (i32, []),
+}
+
+_orc = None
+_ref = None
+
+
+def _load(path, sigs):
+    L = C.CDLL(str(path))
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        if not ORACLE_LIB.exists():
+            raise RuntimeError(f"oracle not built: {ORACLE_LIB}")
+        _orc = _load(ORACLE_LIB, _ORC_SIGS)
+    return _orc
+
+
+def ref_available():
+    return REF_LIB.exists()
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not REF_LIB.exists():
+            raise RuntimeError(f"reference build not available: {REF_LIB}")
+        _ref = _load(REF_LIB, _REF_SIGS)
+    return _ref
+
+
+def p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def cu32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+class CsrHost:
+    """Host CSR arrays kept alive together with the oracle descriptor."""
+
+    def __init__(self, row_ptr, col, val):
+        self.row_ptr, self.col, self.val = cu32(row_ptr), cu32(col), c64(val)
+        self.n = self.row_ptr.size - 1
+        self.desc = _Csr(self.n, p(self.row_ptr), p(self.col), p(self.val))
+
+
+class SellHost:
+    def __init__(self, row_ptr, col, val, C_):
+        row_ptr, col, val = cu32(row_ptr), cu32(col), c64(val)
+        n = row_ptr.size - 1
+        ns, stored = u32(), u64()
+        orc().orc_sell_sizes(n, C_, p(row_ptr), C.byref(ns), C.byref(stored))
+        self.cpb = np.zeros(ns.value, np.uint32)
+        self.bs = np.zeros(ns.value, np.uint32)
+        self.ci = np.zeros(max(stored.value, 1), np.uint32)
+        self.el = np.zeros(max(stored.value, 1), np.float64)
+        orc().orc_sell_from_csr_f64(n, C_, p(row_ptr), p(col), p(val), p(self.cpb), p(self.ci), p(self.bs), p(self.el))
+        self.ci, self.el = self.ci[: stored.value], self.el[: stored.value]
+        self.n, self.C = n, C_
+        self.desc = _Sell(n, C_, p(self.cpb), p(self.ci), p(self.bs), p(self.el))
+
+
+# ------------------------------------------------------------------ oracle wrappers
+
+def set_threads(n):
+    orc().orc_set_num_threads(n)
+    if ref_available():
+        ref().ref_set_num_threads(n)
+
+
+def csr_spmv(rp, col, val, x, alpha=1.0, y=None, beta=0.0, xs=0, xi=1, ys=0, yi=1, dtype=np.float64):
+    rp, col = cu32(rp), cu32(col)
+    val = np.ascontiguousarray(val, dtype=dtype)
+    x = np.ascontiguousarray(x, dtype=dtype)
+    n = rp.size - 1
+    y = np.zeros(ys + (n - 1) * yi + 1 if n else 0, dtype) if y is None else np.array(y, dtype=dtype)
+    fn = orc().orc_csr_spmv_f64 if dtype == np.float64 else orc().orc_csr_spmv_f32
+    fn(n, p(rp), p(col), p(val), p(x), xs, xi, alpha, p(y), ys, yi, beta)
+    return y
+
+
+def sell_spmv(S, x, alpha=1.0, y=None, beta=0.0, dtype=np.float64):
+    x = np.ascontiguousarray(x, dtype=dtype)
+    y = np.zeros(S.n, dtype) if y is None else np.array(y, dtype=dtype)
+    if dtype == np.float64:
+        orc().orc_sell_spmv_f64(S.n, S.C, p(S.cpb), p(S.ci), p(S.bs), p(S.el), p(x), 0, 1, alpha, p(y), 0, 1, beta)
+    else:
+        el = S.el.astype(np.float32)
+        orc().orc_sell_spmv_f32(S.n, S.C, p(S.cpb), p(S.ci), p(S.bs), p(el), p(x), 0, 1, alpha, p(y), 0, 1, beta)
+    return y
+
+
+def row_info(rp, col, val, info):
+    rp, col, val = cu32(rp), cu32(col), c64(val)
+    out = np.zeros(rp.size - 1)
+    orc().orc_csr_row_info_f64(rp.size - 1, p(rp), p(col), p(val), p(out), info)
+    return out
+
+
+def dot(x, y):
+    x, y = c64(x), c64(y)
+    return orc().orc_dot_f64(x.size, p(x), p(y))
+
+
+def norm2(x):
+    x = c64(x)
+    return orc().orc_norm2_f64(x.size, p(x))
+
+
+def cg_pipelined(A, b, tol, max_iters, abs_tol=0.0, history=False):
+    b = c64(b)
+    x = np.zeros_like(b)
+    h = np.zeros(max(max_iters, 1)) if history else None
+    if isinstance(A, SellHost):
+        r = orc().orc_cg_pipelined_sell(C.byref(A.desc), p(b), p(x), tol, abs_tol, max_iters, p(h))
+    else:
+        r = orc().orc_cg_pipelined_csr(C.byref(A.desc), p(b), p(x), tol, abs_tol, max_iters, p(h))
+    return x, r.iters, r.error, (h[: r.iters] if history else None)
+
+
+def cg_classic(A, b, tol, max_iters, diag=None, abs_tol=0.0, history=False):
+    b = c64(b)
+    x = np.zeros_like(b)
+    d = c64(diag) if diag is not None else None
+    h = np.zeros(max(max_iters, 1)) if history else None
+    r = orc().orc_cg_classic_csr(C.byref(A.desc), p(b), p(x), p(d), tol, abs_tol, max_iters, p(h))
+    return x, r.iters, r.error, (h[: r.iters] if history else None)
+
+
+def bicgstab_pipelined(A, b, tol, max_iters, abs_tol=0.0, history=False):
+    b = c64(b)
+    x = np.zeros_like(b)
+    h = np.zeros(max(max_iters, 1)) if history else None
+    if isinstance(A, SellHost):
+        r = orc().orc_bicgstab_pipelined_sell(C.byref(A.desc), p(b), p(x), tol, abs_tol, max_iters, p(h))
+    else:
+        r = orc().orc_bicgstab_pipelined_csr(C.byref(A.desc), p(b), p(x), tol, abs_tol, max_iters, p(h))
+    return x, r.iters, r.error, (h[: r.iters] if history else None)
+
+
+def bicgstab_classic(A, b, tol, max_iters, diag=None, max_before_restart=200, abs_tol=0.0, history=False):
+    b = c64(b)
+    x = np.zeros_like(b)
+    d = c64(diag) if diag is not None else None
+    h = np.zeros(max(max_iters, 1)) if history else None
+    r = orc().orc_bicgstab_classic_csr(C.byref(A.desc), p(b), p(x), p(d), tol, abs_tol, max_iters,
+                                       max_before_restart, p(h))
+    return x, r.iters, r.error, (h[: r.iters] if history else None)
+
+
+def mixed_cg(A, b, tol, max_iters, inner_tol=1e-2):
+    b = c64(b)
+    x = np.zeros_like(b)
+    rest = u32()
+    r = orc().orc_mixed_cg_csr(C.byref(A.desc), p(b), p(x), tol, max_iters, inner_tol, C.byref(rest))
+    return x, r.iters, r.error, rest.value
+
+
+def batched_cg(sys_ptr, A, b, tol, max_iters):
+    sys_ptr = cu32(sys_ptr)
+    b = c64(b)
+    x = np.zeros_like(b)
+    ns = sys_ptr.size - 1
+    iters = np.zeros(ns, np.uint32)
+    errs = np.zeros(ns)
+    orc().orc_batched_cg_csr(ns, p(sys_ptr), C.byref(A.desc), p(b), p(x), tol, max_iters, p(iters), p(errs))
+    return x, iters, errs
+
+
+def adjoint_grad(rp, col, lam, x):
+    rp, col, lam, x = cu32(rp), cu32(col), c64(lam), c64(x)
+    out = np.zeros(col.size)
+    orc().orc_adjoint_grad_csr(rp.size - 1, p(rp), p(col), p(lam), p(x), p(out))
+    return out
+
+
+# ------------------------------------------------------------------ reference-build wrappers
+
+def ref_cg(rp, col, val, b, tol, max_iters, jacobi=False, abs_tol=0.0):
+    rp, col, val, b = cu32(rp), cu32(col), c64(val), c64(b)
+    x = np.zeros_like(b)
+    it, err = u32(), f64()
+    ref().ref_cg_csr(rp.size - 1, p(rp), p(col), p(val), p(b), p(x), tol, abs_tol, max_iters, int(jacobi),
+                     C.byref(it), C.byref(err))
+    return x, it.value, err.value
+
+
+def ref_cg_sell(rp, col, val, C_, b, tol, max_iters, abs_tol=0.0):
+    rp, col, val, b = cu32(rp), cu32(col), c64(val), c64(b)
+    x = np.zeros_like(b)
+    it, err = u32(), f64()
+    ref().ref_cg_sell(rp.size - 1, C_, p(rp), p(col), p(val), p(b), p(x), tol, abs_tol, max_iters,
+                      C.byref(it), C.byref(err))
+    return x, it.value, err.value
+
+
+def ref_bicgstab(rp, col, val, b, tol, max_iters, jacobi=False, max_before_restart=200, abs_tol=0.0):
+    rp, col, val, b = cu32(rp), cu32(col), c64(val), c64(b)
+    x = np.zeros_like(b)
+    it, err = u32(), f64()
+    ref().ref_bicgstab_csr(rp.size - 1, p(rp), p(col), p(val), p(b), p(x), tol, abs_tol, max_iters,
+                           max_before_restart, int(jacobi), C.byref(it), C.byref(err))
+    return x, it.value, err.value
+
+
+def ref_bicgstab_sell(rp, col, val, C_, b, tol, max_iters, abs_tol=0.0):
+    rp, col, val, b = cu32(rp), cu32(col), c64(val), c64(b)
+    x = np.zeros_like(b)
+    it, err = u32(), f64()
+    ref().ref_bicgstab_sell(rp.size - 1, C_, p(rp), p(col), p(val), p(b), p(x), tol, abs_tol, max_iters,
+                            C.byref(it), C.byref(err))
+    return x, it.value, err.value
+
+
+def ref_mixed_cg(rp, col, val, b, tol, max_iters, inner_tol=1e-2):
+    rp, col, val, b = cu32(rp), cu32(col), c64(val), c64(b)
+    x = np.zeros_like(b)
+    it, err = u32(), f64()
+    ref().ref_mixed_cg_csr(rp.size - 1, p(rp), p(col), p(val), p(b), p(x), tol, max_iters, inner_tol,
+                           C.byref(it), C.byref(err))
+    return x, it.value, err.value

@@ -1,0 +1,391 @@
+// ref_shim.cpp -- C entry points over the REAL ViennaCL 1.7.1 host_based
+// backend, compiled from the reference headers where they lie (oracle/Makefile
+// passes -I<reference root>).  TEST INFRASTRUCTURE ONLY: used to pin the oracle
+// restatement (vcl_oracle.c) and as the "reference" CPU baseline in bench.py.
+// The output library lives in oracle/_ref/ and is never linked by the product.
+//
+// Everything here is glue written for this project: it converts raw CSR arrays
+// into the reference's own types and calls the reference's public API.
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "viennacl/compressed_matrix.hpp"
+#include "viennacl/sliced_ell_matrix.hpp"
+#include "viennacl/vector.hpp"
+#include "viennacl/linalg/prod.hpp"
+#include "viennacl/linalg/inner_prod.hpp"
+#include "viennacl/linalg/norm_2.hpp"
+#include "viennacl/linalg/cg.hpp"
+#include "viennacl/linalg/bicgstab.hpp"
+#include "viennacl/linalg/mixed_precision_cg.hpp"
+#include "viennacl/linalg/jacobi_precond.hpp"
+#include "viennacl/linalg/iterative_operations.hpp"
+#include "viennacl/tools/matrix_generation.hpp"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+typedef std::vector<std::map<unsigned int, double> > stl_matrix;
+
+void fill_csr(viennacl::compressed_matrix<double>& A, uint32_t n, const uint32_t* rp, const uint32_t* col,
+              const double* val)
+{
+  std::vector<unsigned int> r(rp, rp + n + 1), c(col, col + rp[n]);
+  A.set(&r[0], &c[0], val, n, n, rp[n]);
+}
+
+stl_matrix to_stl(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val)
+{
+  stl_matrix m(n);
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
+      m[i][col[k]] = val[k];
+  return m;
+}
+
+void fill_sell(viennacl::sliced_ell_matrix<double>& A, uint32_t n, const uint32_t* rp, const uint32_t* col,
+               const double* val)
+{
+  viennacl::copy(to_stl(n, rp, col, val), A);
+}
+
+void to_vcl(const double* src, viennacl::vector<double>& v)
+{
+  viennacl::fast_copy(src, src + v.size(), v.begin());
+}
+
+void from_vcl(const viennacl::vector<double>& v, double* dst)
+{
+  viennacl::fast_copy(v.begin(), v.end(), dst);
+}
+
+// Minimal matrix adapter with the clear/resize/operator() interface that
+// tools::generate_fdm_laplace (tools/matrix_generation.hpp:47-86) writes into.
+struct map_matrix {
+  stl_matrix rows;
+  struct ref {
+    map_matrix* m; std::size_t r, c;
+    ref& operator=(double v) { m->rows[r][(unsigned int)c] = v; return *this; }
+  };
+  void clear() { rows.clear(); }
+  void resize(std::size_t r, std::size_t, bool) { rows.assign(r, std::map<unsigned int, double>()); }
+  ref operator()(std::size_t r, std::size_t c) { ref x = {this, r, c}; return x; }
+};
+
+}  // namespace
+
+extern "C" {
+
+void ref_set_num_threads(int n)
+{
+#ifdef _OPENMP
+  omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+// tools/matrix_generation.hpp:47-86 -> CSR; returns nnz (arrays sized by caller: n+1, 5n)
+uint32_t ref_fdm_laplace(uint32_t px, uint32_t py, uint32_t* rp, uint32_t* col, double* val)
+{
+  map_matrix m;
+  viennacl::tools::generate_fdm_laplace(m, px, py);
+  uint32_t k = 0;
+  rp[0] = 0;
+  for (std::size_t i = 0; i < m.rows.size(); ++i) {
+    for (std::map<unsigned int, double>::const_iterator it = m.rows[i].begin(); it != m.rows[i].end(); ++it) {
+      col[k] = it->first;
+      val[k] = it->second;
+      ++k;
+    }
+    rp[i + 1] = k;
+  }
+  return k;
+}
+
+// y = alpha*A*x + beta*y through linalg::prod_impl (linalg/sparse_matrix_operations.hpp:90-121)
+void ref_csr_spmv_f64(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val,
+                      const double* x, double alpha, double* y, double beta)
+{
+  viennacl::compressed_matrix<double> A;
+  fill_csr(A, n, rp, col, val);
+  viennacl::vector<double> vx(n), vy(n);
+  to_vcl(x, vx);
+  to_vcl(y, vy);
+  viennacl::linalg::prod_impl(A, vx, alpha, vy, beta);
+  from_vcl(vy, y);
+}
+
+void ref_sell_spmv_f64(uint32_t n, uint32_t C, const uint32_t* rp, const uint32_t* col, const double* val,
+                       const double* x, double alpha, double* y, double beta)
+{
+  viennacl::sliced_ell_matrix<double> A(n, n, C);
+  fill_sell(A, n, rp, col, val);
+  viennacl::vector<double> vx(n), vy(n);
+  to_vcl(x, vx);
+  to_vcl(y, vy);
+  viennacl::linalg::prod_impl(A, vx, alpha, vy, beta);
+  from_vcl(vy, y);
+}
+
+// The reference's SELL arrays (sliced_ell_matrix.hpp:140-214), for byte comparison.
+void ref_sell_arrays(uint32_t n, uint32_t C, const uint32_t* rp, const uint32_t* col, const double* val,
+                     uint32_t* cpb, uint32_t* ci, uint32_t* bs, double* el, uint64_t stored)
+{
+  viennacl::sliced_ell_matrix<double> A(n, n, C);
+  fill_sell(A, n, rp, col, val);
+  uint32_t ns = (n + C - 1) / C;
+  viennacl::backend::memory_read(A.handle1(), 0, ns * sizeof(uint32_t), cpb);
+  viennacl::backend::memory_read(A.handle2(), 0, stored * sizeof(uint32_t), ci);
+  viennacl::backend::memory_read(A.handle3(), 0, ns * sizeof(uint32_t), bs);
+  viennacl::backend::memory_read(A.handle(), 0, stored * sizeof(double), el);
+}
+
+// compressed_matrix::generate_row_block_information (compressed_matrix.hpp:1152-1188)
+uint32_t ref_row_blocks(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val, uint32_t* out)
+{
+  viennacl::compressed_matrix<double> A;
+  fill_csr(A, n, rp, col, val);
+  uint32_t nb = (uint32_t)A.blocks1();
+  viennacl::backend::memory_read(A.handle3(), 0, (nb + 1) * sizeof(uint32_t), out);
+  return nb;
+}
+
+void ref_row_info_diag(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val, double* out)
+{
+  viennacl::compressed_matrix<double> A;
+  fill_csr(A, n, rp, col, val);
+  viennacl::vector<double> d(n);
+  viennacl::linalg::detail::row_info(A, d, viennacl::linalg::detail::SPARSE_ROW_DIAGONAL);
+  from_vcl(d, out);
+}
+
+double ref_dot_f64(uint32_t n, const double* x, const double* y)
+{
+  viennacl::vector<double> vx(n), vy(n);
+  to_vcl(x, vx);
+  to_vcl(y, vy);
+  return viennacl::linalg::inner_prod(vx, vy);
+}
+
+double ref_norm2_f64(uint32_t n, const double* x)
+{
+  viennacl::vector<double> vx(n);
+  to_vcl(x, vx);
+  return viennacl::linalg::norm_2(vx);
+}
+
+// linalg::pipelined_cg_vector_update (iterative_operations.hpp:58-87); buf holds 3*256 doubles
+void ref_pipelined_cg_vector_update(uint32_t n, double* x, double alpha, double* p, double* r, const double* Ap,
+                                    double beta, double* buf)
+{
+  viennacl::vector<double> vx(n), vp(n), vr(n), vAp(n), vb(768);
+  to_vcl(x, vx); to_vcl(p, vp); to_vcl(r, vr); to_vcl(Ap, vAp); to_vcl(buf, vb);
+  viennacl::linalg::pipelined_cg_vector_update(vx, alpha, vp, vr, vAp, beta, vb);
+  from_vcl(vx, x); from_vcl(vp, p); from_vcl(vr, r); from_vcl(vb, buf);
+}
+
+// linalg::pipelined_cg_prod (iterative_operations.hpp:96-122)
+void ref_pipelined_cg_prod_csr(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val,
+                               const double* p, double* Ap, double* buf)
+{
+  viennacl::compressed_matrix<double> A;
+  fill_csr(A, n, rp, col, val);
+  viennacl::vector<double> vp(n), vAp(n), vb(768);
+  to_vcl(p, vp); to_vcl(Ap, vAp); to_vcl(buf, vb);
+  viennacl::linalg::pipelined_cg_prod(A, vp, vAp, vb);
+  from_vcl(vAp, Ap); from_vcl(vb, buf);
+}
+
+// linalg::pipelined_bicgstab_* (iterative_operations.hpp:133-236); buf holds 6*256 doubles
+void ref_pipelined_bicgstab_prod_csr(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val,
+                                     const double* p, double* Ap, const double* r0, double* buf, uint32_t offset)
+{
+  viennacl::compressed_matrix<double> A;
+  fill_csr(A, n, rp, col, val);
+  viennacl::vector<double> vp(n), vAp(n), vr0(n), vb(1536);
+  to_vcl(p, vp); to_vcl(Ap, vAp); to_vcl(r0, vr0); to_vcl(buf, vb);
+  viennacl::linalg::pipelined_bicgstab_prod(A, vp, vAp, vr0, vb, 256, offset);
+  from_vcl(vAp, Ap); from_vcl(vb, buf);
+}
+
+void ref_pipelined_bicgstab_update_s(uint32_t n, double* s, double* r, const double* Ap, double* buf, uint32_t offset)
+{
+  viennacl::vector<double> vs(n), vr(n), vAp(n), vb(1536);
+  to_vcl(s, vs); to_vcl(r, vr); to_vcl(Ap, vAp); to_vcl(buf, vb);
+  viennacl::linalg::pipelined_bicgstab_update_s(vs, vr, vAp, vb, 256, offset);
+  from_vcl(vs, s); from_vcl(vb, buf);
+}
+
+void ref_pipelined_bicgstab_vector_update(uint32_t n, double* x, double alpha, double* p, double omega,
+                                          const double* s, double* r, const double* As, double beta,
+                                          const double* Ap, const double* r0, double* buf)
+{
+  viennacl::vector<double> vx(n), vp(n), vs(n), vr(n), vAs(n), vAp(n), vr0(n), vb(1536);
+  to_vcl(x, vx); to_vcl(p, vp); to_vcl(s, vs); to_vcl(r, vr); to_vcl(As, vAs); to_vcl(Ap, vAp);
+  to_vcl(r0, vr0); to_vcl(buf, vb);
+  viennacl::linalg::pipelined_bicgstab_vector_update(vx, alpha, vp, omega, vs, vr, vAs, beta, vAp, vr0, vb, 256);
+  from_vcl(vx, x); from_vcl(vp, p); from_vcl(vr, r); from_vcl(vb, buf);
+}
+
+// ---- timing of the reference's own loops (CPU baseline): only the calls into the reference are timed
+
+double ref_bench_csr_spmv(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val, const double* x,
+                          double* y, uint32_t reps)
+{
+  viennacl::compressed_matrix<double> A;
+  fill_csr(A, n, rp, col, val);
+  viennacl::vector<double> vx(n), vy(n);
+  to_vcl(x, vx);
+  vy = viennacl::linalg::prod(A, vx);  // warm-up
+  viennacl::backend::finish();
+  double t0 = omp_get_wtime();
+  for (uint32_t r = 0; r < reps; ++r)
+    vy = viennacl::linalg::prod(A, vx);
+  viennacl::backend::finish();
+  double t1 = omp_get_wtime();
+  from_vcl(vy, y);
+  return t1 - t0;
+}
+
+// Solve time of cg (jacobi = 0/1) or bicgstab (kind = 1); iterations and error returned.
+double ref_bench_solve(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val, const double* b,
+                       double* x, double tol, uint32_t max_iters, int kind, int jacobi, uint32_t* iters, double* error)
+{
+  viennacl::compressed_matrix<double> A;
+  fill_csr(A, n, rp, col, val);
+  viennacl::vector<double> vb(n), vx;
+  to_vcl(b, vb);
+  double t0 = 0, t1 = 0;
+  if (kind == 0) {
+    viennacl::linalg::cg_tag tag(tol, max_iters);
+    if (jacobi) {
+      viennacl::linalg::jacobi_precond<viennacl::compressed_matrix<double> > J(A, viennacl::linalg::jacobi_tag());
+      t0 = omp_get_wtime();
+      vx = viennacl::linalg::solve(A, vb, tag, J);
+      t1 = omp_get_wtime();
+    } else {
+      t0 = omp_get_wtime();
+      vx = viennacl::linalg::solve(A, vb, tag);
+      t1 = omp_get_wtime();
+    }
+    *iters = tag.iters();
+    *error = tag.error();
+  } else {
+    viennacl::linalg::bicgstab_tag tag(tol, max_iters);
+    t0 = omp_get_wtime();
+    vx = viennacl::linalg::solve(A, vb, tag);
+    t1 = omp_get_wtime();
+    *iters = (uint32_t)tag.iters();
+    *error = tag.error();
+  }
+  from_vcl(vx, x);
+  return t1 - t0;
+}
+
+int ref_max_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// ---- solvers through linalg::solve (cg.hpp:338-375, bicgstab.hpp:495-533, mixed_precision_cg.hpp:95-193)
+
+void ref_cg_csr(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val, const double* b,
+                double* x, double tol, double abs_tol, uint32_t max_iters, int jacobi,
+                uint32_t* iters, double* error)
+{
+  viennacl::compressed_matrix<double> A;
+  fill_csr(A, n, rp, col, val);
+  viennacl::vector<double> vb(n);
+  to_vcl(b, vb);
+  viennacl::linalg::cg_tag tag(tol, max_iters);
+  tag.abs_tolerance(abs_tol);
+  viennacl::vector<double> vx;
+  if (jacobi) {
+    viennacl::linalg::jacobi_precond<viennacl::compressed_matrix<double> > J(A, viennacl::linalg::jacobi_tag());
+    vx = viennacl::linalg::solve(A, vb, tag, J);
+  } else {
+    vx = viennacl::linalg::solve(A, vb, tag);
+  }
+  from_vcl(vx, x);
+  *iters = tag.iters();
+  *error = tag.error();
+}
+
+void ref_cg_sell(uint32_t n, uint32_t C, const uint32_t* rp, const uint32_t* col, const double* val, const double* b,
+                 double* x, double tol, double abs_tol, uint32_t max_iters, uint32_t* iters, double* error)
+{
+  viennacl::sliced_ell_matrix<double> A(n, n, C);
+  fill_sell(A, n, rp, col, val);
+  viennacl::vector<double> vb(n);
+  to_vcl(b, vb);
+  viennacl::linalg::cg_tag tag(tol, max_iters);
+  tag.abs_tolerance(abs_tol);
+  viennacl::vector<double> vx = viennacl::linalg::solve(A, vb, tag);
+  from_vcl(vx, x);
+  *iters = tag.iters();
+  *error = tag.error();
+}
+
+void ref_bicgstab_csr(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val, const double* b,
+                      double* x, double tol, double abs_tol, uint32_t max_iters, uint32_t max_before_restart,
+                      int jacobi, uint32_t* iters, double* error)
+{
+  viennacl::compressed_matrix<double> A;
+  fill_csr(A, n, rp, col, val);
+  viennacl::vector<double> vb(n);
+  to_vcl(b, vb);
+  viennacl::linalg::bicgstab_tag tag(tol, max_iters, max_before_restart);
+  tag.abs_tolerance(abs_tol);
+  viennacl::vector<double> vx;
+  if (jacobi) {
+    viennacl::linalg::jacobi_precond<viennacl::compressed_matrix<double> > J(A, viennacl::linalg::jacobi_tag());
+    vx = viennacl::linalg::solve(A, vb, tag, J);
+  } else {
+    vx = viennacl::linalg::solve(A, vb, tag);
+  }
+  from_vcl(vx, x);
+  *iters = (uint32_t)tag.iters();
+  *error = tag.error();
+}
+
+void ref_bicgstab_sell(uint32_t n, uint32_t C, const uint32_t* rp, const uint32_t* col, const double* val,
+                       const double* b, double* x, double tol, double abs_tol, uint32_t max_iters,
+                       uint32_t* iters, double* error)
+{
+  viennacl::sliced_ell_matrix<double> A(n, n, C);
+  fill_sell(A, n, rp, col, val);
+  viennacl::vector<double> vb(n);
+  to_vcl(b, vb);
+  viennacl::linalg::bicgstab_tag tag(tol, max_iters);
+  tag.abs_tolerance(abs_tol);
+  viennacl::vector<double> vx = viennacl::linalg::solve(A, vb, tag);
+  from_vcl(vx, x);
+  *iters = (uint32_t)tag.iters();
+  *error = tag.error();
+}
+
+void ref_mixed_cg_csr(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val, const double* b,
+                      double* x, double tol, uint32_t max_iters, float inner_tol, uint32_t* iters, double* error)
+{
+  viennacl::compressed_matrix<double> A;
+  fill_csr(A, n, rp, col, val);
+  viennacl::vector<double> vb(n);
+  to_vcl(b, vb);
+  viennacl::linalg::mixed_precision_cg_tag tag(tol, max_iters, inner_tol);
+  viennacl::vector<double> vx = viennacl::linalg::solve(A, vb, tag);
+  from_vcl(vx, x);
+  *iters = tag.iters();
+  *error = tag.error();
+}
+
+}  // extern "C"

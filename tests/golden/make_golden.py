@@ -1,0 +1,166 @@
+"""Writes the golden fixtures of tests/golden/*.npz from the REAL ViennaCL 1.7.1 host backend.
+
+The reference is compiled in place by oracle/Makefile (target `ref`, output oracle/_ref/) and
+called single-threaded through its public API (oracle/ref_shim.cpp).  Each fixture stores the
+inputs and the reference's outputs, so the tests pin the oracle and the HIP path without the
+reference being present.  The reference's own SpMV test matrix (examples/testdata/mat65k.mtx) is
+not available, so these seeded problems stand in for it.
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+import pathlib
+import sys
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT / "oracle"))
+sys.path.insert(0, str(ROOT / "viennacl-dev_amd"))
+
+import pyoracle as po  # noqa: E402
+import vclhip as vh  # noqa: E402
+
+OUT = pathlib.Path(__file__).resolve().parent
+
+
+def save(name, **arrays):
+    np.savez_compressed(OUT / f"{name}.npz", **arrays)
+    print("wrote", name, {k: getattr(v, "shape", v) for k, v in arrays.items()})
+
+
+def with_empty_rows(rp, col, val, rows):
+    """Removes all entries of the given rows (empty CSR rows)."""
+    keep = np.ones(col.size, bool)
+    for r in rows:
+        keep[rp[r]:rp[r + 1]] = False
+    lens = np.diff(rp.astype(np.int64))
+    for r in rows:
+        lens[r] = 0
+    nrp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    return nrp, col[keep], val[keep]
+
+
+def main():
+    po.set_threads(1)
+    R = po.ref()
+    p = po.p
+
+    # generate_fdm_laplace (tools/matrix_generation.hpp:47-86)
+    px, py = 12, 9
+    n = px * py
+    rp = np.zeros(n + 1, np.uint32)
+    col = np.zeros(5 * n, np.uint32)
+    val = np.zeros(5 * n)
+    nnz = R.ref_fdm_laplace(px, py, p(rp), p(col), p(val))
+    save("fdm_laplace_12x9", px=px, py=py, row_ptr=rp, col=col[:nnz], val=val[:nnz])
+
+    # SpMV y = alpha A x + beta y (linalg::prod_impl)
+    cases = {
+        "lap2d_30x20": vh.fdm_laplace_2d(30, 20),
+        "powerlaw_3000": vh.powerlaw(3000, max_len=300, seed=5),
+    }
+    rp0, col0, val0 = vh.fdm_laplace_2d(20, 20)
+    cases["empty_rows_400"] = with_empty_rows(rp0, col0, val0, [0, 7, 133, 399])
+    for name, (rp, col, val) in cases.items():
+        n = rp.size - 1
+        x = vh.random_uniform(11, n)
+        y0 = vh.random_uniform(12, n)
+        outs = {}
+        for tag, alpha, beta in (("a1b0", 1.0, 0.0), ("a2b05", 2.0, 0.5), ("am1b1", -1.0, 1.0)):
+            y = y0.copy()
+            R.ref_csr_spmv_f64(n, p(rp), p(col), p(val), p(x), alpha, p(y), beta)
+            outs[f"y_{tag}"] = y
+        save(f"spmv_{name}", row_ptr=rp, col=col, val=val, x=x, y0=y0, **outs)
+
+    # SELL-C arrays and SpMV (sliced_ell_matrix.hpp:140-214, host_based :1796-1868); n % C != 0
+    for name, C_, (rp, col, val) in (("lap2d_37x29_c32", 32, vh.fdm_laplace_2d(37, 29)),
+                                     ("powerlaw_1001_c64", 64, vh.powerlaw(1001, max_len=200, seed=6))):
+        n = rp.size - 1
+        ns = (n + C_ - 1) // C_
+        S = po.SellHost(rp, col, val, C_)
+        stored = S.el.size
+        cpb = np.zeros(ns, np.uint32)
+        bs = np.zeros(ns, np.uint32)
+        ci = np.zeros(stored, np.uint32)
+        el = np.zeros(stored)
+        R.ref_sell_arrays(n, C_, p(rp), p(col), p(val), p(cpb), p(ci), p(bs), p(el), stored)
+        x = vh.random_uniform(13, n)
+        y0 = vh.random_uniform(14, n)
+        y = y0.copy()
+        R.ref_sell_spmv_f64(n, C_, p(rp), p(col), p(val), p(x), 1.5, p(y), -0.25)
+        save(f"sell_{name}", C=C_, row_ptr=rp, col=col, val=val, columns_per_block=cpb, column_indices=ci,
+             block_start=bs, elements=el, x=x, y0=y0, y=y, alpha=1.5, beta=-0.25)
+
+    # row blocks (compressed_matrix.hpp:1152-1188) and the Jacobi diagonal (row_info)
+    rp, col, val = vh.powerlaw(5000, max_len=1500, seed=7)
+    n = rp.size - 1
+    out = np.zeros(n + 1, np.uint32)
+    nb = R.ref_row_blocks(n, p(rp), p(col), p(val), p(out))
+    d = np.zeros(n)
+    R.ref_row_info_diag(n, p(rp), p(col), p(val), p(d))
+    save("row_blocks_powerlaw_5000", row_ptr=rp, col=col, val=val, row_blocks=out[: nb + 1], diag=d)
+
+    # pipelined kernels (iterative_operations.hpp), one call each
+    rp, col, val = vh.fdm_laplace_2d(25, 17)
+    n = rp.size - 1
+    g = [vh.random_uniform(20 + i, n) for i in range(8)]
+    x, pv, r, Ap = g[0].copy(), g[1].copy(), g[2].copy(), g[3].copy()
+    buf = np.zeros(768)
+    R.ref_pipelined_cg_vector_update(n, p(x), 0.37, p(pv), p(r), p(Ap), -0.61, p(buf))
+    Ap2 = np.zeros(n)
+    buf2 = np.zeros(768)
+    R.ref_pipelined_cg_prod_csr(n, p(rp), p(col), p(val), p(g[1]), p(Ap2), p(buf2))
+    Ap3 = np.zeros(n)
+    buf3 = np.zeros(1536)
+    R.ref_pipelined_bicgstab_prod_csr(n, p(rp), p(col), p(val), p(g[1]), p(Ap3), p(g[4]), p(buf3), 768)
+    s = np.zeros(n)
+    buf4 = np.zeros(1536)
+    buf4[0] = 3.5
+    buf4[768] = -1.25
+    R.ref_pipelined_bicgstab_update_s(n, p(s), p(g[2]), p(g[3]), p(buf4), 1280)
+    x5, p5, r5 = g[0].copy(), g[1].copy(), g[2].copy()
+    buf5 = np.zeros(1536)
+    R.ref_pipelined_bicgstab_vector_update(n, p(x5), 0.3, p(p5), 0.7, p(g[5]), p(r5), p(g[6]), -0.2, p(g[7]),
+                                           p(g[4]), p(buf5))
+    save("pipelined_ops_25x17", row_ptr=rp, col=col, val=val, g=np.stack(g),
+         cgvu_x=x, cgvu_p=pv, cgvu_r=r, cgvu_buf=buf, cgprod_Ap=Ap2, cgprod_buf=buf2,
+         bprod_Ap=Ap3, bprod_buf=buf3, upds_s=s, upds_buf=buf4, bvu_x=x5, bvu_p=p5, bvu_r=r5, bvu_buf=buf5)
+
+    # solvers through linalg::solve
+    def solver_case(name, rp, col, val, b, **meta):
+        save(name, row_ptr=rp, col=col, val=val, b=b, **meta)
+
+    rp, col, val = vh.fdm_laplace_2d(40, 40)
+    b = vh.random_uniform(0, rp.size - 1)
+    x, it, err = po.ref_cg(rp, col, val, b, 1e-8, 2000)
+    xb, itb, errb = po.ref_bicgstab(rp, col, val, b, 1e-8, 2000)
+    xm, itm, errm = po.ref_mixed_cg(rp, col, val, b, 1e-8, 2000)
+    x2, it2, err2 = po.ref_cg(rp, col, val, b, 1e-8, 40)  # stops at max_iters
+    solver_case("solve_lap2d_40x40", rp, col, val, b, cg_x=x, cg_iters=it, cg_error=err,
+                bicgstab_x=xb, bicgstab_iters=itb, bicgstab_error=errb,
+                mixed_x=xm, mixed_iters=itm, mixed_error=errm,
+                cg40_x=x2, cg40_iters=it2, cg40_error=err2)
+
+    k = np.exp(vh.random_normal(1, 30 * 30, 0.0, 1.0))
+    rp, col, val = vh.diffusion(30, 30, 1, k)
+    b = vh.random_uniform(0, rp.size - 1)
+    x, it, err = po.ref_cg(rp, col, val, b, 1e-10, 3000, jacobi=True)
+    xb, itb, errb = po.ref_bicgstab(rp, col, val, b, 1e-10, 3000, jacobi=True, max_before_restart=20)
+    solver_case("solve_diffusion2d_30x30", rp, col, val, b, jcg_x=x, jcg_iters=it, jcg_error=err,
+                jbicgstab_x=xb, jbicgstab_iters=itb, jbicgstab_error=errb, restart_every=20)
+
+    rp, col, val = vh.fdm_laplace_2d(37, 29)
+    b = vh.random_uniform(0, rp.size - 1)
+    x, it, err = po.ref_cg_sell(rp, col, val, 32, b, 1e-8, 2000)
+    xb, itb, errb = po.ref_bicgstab_sell(rp, col, val, 32, b, 1e-8, 2000)
+    solver_case("solve_sell_lap2d_37x29", rp, col, val, b, C=32, cg_x=x, cg_iters=it, cg_error=err,
+                bicgstab_x=xb, bicgstab_iters=itb, bicgstab_error=errb)
+
+    rp, col, val = vh.powerlaw(2000, max_len=512, seed=1)
+    b = vh.random_uniform(0, rp.size - 1)
+    xb, itb, errb = po.ref_bicgstab(rp, col, val, b, 1e-10, 400)
+    solver_case("solve_powerlaw_2000", rp, col, val, b, bicgstab_x=xb, bicgstab_iters=itb, bicgstab_error=errb)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,244 @@
+"""GPU parity of the pipelined operations and the CG / BiCGStab solvers.
+
+Bars (north star): iteration counts within 2% of the reference (golden fixtures / oracle), the same
+stopping quantity below the tag tolerance, and the FP64 true residual ||b - A x|| / ||b|| verified.
+Element-wise updates are bit-exact against the reference (no contraction, same expressions);
+inner products differ only in summation order.
+"""
+import ctypes as C
+import pathlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+
+
+def load(name):
+    return dict(np.load(GOLDEN / f"{name}.npz", allow_pickle=False))
+
+
+def close_sum(a, b, scale):
+    return abs(a - b) <= 1e-12 * scale
+
+
+def iters_close(got, want):
+    return abs(int(got) - int(want)) <= max(2, int(np.ceil(0.02 * want)))
+
+
+def true_relres(rp, col, val, x, b):
+    import pyoracle
+    r = b - pyoracle.csr_spmv(rp, col, val, x)
+    return np.linalg.norm(r) / np.linalg.norm(b)
+
+
+def test_pipelined_ops_golden(dev, vh):
+    g = load("pipelined_ops_25x17")
+    G = [np.ascontiguousarray(v) for v in g["g"]]
+    n = G[0].size
+    A = vh.CsrMatrix(g["row_ptr"], g["col"], g["val"])
+    d = [vh.DeviceArray.from_numpy(v) for v in G]
+    # pipelined_cg_vector_update
+    x, p, r, Ap = (vh.DeviceArray.from_numpy(G[i]) for i in range(4))
+    buf = vh.DeviceArray.zeros(768, np.float64)
+    vh.call("vclhip_pipelined_cg_vector_update_f64", n, x.ptr, 0.37, p.ptr, r.ptr, Ap.ptr, -0.61, buf.ptr, 256, None)
+    assert np.array_equal(x.to_numpy(), g["cgvu_x"]) and np.array_equal(p.to_numpy(), g["cgvu_p"])
+    assert np.array_equal(r.to_numpy(), g["cgvu_r"])
+    hb = buf.to_numpy()
+    assert close_sum(hb[0], g["cgvu_buf"][0], n) and not hb[1:].any()
+    # pipelined_cg_prod
+    Ap2 = vh.DeviceArray.zeros(n, np.float64)
+    buf2 = vh.DeviceArray.zeros(768, np.float64)
+    vh.call("vclhip_pipelined_cg_prod_csr_f64", C.byref(A.desc), d[1].ptr, Ap2.ptr, buf2.ptr, 256, None)
+    assert np.array_equal(Ap2.to_numpy(), g["cgprod_Ap"])
+    hb, gb = buf2.to_numpy(), g["cgprod_buf"]
+    assert close_sum(hb[256], gb[256], 100 * n) and close_sum(hb[512], gb[512], 100 * n)
+    assert not hb[257:512].any() and not hb[513:].any()
+    # pipelined_bicgstab_prod with offset 768
+    Ap3 = vh.DeviceArray.zeros(n, np.float64)
+    buf3 = vh.DeviceArray.zeros(1536, np.float64)
+    vh.call("vclhip_pipelined_bicgstab_prod_csr_f64", C.byref(A.desc), d[1].ptr, Ap3.ptr, d[4].ptr, buf3.ptr, 256, 768,
+            None)
+    assert np.array_equal(Ap3.to_numpy(), g["bprod_Ap"])
+    hb, gb = buf3.to_numpy(), g["bprod_buf"]
+    for k in (256, 512, 768):
+        assert close_sum(hb[k], gb[k], 100 * n)
+    # pipelined_bicgstab_update_s: alpha = 3.5 / -1.25 from the buffer, on the device
+    s = vh.DeviceArray.zeros(n, np.float64)
+    b4 = np.zeros(1536)
+    b4[0], b4[768] = 3.5, -1.25
+    buf4 = vh.DeviceArray.from_numpy(b4)
+    vh.call("vclhip_pipelined_bicgstab_update_s_f64", n, s.ptr, d[2].ptr, d[3].ptr, buf4.ptr, 256, 1280, None)
+    assert np.array_equal(s.to_numpy(), g["upds_s"])
+    assert close_sum(buf4.to_numpy()[1280], g["upds_buf"][1280], 100 * n)
+    # pipelined_bicgstab_vector_update
+    x5, p5, r5 = (vh.DeviceArray.from_numpy(G[i]) for i in range(3))
+    buf5 = vh.DeviceArray.zeros(1536, np.float64)
+    vh.call("vclhip_pipelined_bicgstab_vector_update_f64", n, x5.ptr, 0.3, p5.ptr, 0.7, d[5].ptr, r5.ptr, d[6].ptr,
+            -0.2, d[7].ptr, d[4].ptr, buf5.ptr, 256, None)
+    assert np.array_equal(x5.to_numpy(), g["bvu_x"]) and np.array_equal(p5.to_numpy(), g["bvu_p"])
+    assert np.array_equal(r5.to_numpy(), g["bvu_r"])
+    assert close_sum(buf5.to_numpy()[0], g["bvu_buf"][0], n)
+
+
+def solve(vh, kind, A, b, precond=0, **kw):
+    db = vh.DeviceArray.from_numpy(b)
+    dx = vh.DeviceArray.zeros(b.size, np.float64)
+    fn = vh.cg if kind == "cg" else vh.bicgstab
+    res = fn(A, db, dx, precond=precond, **kw)
+    return dx.to_numpy(), res
+
+
+@pytest.mark.parametrize("fmt", ["csr", "sell"])
+def test_cg_lap2d_golden(dev, vh, fmt):
+    if fmt == "csr":
+        g = load("solve_lap2d_40x40")
+        A = vh.CsrMatrix(g["row_ptr"], g["col"], g["val"])
+    else:
+        g = load("solve_sell_lap2d_37x29")
+        A = vh.SellMatrix(g["row_ptr"], g["col"], g["val"], int(g["C"]))
+    x, res = solve(vh, "cg", A, g["b"], tol=1e-8, max_iters=2000)
+    assert iters_close(res.iters, g["cg_iters"]) and res.converged
+    assert res.error < 1e-8
+    assert true_relres(g["row_ptr"], g["col"], g["val"], x, g["b"]) < 1.05e-8
+    assert np.linalg.norm(x - g["cg_x"]) / np.linalg.norm(g["cg_x"]) < 1e-6
+
+
+def test_cg_max_iters_golden(dev, vh):
+    g = load("solve_lap2d_40x40")
+    A = vh.CsrMatrix(g["row_ptr"], g["col"], g["val"])
+    x, res = solve(vh, "cg", A, g["b"], tol=1e-8, max_iters=40)
+    assert res.iters == 40 and not res.converged
+    assert abs(res.error - g["cg40_error"]) <= 1e-9 * g["cg40_error"]
+    assert np.linalg.norm(x - g["cg40_x"]) / np.linalg.norm(g["cg40_x"]) < 1e-10
+
+
+@pytest.mark.parametrize("fmt", ["csr", "sell"])
+def test_bicgstab_golden(dev, vh, fmt):
+    if fmt == "csr":
+        g = load("solve_lap2d_40x40")
+        A = vh.CsrMatrix(g["row_ptr"], g["col"], g["val"])
+    else:
+        g = load("solve_sell_lap2d_37x29")
+        A = vh.SellMatrix(g["row_ptr"], g["col"], g["val"], int(g["C"]))
+    x, res = solve(vh, "bicgstab", A, g["b"], tol=1e-8, max_iters=2000)
+    assert iters_close(res.iters, g["bicgstab_iters"]) and res.converged and res.error < 1e-8
+    # x includes the final update, so the true residual is the tested one
+    assert true_relres(g["row_ptr"], g["col"], g["val"], x, g["b"]) < 1.1e-8
+
+
+def test_jacobi_cg_and_bicgstab_golden(dev, vh):
+    g = load("solve_diffusion2d_30x30")
+    A = vh.CsrMatrix(g["row_ptr"], g["col"], g["val"])
+    x, res = solve(vh, "cg", A, g["b"], precond=vh.PRECOND_JACOBI, tol=1e-10, max_iters=3000)
+    assert iters_close(res.iters, g["jcg_iters"]) and res.converged and res.error < 1e-10
+    assert true_relres(g["row_ptr"], g["col"], g["val"], x, g["b"]) < 1e-8
+    x, res = solve(vh, "bicgstab", A, g["b"], precond=vh.PRECOND_JACOBI, tol=1e-10, max_iters=3000,
+                   max_iters_before_restart=int(g["restart_every"]))
+    assert iters_close(res.iters, g["jbicgstab_iters"]) and res.converged and res.error < 1e-10
+
+
+def test_powerlaw_bicgstab_golden(dev, vh):
+    g = load("solve_powerlaw_2000")
+    A = vh.CsrMatrix(g["row_ptr"], g["col"], g["val"])
+    x, res = solve(vh, "bicgstab", A, g["b"], tol=1e-10, max_iters=400)
+    assert res.iters == g["bicgstab_iters"] == 400 and not res.converged
+    assert 0.25 * g["bicgstab_error"] < res.error < 4 * g["bicgstab_error"]
+
+
+def test_residual_history_tracks_oracle(dev, vh, oracle):
+    rp, col, val = vh.laplace_3d(20, 18, 16)
+    b = vh.random_uniform(0, rp.size - 1)
+    A = vh.CsrMatrix(rp, col, val)
+    hist = np.zeros(1000)
+    x, res = solve(vh, "cg", A, b, tol=1e-10, max_iters=1000, history=hist)
+    _, it, err, ho = oracle.cg_pipelined(oracle.CsrHost(rp, col, val), b, 1e-10, 1000, history=True)
+    assert iters_close(res.iters, it)
+    m = min(res.iters, it)
+    assert np.allclose(hist[:m // 2], ho[:m // 2], rtol=1e-7)
+
+
+def test_launch_modes_are_deterministic(dev, vh):
+    rp, col, val = vh.laplace_3d(24)
+    b = vh.random_uniform(0, rp.size - 1)
+    A = vh.CsrMatrix(rp, col, val)
+    ref_x, ref_res = solve(vh, "cg", A, b, tol=1e-9, max_iters=500, chunk=16)
+    for kw in (dict(chunk=1), dict(chunk=7), dict(chunk=16, use_graph=True), dict(chunk=5, use_graph=True)):
+        x, res = solve(vh, "cg", A, b, tol=1e-9, max_iters=500, **kw)
+        assert res.iters == ref_res.iters and res.error == ref_res.error and np.array_equal(x, ref_x), kw
+    bx, bres = solve(vh, "bicgstab", A, b, tol=1e-9, max_iters=500, chunk=16)
+    x, res = solve(vh, "bicgstab", A, b, tol=1e-9, max_iters=500, chunk=3, use_graph=True)
+    assert res.iters == bres.iters and np.array_equal(x, bx)
+
+
+def test_edge_cases(dev, vh):
+    rp, col, val = vh.fdm_laplace_2d(16, 16)
+    A = vh.CsrMatrix(rp, col, val)
+    b0 = np.zeros(256)
+    x, res = solve(vh, "cg", A, b0, tol=1e-8, max_iters=100)
+    assert res.iters == 0 and not x.any()
+    x, res = solve(vh, "bicgstab", A, b0, tol=1e-8, max_iters=100)
+    assert res.iters == 0 and not x.any()
+    b = vh.random_uniform(0, 256)
+    x, res = solve(vh, "cg", A, b, tol=1e-8, max_iters=0)
+    assert res.iters == 0 and res.error == 1.0 and not x.any()
+    # a NaN in b must not hang: the iteration cap ends the solve
+    bn = b.copy()
+    bn[3] = np.nan
+    x, res = solve(vh, "cg", A, bn, tol=1e-8, max_iters=50)
+    assert res.iters == 50 and not res.converged
+    # zero on the diagonal: Jacobi refuses (zero_on_diagonal_exception)
+    d_idx = [k for k in range(rp[5], rp[6]) if col[k] == 5][0]
+    val2 = val.copy()
+    val2[d_idx] = 0.0
+    A2 = vh.CsrMatrix(rp, col, val2)
+    with pytest.raises(vh.VclHipError) as e:
+        solve(vh, "cg", A2, b, precond=vh.PRECOND_JACOBI, tol=1e-8, max_iters=10)
+    assert e.value.status == -2
+
+
+def test_monitor_stops_and_sees_every_iteration(dev, vh):
+    rp, col, val = vh.fdm_laplace_2d(32, 32)
+    A = vh.CsrMatrix(rp, col, val)
+    b = vh.random_uniform(0, 1024)
+    seen = []
+
+    @vh.MONITOR
+    def mon(xptr, rel, user):
+        seen.append(rel)
+        return 1 if len(seen) == 17 else 0
+
+    x, res = solve(vh, "cg", A, b, tol=1e-12, max_iters=1000, monitor=mon)
+    assert res.iters == 17 and len(seen) == 17
+    assert all(np.diff(seen[:5]) < 1.0)
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_3d_variable_coefficient_vs_oracle(dev, vh, oracle, precond):
+    N = 40
+    k = np.exp(vh.random_normal(1, N ** 3, 0.0, 1.0))
+    rp, col, val = vh.diffusion(N, N, N, k)
+    b = vh.random_uniform(0, N ** 3)
+    A = vh.CsrMatrix(rp, col, val)
+    x, res = solve(vh, "cg", A, b, precond=precond, tol=1e-10, max_iters=5000)
+    H = oracle.CsrHost(rp, col, val)
+    if precond:
+        _, it, err, _ = oracle.cg_classic(H, b, 1e-10, 5000, diag=oracle.row_info(rp, col, val, 3))
+    else:
+        _, it, err, _ = oracle.cg_pipelined(H, b, 1e-10, 5000)
+    assert iters_close(res.iters, it), (res.iters, it)
+    assert res.converged and res.error < 1e-10
+    assert true_relres(rp, col, val, x, b) < 1e-9
+
+
+def test_jacobi_bicgstab_powerlaw_vs_oracle(dev, vh, oracle):
+    rp, col, val = vh.powerlaw(30000, max_len=512, seed=1)
+    b = vh.random_uniform(0, rp.size - 1)
+    A = vh.CsrMatrix(rp, col, val)
+    x, res = solve(vh, "bicgstab", A, b, precond=vh.PRECOND_JACOBI, tol=1e-10, max_iters=1000)
+    _, it, err, _ = oracle.bicgstab_classic(oracle.CsrHost(rp, col, val), b, 1e-10, 1000,
+                                            diag=oracle.row_info(rp, col, val, 3))
+    assert iters_close(res.iters, it), (res.iters, it)
+    assert res.converged and true_relres(rp, col, val, x, b) < 1e-8

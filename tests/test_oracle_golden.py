@@ -1,0 +1,155 @@
+"""Pins the oracle (oracle/vcl_oracle.c) to the real ViennaCL host backend.
+
+Two sources: the committed golden fixtures (tests/golden/*.npz, written by make_golden.py from the
+reference build), and -- where oracle/_ref is built -- direct calls into the reference build.
+Single-threaded results must agree bit for bit.
+"""
+import pathlib
+
+import numpy as np
+import pytest
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+
+
+def load(name):
+    return dict(np.load(GOLDEN / f"{name}.npz", allow_pickle=False))
+
+
+@pytest.mark.parametrize("case", ["lap2d_30x20", "powerlaw_3000", "empty_rows_400"])
+def test_csr_spmv_matches_reference(oracle, case):
+    g = load(f"spmv_{case}")
+    for tag, alpha, beta in (("a1b0", 1.0, 0.0), ("a2b05", 2.0, 0.5), ("am1b1", -1.0, 1.0)):
+        y = oracle.csr_spmv(g["row_ptr"], g["col"], g["val"], g["x"], alpha, g["y0"], beta)
+        assert np.array_equal(y, g[f"y_{tag}"]), tag
+
+
+@pytest.mark.parametrize("case", ["lap2d_37x29_c32", "powerlaw_1001_c64"])
+def test_sell_arrays_and_spmv_match_reference(oracle, vh, case):
+    g = load(f"sell_{case}")
+    C_ = int(g["C"])
+    S = oracle.SellHost(g["row_ptr"], g["col"], g["val"], C_)
+    assert np.array_equal(S.cpb, g["columns_per_block"])
+    assert np.array_equal(S.bs, g["block_start"])
+    assert np.array_equal(S.ci, g["column_indices"])
+    assert np.array_equal(S.el, g["elements"])
+    # the library's host-side converter builds the same arrays
+    cpb, ci, bs, el = vh.sell_from_csr(g["row_ptr"], g["col"], g["val"], C_)
+    assert np.array_equal(cpb, g["columns_per_block"]) and np.array_equal(bs, g["block_start"])
+    assert np.array_equal(ci, g["column_indices"]) and np.array_equal(el, g["elements"])
+    y = oracle.sell_spmv(S, g["x"], float(g["alpha"]), g["y0"], float(g["beta"]))
+    assert np.array_equal(y, g["y"])
+
+
+def test_row_blocks_and_diag_match_reference(oracle, vh):
+    g = load("row_blocks_powerlaw_5000")
+    rb = np.zeros(g["row_ptr"].size, np.uint32)
+    nb = oracle.orc().orc_row_blocks(g["row_ptr"].size - 1, oracle.p(oracle.cu32(g["row_ptr"])), oracle.p(rb))
+    assert np.array_equal(rb[: nb + 1], g["row_blocks"])
+    assert np.array_equal(vh.reference_row_blocks(g["row_ptr"]), g["row_blocks"])
+    d = oracle.row_info(g["row_ptr"], g["col"], g["val"], 3)
+    assert np.array_equal(d, g["diag"])
+
+
+def test_fdm_laplace_generator_matches_reference(vh):
+    g = load("fdm_laplace_12x9")
+    rp, col, val = vh.fdm_laplace_2d(int(g["px"]), int(g["py"]))
+    assert np.array_equal(rp, g["row_ptr"]) and np.array_equal(col, g["col"]) and np.array_equal(val, g["val"])
+
+
+def test_pipelined_ops_match_reference(oracle):
+    g = load("pipelined_ops_25x17")
+    o = oracle.orc()
+    p = oracle.p
+    rp, col, val = oracle.cu32(g["row_ptr"]), oracle.cu32(g["col"]), oracle.c64(g["val"])
+    G = [np.ascontiguousarray(v) for v in g["g"]]
+    n = G[0].size
+    x, pv, r, Ap = G[0].copy(), G[1].copy(), G[2].copy(), G[3].copy()
+    buf = np.zeros(768)
+    o.orc_pipelined_cg_vector_update_f64(n, p(x), 0.37, p(pv), p(r), p(Ap), -0.61, p(buf))
+    assert np.array_equal(x, g["cgvu_x"]) and np.array_equal(pv, g["cgvu_p"]) and np.array_equal(r, g["cgvu_r"])
+    assert np.array_equal(buf, g["cgvu_buf"])
+    Ap2, buf2 = np.zeros(n), np.zeros(768)
+    o.orc_pipelined_prod_csr_f64(n, p(rp), p(col), p(val), p(G[1]), p(Ap2), None, p(buf2), 256, 0)
+    assert np.array_equal(Ap2, g["cgprod_Ap"]) and np.array_equal(buf2, g["cgprod_buf"])
+    Ap3, buf3 = np.zeros(n), np.zeros(1536)
+    o.orc_pipelined_prod_csr_f64(n, p(rp), p(col), p(val), p(G[1]), p(Ap3), p(G[4]), p(buf3), 256, 768)
+    assert np.array_equal(Ap3, g["bprod_Ap"]) and np.array_equal(buf3, g["bprod_buf"])
+    s, buf4 = np.zeros(n), np.zeros(1536)
+    buf4[0], buf4[768] = 3.5, -1.25
+    o.orc_pipelined_bicgstab_update_s_f64(n, p(s), p(G[2]), p(G[3]), p(buf4), 256, 1280)
+    assert np.array_equal(s, g["upds_s"]) and np.array_equal(buf4, g["upds_buf"])
+    x5, p5, r5, buf5 = G[0].copy(), G[1].copy(), G[2].copy(), np.zeros(1536)
+    o.orc_pipelined_bicgstab_vector_update_f64(n, p(x5), 0.3, p(p5), 0.7, p(G[5]), p(r5), p(G[6]), -0.2, p(G[7]),
+                                               p(G[4]), p(buf5))
+    assert np.array_equal(x5, g["bvu_x"]) and np.array_equal(p5, g["bvu_p"]) and np.array_equal(r5, g["bvu_r"])
+    assert np.array_equal(buf5, g["bvu_buf"])
+
+
+def test_solvers_match_reference_lap2d(oracle):
+    g = load("solve_lap2d_40x40")
+    A = oracle.CsrHost(g["row_ptr"], g["col"], g["val"])
+    x, it, err, _ = oracle.cg_pipelined(A, g["b"], 1e-8, 2000)
+    assert it == g["cg_iters"] and err == g["cg_error"] and np.array_equal(x, g["cg_x"])
+    x, it, err, _ = oracle.cg_pipelined(A, g["b"], 1e-8, 40)
+    assert it == g["cg40_iters"] == 40 and err == g["cg40_error"] and np.array_equal(x, g["cg40_x"])
+    x, it, err, _ = oracle.bicgstab_pipelined(A, g["b"], 1e-8, 2000)
+    assert it == g["bicgstab_iters"] and err == g["bicgstab_error"] and np.array_equal(x, g["bicgstab_x"])
+    x, it, err, _ = oracle.mixed_cg(A, g["b"], 1e-8, 2000)
+    assert it == g["mixed_iters"] and err == g["mixed_error"] and np.array_equal(x, g["mixed_x"])
+
+
+def test_preconditioned_solvers_match_reference(oracle):
+    g = load("solve_diffusion2d_30x30")
+    A = oracle.CsrHost(g["row_ptr"], g["col"], g["val"])
+    d = oracle.row_info(g["row_ptr"], g["col"], g["val"], 3)
+    x, it, err, _ = oracle.cg_classic(A, g["b"], 1e-10, 3000, diag=d)
+    assert it == g["jcg_iters"] and err == g["jcg_error"] and np.array_equal(x, g["jcg_x"])
+    x, it, err, _ = oracle.bicgstab_classic(A, g["b"], 1e-10, 3000, diag=d, max_before_restart=int(g["restart_every"]))
+    assert it == g["jbicgstab_iters"] and err == g["jbicgstab_error"] and np.array_equal(x, g["jbicgstab_x"])
+
+
+def test_sell_solvers_match_reference(oracle):
+    g = load("solve_sell_lap2d_37x29")
+    S = oracle.SellHost(g["row_ptr"], g["col"], g["val"], int(g["C"]))
+    x, it, err, _ = oracle.cg_pipelined(S, g["b"], 1e-8, 2000)
+    assert it == g["cg_iters"] and err == g["cg_error"] and np.array_equal(x, g["cg_x"])
+    x, it, err, _ = oracle.bicgstab_pipelined(S, g["b"], 1e-8, 2000)
+    assert it == g["bicgstab_iters"] and err == g["bicgstab_error"] and np.array_equal(x, g["bicgstab_x"])
+
+
+def test_nonsymmetric_bicgstab_matches_reference(oracle):
+    g = load("solve_powerlaw_2000")
+    A = oracle.CsrHost(g["row_ptr"], g["col"], g["val"])
+    x, it, err, _ = oracle.bicgstab_pipelined(A, g["b"], 1e-10, 400)
+    assert it == g["bicgstab_iters"] and err == g["bicgstab_error"] and np.array_equal(x, g["bicgstab_x"])
+
+
+needs_ref = pytest.mark.skipif(not (pathlib.Path(__file__).resolve().parents[1] / "oracle/_ref/libvclref.so").exists(),
+                               reason="reference build (oracle/_ref) not present")
+
+
+@needs_ref
+@pytest.mark.parametrize("seed", [3, 4])
+def test_oracle_equals_reference_build_on_fresh_inputs(oracle, vh, seed):
+    """Fresh seeded inputs beyond the fixtures, straight against the reference build."""
+    rp, col, val = vh.powerlaw(1500, max_len=100, diag_factor=1.3, seed=seed)
+    b = vh.random_uniform(seed, rp.size - 1)
+    A = oracle.CsrHost(rp, col, val)
+    x, it, err, _ = oracle.bicgstab_pipelined(A, b, 1e-9, 300)
+    xr, itr, errr = oracle.ref_bicgstab(rp, col, val, b, 1e-9, 300)
+    assert (it, err) == (itr, errr) and np.array_equal(x, xr)
+    k = np.exp(vh.random_normal(seed, 12 * 11 * 7, 0.0, 1.0))
+    rp, col, val = vh.diffusion(12, 11, 7, k)
+    b = vh.random_uniform(seed, rp.size - 1)
+    A = oracle.CsrHost(rp, col, val)
+    x, it, err, _ = oracle.cg_pipelined(A, b, 1e-10, 3000)
+    xr, itr, errr = oracle.ref_cg(rp, col, val, b, 1e-10, 3000)
+    assert (it, err) == (itr, errr) and np.array_equal(x, xr)
+    x, it, err, _ = oracle.mixed_cg(A, b, 1e-10, 3000)
+    xr, itr, errr = oracle.ref_mixed_cg(rp, col, val, b, 1e-10, 3000)
+    assert (it, err) == (itr, errr) and np.array_equal(x, xr)
+    d = oracle.row_info(rp, col, val, 3)
+    x, it, err, _ = oracle.bicgstab_classic(A, b, 1e-10, 3000, diag=d, max_before_restart=15)
+    xr, itr, errr = oracle.ref_bicgstab(rp, col, val, b, 1e-10, 3000, jacobi=True, max_before_restart=15)
+    assert (it, err) == (itr, errr) and np.array_equal(x, xr)

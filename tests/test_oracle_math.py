@@ -1,0 +1,205 @@
+"""Pins the oracle against what the mathematics fixes, independently of the reference:
+closed-form Laplacian eigenpairs and row sums, dense numpy products and solves, finite
+termination of CG, the Chronopoulos-Gear recurrence, and central finite differences.
+Each check is chosen so that a dropped term, a wrong sign or index, or a transposed operand in
+the oracle fails it; test_planted_mistakes_are_caught demonstrates this on the CPU.
+"""
+import numpy as np
+import pytest
+
+
+def dense(rp, col, val, n=None):
+    n = rp.size - 1 if n is None else n
+    A = np.zeros((rp.size - 1, n))
+    for i in range(rp.size - 1):
+        for k in range(rp[i], rp[i + 1]):
+            A[i, col[k]] += val[k]
+    return A
+
+
+def test_2d_laplacian_eigenpair(oracle, vh):
+    px, py = 23, 17
+    rp, col, val = vh.fdm_laplace_2d(px, py)
+    j, k = 3, 5
+    i_idx = np.arange(px)[None, :]
+    j_idx = np.arange(py)[:, None]
+    v = (np.sin((i_idx + 1) * j * np.pi / (px + 1)) * np.sin((j_idx + 1) * k * np.pi / (py + 1))).ravel()
+    lam = 4 - 2 * np.cos(j * np.pi / (px + 1)) - 2 * np.cos(k * np.pi / (py + 1))
+    y = oracle.csr_spmv(rp, col, val, v)
+    assert np.max(np.abs(y - lam * v)) < 1e-13
+
+
+def test_3d_laplacian_eigenpair_and_row_sums(oracle, vh):
+    nx, ny, nz = 9, 7, 6
+    rp, col, val = vh.laplace_3d(nx, ny, nz)
+    a, b, c = 2, 3, 1
+    X, Y, Z = np.meshgrid(np.arange(nx) + 1, np.arange(ny) + 1, np.arange(nz) + 1, indexing="ij")
+    v = (np.sin(X * a * np.pi / (nx + 1)) * np.sin(Y * b * np.pi / (ny + 1)) * np.sin(Z * c * np.pi / (nz + 1)))
+    v = v.transpose(2, 1, 0).ravel()  # row = x + nx (y + ny z)
+    lam = sum(2 - 2 * np.cos(m * np.pi / (N + 1)) for m, N in ((a, nx), (b, ny), (c, nz)))
+    assert np.max(np.abs(oracle.csr_spmv(rp, col, val, v) - lam * v)) < 1e-13
+    # A * 1 = number of boundary faces of each cell
+    ones = oracle.csr_spmv(rp, col, val, np.ones(rp.size - 1))
+    bnd = ((X == 1).astype(int) + (X == nx) + (Y == 1) + (Y == ny) + (Z == 1) + (Z == nz)).transpose(2, 1, 0).ravel()
+    assert np.array_equal(ones, bnd.astype(float))
+
+
+def test_spmv_alpha_beta_strides_against_dense(oracle, vh):
+    rp, col, val = vh.powerlaw(300, max_len=40, seed=9)
+    A = dense(rp, col, val)
+    x = vh.random_uniform(1, 300)
+    y0 = vh.random_uniform(2, 300)
+    y = oracle.csr_spmv(rp, col, val, x, -0.75, y0, 2.5)
+    assert np.allclose(y, -0.75 * A @ x + 2.5 * y0, rtol=1e-13, atol=1e-13)
+    # strided operands: x read at 3i+1, y written at 2i+5
+    xs = np.zeros(3 * 300 + 1)
+    xs[1::3] = x
+    ys = np.full(2 * 300 + 6, 7.0)
+    ys = oracle.csr_spmv(rp, col, val, xs, 1.0, ys, 0.0, xs=1, xi=3, ys=5, yi=2)
+    assert np.allclose(ys[5::2][:300], A @ x, rtol=1e-13, atol=1e-13)
+    assert np.all(ys[:5] == 7.0) and np.all(ys[6::2] == 7.0)
+
+
+def test_sell_equals_csr_bitwise(oracle, vh):
+    for C_ in (32, 64, 128):
+        rp, col, val = vh.powerlaw(777, max_len=60, seed=C_)
+        S = oracle.SellHost(rp, col, val, C_)
+        x = vh.random_uniform(5, 777)
+        assert np.array_equal(oracle.sell_spmv(S, x), oracle.csr_spmv(rp, col, val, x))
+
+
+def test_cg_finite_termination_on_diagonal(oracle):
+    # a diagonal SPD matrix with 5 distinct eigenvalues: CG converges in at most 5 steps
+    n = 200
+    d = np.repeat([1.0, 2.0, 5.0, 11.0, 30.0], n // 5)
+    rp = np.arange(n + 1, dtype=np.uint32)
+    A = oracle.CsrHost(rp, np.arange(n, dtype=np.uint32), d)
+    b = np.linspace(-1, 1, n) + 0.3
+    x, it, err, _ = oracle.cg_pipelined(A, b, 1e-10, 50)
+    assert it <= 6 and err < 1e-10
+    assert np.allclose(x, b / d, rtol=1e-9)
+
+
+@pytest.mark.parametrize("variant", ["pipelined", "classic", "jacobi"])
+def test_cg_solves_against_numpy(oracle, vh, variant):
+    k = np.exp(vh.random_normal(3, 15 * 14, 0.0, 1.0))
+    rp, col, val = vh.diffusion(15, 14, 1, k)
+    A = dense(rp, col, val)
+    b = vh.random_uniform(0, rp.size - 1)
+    H = oracle.CsrHost(rp, col, val)
+    if variant == "pipelined":
+        x, it, err, hist = oracle.cg_pipelined(H, b, 1e-12, 2000, history=True)
+    elif variant == "classic":
+        x, it, err, hist = oracle.cg_classic(H, b, 1e-12, 2000, history=True)
+    else:
+        x, it, err, hist = oracle.cg_classic(H, b, 1e-12, 2000, diag=np.diag(A).copy(), history=True)
+    xs = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-9
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10
+    assert err < 1e-12 and hist.size == it
+
+
+def test_chronopoulos_gear_recurrence(oracle, vh):
+    """rr_{k+1} predicted by beta = (alpha^2 <Ap,Ap> - rr)/rr equals the explicit <r,r> (cg.hpp:179-180)."""
+    rp, col, val = vh.fdm_laplace_2d(20, 20)
+    b = vh.random_uniform(0, 400)
+    H = oracle.CsrHost(rp, col, val)
+    _, it, _, hist = oracle.cg_pipelined(H, b, 1e-12, 500, history=True)
+    _, itc, _, histc = oracle.cg_classic(H, b, 1e-12, 500, history=True)
+    m = min(it, itc, 30)
+    assert np.allclose(hist[:m], histc[:m], rtol=1e-6)
+    assert abs(it - itc) <= 2
+
+
+@pytest.mark.parametrize("jacobi", [False, True])
+def test_bicgstab_against_numpy(oracle, vh, jacobi):
+    rp, col, val = vh.powerlaw(400, max_len=30, diag_factor=1.5, seed=11)
+    A = dense(rp, col, val)
+    b = vh.random_uniform(0, 400)
+    H = oracle.CsrHost(rp, col, val)
+    if jacobi:
+        x, it, err, _ = oracle.bicgstab_classic(H, b, 1e-11, 500, diag=np.diag(A).copy())
+    else:
+        x, it, err, _ = oracle.bicgstab_pipelined(H, b, 1e-11, 500)
+    xs = np.linalg.solve(A, b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
+    assert err < 1e-11
+
+
+def test_mixed_cg_true_residual(oracle, vh):
+    rp, col, val = vh.laplace_3d(10, 9, 8)
+    A = dense(rp, col, val)
+    b = vh.random_uniform(0, rp.size - 1)
+    x, it, err, restarts = oracle.mixed_cg(oracle.CsrHost(rp, col, val), b, 1e-10, 3000)
+    rel = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    assert rel < 1e-10 and abs(rel - err) < 1e-12 and restarts >= 2
+
+
+def test_batched_oracle_per_system(oracle, vh):
+    sys_ptr, rp, col, val = vh.graph_laplacian_batch(5, 10, 40, seed=2)
+    b = vh.random_uniform(0, rp.size - 1)
+    x, iters, errs = oracle.batched_cg(sys_ptr, oracle.CsrHost(rp, col, val), b, 1e-10, 500)
+    A = dense(rp, col, val)
+    for s in range(5):
+        r0, r1 = sys_ptr[s], sys_ptr[s + 1]
+        As = A[r0:r1, r0:r1]
+        assert np.allclose(A[r0:r1, :r0], 0) and np.allclose(A[r0:r1, r1:], 0)
+        xs = np.linalg.solve(As, b[r0:r1])
+        assert np.linalg.norm(x[r0:r1] - xs) / np.linalg.norm(xs) < 1e-7
+        assert errs[s] < 1e-10 and iters[s] >= 1
+
+
+def test_adjoint_gradient_central_differences(oracle, vh):
+    """dL/dA_ij = -lambda_i x_j with A^T lambda = dL/dx, for L = 0.5 ||x - t||^2 (finite differences)."""
+    k = np.exp(vh.random_normal(4, 8 * 7, 0.0, 0.5))
+    rp, col, val = vh.diffusion(8, 7, 1, k)
+    n = rp.size - 1
+    b = vh.random_uniform(0, n)
+    t = vh.random_uniform(2, n)
+
+    def loss(v):
+        x = np.linalg.solve(dense(rp, col, v), b)
+        return 0.5 * np.sum((x - t) ** 2)
+
+    A = dense(rp, col, val)
+    x = np.linalg.solve(A, b)
+    lam = np.linalg.solve(A.T, x - t)
+    g = oracle.adjoint_grad(rp, col, lam, x)
+    h = 1e-6
+    for kk in np.random.default_rng(0).choice(col.size, 12, replace=False):
+        vp, vm = val.copy(), val.copy()
+        vp[kk] += h
+        vm[kk] -= h
+        fd = (loss(vp) - loss(vm)) / (2 * h)
+        assert abs(fd - g[kk]) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_planted_mistakes_are_caught(oracle, vh):
+    """Plants typical mistakes in a copy of the oracle's SpMV and CG (on the CPU) and checks that the
+    closed-form / dense pins above would reject them."""
+    rp, col, val = vh.laplace_3d(6, 5, 4)
+    A = dense(rp, col, val)
+    x = vh.random_uniform(1, rp.size - 1)
+    good = oracle.csr_spmv(rp, col, val, x)
+    assert np.allclose(good, A @ x, rtol=1e-13, atol=1e-13)
+    # transposed operand: A^T x differs from A x for a nonsymmetric matrix
+    rp2, col2, val2 = vh.powerlaw(200, max_len=20, seed=3)
+    A2 = dense(rp2, col2, val2)
+    x2 = vh.random_uniform(4, 200)
+    assert not np.allclose(A2.T @ x2, oracle.csr_spmv(rp2, col2, val2, x2), rtol=1e-6)
+    # dropped last entry of each row
+    bad = np.array([sum(val[k] * x[col[k]] for k in range(rp[i], rp[i + 1] - 1)) for i in range(rp.size - 1)])
+    assert not np.allclose(bad, A @ x, rtol=1e-6)
+    # wrong beta sign in CG: the iteration no longer solves the system
+    b = vh.random_uniform(0, rp.size - 1)
+    xx, r, pp = np.zeros_like(b), b.copy(), b.copy()
+    rr = r @ r
+    for _ in range(200):
+        Ap = A @ pp
+        alpha = rr / (pp @ Ap)
+        xx += alpha * pp
+        r -= alpha * Ap
+        rr_new = r @ r
+        pp = r - (rr_new / rr) * pp  # planted sign error
+        rr = rr_new
+    assert np.linalg.norm(b - A @ xx) / np.linalg.norm(b) > 1e-6

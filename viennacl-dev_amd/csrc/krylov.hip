@@ -1,0 +1,1149 @@
+// krylov.hip -- fused pipelined CG / BiCGStab kernels, the reference's pipelined operations
+// (iterative_operations.hpp) as entry points, and the device-resident solver drivers.
+//
+// Solver design.  Each iteration is a short fixed sequence of kernels; all scalars (alpha, beta,
+// omega, inner products, iteration count, convergence flags) live in a device-resident state
+// record written only by single-workgroup "scalar" kernels.  The host enqueues iterations in
+// chunks without waiting, polls the state of the previous chunk while the next one runs, and
+// every kernel returns at once after the state says the solve has ended.  There is therefore no
+// per-iteration device-to-host copy (the reference copies its partial buffer back every
+// iteration, cg.hpp:168 and bicgstab.hpp:180).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+#include "spmv_kernels.hpp"
+
+namespace vclhip {
+
+// ------------------------------------------------------------------ solver state
+
+enum : uint32_t {
+  kFlagDone = 1u,       // every kernel of the solve returns at once
+  kFlagFinishing = 2u,  // BiCGStab: the final vector update is pending, everything else stops
+};
+
+constexpr uint32_t kSlots = 8;  // partial-sum slots of kMaxPartials doubles each
+
+struct KState {
+  double alpha, beta, omega;
+  double rr;        // CG: <r,r> or <r,M^-1 r>;  BiCGStab: <r,r0*>
+  double nrs;       // CG: norm of the right-hand side squared;  BiCGStab: ||b||
+  double res;       // BiCGStab residual-norm estimate
+  double ap_r0;     // BiCGStab <Ap,r0*> of the current iteration
+  double tol, abs_tol;
+  double* history;  // device array or null
+  uint32_t history_len;
+  uint32_t iter, max_iters;
+  uint32_t flags;
+  uint32_t converged;
+  uint32_t final_mode;        // BiCGStab: 0 full update, 1 converged (x only), 2 last iteration (full)
+  uint32_t rr_from_partials;  // BiCGStab: <r,r0*> comes from the previous vector update's partials
+  uint32_t breakdown;
+};
+
+__host__ __device__ __forceinline__ double* slot_ptr(double* part, uint32_t slot) { return part + slot * kMaxPartials; }
+__host__ __device__ __forceinline__ const double* slot_ptr(const double* part, uint32_t slot)
+{
+  return part + slot * kMaxPartials;
+}
+
+__device__ __forceinline__ void record_history(KState* st, uint32_t iter, double v)
+{
+  if (st->history && iter >= 1 && iter <= st->history_len) st->history[iter - 1] = v;
+}
+
+// ------------------------------------------------------------------ epilogues
+
+// CG: Ap = A p; <Ap,Ap> (or <Ap, D^-1 Ap>) and <p,Ap>  (host_based/iterative_operations.hpp:83-97)
+template <typename T, bool Jac>
+struct EpiCg {
+  using acc_t = double;
+  static constexpr int K = 2;
+  T* __restrict__ Ap;
+  const T* __restrict__ p;
+  const T* __restrict__ d;
+  __device__ __forceinline__ void operator()(uint32_t row, T s, acc_t* acc) const
+  {
+    Ap[row] = s;
+    if (Jac)
+      acc[0] += static_cast<double>(s * (s / d[row]));
+    else
+      acc[0] += static_cast<double>(s * s);
+    acc[1] += static_cast<double>(p[row] * s);
+  }
+};
+
+// BiCGStab: t = A p (left-preconditioned: t = D^-1 A p); Full: <t,t>, <p,t>, <t,r0*>; else <t,r0*> only.
+template <typename T, bool Jac, bool Full>
+struct EpiBicg {
+  using acc_t = double;
+  static constexpr int K = Full ? 3 : 1;
+  T* __restrict__ out;
+  const T* __restrict__ p;
+  const T* __restrict__ r0;
+  const T* __restrict__ d;
+  __device__ __forceinline__ void operator()(uint32_t row, T s, acc_t* acc) const
+  {
+    const T t = Jac ? s / d[row] : s;
+    out[row] = t;
+    if (Full) {
+      acc[0] += static_cast<double>(t * t);
+      acc[1] += static_cast<double>(p[row] * t);
+      acc[2] += static_cast<double>(t * r0[row]);
+    } else {
+      acc[0] += static_cast<double>(t * r0[row]);
+    }
+  }
+};
+
+// BiCGStab restart (bicgstab.hpp:434-445): r = M^-1 (b - A x); p = r0* = r; <r,r>
+template <typename T, bool Jac>
+struct EpiRestart {
+  using acc_t = double;
+  static constexpr int K = 1;
+  const T* __restrict__ b;
+  const T* __restrict__ d;
+  T* __restrict__ r;
+  T* __restrict__ p;
+  T* __restrict__ r0;
+  __device__ __forceinline__ void operator()(uint32_t row, T s, acc_t* acc) const
+  {
+    T v = b[row] - s;
+    if (Jac) v = v / d[row];
+    r[row] = v;
+    p[row] = v;
+    r0[row] = v;
+    acc[0] += static_cast<double>(v * v);
+  }
+};
+
+// ------------------------------------------------------------------ CG kernels
+
+// x = 0; r = b; p = M^-1 b; partial <b, M^-1 b>  (cg.hpp:138-149, :268-279)
+template <typename T, bool Jac>
+__global__ __launch_bounds__(kBlock) void cg_init_kernel(uint32_t n, const T* __restrict__ b, T* __restrict__ x,
+                                                         T* __restrict__ r, T* __restrict__ p,
+                                                         const T* __restrict__ d, double* __restrict__ part)
+{
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range(n, lw, gridDim.x, c0, c1);
+  double acc[1] = {0};
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    const T vb = b[i];
+    x[i] = 0;
+    r[i] = vb;
+    const T z = Jac ? vb / d[i] : vb;
+    p[i] = z;
+    acc[0] += static_cast<double>(vb * z);
+  }
+  block_store_partials<1, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
+}
+
+// cg.hpp:149-156 (pipelined start) / :279-287 (preconditioned start)
+template <bool Jac>
+__global__ __launch_bounds__(kBlock) void cg_setup_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                          uint32_t g_vec, uint32_t g_prod)
+{
+  const double bb = block_sum_partials(slot_ptr(part, 0), g_vec);
+  const double ApAp = block_sum_partials(slot_ptr(part, 1), g_prod);
+  const double pAp = block_sum_partials(slot_ptr(part, 2), g_prod);
+  if (threadIdx.x != 0) return;
+  double nrs;
+  if (Jac) {
+    nrs = bb;
+  } else {
+    const double nrm = sqrt(bb);  // norm_2 then squared, cg.hpp:149
+    nrs = nrm * nrm;
+  }
+  st->nrs = nrs;
+  st->rr = nrs;
+  st->iter = 0;
+  const double at2 = st->abs_tol * st->abs_tol;
+  if ((Jac ? fabs(nrs) : nrs) <= at2) {
+    st->flags = kFlagDone;
+    st->converged = 1;
+    return;
+  }
+  const double alpha = nrs / pAp;
+  double beta;
+  if (Jac) {
+    beta = (alpha * alpha * ApAp - nrs) / nrs;
+  } else {
+    const double nap = sqrt(ApAp);
+    beta = (alpha * alpha * nap * nap - nrs) / nrs;
+  }
+  st->alpha = alpha;
+  st->beta = beta;
+  if (st->max_iters == 0) st->flags = kFlagDone;
+}
+
+// x += alpha p; r -= alpha Ap; z = M^-1 r; p = z + beta p; partial <r,z>  (host_based/iterative_operations.hpp:402-415)
+template <typename T, bool Jac>
+__global__ __launch_bounds__(kBlock) void cg_update_kernel(uint32_t n, T* __restrict__ x, T* __restrict__ p,
+                                                           T* __restrict__ r, const T* __restrict__ Ap,
+                                                           const T* __restrict__ d, const KState* __restrict__ st,
+                                                           double* __restrict__ part)
+{
+  if (st->flags & kFlagDone) return;
+  const T alpha = static_cast<T>(st->alpha);
+  const T beta = static_cast<T>(st->beta);
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range(n, lw, gridDim.x, c0, c1);
+  double acc[1] = {0};
+#pragma unroll 2
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    T vp = p[i];
+    T vr = r[i];
+    x[i] += alpha * vp;
+    vr -= alpha * Ap[i];
+    const T z = Jac ? vr / d[i] : vr;
+    vp = z + beta * vp;
+    acc[0] += static_cast<double>(vr * z);
+    p[i] = vp;
+    r[i] = vr;
+  }
+  block_store_partials<1, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
+}
+
+// cg.hpp:170-180: reduce, test, next alpha/beta
+__global__ __launch_bounds__(kBlock) void cg_iter_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                         uint32_t g_vec, uint32_t g_prod)
+{
+  if (st->flags & kFlagDone) return;
+  const double rr = block_sum_partials(slot_ptr(part, 0), g_vec);
+  const double ApAp = block_sum_partials(slot_ptr(part, 1), g_prod);
+  const double pAp = block_sum_partials(slot_ptr(part, 2), g_prod);
+  if (threadIdx.x != 0) return;
+  const uint32_t iter = st->iter + 1;
+  st->iter = iter;
+  st->rr = rr;
+  record_history(st, iter, rr);
+  if (fabs(rr / st->nrs) < st->tol * st->tol || fabs(rr) < st->abs_tol * st->abs_tol) {
+    st->flags |= kFlagDone;
+    st->converged = 1;
+    return;
+  }
+  if (iter >= st->max_iters) {
+    st->flags |= kFlagDone;
+    return;
+  }
+  const double alpha = rr / pAp;
+  st->alpha = alpha;
+  st->beta = (alpha * alpha * ApAp - rr) / rr;
+}
+
+// ------------------------------------------------------------------ BiCGStab kernels
+
+// x = 0; r = p = r0* = M^-1 b; partials <b,b>, <r,r>  (bicgstab.hpp:105-131, :408-445)
+template <typename T, bool Jac>
+__global__ __launch_bounds__(kBlock) void bicg_init_kernel(uint32_t n, const T* __restrict__ b, T* __restrict__ x,
+                                                           T* __restrict__ r, T* __restrict__ p, T* __restrict__ r0,
+                                                           const T* __restrict__ d, double* __restrict__ part)
+{
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range(n, lw, gridDim.x, c0, c1);
+  double acc[2] = {0, 0};
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    const T vb = b[i];
+    const T v = Jac ? vb / d[i] : vb;
+    x[i] = 0;
+    r[i] = v;
+    p[i] = v;
+    r0[i] = v;
+    acc[0] += static_cast<double>(vb * vb);
+    acc[1] += static_cast<double>(v * v);
+  }
+  block_store_partials<2, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
+}
+
+__global__ __launch_bounds__(kBlock) void bicg_setup_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                            uint32_t g_vec)
+{
+  const double bb = block_sum_partials(slot_ptr(part, 0), g_vec);
+  const double vv = block_sum_partials(slot_ptr(part, 1), g_vec);
+  if (threadIdx.x != 0) return;
+  const double norm_rhs = sqrt(bb);
+  const double nv = sqrt(vv);
+  st->nrs = norm_rhs;
+  st->res = norm_rhs;
+  st->rr = nv * nv;  // norm_2 then squared (bicgstab.hpp:131, :441-442)
+  st->rr_from_partials = 0;
+  st->iter = 0;
+  if (norm_rhs <= st->abs_tol) {
+    st->flags = kFlagDone;
+    st->converged = 1;
+    return;
+  }
+  if (st->max_iters == 0) st->flags = kFlagDone;
+}
+
+// alpha = <r,r0*> / <Ap,r0*>  (host_based/iterative_operations.hpp:536-544).  Also turns a pending
+// finish into done, so the kernels of the iteration after the last one all return at once.
+__global__ __launch_bounds__(kBlock) void bicg_alpha_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                            uint32_t g_vec, uint32_t g_prod, int detect_breakdown)
+{
+  const uint32_t flags = st->flags;
+  if (flags & kFlagDone) return;
+  if (flags & kFlagFinishing) {
+    if (threadIdx.x == 0) st->flags = flags | kFlagDone;
+    return;
+  }
+  const bool from_part = st->rr_from_partials != 0;
+  const double rsum = block_sum_partials(slot_ptr(part, 0), from_part ? g_vec : 0u);
+  const double ap_r0 = block_sum_partials(slot_ptr(part, 2), g_prod);
+  if (threadIdx.x != 0) return;
+  const double rr = from_part ? rsum : st->rr;
+  st->rr = rr;
+  st->ap_r0 = ap_r0;
+  if (detect_breakdown && rr <= 0 && rr >= 0) {  // bicgstab.hpp:473: <r,r0*> == 0 asks for a restart
+    st->breakdown = 1;
+    st->flags = flags | kFlagDone;
+    return;
+  }
+  st->alpha = rr / ap_r0;
+}
+
+// s = r - alpha Ap; partial <s,s>  (host_based/iterative_operations.hpp:552-560)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bicg_update_s_kernel(uint32_t n, T* __restrict__ s, const T* __restrict__ r,
+                                                               const T* __restrict__ Ap, const KState* __restrict__ st,
+                                                               double* __restrict__ part)
+{
+  if (st->flags & (kFlagDone | kFlagFinishing)) return;
+  const T alpha = static_cast<T>(st->alpha);
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range(n, lw, gridDim.x, c0, c1);
+  double acc[1] = {0};
+#pragma unroll 2
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    const T vs = r[i] - alpha * Ap[i];
+    acc[0] += static_cast<double>(vs * vs);
+    s[i] = vs;
+  }
+  block_store_partials<1, double>(acc, slot_ptr(part, 3), kMaxPartials, lw);
+}
+
+// bicgstab.hpp:184-198: reduce, alpha/beta/omega, residual estimate, test
+__global__ __launch_bounds__(kBlock) void bicg_scalars_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                              uint32_t g_vec, uint32_t g_prod, int detect_breakdown)
+{
+  if (st->flags & (kFlagDone | kFlagFinishing)) return;
+  const double s_dot_s = block_sum_partials(slot_ptr(part, 3), g_vec);
+  const double As_dot_As = block_sum_partials(slot_ptr(part, 4), g_prod);
+  const double As_dot_s = block_sum_partials(slot_ptr(part, 5), g_prod);
+  const double As_dot_r0 = block_sum_partials(slot_ptr(part, 6), g_prod);
+  if (threadIdx.x != 0) return;
+  const double r_dot_r0 = st->rr;
+  const double Ap_dot_r0 = st->ap_r0;
+  const double alpha = r_dot_r0 / Ap_dot_r0;
+  const double beta = -As_dot_r0 / Ap_dot_r0;
+  const double omega = As_dot_s / As_dot_As;
+  const double res = sqrt(s_dot_s - 2.0 * omega * As_dot_s + omega * omega * As_dot_As);
+  st->alpha = alpha;
+  st->beta = beta;
+  st->omega = omega;
+  st->res = res;
+  const uint32_t iter = st->iter + 1;
+  st->iter = iter;
+  record_history(st, iter, res);
+  st->rr_from_partials = 1;
+  if (fabs(res / st->nrs) < st->tol || res < st->abs_tol) {
+    st->final_mode = 1;
+    st->converged = 1;
+    st->flags |= kFlagFinishing;
+  } else if (iter >= st->max_iters) {
+    st->final_mode = 2;
+    st->flags |= kFlagFinishing;
+  } else {
+    st->final_mode = 0;
+    if (detect_breakdown && omega <= 0 && omega >= 0) {  // bicgstab.hpp:473: omega == 0 asks for a restart
+      st->breakdown = 1;
+      st->flags |= kFlagFinishing;
+    }
+  }
+}
+
+// x += alpha p + omega s; r = s - omega As; p = r + beta (p - omega Ap); partial <r,r0*>
+// (host_based/iterative_operations.hpp:598-617).  final_mode 1 applies only the x update.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bicg_vec_update_kernel(uint32_t n, T* __restrict__ x, T* __restrict__ p,
+                                                                 const T* __restrict__ s, T* __restrict__ r,
+                                                                 const T* __restrict__ As, const T* __restrict__ Ap,
+                                                                 const T* __restrict__ r0,
+                                                                 const KState* __restrict__ st,
+                                                                 double* __restrict__ part)
+{
+  if (st->flags & kFlagDone) return;
+  const T alpha = static_cast<T>(st->alpha);
+  const T beta = static_cast<T>(st->beta);
+  const T omega = static_cast<T>(st->omega);
+  const uint32_t mode = st->final_mode;
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range(n, lw, gridDim.x, c0, c1);
+  if (mode == 1) {
+    for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) x[i] += alpha * p[i] + omega * s[i];
+    return;
+  }
+  double acc[1] = {0};
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    T vx = x[i];
+    T vp = p[i];
+    const T vs = s[i];
+    vx += alpha * vp + omega * vs;
+    const T vr = vs - omega * As[i];
+    vp = vr + beta * (vp - omega * Ap[i]);
+    acc[0] += static_cast<double>(vr * r0[i]);
+    x[i] = vx;
+    r[i] = vr;
+    p[i] = vp;
+  }
+  block_store_partials<1, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
+}
+
+// restart: <r,r> -> rr (norm_2 then squared, bicgstab.hpp:441-442)
+__global__ __launch_bounds__(kBlock) void bicg_restart_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                              uint32_t g_prod)
+{
+  if (st->flags & (kFlagDone | kFlagFinishing)) return;
+  const double vv = block_sum_partials(slot_ptr(part, 7), g_prod);
+  if (threadIdx.x != 0) return;
+  const double nv = sqrt(vv);
+  st->rr = nv * nv;
+  st->rr_from_partials = 0;
+}
+
+// ------------------------------------------------------------------ matrix wrapper
+
+template <typename T>
+struct DevOp {
+  bool sell = false;
+  CsrDev<T> csr{};
+  SellDev<T> sm{};
+  uint32_t n = 0;
+  uint32_t grid = 8;
+
+  template <class Epi>
+  void launch(hipStream_t s, const T* x, const Epi& epi, double* part, const uint32_t* stop, uint32_t mask) const
+  {
+    if (sell)
+      hipLaunchKernelGGL((sell_kernel<T, true, Epi>), dim3(grid), dim3(kBlock), 0, s, sm, x, 0u, 1u, epi, part,
+                         kMaxPartials, stop, mask);
+    else
+      hipLaunchKernelGGL((csr_kernel<T, true, Epi>), dim3(grid), dim3(kBlock), 0, s, csr, x, 0u, 1u, epi, part,
+                         kMaxPartials, stop, mask);
+  }
+};
+
+template <typename T>
+static int make_op(const vclhip_csr* A, DevOp<T>& op)
+{
+  if (!A || !A->row_ptr || !A->blocks || !A->col_idx || !A->values) return invalid("CSR matrix arrays");
+  if (A->n_rows == 0 || A->n_rows != A->n_cols) return invalid("solvers need a nonempty square matrix");
+  if (A->n_rows >= kPlanRowLimit || A->num_blocks == 0 || A->num_blocks > A->n_rows) return invalid("CSR plan");
+  op.sell = false;
+  op.csr.row_ptr = A->row_ptr;
+  op.csr.col = A->col_idx;
+  op.csr.val = static_cast<const T*>(A->values);
+  op.csr.blocks = A->blocks;
+  op.csr.num_blocks = A->num_blocks;
+  op.csr.n_rows = A->n_rows;
+  op.n = A->n_rows;
+  op.grid = chunk_grid(A->num_blocks, csr_gmax());
+  return VCLHIP_SUCCESS;
+}
+
+template <typename T>
+static int make_op(const vclhip_sell* A, DevOp<T>& op)
+{
+  if (!A || !A->columns_per_block || !A->column_indices || !A->block_start || !A->elements)
+    return invalid("SELL matrix arrays");
+  if (A->n_rows == 0 || A->n_rows != A->n_cols || A->C == 0) return invalid("solvers need a nonempty square matrix");
+  if (A->num_slices != (A->n_rows + A->C - 1) / A->C) return invalid("num_slices");
+  op.sell = true;
+  op.sm.cpb = A->columns_per_block;
+  op.sm.col = A->column_indices;
+  op.sm.start = A->block_start;
+  op.sm.val = static_cast<const T*>(A->elements);
+  op.sm.n_rows = A->n_rows;
+  op.sm.C = A->C;
+  op.n = A->n_rows;
+  op.grid = chunk_grid((A->n_rows + kBlock - 1) / kBlock, sell_gmax());
+  return VCLHIP_SUCCESS;
+}
+
+static uint32_t vec_grid_n(uint32_t n) { return chunk_grid((n + 4 * kBlock - 1) / (4 * kBlock), vec_gmax()); }
+
+// ------------------------------------------------------------------ solve stream
+
+// Graph capture needs a real stream.  When the caller passes the null stream and asks for graphs,
+// the solve runs on a private non-blocking stream, ordered after all earlier device work.
+class SolveStream {
+ public:
+  SolveStream(void* user, bool need_capture)
+  {
+    s_ = static_cast<hipStream_t>(user);
+    if (!s_ && need_capture) {
+      status_ = hip_status(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      if (!status_) status_ = hip_status(hipStreamCreateWithFlags(&own_, hipStreamNonBlocking), "hipStreamCreate");
+      s_ = own_;
+    }
+  }
+  ~SolveStream()
+  {
+    if (own_) {
+      (void)hipStreamSynchronize(own_);
+      (void)hipStreamDestroy(own_);
+    }
+  }
+  hipStream_t get() const { return s_; }
+  int status() const { return status_; }
+
+ private:
+  hipStream_t s_ = nullptr;
+  hipStream_t own_ = nullptr;
+  int status_ = VCLHIP_SUCCESS;
+};
+
+// ------------------------------------------------------------------ device workspace
+
+// Stream-ordered allocations released when the object goes out of scope.
+class Workspace {
+ public:
+  explicit Workspace(hipStream_t s) : s_(s) {}
+  ~Workspace()
+  {
+    for (void* p : ptrs_) (void)hipFreeAsync(p, s_);
+  }
+  template <typename U>
+  int alloc(U** p, size_t count)
+  {
+    void* v = nullptr;
+    VCLHIP_HIP(hipMallocAsync(&v, count * sizeof(U) + 256, s_));
+    ptrs_.push_back(v);
+    *p = static_cast<U*>(v);
+    return VCLHIP_SUCCESS;
+  }
+
+ private:
+  hipStream_t s_;
+  std::vector<void*> ptrs_;
+};
+
+// Diagonal of A for Jacobi (linalg/detail row_info with SPARSE_ROW_DIAGONAL), failing on a zero entry.
+__global__ __launch_bounds__(kBlock) void count_zero_kernel(uint32_t n, const double* __restrict__ d,
+                                                            uint32_t* __restrict__ count)
+{
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n && !(d[i] < 0 || d[i] > 0)) atomicAdd(count, 1u);
+}
+
+__global__ __launch_bounds__(kBlock) void sell_diag_kernel(SellDev<double> A, double* __restrict__ d)
+{
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= A.n_rows) return;
+  const uint32_t s = row / A.C;
+  const uint32_t lane = row - s * A.C;
+  const size_t base = static_cast<size_t>(A.start[s]) + lane;
+  double v = 0;
+  for (uint32_t k = 0; k < A.cpb[s]; ++k) {
+    const size_t off = base + static_cast<size_t>(k) * A.C;
+    if (A.col[off] == row && (A.val[off] < 0 || A.val[off] > 0)) { v = A.val[off]; break; }
+  }
+  d[row] = v;
+}
+
+static int jacobi_diag(const DevOp<double>& op, const vclhip_csr* Acsr, double* d, uint32_t* zero_count,
+                       hipStream_t s)
+{
+  if (!op.sell) {
+    VCLHIP_TRY(vclhip_csr_row_info_f64(Acsr, d, 3, s));
+  } else {
+    hipLaunchKernelGGL(sell_diag_kernel, dim3((op.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, op.sm, d);
+    VCLHIP_TRY(check_launch("sell_diag_kernel"));
+  }
+  VCLHIP_HIP(hipMemsetAsync(zero_count, 0, sizeof(uint32_t), s));
+  hipLaunchKernelGGL(count_zero_kernel, dim3((op.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, op.n, d, zero_count);
+  VCLHIP_TRY(check_launch("count_zero_kernel"));
+  uint32_t zeros = 0;
+  VCLHIP_HIP(hipMemcpyAsync(&zeros, zero_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  VCLHIP_HIP(hipStreamSynchronize(s));
+  if (zeros) {
+    set_error("Jacobi preconditioner: %u zero diagonal entries", zeros);
+    return VCLHIP_ERROR_ZERO_DIAGONAL;
+  }
+  return VCLHIP_SUCCESS;
+}
+
+// ------------------------------------------------------------------ iteration runner
+
+struct PinnedState {
+  KState* h = nullptr;
+  ~PinnedState()
+  {
+    if (h) (void)hipHostFree(h);
+  }
+};
+
+struct EventPair {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  ~EventPair()
+  {
+    for (auto ev : e)
+      if (ev) (void)hipEventDestroy(ev);
+  }
+};
+
+// Enqueues iterations in chunks and polls the state of the previous chunk.  launch_iter(i) enqueues
+// iteration i (0-based); needs_hook(i) says whether iteration i needs extra work before it (a
+// restart), which keeps it out of a captured graph; hook(i) enqueues that work.
+template <class LaunchIter, class NeedsHook, class Hook>
+static int run_iterations(hipStream_t s, KState* st_dev, uint32_t first_iter, uint32_t max_iters,
+                          const vclhip_solver_params* P, const double* x_dev, LaunchIter&& launch_iter,
+                          NeedsHook&& needs_hook, Hook&& hook, KState& final_state, bool bicg)
+{
+  PinnedState pin;
+  VCLHIP_HIP(hipHostMalloc(reinterpret_cast<void**>(&pin.h), 2 * sizeof(KState), hipHostMallocDefault));
+  EventPair ev;
+  VCLHIP_HIP(hipEventCreateWithFlags(&ev.e[0], hipEventDisableTiming));
+  VCLHIP_HIP(hipEventCreateWithFlags(&ev.e[1], hipEventDisableTiming));
+
+  const bool monitor = P && P->monitor;
+  uint32_t chunk = (P && P->chunk) ? P->chunk : 16;
+  if (monitor) chunk = 1;
+  const bool want_graph = P && P->use_graph && !monitor && chunk > 1;
+
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  auto cleanup_graph = [&]() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  };
+
+  const uint32_t end_mask = bicg ? (kFlagDone | kFlagFinishing) : kFlagDone;
+  uint32_t launched = first_iter;
+  int pending = -1;
+  int slot = 0;
+  bool finished = false;
+  int status = VCLHIP_SUCCESS;
+  while (!finished && launched < max_iters) {
+    const uint32_t cnt = std::min(chunk, max_iters - launched);
+    bool hook_inside = false;
+    for (uint32_t k = 0; k < cnt; ++k) hook_inside = hook_inside || needs_hook(launched + k);
+    if (want_graph && cnt == chunk && !hook_inside) {
+      if (!gexec) {
+        status = hip_status(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        if (status) break;
+        int st2 = VCLHIP_SUCCESS;
+        for (uint32_t k = 0; k < cnt && !st2; ++k) st2 = launch_iter(launched + k);
+        status = hip_status(hipStreamEndCapture(s, &graph), "hipStreamEndCapture");
+        if (st2) status = st2;
+        if (status) break;
+        status = hip_status(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        if (status) break;
+      }
+      status = hip_status(hipGraphLaunch(gexec, s), "hipGraphLaunch");
+      if (status) break;
+    } else {
+      for (uint32_t k = 0; k < cnt && !status; ++k) {
+        if (needs_hook(launched + k)) status = hook(launched + k);
+        if (!status) status = launch_iter(launched + k);
+      }
+      if (status) break;
+    }
+    launched += cnt;
+    status = hip_status(hipMemcpyAsync(&pin.h[slot], st_dev, sizeof(KState), hipMemcpyDeviceToHost, s), "state D2H");
+    if (!status) status = hip_status(hipEventRecord(ev.e[slot], s), "hipEventRecord");
+    if (status) break;
+    if (monitor) {
+      status = hip_status(hipEventSynchronize(ev.e[slot]), "hipEventSynchronize");
+      if (status) break;
+      const KState& h = pin.h[slot];
+      if (h.iter > 0 && !h.breakdown) {
+        const double rel = bicg ? std::fabs(h.res / h.nrs) : std::sqrt(std::fabs(h.rr / h.nrs));
+        if (P->monitor(x_dev, rel, P->monitor_user)) {
+          uint32_t done = kFlagDone;
+          status = hip_status(hipMemcpyAsync(&st_dev->flags, &done, sizeof(uint32_t), hipMemcpyHostToDevice, s),
+                              "stop flag");
+          finished = true;
+        }
+      }
+      if (h.flags & end_mask) finished = true;
+    } else if (pending >= 0) {
+      status = hip_status(hipEventSynchronize(ev.e[pending]), "hipEventSynchronize");
+      if (status) break;
+      if (pin.h[pending].flags & end_mask) finished = true;
+    }
+    pending = slot;
+    slot ^= 1;
+  }
+  cleanup_graph();
+  int st_sync = hip_status(hipStreamSynchronize(s), "solver sync");
+  if (!status) status = st_sync;
+  if (!status)
+    status = hip_status(hipMemcpy(&final_state, st_dev, sizeof(KState), hipMemcpyDeviceToHost), "final state");
+  return status;
+}
+
+static void init_state(KState& h, const vclhip_solver_params* P, double* history_dev)
+{
+  std::memset(&h, 0, sizeof(h));
+  h.tol = P->tol;
+  h.abs_tol = P->abs_tol;
+  h.max_iters = P->max_iters;
+  h.history = history_dev;
+  h.history_len = history_dev ? std::min(P->history_len, P->max_iters) : 0;
+}
+
+static int upload_state(KState* st_dev, const KState& h, hipStream_t s)
+{
+  VCLHIP_HIP(hipMemcpyAsync(st_dev, &h, sizeof(KState), hipMemcpyHostToDevice, s));
+  VCLHIP_HIP(hipStreamSynchronize(s));  // h lives on the host stack
+  return VCLHIP_SUCCESS;
+}
+
+static int copy_history(const vclhip_solver_params* P, const double* history_dev, uint32_t iters, hipStream_t s)
+{
+  if (!P->history_host || !history_dev) return VCLHIP_SUCCESS;
+  const uint32_t cnt = std::min(std::min(P->history_len, P->max_iters), iters);
+  if (cnt == 0) return VCLHIP_SUCCESS;
+  VCLHIP_HIP(hipMemcpyAsync(P->history_host, history_dev, cnt * sizeof(double), hipMemcpyDeviceToHost, s));
+  VCLHIP_HIP(hipStreamSynchronize(s));
+  return VCLHIP_SUCCESS;
+}
+
+// ------------------------------------------------------------------ CG driver
+
+template <class M>
+static int cg_solve(const M* Ain, const double* b, double* x, int precond, const vclhip_solver_params* P,
+                    vclhip_solver_result* R, void* stream)
+{
+  if (!P || !R || !b || !x) return invalid("null argument");
+  if (precond != VCLHIP_PRECOND_NONE && precond != VCLHIP_PRECOND_JACOBI) return invalid("precond");
+  DevOp<double> op;
+  VCLHIP_TRY(make_op(Ain, op));
+  SolveStream ss(stream, P->use_graph != 0);
+  VCLHIP_TRY(ss.status());
+  hipStream_t s = ss.get();
+  const uint32_t n = op.n;
+  const bool jac = precond == VCLHIP_PRECOND_JACOBI;
+  Workspace ws(s);
+  double *r, *p, *Ap, *d = nullptr, *part, *hist = nullptr;
+  KState* st;
+  VCLHIP_TRY(ws.alloc(&r, n));
+  VCLHIP_TRY(ws.alloc(&p, n));
+  VCLHIP_TRY(ws.alloc(&Ap, n));
+  VCLHIP_TRY(ws.alloc(&part, kSlots * kMaxPartials));
+  VCLHIP_TRY(ws.alloc(&st, 1));
+  if (P->history_host && P->history_len && P->max_iters) VCLHIP_TRY(ws.alloc(&hist, std::min(P->history_len, P->max_iters)));
+  if (jac) {
+    uint32_t* zc;
+    VCLHIP_TRY(ws.alloc(&d, n));
+    VCLHIP_TRY(ws.alloc(&zc, 1));
+    VCLHIP_TRY(jacobi_diag(op, reinterpret_cast<const vclhip_csr*>(Ain), d, zc, s));
+  }
+  KState h;
+  init_state(h, P, hist);
+  VCLHIP_TRY(upload_state(st, h, s));
+
+  const uint32_t gv = vec_grid_n(n);
+  // setup: x = 0, r = b, p = M^-1 b, Ap = A p, scalars
+  if (jac)
+    hipLaunchKernelGGL((cg_init_kernel<double, true>), dim3(gv), dim3(kBlock), 0, s, n, b, x, r, p, d, part);
+  else
+    hipLaunchKernelGGL((cg_init_kernel<double, false>), dim3(gv), dim3(kBlock), 0, s, n, b, x, r, p, d, part);
+  VCLHIP_TRY(check_launch("cg_init_kernel"));
+  if (jac)
+    op.launch(s, p, EpiCg<double, true>{Ap, p, d}, slot_ptr(part, 1), nullptr, 0u);
+  else
+    op.launch(s, p, EpiCg<double, false>{Ap, p, d}, slot_ptr(part, 1), nullptr, 0u);
+  VCLHIP_TRY(check_launch("cg prod"));
+  if (jac)
+    hipLaunchKernelGGL(cg_setup_kernel<true>, dim3(1), dim3(kBlock), 0, s, st, part, gv, op.grid);
+  else
+    hipLaunchKernelGGL(cg_setup_kernel<false>, dim3(1), dim3(kBlock), 0, s, st, part, gv, op.grid);
+  VCLHIP_TRY(check_launch("cg_setup_kernel"));
+
+  uint32_t* flags = &st->flags;
+  auto launch_iter = [&](uint32_t) -> int {
+    if (jac)
+      hipLaunchKernelGGL((cg_update_kernel<double, true>), dim3(gv), dim3(kBlock), 0, s, n, x, p, r, Ap, d, st, part);
+    else
+      hipLaunchKernelGGL((cg_update_kernel<double, false>), dim3(gv), dim3(kBlock), 0, s, n, x, p, r, Ap, d, st, part);
+    if (jac)
+      op.launch(s, p, EpiCg<double, true>{Ap, p, d}, slot_ptr(part, 1), flags, kFlagDone);
+    else
+      op.launch(s, p, EpiCg<double, false>{Ap, p, d}, slot_ptr(part, 1), flags, kFlagDone);
+    hipLaunchKernelGGL(cg_iter_kernel, dim3(1), dim3(kBlock), 0, s, st, part, gv, op.grid);
+    return check_launch("cg iteration");
+  };
+  KState fin;
+  VCLHIP_TRY(run_iterations(
+      s, st, 0u, P->max_iters, P, x, launch_iter, [](uint32_t) { return false; }, [](uint32_t) { return 0; }, fin,
+      false));
+  R->iters = fin.iter;
+  R->error = fin.nrs != 0 ? std::sqrt(std::fabs(fin.rr) / fin.nrs) : 0.0;
+  if (jac && fin.nrs != 0) R->error = std::sqrt(std::fabs(fin.rr / fin.nrs));
+  R->restarts = 0;
+  R->converged = static_cast<int>(fin.converged);
+  return copy_history(P, hist, fin.iter, s);
+}
+
+// ------------------------------------------------------------------ BiCGStab driver
+
+template <class M>
+static int bicgstab_solve(const M* Ain, const double* b, double* x, int precond, const vclhip_solver_params* P,
+                          vclhip_solver_result* R, void* stream)
+{
+  if (!P || !R || !b || !x) return invalid("null argument");
+  if (precond != VCLHIP_PRECOND_NONE && precond != VCLHIP_PRECOND_JACOBI) return invalid("precond");
+  DevOp<double> op;
+  VCLHIP_TRY(make_op(Ain, op));
+  SolveStream ss(stream, P->use_graph != 0);
+  VCLHIP_TRY(ss.status());
+  hipStream_t s = ss.get();
+  const uint32_t n = op.n;
+  const bool jac = precond == VCLHIP_PRECOND_JACOBI;
+  Workspace ws(s);
+  double *r, *p, *r0, *Ap, *sv, *As, *d = nullptr, *part, *hist = nullptr;
+  KState* st;
+  VCLHIP_TRY(ws.alloc(&r, n));
+  VCLHIP_TRY(ws.alloc(&p, n));
+  VCLHIP_TRY(ws.alloc(&r0, n));
+  VCLHIP_TRY(ws.alloc(&Ap, n));
+  VCLHIP_TRY(ws.alloc(&sv, n));
+  VCLHIP_TRY(ws.alloc(&As, n));
+  VCLHIP_TRY(ws.alloc(&part, kSlots * kMaxPartials));
+  VCLHIP_TRY(ws.alloc(&st, 1));
+  if (P->history_host && P->history_len && P->max_iters) VCLHIP_TRY(ws.alloc(&hist, std::min(P->history_len, P->max_iters)));
+  if (jac) {
+    uint32_t* zc;
+    VCLHIP_TRY(ws.alloc(&d, n));
+    VCLHIP_TRY(ws.alloc(&zc, 1));
+    VCLHIP_TRY(jacobi_diag(op, reinterpret_cast<const vclhip_csr*>(Ain), d, zc, s));
+  }
+  KState h;
+  init_state(h, P, hist);
+  VCLHIP_TRY(upload_state(st, h, s));
+
+  const uint32_t gv = vec_grid_n(n);
+  if (jac)
+    hipLaunchKernelGGL((bicg_init_kernel<double, true>), dim3(gv), dim3(kBlock), 0, s, n, b, x, r, p, r0, d, part);
+  else
+    hipLaunchKernelGGL((bicg_init_kernel<double, false>), dim3(gv), dim3(kBlock), 0, s, n, b, x, r, p, r0, d, part);
+  VCLHIP_TRY(check_launch("bicg_init_kernel"));
+  hipLaunchKernelGGL(bicg_setup_kernel, dim3(1), dim3(kBlock), 0, s, st, part, gv);
+  VCLHIP_TRY(check_launch("bicg_setup_kernel"));
+
+  uint32_t* flags = &st->flags;
+  const uint32_t stop_all = kFlagDone | kFlagFinishing;
+  const int brk = jac ? 1 : 0;
+  auto launch_iter = [&](uint32_t) -> int {
+    if (jac)
+      op.launch(s, p, EpiBicg<double, true, false>{Ap, p, r0, d}, slot_ptr(part, 2), flags, stop_all);
+    else
+      op.launch(s, p, EpiBicg<double, false, false>{Ap, p, r0, d}, slot_ptr(part, 2), flags, stop_all);
+    hipLaunchKernelGGL(bicg_alpha_kernel, dim3(1), dim3(kBlock), 0, s, st, part, gv, op.grid, brk);
+    hipLaunchKernelGGL(bicg_update_s_kernel<double>, dim3(gv), dim3(kBlock), 0, s, n, sv, r, Ap, st, part);
+    if (jac)
+      op.launch(s, sv, EpiBicg<double, true, true>{As, sv, r0, d}, slot_ptr(part, 4), flags, stop_all);
+    else
+      op.launch(s, sv, EpiBicg<double, false, true>{As, sv, r0, d}, slot_ptr(part, 4), flags, stop_all);
+    hipLaunchKernelGGL(bicg_scalars_kernel, dim3(1), dim3(kBlock), 0, s, st, part, gv, op.grid, brk);
+    hipLaunchKernelGGL(bicg_vec_update_kernel<double>, dim3(gv), dim3(kBlock), 0, s, n, x, p, sv, r, As, Ap, r0, st,
+                       part);
+    return check_launch("bicgstab iteration");
+  };
+  // Restarts of the preconditioned iteration (bicgstab.hpp:431-445, :473): iteration i restarts when
+  // i - last_restart - 1 > max_iters_before_restart.
+  const uint64_t mr = P->max_iters_before_restart;
+  uint64_t last_restart = 0;
+  auto needs_restart = [&](uint32_t i) { return jac && static_cast<uint64_t>(i) > last_restart + mr + 1; };
+  auto restart = [&](uint32_t i) -> int {
+    last_restart = i;
+    if (jac)
+      op.launch(s, x, EpiRestart<double, true>{b, d, r, p, r0}, slot_ptr(part, 7), flags, stop_all);
+    else
+      op.launch(s, x, EpiRestart<double, false>{b, d, r, p, r0}, slot_ptr(part, 7), flags, stop_all);
+    hipLaunchKernelGGL(bicg_restart_kernel, dim3(1), dim3(kBlock), 0, s, st, part, op.grid);
+    return check_launch("bicgstab restart");
+  };
+
+  KState fin;
+  uint32_t restarts = 0;
+  uint32_t first = 0;
+  for (;;) {
+    VCLHIP_TRY(run_iterations(s, st, first, P->max_iters, P, x, launch_iter, needs_restart, restart, fin, true));
+    if (!(jac && fin.breakdown) || fin.iter >= P->max_iters) break;
+    // breakdown: restart from the current x at iteration fin.iter and continue
+    fin.breakdown = 0;
+    fin.flags = 0;
+    fin.final_mode = 0;
+    VCLHIP_TRY(upload_state(st, fin, s));
+    VCLHIP_TRY(restart(fin.iter));
+    ++restarts;
+    first = fin.iter;
+  }
+  R->iters = fin.iter;
+  R->error = fin.nrs != 0 ? fin.res / fin.nrs : 0.0;
+  R->restarts = restarts;
+  R->converged = static_cast<int>(fin.converged);
+  return copy_history(P, hist, fin.iter, s);
+}
+
+// ------------------------------------------------------------------ pipelined ops with the reference's buffer
+
+// Sums slot q over `count` partials into buf[dest[q]] and zeroes buf[dest[q]+1 .. dest[q]+chunk).
+__global__ __launch_bounds__(kBlock) void write_chunks_kernel(const double* __restrict__ part, uint32_t count, int K,
+                                                              uint32_t d0, uint32_t d1, uint32_t d2,
+                                                              double* __restrict__ buf, uint32_t chunk)
+{
+  const uint32_t dest[3] = {d0, d1, d2};
+  for (int q = 0; q < K; ++q) {
+    const double v = block_sum_partials(slot_ptr(part, q), count);
+    for (uint32_t i = threadIdx.x; i < chunk; i += kBlock) buf[dest[q] + i] = (i == 0) ? v : 0.0;
+    __syncthreads();
+  }
+}
+
+// alpha = sum(buf[0..chunk)) / sum(buf[3chunk..4chunk)), sequential as host_based :539-544
+__global__ void bicg_buffer_alpha_kernel(const double* __restrict__ buf, uint32_t chunk, double* __restrict__ alpha)
+{
+  if (threadIdx.x != 0) return;
+  double a = 0, c = 0;
+  for (uint32_t i = 0; i < chunk; ++i) {
+    a += buf[i];
+    c += buf[i + 3 * chunk];
+  }
+  *alpha = a / c;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bicg_update_s_ptr_kernel(uint32_t n, T* __restrict__ s,
+                                                                   const T* __restrict__ r, const T* __restrict__ Ap,
+                                                                   const double* __restrict__ alpha_dev,
+                                                                   double* __restrict__ part)
+{
+  const T alpha = static_cast<T>(*alpha_dev);
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range(n, lw, gridDim.x, c0, c1);
+  double acc[1] = {0};
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    const T vs = r[i] - alpha * Ap[i];
+    acc[0] += static_cast<double>(vs * vs);
+    s[i] = vs;
+  }
+  block_store_partials<1, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_update_scalar_kernel(uint32_t n, T* __restrict__ x, T alpha,
+                                                                  T* __restrict__ p, T* __restrict__ r,
+                                                                  const T* __restrict__ Ap, T beta,
+                                                                  double* __restrict__ part)
+{
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range(n, lw, gridDim.x, c0, c1);
+  double acc[1] = {0};
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    T vp = p[i];
+    T vr = r[i];
+    x[i] += alpha * vp;
+    vr -= alpha * Ap[i];
+    vp = vr + beta * vp;
+    acc[0] += static_cast<double>(vr * vr);
+    p[i] = vp;
+    r[i] = vr;
+  }
+  block_store_partials<1, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bicg_vec_update_scalar_kernel(uint32_t n, T* __restrict__ x, T alpha,
+                                                                        T* __restrict__ p, T omega,
+                                                                        const T* __restrict__ s, T* __restrict__ r,
+                                                                        const T* __restrict__ As, T beta,
+                                                                        const T* __restrict__ Ap,
+                                                                        const T* __restrict__ r0,
+                                                                        double* __restrict__ part)
+{
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range(n, lw, gridDim.x, c0, c1);
+  double acc[1] = {0};
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    T vx = x[i];
+    T vp = p[i];
+    const T vs = s[i];
+    vx += alpha * vp + omega * vs;
+    const T vr = vs - omega * As[i];
+    vp = vr + beta * (vp - omega * Ap[i]);
+    acc[0] += static_cast<double>(vr * r0[i]);
+    x[i] = vx;
+    r[i] = vr;
+    p[i] = vp;
+  }
+  block_store_partials<1, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
+}
+
+template <class M>
+static int pipelined_prod(const M* A, const double* p, double* Ap, const double* r0, double* buf, uint32_t chunk,
+                          uint32_t offset, void* stream)
+{
+  if (!p || !Ap || !buf || chunk == 0) return invalid("null argument");
+  DevOp<double> op;
+  VCLHIP_TRY(make_op(A, op));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Workspace ws(s);
+  double* part;
+  VCLHIP_TRY(ws.alloc(&part, kSlots * kMaxPartials));
+  if (r0) {
+    op.launch(s, p, EpiBicg<double, false, true>{Ap, p, r0, nullptr}, slot_ptr(part, 0), nullptr, 0u);
+    VCLHIP_TRY(check_launch("pipelined bicgstab prod"));
+    hipLaunchKernelGGL(write_chunks_kernel, dim3(1), dim3(kBlock), 0, s, part, op.grid, 3, chunk, 2 * chunk, offset,
+                       buf, chunk);
+  } else {
+    op.launch(s, p, EpiCg<double, false>{Ap, p, nullptr}, slot_ptr(part, 0), nullptr, 0u);
+    VCLHIP_TRY(check_launch("pipelined cg prod"));
+    hipLaunchKernelGGL(write_chunks_kernel, dim3(1), dim3(kBlock), 0, s, part, op.grid, 2, chunk, 2 * chunk, 0u, buf,
+                       chunk);
+  }
+  return check_launch("write_chunks_kernel");
+}
+
+}  // namespace vclhip
+
+using namespace vclhip;
+
+extern "C" {
+
+void vclhip_solver_params_default(vclhip_solver_params* P)
+{
+  if (!P) return;
+  std::memset(P, 0, sizeof(*P));
+  P->tol = 1e-8;                     // cg_tag / bicgstab_tag / mixed_precision_cg_tag default
+  P->abs_tol = 0;
+  P->max_iters = 300;                // cg_tag default (cg.hpp:56)
+  P->max_iters_before_restart = 200; // bicgstab_tag default (bicgstab.hpp:56)
+  P->inner_tol = 1e-2f;              // mixed_precision_cg_tag default (mixed_precision_cg.hpp:56)
+  P->chunk = 16;
+  P->use_graph = 0;
+  P->mixed_mode = 0;
+}
+
+int vclhip_cg_solve_csr_f64(const vclhip_csr* A, const double* b, double* x, int precond,
+                            const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return cg_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_cg_solve_sell_f64(const vclhip_sell* A, const double* b, double* x, int precond,
+                             const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return cg_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_bicgstab_solve_csr_f64(const vclhip_csr* A, const double* b, double* x, int precond,
+                                  const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return bicgstab_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_bicgstab_solve_sell_f64(const vclhip_sell* A, const double* b, double* x, int precond,
+                                   const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return bicgstab_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_pipelined_cg_vector_update_f64(uint32_t n, double* x, double alpha, double* p, double* r, const double* Ap,
+                                          double beta, double* buf, uint32_t chunk, void* stream)
+{
+  if (!x || !p || !r || !Ap || !buf || chunk == 0) return invalid("null argument");
+  if (n == 0) return VCLHIP_SUCCESS;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Workspace ws(s);
+  double* part;
+  VCLHIP_TRY(ws.alloc(&part, kMaxPartials));
+  const uint32_t gv = vec_grid_n(n);
+  hipLaunchKernelGGL(cg_update_scalar_kernel<double>, dim3(gv), dim3(kBlock), 0, s, n, x, alpha, p, r, Ap, beta, part);
+  VCLHIP_TRY(check_launch("cg_update_scalar_kernel"));
+  hipLaunchKernelGGL(write_chunks_kernel, dim3(1), dim3(kBlock), 0, s, part, gv, 1, 0u, 0u, 0u, buf, chunk);
+  return check_launch("write_chunks_kernel");
+}
+
+int vclhip_pipelined_cg_prod_csr_f64(const vclhip_csr* A, const double* p, double* Ap, double* buf, uint32_t chunk,
+                                     void* stream)
+{
+  return pipelined_prod(A, p, Ap, nullptr, buf, chunk, 0u, stream);
+}
+
+int vclhip_pipelined_cg_prod_sell_f64(const vclhip_sell* A, const double* p, double* Ap, double* buf, uint32_t chunk,
+                                      void* stream)
+{
+  return pipelined_prod(A, p, Ap, nullptr, buf, chunk, 0u, stream);
+}
+
+int vclhip_pipelined_bicgstab_prod_csr_f64(const vclhip_csr* A, const double* p, double* Ap, const double* r0,
+                                           double* buf, uint32_t chunk, uint32_t offset, void* stream)
+{
+  if (!r0) return invalid("r0star is NULL");
+  return pipelined_prod(A, p, Ap, r0, buf, chunk, offset, stream);
+}
+
+int vclhip_pipelined_bicgstab_prod_sell_f64(const vclhip_sell* A, const double* p, double* Ap, const double* r0,
+                                            double* buf, uint32_t chunk, uint32_t offset, void* stream)
+{
+  if (!r0) return invalid("r0star is NULL");
+  return pipelined_prod(A, p, Ap, r0, buf, chunk, offset, stream);
+}
+
+int vclhip_pipelined_bicgstab_update_s_f64(uint32_t n, double* s_vec, const double* r, const double* Ap, double* buf,
+                                           uint32_t chunk, uint32_t offset, void* stream)
+{
+  if (!s_vec || !r || !Ap || !buf || chunk == 0) return invalid("null argument");
+  if (n == 0) return VCLHIP_SUCCESS;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Workspace ws(s);
+  double *part, *alpha;
+  VCLHIP_TRY(ws.alloc(&part, kMaxPartials));
+  VCLHIP_TRY(ws.alloc(&alpha, 1));
+  hipLaunchKernelGGL(bicg_buffer_alpha_kernel, dim3(1), dim3(64), 0, s, buf, chunk, alpha);
+  VCLHIP_TRY(check_launch("bicg_buffer_alpha_kernel"));
+  const uint32_t gv = vec_grid_n(n);
+  hipLaunchKernelGGL(bicg_update_s_ptr_kernel<double>, dim3(gv), dim3(kBlock), 0, s, n, s_vec, r, Ap, alpha, part);
+  VCLHIP_TRY(check_launch("bicg_update_s_ptr_kernel"));
+  hipLaunchKernelGGL(write_chunks_kernel, dim3(1), dim3(kBlock), 0, s, part, gv, 1, offset, 0u, 0u, buf, chunk);
+  return check_launch("write_chunks_kernel");
+}
+
+int vclhip_pipelined_bicgstab_vector_update_f64(uint32_t n, double* x, double alpha, double* p, double omega,
+                                                const double* s_vec, double* r, const double* As, double beta,
+                                                const double* Ap, const double* r0, double* buf, uint32_t chunk,
+                                                void* stream)
+{
+  if (!x || !p || !s_vec || !r || !As || !Ap || !r0 || !buf || chunk == 0) return invalid("null argument");
+  if (n == 0) return VCLHIP_SUCCESS;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Workspace ws(s);
+  double* part;
+  VCLHIP_TRY(ws.alloc(&part, kMaxPartials));
+  const uint32_t gv = vec_grid_n(n);
+  hipLaunchKernelGGL(bicg_vec_update_scalar_kernel<double>, dim3(gv), dim3(kBlock), 0, s, n, x, alpha, p, omega, s_vec,
+                     r, As, beta, Ap, r0, part);
+  VCLHIP_TRY(check_launch("bicg_vec_update_scalar_kernel"));
+  hipLaunchKernelGGL(write_chunks_kernel, dim3(1), dim3(kBlock), 0, s, part, gv, 1, 0u, 0u, 0u, buf, chunk);
+  return check_launch("write_chunks_kernel");
+}
+
+}  // extern "C"

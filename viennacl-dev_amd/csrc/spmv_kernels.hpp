@@ -1,0 +1,202 @@
+// spmv_kernels.hpp -- CSR and SELL-C SpMV kernel templates with a pluggable per-row epilogue.
+//
+// The epilogue receives each finished row sum; plain SpMV stores alpha*s + beta*y, the Krylov
+// kernels also store the row and accumulate the inner products that the solver needs next
+// (reference: cuda/iterative_operations.hpp:113-274, 543-607, 895-1074, 1381-1453 fuse the same
+// products into their SpMV kernels).  Epilogue interface:
+//   using acc_t = ...;  static constexpr int K = number of inner products (0..3);
+//   __device__ void operator()(uint32_t row, T sum, acc_t* acc) const;
+// Workgroups write their K partial sums to part[q * pstride + logical_block].  A kernel returns
+// at once when (*stop & stop_mask) is nonzero, which lets a solver enqueue iterations ahead of
+// its convergence test.
+//
+// Every kernel here is launched with 256 lanes per workgroup and a grid that is a multiple of 8
+// (see xcd_logical_block).  Floating-point contraction is disabled for the whole library
+// (-ffp-contract=off) so that a row reduced by one lane sums its products in the same order and
+// with the same roundings as the reference's sequential host loop.
+#pragma once
+
+#include "device.hpp"
+#include "plan.hpp"
+
+namespace vclhip {
+
+template <typename T>
+struct CsrDev {
+  const uint32_t* __restrict__ row_ptr;
+  const uint32_t* __restrict__ col;
+  const T* __restrict__ val;
+  const uint32_t* __restrict__ blocks;
+  uint32_t num_blocks;
+  uint32_t n_rows;
+};
+
+template <typename T>
+struct SellDev {
+  const uint32_t* __restrict__ cpb;
+  const uint32_t* __restrict__ col;
+  const uint32_t* __restrict__ start;
+  const T* __restrict__ val;
+  uint32_t n_rows;
+  uint32_t C;
+};
+
+template <typename T, bool Contig>
+__device__ __forceinline__ T load_x(const T* __restrict__ x, uint32_t c, uint32_t xs, uint32_t xi)
+{
+  if (Contig) return x[c];
+  return x[static_cast<size_t>(c) * xi + xs];
+}
+
+// y = alpha*s + beta*y with the reference's beta test (host_based/sparse_matrix_operations.hpp:177-183).
+template <typename T, bool Contig>
+struct EpiAxpby {
+  using acc_t = T;
+  static constexpr int K = 0;
+  T* __restrict__ y;
+  uint32_t ys, yi;
+  T alpha, beta;
+  __device__ __forceinline__ void operator()(uint32_t row, T s, acc_t*) const
+  {
+    const size_t idx = Contig ? row : static_cast<size_t>(row) * yi + ys;
+    if (beta < 0 || beta > 0)
+      y[idx] = alpha * s + beta * y[idx];
+    else
+      y[idx] = alpha * s;
+  }
+};
+
+// CSR SpMV over the row-block plan.  Each logical workgroup walks a contiguous chunk of blocks.
+template <typename T, bool Contig, class Epi>
+__global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __restrict__ x, uint32_t xs, uint32_t xi,
+                                                     Epi epi, typename Epi::acc_t* __restrict__ part,
+                                                     uint32_t pstride, const uint32_t* __restrict__ stop,
+                                                     uint32_t stop_mask)
+{
+  using acc_t = typename Epi::acc_t;
+  constexpr int KA = Epi::K > 0 ? Epi::K : 1;
+  if (stop && (*stop & stop_mask)) return;
+
+  __shared__ T stage[kCsrNnzCap];
+  __shared__ uint32_t srp[kCsrWG + 1];
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lw = xcd_logical_block();
+  uint32_t b0, b1;
+  chunk_range(A.num_blocks, lw, gridDim.x, b0, b1);
+
+  acc_t acc[KA];
+#pragma unroll
+  for (int q = 0; q < KA; ++q) acc[q] = 0;
+
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint32_t d0 = A.blocks[b];
+    const uint32_t r0 = d0 & (kPlanRowLimit - 1);
+    const uint32_t mode = d0 >> 28;
+    const uint32_t r1 = A.blocks[b + 1] & (kPlanRowLimit - 1);
+    const uint32_t e0 = A.row_ptr[r0];
+    const uint32_t e1 = A.row_ptr[r1];
+
+    if (mode == kModeLongRow) {
+      // one row longer than the staging buffer: strided partial sums, then a workgroup reduction
+      T s = 0;
+      for (uint32_t e = e0 + tid; e < e1; e += kBlock) s += A.val[e] * load_x<T, Contig>(x, A.col[e], xs, xi);
+      s = block_sum(s);
+      if (tid == 0) epi(r0, s, acc);
+      continue;
+    }
+
+    // Phase 1: coalesced loads of the block's entries; products into LDS.
+    const uint32_t cnt = e1 - e0;
+    const uint32_t nrows = r1 - r0;
+#pragma unroll
+    for (uint32_t k = 0; k < kCsrNnzCap / kBlock; ++k) {
+      const uint32_t i = tid + k * kBlock;
+      if (i < cnt) {
+        const uint32_t e = e0 + i;
+        stage[i] = A.val[e] * load_x<T, Contig>(x, A.col[e], xs, xi);
+      }
+    }
+    if (tid < nrows) srp[tid + 1] = A.row_ptr[r0 + tid + 1] - e0;
+    if (tid == 0) srp[0] = 0;
+    __syncthreads();
+
+    // Phase 2: 2^mode lanes per row sum the staged products.
+    const uint32_t L = 1u << mode;
+    const uint32_t g = tid >> mode;
+    const uint32_t lane = tid & (L - 1);
+    if (g < nrows) {
+      const uint32_t a = srp[g];
+      const uint32_t z = srp[g + 1];
+      T s = 0;
+      for (uint32_t k = a + lane; k < z; k += L) s += stage[k];
+      s = group_sum(s, L);
+      if (lane == 0) epi(r0 + g, s, acc);
+    }
+    __syncthreads();
+  }
+
+  if constexpr (Epi::K > 0) block_store_partials<Epi::K, acc_t>(acc, part, pstride, lw);
+}
+
+// SELL-C SpMV, one lane per row; with C = 64 a wavefront is exactly one slice, so all lanes of a
+// wavefront run the same number of columns and every load is one contiguous 64-lane access.
+template <typename T, bool Contig, class Epi>
+__global__ __launch_bounds__(kBlock) void sell_kernel(SellDev<T> A, const T* __restrict__ x, uint32_t xs,
+                                                      uint32_t xi, Epi epi, typename Epi::acc_t* __restrict__ part,
+                                                      uint32_t pstride, const uint32_t* __restrict__ stop,
+                                                      uint32_t stop_mask)
+{
+  using acc_t = typename Epi::acc_t;
+  constexpr int KA = Epi::K > 0 ? Epi::K : 1;
+  if (stop && (*stop & stop_mask)) return;
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lw = xcd_logical_block();
+  const uint32_t tiles = (A.n_rows + kBlock - 1) / kBlock;
+  uint32_t t0, t1;
+  chunk_range(tiles, lw, gridDim.x, t0, t1);
+
+  acc_t acc[KA];
+#pragma unroll
+  for (int q = 0; q < KA; ++q) acc[q] = 0;
+
+  for (uint32_t t = t0; t < t1; ++t) {
+    const uint32_t row = t * kBlock + tid;
+    if (row < A.n_rows) {
+      const uint32_t s = row / A.C;
+      const uint32_t lane = row - s * A.C;
+      const uint32_t ncols = A.cpb[s];
+      const size_t base = static_cast<size_t>(A.start[s]) + lane;
+      const T* __restrict__ ev = A.val + base;
+      const uint32_t* __restrict__ cv = A.col + base;
+      T sum = 0;
+#pragma unroll 4
+      for (uint32_t k = 0; k < ncols; ++k) {
+        const size_t off = static_cast<size_t>(k) * A.C;
+        const T v = ev[off];
+        const T xv = load_x<T, Contig>(x, cv[off], xs, xi);
+        sum += (v > 0 || v < 0) ? xv * v : T(0);  // host_based/sparse_matrix_operations.hpp:1831-1833
+      }
+      epi(row, sum, acc);
+    }
+  }
+
+  if constexpr (Epi::K > 0) block_store_partials<Epi::K, acc_t>(acc, part, pstride, lw);
+}
+
+// Grid for kernels that take contiguous chunks per logical workgroup: a multiple of 8,
+// at most `gmax`, at least 8.
+inline uint32_t chunk_grid(uint32_t units, uint32_t gmax)
+{
+  uint32_t g = units < gmax ? units : gmax;
+  g = (g + 7u) & ~7u;
+  return g < 8u ? 8u : g;
+}
+
+// Tunables read once from the environment (VCLHIP_CSR_GMAX, VCLHIP_SELL_GMAX, VCLHIP_VEC_GMAX).
+uint32_t csr_gmax();
+uint32_t sell_gmax();
+uint32_t vec_gmax();
+
+}  // namespace vclhip

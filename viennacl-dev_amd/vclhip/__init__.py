@@ -1,0 +1,296 @@
+"""Python binding of libvclhip (the C ABI in include/vclhip.h).
+
+Thin layer for tests, benchmarks and the torch autograd hook: device buffers, CSR/SELL matrices
+on the device, and the SpMV / vector / solver entry points.  Every compute call goes through the
+HIP library; nothing here computes on the CPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import (Csr, Sell, SolverParams, SolverResult, MONITOR, VclHipError, call, lib,
+                   PRECOND_NONE, PRECOND_JACOBI, LIB_PATH, declared_functions)
+
+__all__ = ["DeviceArray", "Stream", "CsrMatrix", "SellMatrix", "lib", "call", "VclHipError",
+           "device_count", "PRECOND_NONE", "PRECOND_JACOBI", "SolverParams", "SolverResult"]
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def device_count():
+    c = C.c_int(0)
+    call("vclhip_device_count", C.byref(c))
+    return c.value
+
+
+class Stream:
+    """A non-blocking HIP stream owned by the caller (None = null stream)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        call("vclhip_stream_create", C.byref(h))
+        self.handle = h
+
+    def synchronize(self):
+        call("vclhip_stream_synchronize", self.handle)
+
+    def close(self):
+        if self.handle:
+            call("vclhip_stream_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device allocation holding `size` elements of `dtype`."""
+
+    def __init__(self, size, dtype):
+        self.dtype = np.dtype(dtype)
+        self.size = int(size)
+        self.nbytes = self.size * self.dtype.itemsize
+        p = C.c_void_p()
+        call("vclhip_malloc", C.byref(p), C.c_size_t(self.nbytes))
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        call("vclhip_memcpy_h2d", d.ptr, _ptr(a), C.c_size_t(d.nbytes), stream)
+        return d
+
+    @classmethod
+    def zeros(cls, size, dtype, stream=None):
+        d = cls(size, dtype)
+        call("vclhip_fill_zero", d.ptr, C.c_size_t(d.nbytes), stream)
+        return d
+
+    def to_numpy(self, stream=None):
+        out = np.empty(self.size, self.dtype)
+        call("vclhip_memcpy_d2h", _ptr(out), self.ptr, C.c_size_t(self.nbytes), stream)
+        return out
+
+    def upload(self, a, stream=None):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.size
+        call("vclhip_memcpy_h2d", self.ptr, _ptr(a), C.c_size_t(self.nbytes), stream)
+
+    def free(self):
+        if self.ptr:
+            call("vclhip_free", self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def plan_blocks(row_ptr):
+    """Row-block plan of the CSR kernels (host)."""
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    n = row_ptr.size - 1
+    blocks = np.zeros(n + 1, np.uint32)
+    nb = C.c_uint32()
+    call("vclhip_csr_plan_blocks", _ptr(row_ptr), n, _ptr(blocks), C.byref(nb))
+    return blocks[: nb.value + 1].copy()
+
+
+def reference_row_blocks(row_ptr):
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    n = row_ptr.size - 1
+    out = np.zeros(n + 1, np.uint32)
+    nb = C.c_uint32()
+    call("vclhip_csr_reference_row_blocks", _ptr(row_ptr), n, _ptr(out), C.byref(nb))
+    return out[: nb.value + 1].copy()
+
+
+def sell_from_csr(row_ptr, col, val, C_):
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    n = row_ptr.size - 1
+    ns = C.c_uint32()
+    stored = C.c_uint64()
+    call("vclhip_sell_from_csr_f64", n, C_, _ptr(row_ptr), _ptr(col), _ptr(val), C.byref(ns), C.byref(stored),
+         None, None, None, None)
+    cpb = np.zeros(ns.value, np.uint32)
+    bs = np.zeros(ns.value, np.uint32)
+    ci = np.zeros(max(stored.value, 1), np.uint32)
+    el = np.zeros(max(stored.value, 1), np.float64)
+    call("vclhip_sell_from_csr_f64", n, C_, _ptr(row_ptr), _ptr(col), _ptr(val), C.byref(ns), C.byref(stored),
+         _ptr(cpb), _ptr(ci), _ptr(bs), _ptr(el))
+    return cpb, ci[: stored.value], bs, el[: stored.value]
+
+
+class CsrMatrix:
+    """compressed_matrix on the device: row_ptr, col, values (float64 or float32) and the block plan."""
+
+    def __init__(self, row_ptr, col, val, dtype=np.float64, n_cols=None, stream=None):
+        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        val = np.ascontiguousarray(val, dtype=dtype)
+        self.n = row_ptr.size - 1
+        self.n_cols = self.n if n_cols is None else n_cols
+        self.nnz = int(row_ptr[-1])
+        self.dtype = np.dtype(dtype)
+        self.blocks_host = plan_blocks(row_ptr)
+        self.d_row_ptr = DeviceArray.from_numpy(row_ptr, stream)
+        self.d_col = DeviceArray.from_numpy(col if col.size else np.zeros(1, np.uint32), stream)
+        self.d_val = DeviceArray.from_numpy(val if val.size else np.zeros(1, dtype), stream)
+        self.d_blocks = DeviceArray.from_numpy(self.blocks_host, stream)
+        self.desc = Csr(self.n, self.n_cols, self.nnz, self.blocks_host.size - 1, self.d_row_ptr.ptr,
+                        self.d_col.ptr, self.d_val.ptr, self.d_blocks.ptr)
+
+    @property
+    def num_blocks(self):
+        return self.blocks_host.size - 1
+
+    def spmv(self, x, y, alpha=1.0, beta=0.0, xs=0, xi=1, ys=0, yi=1, stream=None):
+        fn = "vclhip_csr_spmv_f64" if self.dtype == np.float64 else "vclhip_csr_spmv_f32"
+        call(fn, C.byref(self.desc), x.ptr, xs, xi, alpha, y.ptr, ys, yi, beta, stream)
+
+    def row_info(self, out, info, stream=None):
+        fn = "vclhip_csr_row_info_f64" if self.dtype == np.float64 else "vclhip_csr_row_info_f32"
+        call(fn, C.byref(self.desc), out.ptr, info, stream)
+
+
+class SellMatrix:
+    """sliced_ell_matrix (sigma = 1) on the device, built from CSR arrays."""
+
+    def __init__(self, row_ptr, col, val, C_=64, dtype=np.float64, stream=None):
+        cpb, ci, bs, el = sell_from_csr(row_ptr, col, val, C_)
+        self.n = len(row_ptr) - 1
+        self.C = C_
+        self.dtype = np.dtype(dtype)
+        self.stored = el.size
+        self.host = (cpb, ci, bs, el)
+        self.d_cpb = DeviceArray.from_numpy(cpb, stream)
+        self.d_ci = DeviceArray.from_numpy(ci if ci.size else np.zeros(1, np.uint32), stream)
+        self.d_bs = DeviceArray.from_numpy(bs, stream)
+        self.d_el = DeviceArray.from_numpy((el if el.size else np.zeros(1)).astype(dtype), stream)
+        self.desc = Sell(self.n, self.n, C_, cpb.size, self.d_cpb.ptr, self.d_ci.ptr, self.d_bs.ptr, self.d_el.ptr)
+
+    def spmv(self, x, y, alpha=1.0, beta=0.0, xs=0, xi=1, ys=0, yi=1, stream=None):
+        fn = "vclhip_sell_spmv_f64" if self.dtype == np.float64 else "vclhip_sell_spmv_f32"
+        call(fn, C.byref(self.desc), x.ptr, xs, xi, alpha, y.ptr, ys, yi, beta, stream)
+
+
+def solver_params(tol=1e-8, max_iters=300, abs_tol=0.0, max_iters_before_restart=200, inner_tol=1e-2,
+                  chunk=16, use_graph=False, mixed_mode=0, history=None, monitor=None):
+    p = SolverParams()
+    lib().vclhip_solver_params_default(C.byref(p))
+    p.tol = tol
+    p.abs_tol = abs_tol
+    p.max_iters = max_iters
+    p.max_iters_before_restart = max_iters_before_restart
+    p.inner_tol = inner_tol
+    p.chunk = chunk
+    p.use_graph = 1 if use_graph else 0
+    p.mixed_mode = mixed_mode
+    if history is not None:
+        p.history_host = history.ctypes.data_as(C.POINTER(C.c_double))
+        p.history_len = history.size
+    if monitor is not None:
+        p.monitor = monitor
+    return p
+
+
+def _solve(kind, A, b, x, precond, params, stream):
+    res = SolverResult()
+    mat = "csr" if isinstance(A, CsrMatrix) else "sell"
+    call(f"vclhip_{kind}_solve_{mat}_f64", C.byref(A.desc), b.ptr, x.ptr, precond, C.byref(params), C.byref(res),
+         stream)
+    return res
+
+
+def cg(A, b, x, precond=PRECOND_NONE, stream=None, **kw):
+    """solve(A, b, cg_tag [, jacobi_precond]); returns SolverResult (iters, error, converged)."""
+    return _solve("cg", A, b, x, precond, solver_params(**kw), stream)
+
+
+def bicgstab(A, b, x, precond=PRECOND_NONE, stream=None, **kw):
+    kw.setdefault("max_iters", 400)
+    return _solve("bicgstab", A, b, x, precond, solver_params(**kw), stream)
+
+
+# ------------------------------------------------------------------ generators (host)
+
+def random_uniform(seed, n, a=-1.0, b=1.0):
+    out = np.empty(n, np.float64)
+    call("vclhip_random_uniform", C.c_uint64(seed), C.c_uint64(n), a, b, _ptr(out))
+    return out
+
+
+def random_normal(seed, n, mean=0.0, sigma=1.0):
+    out = np.empty(n, np.float64)
+    call("vclhip_random_normal", C.c_uint64(seed), C.c_uint64(n), mean, sigma, _ptr(out))
+    return out
+
+
+def fdm_laplace_2d(px, py):
+    n = px * py
+    rp = np.zeros(n + 1, np.uint32)
+    col = np.zeros(5 * n, np.uint32)
+    val = np.zeros(5 * n, np.float64)
+    nnz = C.c_uint32()
+    call("vclhip_gen_fdm_laplace_2d", px, py, _ptr(rp), _ptr(col), _ptr(val), C.byref(nnz))
+    return rp, col[: nnz.value].copy(), val[: nnz.value].copy()
+
+
+def laplace_3d(nx, ny=None, nz=None):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    n = nx * ny * nz
+    rp = np.zeros(n + 1, np.uint32)
+    col = np.zeros(7 * n, np.uint32)
+    val = np.zeros(7 * n, np.float64)
+    nnz = C.c_uint32()
+    call("vclhip_gen_laplace_3d", nx, ny, nz, _ptr(rp), _ptr(col), _ptr(val), C.byref(nnz))
+    return rp, col[: nnz.value].copy(), val[: nnz.value].copy()
+
+
+def diffusion(nx, ny, nz, k):
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    n = nx * ny * nz
+    assert k.size == n
+    rp = np.zeros(n + 1, np.uint32)
+    col = np.zeros(7 * n, np.uint32)
+    val = np.zeros(7 * n, np.float64)
+    nnz = C.c_uint32()
+    call("vclhip_gen_diffusion", nx, ny, nz, _ptr(k), _ptr(rp), _ptr(col), _ptr(val), C.byref(nnz))
+    return rp, col[: nnz.value].copy(), val[: nnz.value].copy()
+
+
+def powerlaw(n, max_len=512, alpha=2.0, diag_factor=1.1, seed=1):
+    nnz = C.c_uint32()
+    call("vclhip_gen_powerlaw", n, max_len, alpha, diag_factor, C.c_uint64(seed), None, None, None, C.byref(nnz))
+    rp = np.zeros(n + 1, np.uint32)
+    col = np.zeros(nnz.value, np.uint32)
+    val = np.zeros(nnz.value, np.float64)
+    call("vclhip_gen_powerlaw", n, max_len, alpha, diag_factor, C.c_uint64(seed), _ptr(rp), _ptr(col), _ptr(val),
+         C.byref(nnz))
+    return rp, col, val
+
+
+def graph_laplacian_batch(num_systems, n_min=64, n_max=4096, avg_degree=8.0, w_lo=0.1, w_hi=1.0, shift=0.01, seed=1):
+    rows = C.c_uint32()
+    nnz = C.c_uint32()
+    args = (num_systems, n_min, n_max, avg_degree, w_lo, w_hi, shift, C.c_uint64(seed))
+    call("vclhip_gen_graph_laplacian_batch", *args, None, None, None, None, C.byref(rows), C.byref(nnz))
+    sys_ptr = np.zeros(num_systems + 1, np.uint32)
+    rp = np.zeros(rows.value + 1, np.uint32)
+    col = np.zeros(nnz.value, np.uint32)
+    val = np.zeros(nnz.value, np.float64)
+    call("vclhip_gen_graph_laplacian_batch", *args, _ptr(sys_ptr), _ptr(rp), _ptr(col), _ptr(val), C.byref(rows),
+         C.byref(nnz))
+    return sys_ptr, rp, col, val
