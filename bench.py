@@ -2,13 +2,15 @@
 """Benchmark of the sparse Krylov hot path on one MI355X.
 
 Step = one FP64 CSR SpMV y = A x on the 3-D 7-point Laplacian 128^3 (n = 2,097,152,
-nnz = 14,581,760), inputs resident in HBM.  Each timed SpMV is preceded by a 512 MB
-device write that evicts the 256 MB Infinity Cache, so `value` is an HBM figure; the
-back-to-back (cache-assisted) rate is reported as `warm_gbs`.  The JSON line also carries
-the roofline of the SpMV kernel, the CPU baseline (the reference's own OpenMP host backend,
-compiled from its sources into oracle/_ref), and time-to-solution of the FP64 solvers.
+nnz = 14,581,760) with all inputs resident in HBM.  Before each timed SpMV a 1 GiB read
+evicts the 256 MB Infinity Cache (leaving only clean, unrelated lines), so `value` is an HBM
+figure; the back-to-back (cache-assisted) rate is reported as `warm_gbs`.  The JSON line also
+carries the roofline of the SpMV kernel, a streaming reference (z = a x + b y over 1.5 GB), the
+CPU baseline (the reference's own OpenMP host backend, compiled from its sources into
+oracle/_ref), and time-to-solution of the FP64 Krylov solvers.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --kernels-only     # only the SpMV and streaming kernels (for rocprofv3 --pmc)
 """
 import argparse
 import ctypes as C
@@ -25,7 +27,8 @@ sys.path.insert(0, str(ROOT / "viennacl-dev_amd"))
 
 METRIC = "CG time-to-1e-10 rel. residual (FP64 vs mixed, 1 GPU); SpMV GB/s vs HBM peak"
 HBM_PEAK_GBS = 8000.0
-FLUSH_BYTES = 512 << 20
+EVICT_DOUBLES = 128 << 20   # 1 GiB
+STREAM_N = 64 << 20         # 3 x 512 MB for the streaming reference
 
 
 def log(*a):
@@ -38,6 +41,7 @@ def csr_bytes(n, nnz, val_bytes=8, vec_bytes=8):
 
 
 def sell_bytes(stored, slices, n):
+    """Stored values + column indices + per-slice (columns, start) + x once + y written."""
     return 12 * stored + 8 * slices + 16 * n
 
 
@@ -63,24 +67,32 @@ class Events:
             self.vh.call("vclhip_event_destroy", h)
 
 
-def time_spmv(vh, M, x, y, stream, steps, warmup, flush):
-    """Cold (flushed) per-launch times and warm back-to-back time of M.spmv, via HIP events on `stream`."""
+def evict_caches(vh, evict, stream):
+    """Reads 1 GiB (4x the Infinity Cache) so that the caches hold only clean, unrelated lines."""
+    r = C.c_double()
+    vh.call("vclhip_norm2_f64", evict.size, evict.ptr, 0, 1, C.byref(r), stream.handle)
+
+
+def time_launches(vh, launch, stream, steps, warmup, evict=None):
+    """Per-launch HIP-event times (cold when `evict` is given) and the warm back-to-back mean."""
     for _ in range(warmup):
-        M.spmv(x, y, stream=stream.handle)
+        launch()
     stream.synchronize()
     ev = Events(vh, 2 * steps + 2)
     t0 = time.perf_counter()
-    for k in range(steps):
-        vh.call("vclhip_fill_zero", flush.ptr, C.c_size_t(flush.nbytes), stream.handle)
-        ev.record(2 * k, stream.handle)
-        M.spmv(x, y, stream=stream.handle)
-        ev.record(2 * k + 1, stream.handle)
-    stream.synchronize()
+    cold = None
+    if evict is not None:
+        for k in range(steps):
+            evict_caches(vh, evict, stream)
+            ev.record(2 * k, stream.handle)
+            launch()
+            ev.record(2 * k + 1, stream.handle)
+        stream.synchronize()
+        cold = np.array([ev.ms(2 * k, 2 * k + 1) for k in range(steps)])
     wall = time.perf_counter() - t0
-    cold = np.array([ev.ms(2 * k, 2 * k + 1) for k in range(steps)])
     ev.record(2 * steps, stream.handle)
     for _ in range(steps):
-        M.spmv(x, y, stream=stream.handle)
+        launch()
     ev.record(2 * steps + 1, stream.handle)
     stream.synchronize()
     warm = ev.ms(2 * steps, 2 * steps + 1) / steps
@@ -88,9 +100,26 @@ def time_spmv(vh, M, x, y, stream, steps, warmup, flush):
     return cold, warm, wall
 
 
+def stream_reference(vh, stream, steps):
+    """z = a x + b y over 64M doubles (1.5 GB moved per launch): the streaming rate of this box."""
+    x = vh.DeviceArray.zeros(STREAM_N, np.float64)
+    y = vh.DeviceArray.zeros(STREAM_N, np.float64)
+    z = vh.DeviceArray.zeros(STREAM_N, np.float64)
+
+    def launch():
+        vh.call("vclhip_axpby_f64", STREAM_N, 1.5, x.ptr, -0.5, y.ptr, z.ptr, stream.handle)
+
+    _, warm, _ = time_launches(vh, launch, stream, steps, 2)
+    gbs = 24 * STREAM_N / (warm * 1e-3) / 1e9
+    for a in (x, y, z):
+        a.free()
+    return {"kernel": "axpby_kernel<double>", "bytes_per_launch": 24 * STREAM_N, "ms": round(warm, 6),
+            "gbs": round(gbs, 2)}
+
+
 def true_relres(vh, A, b, x, n):
     """||b - A x|| / ||b|| in FP64 on the device."""
-    r = vh.DeviceArray.from_numpy(np.zeros(n))
+    r = vh.DeviceArray.zeros(n, np.float64)
     vh.call("vclhip_memcpy_d2d", r.ptr, b.ptr, C.c_size_t(8 * n), None)
     A.spmv(x, r, -1.0, 1.0)
     nr, nb = C.c_double(), C.c_double()
@@ -133,6 +162,13 @@ def cpu_baseline(rp, col, val, x, n, nnz, seconds):
                       f"host_based OpenMP backend), {t:.1f} s"}, y
 
 
+def device_banner(vh):
+    name = C.create_string_buffer(64)
+    mem, cu, l2, llc = C.c_size_t(), C.c_int(), C.c_int(), C.c_int()
+    vh.call("vclhip_device_info", 0, name, 64, C.byref(mem), C.byref(cu), C.byref(l2), C.byref(llc))
+    log(f"device {name.value.decode()} CUs={cu.value} HBM={mem.value / 2**30:.0f} GiB L2={l2.value >> 20} MiB")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,19 +178,18 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solvers", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only warm CSR/SELL SpMV and the streaming kernel (for counter collection)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding).",
+        print("bench.py measures one GPU only: the sharded (row-partitioned) path is not built; run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
     import vclhip as vh
     assert vh.device_count() > 0, "no HIP device"
     vh.call("vclhip_set_device", 0)
-    name = C.create_string_buffer(64)
-    mem, cu, l2, llc = C.c_size_t(), C.c_int(), C.c_int(), C.c_int()
-    vh.call("vclhip_device_info", 0, name, 64, C.byref(mem), C.byref(cu), C.byref(l2), C.byref(llc))
-    log(f"device {name.value.decode()} CUs={cu.value} HBM={mem.value / 2**30:.0f} GiB L2={l2.value >> 20} MiB")
+    device_banner(vh)
 
     N = args.grid
     t = time.perf_counter()
@@ -167,19 +202,33 @@ def main():
     xh = vh.random_uniform(1, n)
     x = vh.DeviceArray.from_numpy(xh)
     y = vh.DeviceArray.zeros(n, np.float64)
-    flush = vh.DeviceArray(FLUSH_BYTES, np.uint8)
-
-    cold, warm, wall = time_spmv(vh, A, x, y, stream, args.steps, args.warmup, flush)
     bytes_csr = csr_bytes(n, nnz)
+    bytes_sell = sell_bytes(S.stored, (n + 63) // 64, n)
+
+    if args.kernels_only:
+        _, warm, _ = time_launches(vh, lambda: A.spmv(x, y, stream=stream.handle), stream, args.steps, args.warmup)
+        _, swarm, _ = time_launches(vh, lambda: S.spmv(x, y, stream=stream.handle), stream, args.steps, args.warmup)
+        ref = stream_reference(vh, stream, args.steps)
+        print(json.dumps({"csr_spmv_ms": warm, "csr_bytes": bytes_csr, "sell_spmv_ms": swarm,
+                          "sell_bytes": bytes_sell, "stream": ref}), flush=True)
+        return
+
+    evict = vh.DeviceArray(EVICT_DOUBLES, np.float64)
+    vh.call("vclhip_fill_zero", evict.ptr, C.c_size_t(evict.nbytes), None)
+    cold, warm, wall = time_launches(vh, lambda: A.spmv(x, y, stream=stream.handle), stream, args.steps,
+                                     args.warmup, evict)
     ms = float(cold.mean())
     gbs = bytes_csr / (ms * 1e-3) / 1e9
     warm_gbs = bytes_csr / (warm * 1e-3) / 1e9
     log(f"CSR SpMV cold {ms * 1e3:.1f} us ({gbs:.0f} GB/s), warm {warm * 1e3:.1f} us ({warm_gbs:.0f} GB/s)")
-    s_cold, s_warm, _ = time_spmv(vh, S, x, y, stream, args.steps, args.warmup, flush)
-    bytes_sell = sell_bytes(S.stored, (n + 63) // 64, n)
+    s_cold, s_warm, _ = time_launches(vh, lambda: S.spmv(x, y, stream=stream.handle), stream, args.steps,
+                                      args.warmup, evict)
     sell_gbs = bytes_sell / (s_cold.mean() * 1e-3) / 1e9
     log(f"SELL-64 SpMV cold {s_cold.mean() * 1e3:.1f} us ({sell_gbs:.0f} GB/s), warm {s_warm * 1e3:.1f} us")
     y_gpu = y.to_numpy()
+    evict.free()
+    ref = stream_reference(vh, stream, 10)
+    log("streaming reference", ref)
 
     traffic = None
     tfile = ROOT / "profiles" / "csr_spmv_lap3d_128_traffic.json"
@@ -191,27 +240,29 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms, 6), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"FP64 CSR SpMV y=Ax, 3-D 7-point Laplacian {N}^3 (n={n}, nnz={nnz}), "
-                               f"512 MB cache flush before each timed SpMV", "n": n, "nnz": nnz,
-                   "format": "CSR (uint32 indices)", "algorithmic_bytes_per_spmv": bytes_csr},
+                               f"Infinity Cache evicted by a 1 GiB read before each timed SpMV",
+                   "n": n, "nnz": nnz, "format": "CSR (uint32 indices)", "algorithmic_bytes_per_spmv": bytes_csr},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "warm_gbs": round(warm_gbs, 2),
         "timed_region_wall_s": round(wall, 4),
         "sell64": {"cold_gbs": round(sell_gbs, 2), "cold_ms": round(float(s_cold.mean()), 6),
                    "warm_ms": round(s_warm, 6), "algorithmic_bytes": bytes_sell},
+        "stream_reference": ref,
     }
 
     if not args.no_solvers:
         b = vh.DeviceArray.from_numpy(vh.random_uniform(0, n))
         sol = {}
         sol["cg_fp64_csr"] = timed_solve(vh, "cg", A, b, n, tol=1e-10, max_iters=5000)
-        log("CG FP64 CSR", sol["cg_fp64_csr"])
+        sol["cg_fp64_csr_graph"] = timed_solve(vh, "cg", A, b, n, tol=1e-10, max_iters=5000, use_graph=True)
         sol["cg_fp64_sell64"] = timed_solve(vh, "cg", S, b, n, tol=1e-10, max_iters=5000)
         sol["jacobi_cg_fp64_csr"] = timed_solve(vh, "cg", A, b, n, precond=1, tol=1e-10, max_iters=5000)
         sol["bicgstab_fp64_csr"] = timed_solve(vh, "bicgstab", A, b, n, tol=1e-10, max_iters=5000)
         sol["jacobi_bicgstab_fp64_csr"] = timed_solve(vh, "bicgstab", A, b, n, precond=1, tol=1e-10,
                                                       max_iters=5000)
-        log("solvers 128^3", sol)
+        for k, v in sol.items():
+            log(k, v)
         out["solvers_lap3d_128"] = sol
 
     if not args.no_cpu:

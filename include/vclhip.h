@@ -225,12 +225,15 @@ typedef struct vclhip_solver_params {
   uint32_t chunk;                    /* iterations enqueued between convergence polls (0 = default 16) */
   int use_graph;                     /* capture one chunk as a hipGraph and replay it */
   int mixed_mode;                    /* mixed CG: 0 = restart as the reference, 1 = reliable update (keeps p) */
+  int variant;                       /* BiCGStab with Jacobi: 0 = the reference's recurrences (bicgstab.hpp:447-480),
+                                        1 = fully pipelined (fewer passes; rounding differs) */
   double* history_host;              /* optional: per-iteration convergence quantity (may be NULL) */
   uint32_t history_len;
   /* optional monitor, called once per iteration with the device solution and the relative residual
    * estimate; returning nonzero stops the solve (cg.hpp:174-175, cg_solver::set_monitor :420-424).
-   * A monitor forces one host synchronization per iteration. */
-  int (*monitor)(const double* x_dev, double rel_residual, void* user);
+   * A monitor forces one host synchronization per iteration.  x_dev points to the solution vector's
+   * element type (double or float). */
+  int (*monitor)(const void* x_dev, double rel_residual, void* user);
   void* monitor_user;
 } vclhip_solver_params;
 
@@ -256,13 +259,25 @@ int vclhip_cg_solve_csr_f64(const vclhip_csr* A, const double* b_dev, double* x_
 int vclhip_cg_solve_sell_f64(const vclhip_sell* A, const double* b_dev, double* x_dev, int precond,
                              const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
 /* solve(A, b, bicgstab_tag [, jacobi_precond]) (bicgstab.hpp:495-533).  Pipelined BiCGStab of
- * bicgstab.hpp:97-215; with Jacobi the left-preconditioned iteration of bicgstab.hpp:398-489
- * (criterion ||M^-1 r|| / ||b||, restarts every max_iters_before_restart iterations or on
- * breakdown).  Unlike the reference, the last x update is applied before returning, so x
- * matches the residual estimate that stopped the iteration. */
+ * bicgstab.hpp:97-215; unlike the reference, the last x update is applied before returning, so x
+ * matches the residual estimate that stopped the iteration.  With Jacobi: the left-preconditioned
+ * iteration of bicgstab.hpp:398-489 (criterion ||M^-1 r|| / ||b||, restarts every
+ * max_iters_before_restart iterations or on breakdown), with the reference's recurrences unless
+ * params->variant == 1. */
 int vclhip_bicgstab_solve_csr_f64(const vclhip_csr* A, const double* b_dev, double* x_dev, int precond,
                                   const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
 int vclhip_bicgstab_solve_sell_f64(const vclhip_sell* A, const double* b_dev, double* x_dev, int precond,
+                                   const vclhip_solver_params* params, vclhip_solver_result* result,
+                                   void* stream);
+/* FP32 solves: FP32 matrices and vectors, inner products accumulated in FP64, scalars in FP64
+ * (solve(A, b, tag) with compressed_matrix<float> / sliced_ell_matrix<float>). */
+int vclhip_cg_solve_csr_f32(const vclhip_csr* A, const float* b_dev, float* x_dev, int precond,
+                            const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+int vclhip_cg_solve_sell_f32(const vclhip_sell* A, const float* b_dev, float* x_dev, int precond,
+                             const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+int vclhip_bicgstab_solve_csr_f32(const vclhip_csr* A, const float* b_dev, float* x_dev, int precond,
+                                  const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+int vclhip_bicgstab_solve_sell_f32(const vclhip_sell* A, const float* b_dev, float* x_dev, int precond,
                                    const vclhip_solver_params* params, vclhip_solver_result* result,
                                    void* stream);
 

@@ -31,6 +31,7 @@ class _Res(C.Structure):
 _ORC_SIGS = {
     "orc_set_num_threads": (None, [i32]),
     "orc_get_num_threads": (i32, []),
+    "orc_set_reduction_blocks": (None, [i32]),
     "orc_csr_spmv_f64": (None, [u32, vp, vp, vp, vp, sz, sz, f64, vp, sz, sz, f64]),
     "orc_csr_spmv_f32": (None, [u32, vp, vp, vp, vp, sz, sz, f32, vp, sz, sz, f32]),
     "orc_sell_spmv_f64": (None, [u32, u32, vp, vp, vp, vp, vp, sz, sz, f64, vp, sz, sz, f64]),
@@ -162,6 +163,33 @@ def set_threads(n):
     orc().orc_set_num_threads(n)
     if ref_available():
         ref().ref_set_num_threads(n)
+
+
+class reduction_blocks:
+    """Context manager: run the oracle's inner products as k blocks (k = 0: reference order)."""
+
+    def __init__(self, k):
+        self.k = k
+
+    def __enter__(self):
+        orc().orc_set_reduction_blocks(self.k)
+        return self
+
+    def __exit__(self, *exc):
+        orc().orc_set_reduction_blocks(0)
+
+
+ENVELOPE_BLOCKS = (0, 2, 3, 4, 6, 8, 16, 64, 256)
+
+
+def iteration_envelope(run, blocks=ENVELOPE_BLOCKS):
+    """[min, max] of the iteration count returned by run() (a tuple whose second entry is the count)
+    over the reference order and the k-block reduction orders."""
+    its = []
+    for k in blocks:
+        with reduction_blocks(k):
+            its.append(int(run()[1]))
+    return min(its), max(its)
 
 
 def csr_spmv(rp, col, val, x, alpha=1.0, y=None, beta=0.0, xs=0, xi=1, ys=0, yi=1, dtype=np.float64):

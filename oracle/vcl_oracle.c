@@ -203,9 +203,29 @@ void orc_csr_row_info_f64(uint32_t n_rows, const uint32_t* row_ptr, const uint32
 
 /* ------------------------------------------------------------ vector ops */
 
+/* Reduction order.  0 (default): the reference's own loops (sequential with one OpenMP thread).
+ * k > 0: every inner product is summed as k contiguous blocks, each sequential, the block sums added
+ * in order -- the order of the reference's `omp parallel for reduction(+:)` with k threads and a
+ * static schedule, applied regardless of the reference's 5000-element OpenMP threshold.  Used to
+ * measure how far a solver's iteration count moves when only the summation order changes. */
+static int g_red_blocks = 0;
+
+void orc_set_reduction_blocks(int k) { g_red_blocks = k > 0 ? k : 0; }
+
+static size_t blk_lo(size_t n, int b) { return (size_t)(((unsigned long long)n * (unsigned)b) / (unsigned)g_red_blocks); }
+
 /* host_based/vector_operations.hpp:465-544 */
 double orc_dot_f64(size_t n, const double* x, const double* y)
 {
+  if (g_red_blocks) {
+    double tot = 0;
+    for (int b = 0; b < g_red_blocks; ++b) {
+      double t = 0;
+      for (size_t i = blk_lo(n, b); i < blk_lo(n, b + 1); ++i) t += x[i] * y[i];
+      tot += t;
+    }
+    return tot;
+  }
   double temp = 0;
   long i;
   #pragma omp parallel for reduction(+: temp) if (n > 5000)
@@ -216,6 +236,15 @@ double orc_dot_f64(size_t n, const double* x, const double* y)
 
 float orc_dot_f32(size_t n, const float* x, const float* y)
 {
+  if (g_red_blocks) {
+    float tot = 0;
+    for (int b = 0; b < g_red_blocks; ++b) {
+      float t = 0;
+      for (size_t i = blk_lo(n, b); i < blk_lo(n, b + 1); ++i) t += x[i] * y[i];
+      tot += t;
+    }
+    return tot;
+  }
   float temp = 0;
   long i;
   #pragma omp parallel for reduction(+: temp) if (n > 5000)
@@ -227,6 +256,7 @@ float orc_dot_f32(size_t n, const float* x, const float* y)
 /* host_based/vector_operations.hpp:727-833 */
 double orc_norm2_f64(size_t n, const double* x)
 {
+  if (g_red_blocks) return sqrt(orc_dot_f64(n, x, x));
   double temp = 0;
   long i;
   #pragma omp parallel for reduction(+: temp) if (n > 5000)
@@ -239,6 +269,7 @@ double orc_norm2_f64(size_t n, const double* x)
 
 float orc_norm2_f32(size_t n, const float* x)
 {
+  if (g_red_blocks) return sqrtf(orc_dot_f32(n, x, x));
   float temp = 0;
   long i;
   #pragma omp parallel for reduction(+: temp) if (n > 5000)
@@ -256,6 +287,28 @@ void orc_pipelined_prod_csr_f64(uint32_t n_rows, const uint32_t* row_ptr, const 
                                 const double* p, double* Ap, const double* r0star,
                                 double* buf, size_t chunk, size_t offset)
 {
+  if (g_red_blocks) {
+    double tA = 0, tP = 0, tR = 0;
+    for (int b = 0; b < g_red_blocks; ++b) {
+      double a = 0, q = 0, c = 0;
+      for (size_t row = blk_lo(n_rows, b); row < blk_lo(n_rows, b + 1); ++row) {
+        double dot_prod = 0;
+        double vpd = p[row];
+        for (size_t i = row_ptr[row]; i < row_ptr[row + 1]; ++i) dot_prod += val[i] * p[col[i]];
+        Ap[row] = dot_prod;
+        a += dot_prod * dot_prod;
+        q += vpd * dot_prod;
+        c += r0star ? dot_prod * r0star[row] : 0.0;
+      }
+      tA += a;
+      tP += q;
+      tR += c;
+    }
+    buf[chunk] = tA;
+    buf[2 * chunk] = tP;
+    if (r0star) buf[offset] = tR;
+    return;
+  }
   double ApAp = 0, pAp = 0, Apr0 = 0;
   long row;
   #pragma omp parallel for reduction(+: ApAp, pAp, Apr0)
@@ -282,6 +335,13 @@ void orc_pipelined_prod_sell_f64(uint32_t n_rows, uint32_t C, const uint32_t* co
                                  const double* p, double* Ap, const double* r0star,
                                  double* buf, size_t chunk, size_t offset)
 {
+  if (g_red_blocks) {
+    orc_sell_spmv_f64(n_rows, C, columns_per_block, column_indices, block_start, elements, p, 0, 1, 1.0, Ap, 0, 1, 0.0);
+    buf[chunk] = orc_dot_f64(n_rows, Ap, Ap);
+    buf[2 * chunk] = orc_dot_f64(n_rows, p, Ap);
+    if (r0star) buf[offset] = orc_dot_f64(n_rows, Ap, r0star);
+    return;
+  }
   long num_blocks = (long)((n_rows + C - 1) / C);
   double ApAp = 0, pAp = 0, Apr0 = 0;
   long b;
@@ -319,6 +379,18 @@ void orc_pipelined_prod_sell_f64(uint32_t n_rows, uint32_t C, const uint32_t* co
 void orc_pipelined_cg_vector_update_f64(size_t n, double* x, double alpha, double* p, double* r,
                                         const double* Ap, double beta, double* buf)
 {
+  if (g_red_blocks) {
+    for (size_t k = 0; k < n; ++k) {
+      double value_p = p[k];
+      double value_r = r[k];
+      x[k] += alpha * value_p;
+      value_r -= alpha * Ap[k];
+      p[k] = value_r + beta * value_p;
+      r[k] = value_r;
+    }
+    buf[0] = orc_dot_f64(n, r, r);
+    return;
+  }
   double inner_prod_r = 0;
   long i;
   #pragma omp parallel for reduction(+: inner_prod_r)
@@ -345,6 +417,11 @@ void orc_pipelined_bicgstab_update_s_f64(size_t n, double* s, const double* r, c
     Ap_in_r0 += buf[i + 3 * chunk];
   }
   double alpha = r_in_r0 / Ap_in_r0;
+  if (g_red_blocks) {
+    for (size_t k = 0; k < n; ++k) s[k] = r[k] - alpha * Ap[k];
+    buf[offset] = orc_dot_f64(n, s, s);
+    return;
+  }
   double inner_prod_s = 0;
   long i;
   #pragma omp parallel for reduction(+: inner_prod_s)
@@ -361,6 +438,18 @@ void orc_pipelined_bicgstab_vector_update_f64(size_t n, double* x, double alpha,
                                               const double* s, double* r, const double* As, double beta,
                                               const double* Ap, const double* r0star, double* buf)
 {
+  if (g_red_blocks) {
+    for (size_t k = 0; k < n; ++k) {
+      double value_p = p[k];
+      double value_s = s[k];
+      x[k] += alpha * value_p + omega * value_s;
+      double value_residual = value_s - omega * As[k];
+      p[k] = value_residual + beta * (value_p - omega * Ap[k]);
+      r[k] = value_residual;
+    }
+    buf[0] = orc_dot_f64(n, r, r0star);
+    return;
+  }
   double inner_prod_r_r0star = 0;
   long i;
   #pragma omp parallel for reduction(+: inner_prod_r_r0star)

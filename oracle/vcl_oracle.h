@@ -10,9 +10,9 @@
  * reference root).  Floating-point expressions keep the reference's evaluation
  * order and the library is built with -ffp-contract=off, so single-threaded
  * results are bit-identical to the reference built the same way
- * (checked against oracle/_ref by tests/test_oracle_vs_ref.py).
+ * (checked against oracle/_ref by tests/test_oracle_golden.py).
  *
- * Pinning (see DESIGN.md "Oracle pinning"):
+ * Pinning:
  *   - SpMV (CSR/SELL), row_info, dots, norms, pipelined kernels, all solvers:
  *     pinned bit-for-bit against the reference build (oracle/_ref) and the
  *     golden fixtures under tests/golden/, plus closed forms (Laplacian
@@ -35,6 +35,10 @@ extern "C" {
 /* OpenMP thread count for the reductions (1 = reference single-thread order). */
 void orc_set_num_threads(int n);
 int  orc_get_num_threads(void);
+/* 0 (default): the reference's reduction order.  k > 0: every inner product is summed as k
+ * contiguous blocks (sequential inside, block sums added in order), the order of the reference's
+ * OpenMP reduction with k threads, without its 5000-element threshold. */
+void orc_set_reduction_blocks(int k);
 
 /* ---------------------------------------------------------------- SpMV --- */
 /* host_based/sparse_matrix_operations.hpp:110-186 */

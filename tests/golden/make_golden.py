@@ -126,9 +126,19 @@ def main():
          cgvu_x=x, cgvu_p=pv, cgvu_r=r, cgvu_buf=buf, cgprod_Ap=Ap2, cgprod_buf=buf2,
          bprod_Ap=Ap3, bprod_buf=buf3, upds_s=s, upds_buf=buf4, bvu_x=x5, bvu_p=p5, bvu_r=r5, bvu_buf=buf5)
 
-    # solvers through linalg::solve
+    # solvers through linalg::solve.  Besides the single-threaded run, each case records the
+    # envelope [min, max] of the reference's iteration count over OpenMP thread counts 1..8: the
+    # reference's own spread when only the summation order of its inner products changes.
     def solver_case(name, rp, col, val, b, **meta):
         save(name, row_ptr=rp, col=col, val=val, b=b, **meta)
+
+    def envelope(fn):
+        its = []
+        for t in range(1, 9):
+            po.set_threads(t)
+            its.append(fn()[1])
+        po.set_threads(1)
+        return np.array([min(its), max(its)], np.int64)
 
     rp, col, val = vh.fdm_laplace_2d(40, 40)
     b = vh.random_uniform(0, rp.size - 1)
@@ -137,8 +147,11 @@ def main():
     xm, itm, errm = po.ref_mixed_cg(rp, col, val, b, 1e-8, 2000)
     x2, it2, err2 = po.ref_cg(rp, col, val, b, 1e-8, 40)  # stops at max_iters
     solver_case("solve_lap2d_40x40", rp, col, val, b, cg_x=x, cg_iters=it, cg_error=err,
+                cg_env=envelope(lambda: po.ref_cg(rp, col, val, b, 1e-8, 2000)),
                 bicgstab_x=xb, bicgstab_iters=itb, bicgstab_error=errb,
+                bicgstab_env=envelope(lambda: po.ref_bicgstab(rp, col, val, b, 1e-8, 2000)),
                 mixed_x=xm, mixed_iters=itm, mixed_error=errm,
+                mixed_env=envelope(lambda: po.ref_mixed_cg(rp, col, val, b, 1e-8, 2000)),
                 cg40_x=x2, cg40_iters=it2, cg40_error=err2)
 
     k = np.exp(vh.random_normal(1, 30 * 30, 0.0, 1.0))
@@ -147,14 +160,19 @@ def main():
     x, it, err = po.ref_cg(rp, col, val, b, 1e-10, 3000, jacobi=True)
     xb, itb, errb = po.ref_bicgstab(rp, col, val, b, 1e-10, 3000, jacobi=True, max_before_restart=20)
     solver_case("solve_diffusion2d_30x30", rp, col, val, b, jcg_x=x, jcg_iters=it, jcg_error=err,
-                jbicgstab_x=xb, jbicgstab_iters=itb, jbicgstab_error=errb, restart_every=20)
+                jcg_env=envelope(lambda: po.ref_cg(rp, col, val, b, 1e-10, 3000, jacobi=True)),
+                jbicgstab_x=xb, jbicgstab_iters=itb, jbicgstab_error=errb, restart_every=20,
+                jbicgstab_env=envelope(lambda: po.ref_bicgstab(rp, col, val, b, 1e-10, 3000, jacobi=True,
+                                                               max_before_restart=20)))
 
     rp, col, val = vh.fdm_laplace_2d(37, 29)
     b = vh.random_uniform(0, rp.size - 1)
     x, it, err = po.ref_cg_sell(rp, col, val, 32, b, 1e-8, 2000)
     xb, itb, errb = po.ref_bicgstab_sell(rp, col, val, 32, b, 1e-8, 2000)
     solver_case("solve_sell_lap2d_37x29", rp, col, val, b, C=32, cg_x=x, cg_iters=it, cg_error=err,
-                bicgstab_x=xb, bicgstab_iters=itb, bicgstab_error=errb)
+                cg_env=envelope(lambda: po.ref_cg_sell(rp, col, val, 32, b, 1e-8, 2000)),
+                bicgstab_x=xb, bicgstab_iters=itb, bicgstab_error=errb,
+                bicgstab_env=envelope(lambda: po.ref_bicgstab_sell(rp, col, val, 32, b, 1e-8, 2000)))
 
     rp, col, val = vh.powerlaw(2000, max_len=512, seed=1)
     b = vh.random_uniform(0, rp.size - 1)

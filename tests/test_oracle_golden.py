@@ -153,3 +153,24 @@ def test_oracle_equals_reference_build_on_fresh_inputs(oracle, vh, seed):
     x, it, err, _ = oracle.bicgstab_classic(A, b, 1e-10, 3000, diag=d, max_before_restart=15)
     xr, itr, errr = oracle.ref_bicgstab(rp, col, val, b, 1e-10, 3000, jacobi=True, max_before_restart=15)
     assert (it, err) == (itr, errr) and np.array_equal(x, xr)
+
+
+def test_blocked_reduction_orders(oracle, vh):
+    """k-block reduction orders: k = 0 is the reference order (bit-exact with the fixture); other k
+    change only rounding, so CG moves by at most a couple of iterations while the restarted
+    BiCGStab of this fixture spreads much wider (which is why its GPU bar is an envelope)."""
+    g = load("solve_lap2d_40x40")
+    A = oracle.CsrHost(g["row_ptr"], g["col"], g["val"])
+    lo, hi = oracle.iteration_envelope(lambda: oracle.cg_pipelined(A, g["b"], 1e-8, 2000))
+    assert lo <= int(g["cg_iters"]) <= hi and hi - lo <= 2
+    with oracle.reduction_blocks(0):
+        assert oracle.cg_pipelined(A, g["b"], 1e-8, 2000)[1] == g["cg_iters"]
+    with oracle.reduction_blocks(1):  # one block == sequential order
+        x, it, err, _ = oracle.cg_pipelined(A, g["b"], 1e-8, 2000)
+        assert it == g["cg_iters"] and err == g["cg_error"] and np.array_equal(x, g["cg_x"])
+    g = load("solve_diffusion2d_30x30")
+    A = oracle.CsrHost(g["row_ptr"], g["col"], g["val"])
+    d = oracle.row_info(g["row_ptr"], g["col"], g["val"], 3)
+    lo, hi = oracle.iteration_envelope(lambda: oracle.bicgstab_classic(A, g["b"], 1e-10, 3000, diag=d,
+                                                                        max_before_restart=20))
+    assert lo <= int(g["jbicgstab_iters"]) <= hi

@@ -44,15 +44,40 @@ __global__ __launch_bounds__(kBlock) void sum_partials_kernel(const A* __restric
   if (threadIdx.x == 0) *out = s;
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void axpby_kernel(uint32_t n, T a, const T* __restrict__ x, T b,
-                                                       const T* __restrict__ y, T* __restrict__ z)
+// Contiguous sum of x*y with W-wide loads (x, y aligned for the packs).
+template <typename T, typename A, int W>
+__global__ __launch_bounds__(kBlock) void dot_partial_vec_kernel(uint32_t n, const T* __restrict__ x,
+                                                                 const T* __restrict__ y, A* __restrict__ part)
 {
   const uint32_t lw = xcd_logical_block();
   uint32_t c0, c1;
-  chunk_range(n, lw, gridDim.x, c0, c1);
-#pragma unroll 4
-  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) z[i] = x[i] * a + y[i] * b;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  A acc[1] = {0};
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    const Pack<V, T> vx = ldp<V>(x, i);
+    const Pack<V, T> vy = ldp<V>(y, i);
+#pragma unroll
+    for (int k = 0; k < V; ++k) acc[0] += static_cast<A>(vx.v[k]) * static_cast<A>(vy.v[k]);
+  });
+  block_store_partials<1, A>(acc, part, 0, lw);
+}
+
+template <typename T, int W>
+__global__ __launch_bounds__(kBlock) void axpby_kernel(uint32_t n, T a, const T* x, T b, const T* y, T* z)
+{
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    const Pack<V, T> vx = ldp<V>(x, i);
+    const Pack<V, T> vy = ldp<V>(y, i);
+    Pack<V, T> vz;
+#pragma unroll
+    for (int k = 0; k < V; ++k) vz.v[k] = vx.v[k] * a + vy.v[k] * b;
+    stp<V>(z, i, vz);
+  });
 }
 
 template <typename T>
@@ -76,25 +101,38 @@ __global__ __launch_bounds__(kBlock) void fill_kernel(uint32_t n, T* __restrict_
   for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) v[static_cast<size_t>(i) * inc + s] = a;
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void ediv_kernel(uint32_t n, const T* __restrict__ x, const T* __restrict__ y,
-                                                      T* __restrict__ z)
+template <typename T, int W>
+__global__ __launch_bounds__(kBlock) void ediv_kernel(uint32_t n, const T* x, const T* y, T* z)
 {
   const uint32_t lw = xcd_logical_block();
   uint32_t c0, c1;
-  chunk_range(n, lw, gridDim.x, c0, c1);
-#pragma unroll 4
-  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) z[i] = x[i] / y[i];
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    const Pack<V, T> vx = ldp<V>(x, i);
+    const Pack<V, T> vy = ldp<V>(y, i);
+    Pack<V, T> vz;
+#pragma unroll
+    for (int k = 0; k < V; ++k) vz.v[k] = vx.v[k] / vy.v[k];
+    stp<V>(z, i, vz);
+  });
 }
 
-template <typename S, typename D>
+// W source elements per step; S and D widths follow (double <-> float: 2 per 16-byte double pack).
+template <typename S, typename D, int W>
 __global__ __launch_bounds__(kBlock) void convert_kernel(uint32_t n, const S* __restrict__ src, D* __restrict__ dst)
 {
   const uint32_t lw = xcd_logical_block();
   uint32_t c0, c1;
-  chunk_range(n, lw, gridDim.x, c0, c1);
-#pragma unroll 4
-  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) dst[i] = static_cast<D>(src[i]);
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    const Pack<V, S> vs = ldp<V>(src, i);
+    Pack<V, D> vd;
+#pragma unroll
+    for (int k = 0; k < V; ++k) vd.v[k] = static_cast<D>(vs.v[k]);
+    stp<V>(dst, i, vd);
+  });
 }
 
 static uint32_t vec_grid(uint32_t n) { return chunk_grid((n + 4 * kBlock - 1) / (4 * kBlock), vec_gmax()); }
@@ -113,7 +151,11 @@ static int dot_to_host(uint32_t n, const T* x, uint32_t xs, uint32_t xi, const T
   void* ws = nullptr;
   VCLHIP_HIP(hipMallocAsync(&ws, (grid + 1) * sizeof(A), s));
   A* part = static_cast<A*>(ws);
-  hipLaunchKernelGGL((dot_partial_kernel<T, A>), dim3(grid), dim3(kBlock), 0, s, n, x, xs, xi, y, ys, yi, part);
+  if (xi == 1 && yi == 1)
+    VCLHIP_LAUNCH_W(aligned16({x + xs, y + ys}), (dot_partial_vec_kernel<T, A, kVecW<T> >),
+                    (dot_partial_vec_kernel<T, A, 1>), grid, s, n, x + xs, y + ys, part);
+  else
+    hipLaunchKernelGGL((dot_partial_kernel<T, A>), dim3(grid), dim3(kBlock), 0, s, n, x, xs, xi, y, ys, yi, part);
   int st = check_launch("dot_partial_kernel");
   if (st == 0) {
     hipLaunchKernelGGL(sum_partials_kernel<A>, dim3(1), dim3(kBlock), 0, s, part, grid, part + grid);
@@ -166,8 +208,8 @@ int vclhip_axpby_f64(uint32_t n, double a, const double* x, double b, const doub
 {
   if (n == 0) return VCLHIP_SUCCESS;
   if (!x || !y || !z) return invalid("null vector");
-  hipLaunchKernelGGL(axpby_kernel<double>, dim3(vec_grid(n)), dim3(kBlock), 0, static_cast<hipStream_t>(stream), n, a,
-                     x, b, y, z);
+  VCLHIP_LAUNCH_W(aligned16({x, y, z}), (axpby_kernel<double, 2>), (axpby_kernel<double, 1>), vec_grid(n),
+                  static_cast<hipStream_t>(stream), n, a, x, b, y, z);
   return check_launch("axpby_kernel");
 }
 
@@ -175,8 +217,8 @@ int vclhip_axpby_f32(uint32_t n, float a, const float* x, float b, const float* 
 {
   if (n == 0) return VCLHIP_SUCCESS;
   if (!x || !y || !z) return invalid("null vector");
-  hipLaunchKernelGGL(axpby_kernel<float>, dim3(vec_grid(n)), dim3(kBlock), 0, static_cast<hipStream_t>(stream), n, a,
-                     x, b, y, z);
+  VCLHIP_LAUNCH_W(aligned16({x, y, z}), (axpby_kernel<float, 4>), (axpby_kernel<float, 1>), vec_grid(n),
+                  static_cast<hipStream_t>(stream), n, a, x, b, y, z);
   return check_launch("axpby_kernel");
 }
 
@@ -212,8 +254,8 @@ int vclhip_element_div_f64(uint32_t n, const double* x, const double* y, double*
 {
   if (n == 0) return VCLHIP_SUCCESS;
   if (!x || !y || !z) return invalid("null vector");
-  hipLaunchKernelGGL(ediv_kernel<double>, dim3(vec_grid(n)), dim3(kBlock), 0, static_cast<hipStream_t>(stream), n, x,
-                     y, z);
+  VCLHIP_LAUNCH_W(aligned16({x, y, z}), (ediv_kernel<double, 2>), (ediv_kernel<double, 1>), vec_grid(n),
+                  static_cast<hipStream_t>(stream), n, x, y, z);
   return check_launch("ediv_kernel");
 }
 
@@ -221,8 +263,8 @@ int vclhip_element_div_f32(uint32_t n, const float* x, const float* y, float* z,
 {
   if (n == 0) return VCLHIP_SUCCESS;
   if (!x || !y || !z) return invalid("null vector");
-  hipLaunchKernelGGL(ediv_kernel<float>, dim3(vec_grid(n)), dim3(kBlock), 0, static_cast<hipStream_t>(stream), n, x,
-                     y, z);
+  VCLHIP_LAUNCH_W(aligned16({x, y, z}), (ediv_kernel<float, 4>), (ediv_kernel<float, 1>), vec_grid(n),
+                  static_cast<hipStream_t>(stream), n, x, y, z);
   return check_launch("ediv_kernel");
 }
 
@@ -230,8 +272,8 @@ int vclhip_convert_f64_to_f32(uint32_t n, const double* src, float* dst, void* s
 {
   if (n == 0) return VCLHIP_SUCCESS;
   if (!src || !dst) return invalid("null vector");
-  hipLaunchKernelGGL((convert_kernel<double, float>), dim3(vec_grid(n)), dim3(kBlock), 0,
-                     static_cast<hipStream_t>(stream), n, src, dst);
+  VCLHIP_LAUNCH_W(aligned16({src, dst}), (convert_kernel<double, float, 2>), (convert_kernel<double, float, 1>),
+                  vec_grid(n), static_cast<hipStream_t>(stream), n, src, dst);
   return check_launch("convert_kernel");
 }
 
@@ -239,8 +281,8 @@ int vclhip_convert_f32_to_f64(uint32_t n, const float* src, double* dst, void* s
 {
   if (n == 0) return VCLHIP_SUCCESS;
   if (!src || !dst) return invalid("null vector");
-  hipLaunchKernelGGL((convert_kernel<float, double>), dim3(vec_grid(n)), dim3(kBlock), 0,
-                     static_cast<hipStream_t>(stream), n, src, dst);
+  VCLHIP_LAUNCH_W(aligned16({src, dst}), (convert_kernel<float, double, 4>), (convert_kernel<float, double, 1>),
+                  vec_grid(n), static_cast<hipStream_t>(stream), n, src, dst);
   return check_launch("convert_kernel");
 }
 

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <initializer_list>
+#include <type_traits>
 
 namespace vclhip {
 
@@ -69,12 +71,113 @@ __device__ __forceinline__ void block_store_partials(const A (&acc)[K], A* part,
 }
 
 // Sum of part[0..count) by one workgroup in a fixed order; result returned to every lane.
+// Lane t adds part[t], part[t+256], ... in that order; the loads of a batch of eight are issued
+// together so that the latency is paid once per batch rather than once per partial.
 template <typename A>
 __device__ __forceinline__ A block_sum_partials(const A* part, uint32_t count)
 {
+  constexpr uint32_t kBatch = 8;
   A s = 0;
-  for (uint32_t i = threadIdx.x; i < count; i += kBlock) s += part[i];
+  uint32_t i = threadIdx.x;
+  for (; i + (kBatch - 1) * kBlock < count; i += kBatch * kBlock) {
+    A v[kBatch];
+#pragma unroll
+    for (uint32_t k = 0; k < kBatch; ++k) v[k] = part[i + k * kBlock];
+#pragma unroll
+    for (uint32_t k = 0; k < kBatch; ++k) s += v[k];
+  }
+  for (; i < count; i += kBlock) s += part[i];
   return block_sum(s);
+}
+
+// [begin, end) of the lw-th of G chunks of n elements with both ends on multiples of W (the last
+// chunk also takes the tail), so that W-wide vector accesses of a W*sizeof(T)-aligned array stay
+// aligned.
+__device__ __forceinline__ void chunk_range_w(uint32_t n, uint32_t W, uint32_t lw, uint32_t G, uint32_t& begin,
+                                              uint32_t& end)
+{
+  const uint32_t packs = n / W;
+  begin = static_cast<uint32_t>((static_cast<uint64_t>(packs) * lw) / G) * W;
+  end = static_cast<uint32_t>((static_cast<uint64_t>(packs) * (lw + 1)) / G) * W;
+  if (lw == G - 1) end = n;
+}
+
+// Calls f(i, integral_constant<W>) for the W-element packs of [begin, end) (lane-interleaved, so a
+// wavefront touches 64*W consecutive elements per step) and f(i, integral_constant<1>) for the tail.
+template <int W, class F>
+__device__ __forceinline__ void for_packs(uint32_t begin, uint32_t end, F&& f)
+{
+  const uint32_t packed_end = begin + ((end - begin) / W) * W;
+  for (uint32_t i = begin + W * threadIdx.x; i < packed_end; i += W * kBlock) f(i, std::integral_constant<int, W>());
+  if (W > 1)
+    for (uint32_t i = packed_end + threadIdx.x; i < end; i += kBlock) f(i, std::integral_constant<int, 1>());
+}
+
+// W-wide loads and stores (W = 1, 2 for double; 1, 2, 4 for float).
+template <int W, typename T>
+struct Pack {
+  T v[W];
+};
+
+template <int W, typename T>
+__device__ __forceinline__ Pack<W, T> ldp(const T* __restrict__ p, uint32_t i)
+{
+  Pack<W, T> r;
+  if constexpr (W == 2 && sizeof(T) == 8) {
+    const double2 q = *reinterpret_cast<const double2*>(p + i);
+    r.v[0] = q.x;
+    r.v[1] = q.y;
+  } else if constexpr (W == 2 && sizeof(T) == 4) {
+    const float2 q = *reinterpret_cast<const float2*>(p + i);
+    r.v[0] = q.x;
+    r.v[1] = q.y;
+  } else if constexpr (W == 4 && sizeof(T) == 4) {
+    const float4 q = *reinterpret_cast<const float4*>(p + i);
+    r.v[0] = q.x;
+    r.v[1] = q.y;
+    r.v[2] = q.z;
+    r.v[3] = q.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < W; ++k) r.v[k] = p[i + k];
+  }
+  return r;
+}
+
+template <int W, typename T>
+__device__ __forceinline__ void stp(T* __restrict__ p, uint32_t i, const Pack<W, T>& r)
+{
+  if constexpr (W == 2 && sizeof(T) == 8) {
+    *reinterpret_cast<double2*>(p + i) = make_double2(r.v[0], r.v[1]);
+  } else if constexpr (W == 2 && sizeof(T) == 4) {
+    *reinterpret_cast<float2*>(p + i) = make_float2(r.v[0], r.v[1]);
+  } else if constexpr (W == 4 && sizeof(T) == 4) {
+    *reinterpret_cast<float4*>(p + i) = make_float4(r.v[0], r.v[1], r.v[2], r.v[3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < W; ++k) p[i + k] = r.v[k];
+  }
+}
+
+// Vector width used for arrays of T: 16-byte packs.
+template <typename T>
+constexpr int kVecW = 16 / sizeof(T);
+
+// Launches KERNEL_W2 (16-byte packs) when `v2` holds, else KERNEL_W1, with 256-lane workgroups.
+#define VCLHIP_LAUNCH_W(v2, KERNEL_W2, KERNEL_W1, grid, stream, ...)                                  \
+  do {                                                                                               \
+    if (v2)                                                                                          \
+      hipLaunchKernelGGL(KERNEL_W2, dim3(grid), dim3(::vclhip::kBlock), 0, stream, __VA_ARGS__);     \
+    else                                                                                             \
+      hipLaunchKernelGGL(KERNEL_W1, dim3(grid), dim3(::vclhip::kBlock), 0, stream, __VA_ARGS__);     \
+  } while (0)
+
+// True when every pointer is aligned for 16-byte packs.
+inline bool aligned16(std::initializer_list<const void*> ptrs)
+{
+  for (const void* p : ptrs)
+    if (p && (reinterpret_cast<uintptr_t>(p) & 15u)) return false;
+  return true;
 }
 
 // Workgroup w is dispatched to XCD (w mod 8).  Renumber so that each XCD owns a contiguous

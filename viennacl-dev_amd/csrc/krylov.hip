@@ -123,23 +123,33 @@ struct EpiRestart {
 // ------------------------------------------------------------------ CG kernels
 
 // x = 0; r = b; p = M^-1 b; partial <b, M^-1 b>  (cg.hpp:138-149, :268-279)
-template <typename T, bool Jac>
+template <typename T, bool Jac, int W>
 __global__ __launch_bounds__(kBlock) void cg_init_kernel(uint32_t n, const T* __restrict__ b, T* __restrict__ x,
                                                          T* __restrict__ r, T* __restrict__ p,
                                                          const T* __restrict__ d, double* __restrict__ part)
 {
   const uint32_t lw = xcd_logical_block();
   uint32_t c0, c1;
-  chunk_range(n, lw, gridDim.x, c0, c1);
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
   double acc[1] = {0};
-  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
-    const T vb = b[i];
-    x[i] = 0;
-    r[i] = vb;
-    const T z = Jac ? vb / d[i] : vb;
-    p[i] = z;
-    acc[0] += static_cast<double>(vb * z);
-  }
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    const Pack<V, T> vb = ldp<V>(b, i);
+    Pack<V, T> z = vb, zero;
+    if constexpr (Jac) {
+      const Pack<V, T> vd = ldp<V>(d, i);
+#pragma unroll
+      for (int k = 0; k < V; ++k) z.v[k] = vb.v[k] / vd.v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      zero.v[k] = 0;
+      acc[0] += static_cast<double>(vb.v[k] * z.v[k]);
+    }
+    stp<V>(x, i, zero);
+    stp<V>(r, i, vb);
+    stp<V>(p, i, z);
+  });
   block_store_partials<1, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
 }
 
@@ -182,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void cg_setup_kernel(KState* __restrict__ s
 }
 
 // x += alpha p; r -= alpha Ap; z = M^-1 r; p = z + beta p; partial <r,z>  (host_based/iterative_operations.hpp:402-415)
-template <typename T, bool Jac>
+template <typename T, bool Jac, int W>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(uint32_t n, T* __restrict__ x, T* __restrict__ p,
                                                            T* __restrict__ r, const T* __restrict__ Ap,
                                                            const T* __restrict__ d, const KState* __restrict__ st,
@@ -193,20 +203,29 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(uint32_t n, T* __rest
   const T beta = static_cast<T>(st->beta);
   const uint32_t lw = xcd_logical_block();
   uint32_t c0, c1;
-  chunk_range(n, lw, gridDim.x, c0, c1);
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
   double acc[1] = {0};
-#pragma unroll 2
-  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
-    T vp = p[i];
-    T vr = r[i];
-    x[i] += alpha * vp;
-    vr -= alpha * Ap[i];
-    const T z = Jac ? vr / d[i] : vr;
-    vp = z + beta * vp;
-    acc[0] += static_cast<double>(vr * z);
-    p[i] = vp;
-    r[i] = vr;
-  }
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    Pack<V, T> vp = ldp<V>(p, i);
+    Pack<V, T> vr = ldp<V>(r, i);
+    Pack<V, T> vx = ldp<V>(x, i);
+    const Pack<V, T> va = ldp<V>(Ap, i);
+    Pack<V, T> vd;
+    if constexpr (Jac) vd = ldp<V>(d, i);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      vx.v[k] += alpha * vp.v[k];
+      vr.v[k] -= alpha * va.v[k];
+      T z = vr.v[k];
+      if constexpr (Jac) z = vr.v[k] / vd.v[k];
+      vp.v[k] = z + beta * vp.v[k];
+      acc[0] += static_cast<double>(vr.v[k] * z);
+    }
+    stp<V>(x, i, vx);
+    stp<V>(p, i, vp);
+    stp<V>(r, i, vr);
+  });
   block_store_partials<1, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
 }
 
@@ -240,25 +259,35 @@ __global__ __launch_bounds__(kBlock) void cg_iter_kernel(KState* __restrict__ st
 // ------------------------------------------------------------------ BiCGStab kernels
 
 // x = 0; r = p = r0* = M^-1 b; partials <b,b>, <r,r>  (bicgstab.hpp:105-131, :408-445)
-template <typename T, bool Jac>
+template <typename T, bool Jac, int W>
 __global__ __launch_bounds__(kBlock) void bicg_init_kernel(uint32_t n, const T* __restrict__ b, T* __restrict__ x,
                                                            T* __restrict__ r, T* __restrict__ p, T* __restrict__ r0,
                                                            const T* __restrict__ d, double* __restrict__ part)
 {
   const uint32_t lw = xcd_logical_block();
   uint32_t c0, c1;
-  chunk_range(n, lw, gridDim.x, c0, c1);
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
   double acc[2] = {0, 0};
-  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
-    const T vb = b[i];
-    const T v = Jac ? vb / d[i] : vb;
-    x[i] = 0;
-    r[i] = v;
-    p[i] = v;
-    r0[i] = v;
-    acc[0] += static_cast<double>(vb * vb);
-    acc[1] += static_cast<double>(v * v);
-  }
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    const Pack<V, T> vb = ldp<V>(b, i);
+    Pack<V, T> v = vb, zero;
+    if constexpr (Jac) {
+      const Pack<V, T> vd = ldp<V>(d, i);
+#pragma unroll
+      for (int k = 0; k < V; ++k) v.v[k] = vb.v[k] / vd.v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      zero.v[k] = 0;
+      acc[0] += static_cast<double>(vb.v[k] * vb.v[k]);
+      acc[1] += static_cast<double>(v.v[k] * v.v[k]);
+    }
+    stp<V>(x, i, zero);
+    stp<V>(r, i, v);
+    stp<V>(p, i, v);
+    stp<V>(r0, i, v);
+  });
   block_store_partials<2, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
 }
 
@@ -310,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void bicg_alpha_kernel(KState* __restrict__
 }
 
 // s = r - alpha Ap; partial <s,s>  (host_based/iterative_operations.hpp:552-560)
-template <typename T>
+template <typename T, int W>
 __global__ __launch_bounds__(kBlock) void bicg_update_s_kernel(uint32_t n, T* __restrict__ s, const T* __restrict__ r,
                                                                const T* __restrict__ Ap, const KState* __restrict__ st,
                                                                double* __restrict__ part)
@@ -319,14 +348,20 @@ __global__ __launch_bounds__(kBlock) void bicg_update_s_kernel(uint32_t n, T* __
   const T alpha = static_cast<T>(st->alpha);
   const uint32_t lw = xcd_logical_block();
   uint32_t c0, c1;
-  chunk_range(n, lw, gridDim.x, c0, c1);
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
   double acc[1] = {0};
-#pragma unroll 2
-  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
-    const T vs = r[i] - alpha * Ap[i];
-    acc[0] += static_cast<double>(vs * vs);
-    s[i] = vs;
-  }
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    const Pack<V, T> vr = ldp<V>(r, i);
+    const Pack<V, T> va = ldp<V>(Ap, i);
+    Pack<V, T> vs;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      vs.v[k] = vr.v[k] - alpha * va.v[k];
+      acc[0] += static_cast<double>(vs.v[k] * vs.v[k]);
+    }
+    stp<V>(s, i, vs);
+  });
   block_store_partials<1, double>(acc, slot_ptr(part, 3), kMaxPartials, lw);
 }
 
@@ -372,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void bicg_scalars_kernel(KState* __restrict
 
 // x += alpha p + omega s; r = s - omega As; p = r + beta (p - omega Ap); partial <r,r0*>
 // (host_based/iterative_operations.hpp:598-617).  final_mode 1 applies only the x update.
-template <typename T>
+template <typename T, int W>
 __global__ __launch_bounds__(kBlock) void bicg_vec_update_kernel(uint32_t n, T* __restrict__ x, T* __restrict__ p,
                                                                  const T* __restrict__ s, T* __restrict__ r,
                                                                  const T* __restrict__ As, const T* __restrict__ Ap,
@@ -387,25 +422,157 @@ __global__ __launch_bounds__(kBlock) void bicg_vec_update_kernel(uint32_t n, T* 
   const uint32_t mode = st->final_mode;
   const uint32_t lw = xcd_logical_block();
   uint32_t c0, c1;
-  chunk_range(n, lw, gridDim.x, c0, c1);
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
   if (mode == 1) {
-    for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) x[i] += alpha * p[i] + omega * s[i];
+    for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+      constexpr int V = decltype(w)::value;
+      Pack<V, T> vx = ldp<V>(x, i);
+      const Pack<V, T> vp = ldp<V>(p, i);
+      const Pack<V, T> vs = ldp<V>(s, i);
+#pragma unroll
+      for (int k = 0; k < V; ++k) vx.v[k] += alpha * vp.v[k] + omega * vs.v[k];
+      stp<V>(x, i, vx);
+    });
     return;
   }
   double acc[1] = {0};
-  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
-    T vx = x[i];
-    T vp = p[i];
-    const T vs = s[i];
-    vx += alpha * vp + omega * vs;
-    const T vr = vs - omega * As[i];
-    vp = vr + beta * (vp - omega * Ap[i]);
-    acc[0] += static_cast<double>(vr * r0[i]);
-    x[i] = vx;
-    r[i] = vr;
-    p[i] = vp;
-  }
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    Pack<V, T> vx = ldp<V>(x, i);
+    Pack<V, T> vp = ldp<V>(p, i);
+    const Pack<V, T> vs = ldp<V>(s, i);
+    const Pack<V, T> vas = ldp<V>(As, i);
+    const Pack<V, T> vap = ldp<V>(Ap, i);
+    const Pack<V, T> vr0 = ldp<V>(r0, i);
+    Pack<V, T> vr;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      vx.v[k] += alpha * vp.v[k] + omega * vs.v[k];
+      vr.v[k] = vs.v[k] - omega * vas.v[k];
+      vp.v[k] = vr.v[k] + beta * (vp.v[k] - omega * vap.v[k]);
+      acc[0] += static_cast<double>(vr.v[k] * vr0.v[k]);
+    }
+    stp<V>(x, i, vx);
+    stp<V>(r, i, vr);
+    stp<V>(p, i, vp);
+  });
   block_store_partials<1, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
+}
+
+// ---- left-preconditioned BiCGStab with the reference's recurrences (bicgstab.hpp:447-480):
+// explicit <r,r0*> and ||r|| after the x/r update, beta = <r,r0*>_new / <r,r0*> * alpha / omega.
+
+// alpha = ip_rr0star / <M^-1 A p, r0*>  (bicgstab.hpp:448-450)
+__global__ __launch_bounds__(kBlock) void bicgc_alpha_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                             uint32_t g_prod)
+{
+  if (st->flags & kFlagDone) return;
+  const double t0r0 = block_sum_partials(slot_ptr(part, 2), g_prod);
+  if (threadIdx.x != 0) return;
+  st->alpha = st->rr / t0r0;
+}
+
+// omega = <t1,s> / (||t1|| ||t1||)  (bicgstab.hpp:454-457)
+__global__ __launch_bounds__(kBlock) void bicgc_omega_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                             uint32_t g_prod)
+{
+  if (st->flags & kFlagDone) return;
+  const double t1t1 = block_sum_partials(slot_ptr(part, 4), g_prod);
+  const double t1s = block_sum_partials(slot_ptr(part, 5), g_prod);
+  if (threadIdx.x != 0) return;
+  const double nrm = sqrt(t1t1);
+  st->omega = t1s / (nrm * nrm);
+}
+
+// x += alpha p + omega s; r = s - omega t1; partials <r,r>, <r,r0*>  (bicgstab.hpp:459-468)
+template <typename T, int W>
+__global__ __launch_bounds__(kBlock) void bicgc_xr_kernel(uint32_t n, T* __restrict__ x, const T* __restrict__ p,
+                                                          const T* __restrict__ s, T* __restrict__ r,
+                                                          const T* __restrict__ t1, const T* __restrict__ r0,
+                                                          const KState* __restrict__ st, double* __restrict__ part)
+{
+  if (st->flags & kFlagDone) return;
+  const T alpha = static_cast<T>(st->alpha);
+  const T omega = static_cast<T>(st->omega);
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  double acc[2] = {0, 0};
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    Pack<V, T> vx = ldp<V>(x, i);
+    const Pack<V, T> vp = ldp<V>(p, i);
+    const Pack<V, T> vs = ldp<V>(s, i);
+    const Pack<V, T> vt = ldp<V>(t1, i);
+    const Pack<V, T> vr0 = ldp<V>(r0, i);
+    Pack<V, T> vr;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      vx.v[k] += alpha * vp.v[k] + omega * vs.v[k];
+      vr.v[k] = vs.v[k] - omega * vt.v[k];
+      acc[0] += static_cast<double>(vr.v[k] * vr.v[k]);
+      acc[1] += static_cast<double>(vr.v[k] * vr0.v[k]);
+    }
+    stp<V>(x, i, vx);
+    stp<V>(r, i, vr);
+  });
+  block_store_partials<2, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
+}
+
+// ||r||, test, beta, restart request  (bicgstab.hpp:462-474)
+__global__ __launch_bounds__(kBlock) void bicgc_check_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                             uint32_t g_vec)
+{
+  if (st->flags & kFlagDone) return;
+  const double rr2 = block_sum_partials(slot_ptr(part, 0), g_vec);
+  const double rr0 = block_sum_partials(slot_ptr(part, 1), g_vec);
+  if (threadIdx.x != 0) return;
+  const double res = sqrt(rr2);
+  const uint32_t iter = st->iter + 1;
+  st->iter = iter;
+  st->res = res;
+  record_history(st, iter, res);
+  if (res / st->nrs < st->tol || res < st->abs_tol) {
+    st->flags |= kFlagDone;
+    st->converged = 1;
+    return;
+  }
+  if (iter >= st->max_iters) {
+    st->flags |= kFlagDone;
+    return;
+  }
+  const double omega = st->omega;
+  st->beta = rr0 / st->rr * st->alpha / omega;
+  st->rr = rr0;
+  if ((rr0 <= 0 && rr0 >= 0) || (omega <= 0 && omega >= 0)) {  // degenerate direction: restart (:473)
+    st->breakdown = 1;
+    st->flags |= kFlagDone;
+  }
+}
+
+// p = r + beta (p - omega t0), in the reference's two steps (bicgstab.hpp:479-480)
+template <typename T, int W>
+__global__ __launch_bounds__(kBlock) void bicgc_p_kernel(uint32_t n, T* __restrict__ p, const T* __restrict__ r,
+                                                         const T* __restrict__ t0, const KState* __restrict__ st)
+{
+  if (st->flags & kFlagDone) return;
+  const T beta = static_cast<T>(st->beta);
+  const T omega = static_cast<T>(st->omega);
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    Pack<V, T> vp = ldp<V>(p, i);
+    const Pack<V, T> vr = ldp<V>(r, i);
+    const Pack<V, T> vt = ldp<V>(t0, i);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const T pm = vp.v[k] - vt.v[k] * omega;
+      vp.v[k] = vr.v[k] + pm * beta;
+    }
+    stp<V>(p, i, vp);
+  });
 }
 
 // restart: <r,r> -> rr (norm_2 then squared, bicgstab.hpp:441-442)
@@ -538,21 +705,23 @@ class Workspace {
 };
 
 // Diagonal of A for Jacobi (linalg/detail row_info with SPARSE_ROW_DIAGONAL), failing on a zero entry.
-__global__ __launch_bounds__(kBlock) void count_zero_kernel(uint32_t n, const double* __restrict__ d,
+template <typename T>
+__global__ __launch_bounds__(kBlock) void count_zero_kernel(uint32_t n, const T* __restrict__ d,
                                                             uint32_t* __restrict__ count)
 {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i < n && !(d[i] < 0 || d[i] > 0)) atomicAdd(count, 1u);
 }
 
-__global__ __launch_bounds__(kBlock) void sell_diag_kernel(SellDev<double> A, double* __restrict__ d)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void sell_diag_kernel(SellDev<T> A, T* __restrict__ d)
 {
   const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
   if (row >= A.n_rows) return;
   const uint32_t s = row / A.C;
   const uint32_t lane = row - s * A.C;
   const size_t base = static_cast<size_t>(A.start[s]) + lane;
-  double v = 0;
+  T v = 0;
   for (uint32_t k = 0; k < A.cpb[s]; ++k) {
     const size_t off = base + static_cast<size_t>(k) * A.C;
     if (A.col[off] == row && (A.val[off] < 0 || A.val[off] > 0)) { v = A.val[off]; break; }
@@ -560,17 +729,20 @@ __global__ __launch_bounds__(kBlock) void sell_diag_kernel(SellDev<double> A, do
   d[row] = v;
 }
 
-static int jacobi_diag(const DevOp<double>& op, const vclhip_csr* Acsr, double* d, uint32_t* zero_count,
-                       hipStream_t s)
+inline int csr_row_info(const vclhip_csr* A, double* d, hipStream_t s) { return vclhip_csr_row_info_f64(A, d, 3, s); }
+inline int csr_row_info(const vclhip_csr* A, float* d, hipStream_t s) { return vclhip_csr_row_info_f32(A, d, 3, s); }
+
+template <typename T>
+static int jacobi_diag(const DevOp<T>& op, const vclhip_csr* Acsr, T* d, uint32_t* zero_count, hipStream_t s)
 {
   if (!op.sell) {
-    VCLHIP_TRY(vclhip_csr_row_info_f64(Acsr, d, 3, s));
+    VCLHIP_TRY(csr_row_info(Acsr, d, s));
   } else {
-    hipLaunchKernelGGL(sell_diag_kernel, dim3((op.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, op.sm, d);
+    hipLaunchKernelGGL(sell_diag_kernel<T>, dim3((op.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, op.sm, d);
     VCLHIP_TRY(check_launch("sell_diag_kernel"));
   }
   VCLHIP_HIP(hipMemsetAsync(zero_count, 0, sizeof(uint32_t), s));
-  hipLaunchKernelGGL(count_zero_kernel, dim3((op.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, op.n, d, zero_count);
+  hipLaunchKernelGGL(count_zero_kernel<T>, dim3((op.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, op.n, d, zero_count);
   VCLHIP_TRY(check_launch("count_zero_kernel"));
   uint32_t zeros = 0;
   VCLHIP_HIP(hipMemcpyAsync(&zeros, zero_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -606,7 +778,7 @@ struct EventPair {
 // restart), which keeps it out of a captured graph; hook(i) enqueues that work.
 template <class LaunchIter, class NeedsHook, class Hook>
 static int run_iterations(hipStream_t s, KState* st_dev, uint32_t first_iter, uint32_t max_iters,
-                          const vclhip_solver_params* P, const double* x_dev, LaunchIter&& launch_iter,
+                          const vclhip_solver_params* P, const void* x_dev, LaunchIter&& launch_iter,
                           NeedsHook&& needs_hook, Hook&& hook, KState& final_state, bool bicg)
 {
   PinnedState pin;
@@ -723,13 +895,13 @@ static int copy_history(const vclhip_solver_params* P, const double* history_dev
 
 // ------------------------------------------------------------------ CG driver
 
-template <class M>
-static int cg_solve(const M* Ain, const double* b, double* x, int precond, const vclhip_solver_params* P,
+template <typename T, class M>
+static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_solver_params* P,
                     vclhip_solver_result* R, void* stream)
 {
   if (!P || !R || !b || !x) return invalid("null argument");
   if (precond != VCLHIP_PRECOND_NONE && precond != VCLHIP_PRECOND_JACOBI) return invalid("precond");
-  DevOp<double> op;
+  DevOp<T> op;
   VCLHIP_TRY(make_op(Ain, op));
   SolveStream ss(stream, P->use_graph != 0);
   VCLHIP_TRY(ss.status());
@@ -737,7 +909,8 @@ static int cg_solve(const M* Ain, const double* b, double* x, int precond, const
   const uint32_t n = op.n;
   const bool jac = precond == VCLHIP_PRECOND_JACOBI;
   Workspace ws(s);
-  double *r, *p, *Ap, *d = nullptr, *part, *hist = nullptr;
+  T *r, *p, *Ap, *d = nullptr;
+  double *part, *hist = nullptr;
   KState* st;
   VCLHIP_TRY(ws.alloc(&r, n));
   VCLHIP_TRY(ws.alloc(&p, n));
@@ -756,16 +929,19 @@ static int cg_solve(const M* Ain, const double* b, double* x, int precond, const
   VCLHIP_TRY(upload_state(st, h, s));
 
   const uint32_t gv = vec_grid_n(n);
+  const bool v2 = aligned16({b, x});  // workspace vectors are 256-byte aligned
   // setup: x = 0, r = b, p = M^-1 b, Ap = A p, scalars
   if (jac)
-    hipLaunchKernelGGL((cg_init_kernel<double, true>), dim3(gv), dim3(kBlock), 0, s, n, b, x, r, p, d, part);
+    VCLHIP_LAUNCH_W(v2, (cg_init_kernel<T, true, kVecW<T> >), (cg_init_kernel<T, true, 1>), gv, s, n, b, x, r, p,
+                    d, part);
   else
-    hipLaunchKernelGGL((cg_init_kernel<double, false>), dim3(gv), dim3(kBlock), 0, s, n, b, x, r, p, d, part);
+    VCLHIP_LAUNCH_W(v2, (cg_init_kernel<T, false, kVecW<T> >), (cg_init_kernel<T, false, 1>), gv, s, n, b, x, r, p,
+                    d, part);
   VCLHIP_TRY(check_launch("cg_init_kernel"));
   if (jac)
-    op.launch(s, p, EpiCg<double, true>{Ap, p, d}, slot_ptr(part, 1), nullptr, 0u);
+    op.launch(s, p, EpiCg<T, true>{Ap, p, d}, slot_ptr(part, 1), nullptr, 0u);
   else
-    op.launch(s, p, EpiCg<double, false>{Ap, p, d}, slot_ptr(part, 1), nullptr, 0u);
+    op.launch(s, p, EpiCg<T, false>{Ap, p, d}, slot_ptr(part, 1), nullptr, 0u);
   VCLHIP_TRY(check_launch("cg prod"));
   if (jac)
     hipLaunchKernelGGL(cg_setup_kernel<true>, dim3(1), dim3(kBlock), 0, s, st, part, gv, op.grid);
@@ -776,13 +952,15 @@ static int cg_solve(const M* Ain, const double* b, double* x, int precond, const
   uint32_t* flags = &st->flags;
   auto launch_iter = [&](uint32_t) -> int {
     if (jac)
-      hipLaunchKernelGGL((cg_update_kernel<double, true>), dim3(gv), dim3(kBlock), 0, s, n, x, p, r, Ap, d, st, part);
+      VCLHIP_LAUNCH_W(v2, (cg_update_kernel<T, true, kVecW<T> >), (cg_update_kernel<T, true, 1>), gv, s, n, x, p, r,
+                      Ap, d, st, part);
     else
-      hipLaunchKernelGGL((cg_update_kernel<double, false>), dim3(gv), dim3(kBlock), 0, s, n, x, p, r, Ap, d, st, part);
+      VCLHIP_LAUNCH_W(v2, (cg_update_kernel<T, false, kVecW<T> >), (cg_update_kernel<T, false, 1>), gv, s, n, x, p,
+                      r, Ap, d, st, part);
     if (jac)
-      op.launch(s, p, EpiCg<double, true>{Ap, p, d}, slot_ptr(part, 1), flags, kFlagDone);
+      op.launch(s, p, EpiCg<T, true>{Ap, p, d}, slot_ptr(part, 1), flags, kFlagDone);
     else
-      op.launch(s, p, EpiCg<double, false>{Ap, p, d}, slot_ptr(part, 1), flags, kFlagDone);
+      op.launch(s, p, EpiCg<T, false>{Ap, p, d}, slot_ptr(part, 1), flags, kFlagDone);
     hipLaunchKernelGGL(cg_iter_kernel, dim3(1), dim3(kBlock), 0, s, st, part, gv, op.grid);
     return check_launch("cg iteration");
   };
@@ -800,13 +978,13 @@ static int cg_solve(const M* Ain, const double* b, double* x, int precond, const
 
 // ------------------------------------------------------------------ BiCGStab driver
 
-template <class M>
-static int bicgstab_solve(const M* Ain, const double* b, double* x, int precond, const vclhip_solver_params* P,
+template <typename T, class M>
+static int bicgstab_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_solver_params* P,
                           vclhip_solver_result* R, void* stream)
 {
   if (!P || !R || !b || !x) return invalid("null argument");
   if (precond != VCLHIP_PRECOND_NONE && precond != VCLHIP_PRECOND_JACOBI) return invalid("precond");
-  DevOp<double> op;
+  DevOp<T> op;
   VCLHIP_TRY(make_op(Ain, op));
   SolveStream ss(stream, P->use_graph != 0);
   VCLHIP_TRY(ss.status());
@@ -814,7 +992,8 @@ static int bicgstab_solve(const M* Ain, const double* b, double* x, int precond,
   const uint32_t n = op.n;
   const bool jac = precond == VCLHIP_PRECOND_JACOBI;
   Workspace ws(s);
-  double *r, *p, *r0, *Ap, *sv, *As, *d = nullptr, *part, *hist = nullptr;
+  T *r, *p, *r0, *Ap, *sv, *As, *d = nullptr;
+  double *part, *hist = nullptr;
   KState* st;
   VCLHIP_TRY(ws.alloc(&r, n));
   VCLHIP_TRY(ws.alloc(&p, n));
@@ -836,10 +1015,13 @@ static int bicgstab_solve(const M* Ain, const double* b, double* x, int precond,
   VCLHIP_TRY(upload_state(st, h, s));
 
   const uint32_t gv = vec_grid_n(n);
+  const bool v2 = aligned16({b, x});
   if (jac)
-    hipLaunchKernelGGL((bicg_init_kernel<double, true>), dim3(gv), dim3(kBlock), 0, s, n, b, x, r, p, r0, d, part);
+    VCLHIP_LAUNCH_W(v2, (bicg_init_kernel<T, true, kVecW<T> >), (bicg_init_kernel<T, true, 1>), gv, s, n, b, x, r,
+                    p, r0, d, part);
   else
-    hipLaunchKernelGGL((bicg_init_kernel<double, false>), dim3(gv), dim3(kBlock), 0, s, n, b, x, r, p, r0, d, part);
+    VCLHIP_LAUNCH_W(v2, (bicg_init_kernel<T, false, kVecW<T> >), (bicg_init_kernel<T, false, 1>), gv, s, n, b, x, r,
+                    p, r0, d, part);
   VCLHIP_TRY(check_launch("bicg_init_kernel"));
   hipLaunchKernelGGL(bicg_setup_kernel, dim3(1), dim3(kBlock), 0, s, st, part, gv);
   VCLHIP_TRY(check_launch("bicg_setup_kernel"));
@@ -847,20 +1029,37 @@ static int bicgstab_solve(const M* Ain, const double* b, double* x, int precond,
   uint32_t* flags = &st->flags;
   const uint32_t stop_all = kFlagDone | kFlagFinishing;
   const int brk = jac ? 1 : 0;
+  const bool classic = jac && P->variant == 0;
+  auto launch_classic = [&]() -> int {
+    // t0 = M^-1 A p -> Ap; alpha; s = r - alpha t0; t1 = M^-1 A s -> As; omega; x, r; test; p
+    op.launch(s, p, EpiBicg<T, true, false>{Ap, p, r0, d}, slot_ptr(part, 2), flags, kFlagDone);
+    hipLaunchKernelGGL(bicgc_alpha_kernel, dim3(1), dim3(kBlock), 0, s, st, part, op.grid);
+    VCLHIP_LAUNCH_W(v2, (bicg_update_s_kernel<T, kVecW<T> >), (bicg_update_s_kernel<T, 1>), gv, s, n, sv, r, Ap, st,
+                    part);
+    op.launch(s, sv, EpiBicg<T, true, true>{As, sv, r0, d}, slot_ptr(part, 4), flags, kFlagDone);
+    hipLaunchKernelGGL(bicgc_omega_kernel, dim3(1), dim3(kBlock), 0, s, st, part, op.grid);
+    VCLHIP_LAUNCH_W(v2, (bicgc_xr_kernel<T, kVecW<T> >), (bicgc_xr_kernel<T, 1>), gv, s, n, x, p, sv, r, As, r0, st,
+                    part);
+    hipLaunchKernelGGL(bicgc_check_kernel, dim3(1), dim3(kBlock), 0, s, st, part, gv);
+    VCLHIP_LAUNCH_W(v2, (bicgc_p_kernel<T, kVecW<T> >), (bicgc_p_kernel<T, 1>), gv, s, n, p, r, Ap, st);
+    return check_launch("bicgstab iteration");
+  };
   auto launch_iter = [&](uint32_t) -> int {
+    if (classic) return launch_classic();
     if (jac)
-      op.launch(s, p, EpiBicg<double, true, false>{Ap, p, r0, d}, slot_ptr(part, 2), flags, stop_all);
+      op.launch(s, p, EpiBicg<T, true, false>{Ap, p, r0, d}, slot_ptr(part, 2), flags, stop_all);
     else
-      op.launch(s, p, EpiBicg<double, false, false>{Ap, p, r0, d}, slot_ptr(part, 2), flags, stop_all);
+      op.launch(s, p, EpiBicg<T, false, false>{Ap, p, r0, d}, slot_ptr(part, 2), flags, stop_all);
     hipLaunchKernelGGL(bicg_alpha_kernel, dim3(1), dim3(kBlock), 0, s, st, part, gv, op.grid, brk);
-    hipLaunchKernelGGL(bicg_update_s_kernel<double>, dim3(gv), dim3(kBlock), 0, s, n, sv, r, Ap, st, part);
+    VCLHIP_LAUNCH_W(v2, (bicg_update_s_kernel<T, kVecW<T> >), (bicg_update_s_kernel<T, 1>), gv, s, n, sv, r, Ap, st,
+                    part);
     if (jac)
-      op.launch(s, sv, EpiBicg<double, true, true>{As, sv, r0, d}, slot_ptr(part, 4), flags, stop_all);
+      op.launch(s, sv, EpiBicg<T, true, true>{As, sv, r0, d}, slot_ptr(part, 4), flags, stop_all);
     else
-      op.launch(s, sv, EpiBicg<double, false, true>{As, sv, r0, d}, slot_ptr(part, 4), flags, stop_all);
+      op.launch(s, sv, EpiBicg<T, false, true>{As, sv, r0, d}, slot_ptr(part, 4), flags, stop_all);
     hipLaunchKernelGGL(bicg_scalars_kernel, dim3(1), dim3(kBlock), 0, s, st, part, gv, op.grid, brk);
-    hipLaunchKernelGGL(bicg_vec_update_kernel<double>, dim3(gv), dim3(kBlock), 0, s, n, x, p, sv, r, As, Ap, r0, st,
-                       part);
+    VCLHIP_LAUNCH_W(v2, (bicg_vec_update_kernel<T, kVecW<T> >), (bicg_vec_update_kernel<T, 1>), gv, s, n, x, p, sv,
+                    r, As, Ap, r0, st, part);
     return check_launch("bicgstab iteration");
   };
   // Restarts of the preconditioned iteration (bicgstab.hpp:431-445, :473): iteration i restarts when
@@ -871,9 +1070,9 @@ static int bicgstab_solve(const M* Ain, const double* b, double* x, int precond,
   auto restart = [&](uint32_t i) -> int {
     last_restart = i;
     if (jac)
-      op.launch(s, x, EpiRestart<double, true>{b, d, r, p, r0}, slot_ptr(part, 7), flags, stop_all);
+      op.launch(s, x, EpiRestart<T, true>{b, d, r, p, r0}, slot_ptr(part, 7), flags, stop_all);
     else
-      op.launch(s, x, EpiRestart<double, false>{b, d, r, p, r0}, slot_ptr(part, 7), flags, stop_all);
+      op.launch(s, x, EpiRestart<T, false>{b, d, r, p, r0}, slot_ptr(part, 7), flags, stop_all);
     hipLaunchKernelGGL(bicg_restart_kernel, dim3(1), dim3(kBlock), 0, s, st, part, op.grid);
     return check_launch("bicgstab restart");
   };
@@ -1061,6 +1260,30 @@ int vclhip_bicgstab_solve_csr_f64(const vclhip_csr* A, const double* b, double* 
 }
 
 int vclhip_bicgstab_solve_sell_f64(const vclhip_sell* A, const double* b, double* x, int precond,
+                                   const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return bicgstab_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_cg_solve_csr_f32(const vclhip_csr* A, const float* b, float* x, int precond,
+                            const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return cg_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_cg_solve_sell_f32(const vclhip_sell* A, const float* b, float* x, int precond,
+                             const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return cg_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_bicgstab_solve_csr_f32(const vclhip_csr* A, const float* b, float* x, int precond,
+                                  const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return bicgstab_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_bicgstab_solve_sell_f32(const vclhip_sell* A, const float* b, float* x, int precond,
                                    const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
 {
   return bicgstab_solve(A, b, x, precond, P, R, stream);

@@ -28,9 +28,16 @@ uint32_t sell_gmax()
   return g;
 }
 
+// Plain SpMV writes no partials, so by default every row block / row tile gets its own workgroup.
+uint32_t spmv_gmax()
+{
+  static const uint32_t g = env_u32("VCLHIP_SPMV_GMAX", 1u << 24, 8, 1u << 24);
+  return g;
+}
+
 uint32_t vec_gmax()
 {
-  static const uint32_t g = env_u32("VCLHIP_VEC_GMAX", 1024, 8, kMaxPartials);
+  static const uint32_t g = env_u32("VCLHIP_VEC_GMAX", 2048, 8, kMaxPartials);
   return g;
 }
 
@@ -92,7 +99,7 @@ static int csr_spmv(const vclhip_csr* A, const T* x, uint32_t xs, uint32_t xi, T
   if (A->n_rows == 0) return VCLHIP_SUCCESS;
   if (!x || !y || xi == 0 || yi == 0) return invalid("x/y must be non-NULL with nonzero stride");
   if (x == y) return invalid("x and y must not alias");
-  const uint32_t grid = chunk_grid(A->num_blocks, csr_gmax());
+  const uint32_t grid = chunk_grid(A->num_blocks, spmv_gmax());
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (xs == 0 && xi == 1 && ys == 0 && yi == 1) {
     EpiAxpby<T, true> epi{y, 0, 1, alpha, beta};
@@ -115,7 +122,7 @@ static int sell_spmv(const vclhip_sell* A, const T* x, uint32_t xs, uint32_t xi,
   if (!x || !y || xi == 0 || yi == 0) return invalid("x/y must be non-NULL with nonzero stride");
   if (x == y) return invalid("x and y must not alias");
   const uint32_t tiles = (A->n_rows + kBlock - 1) / kBlock;
-  const uint32_t grid = chunk_grid(tiles, sell_gmax());
+  const uint32_t grid = chunk_grid(tiles, spmv_gmax());
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (xs == 0 && xi == 1 && ys == 0 && yi == 1) {
     EpiAxpby<T, true> epi{y, 0, 1, alpha, beta};

@@ -198,5 +198,6 @@ inline uint32_t chunk_grid(uint32_t units, uint32_t gmax)
 uint32_t csr_gmax();
 uint32_t sell_gmax();
 uint32_t vec_gmax();
+uint32_t spmv_gmax();
 
 }  // namespace vclhip

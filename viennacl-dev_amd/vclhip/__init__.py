@@ -186,9 +186,10 @@ class SellMatrix:
 
 
 def solver_params(tol=1e-8, max_iters=300, abs_tol=0.0, max_iters_before_restart=200, inner_tol=1e-2,
-                  chunk=16, use_graph=False, mixed_mode=0, history=None, monitor=None):
+                  chunk=16, use_graph=False, mixed_mode=0, variant=0, history=None, monitor=None):
     p = SolverParams()
     lib().vclhip_solver_params_default(C.byref(p))
+    p.variant = variant
     p.tol = tol
     p.abs_tol = abs_tol
     p.max_iters = max_iters

@@ -49,7 +49,7 @@ MONITOR = C.CFUNCTYPE(C.c_int, vp, f64, vp)
 class SolverParams(C.Structure):
     _fields_ = [("tol", f64), ("abs_tol", f64), ("max_iters", u32),
                 ("max_iters_before_restart", u32), ("inner_tol", f32), ("chunk", u32),
-                ("use_graph", i32), ("mixed_mode", i32), ("history_host", P(f64)),
+                ("use_graph", i32), ("mixed_mode", i32), ("variant", i32), ("history_host", P(f64)),
                 ("history_len", u32), ("monitor", MONITOR), ("monitor_user", vp)]
 
 
@@ -116,6 +116,10 @@ _SIGS = {
     "vclhip_cg_solve_sell_f64": (i32, [P(Sell), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_bicgstab_solve_csr_f64": (i32, [P(Csr), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_bicgstab_solve_sell_f64": (i32, [P(Sell), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_cg_solve_csr_f32": (i32, [P(Csr), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_cg_solve_sell_f32": (i32, [P(Sell), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_bicgstab_solve_csr_f32": (i32, [P(Csr), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_bicgstab_solve_sell_f32": (i32, [P(Sell), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_random_uniform": (i32, [u64, u64, f64, f64, vp]),
     "vclhip_random_normal": (i32, [u64, u64, f64, f64, vp]),
     "vclhip_gen_fdm_laplace_2d": (i32, [u32, u32, vp, vp, vp, P(u32)]),

@@ -1,0 +1,113 @@
+#ifndef VIENNACL_LINALG_SPARSE_MATRIX_OPERATIONS_HPP
+#define VIENNACL_LINALG_SPARSE_MATRIX_OPERATIONS_HPP
+// sparse_matrix_operations.hpp -- prod_impl and row_info for CSR and SELL-C matrices, the
+// operations behind y = prod(A, x), y += prod(A, x), y -= prod(A, x)
+// (reference linalg/sparse_matrix_operations.hpp:48-121 and the op_executors of
+// compressed_matrix.hpp:1231-1318 / sliced_ell_matrix.hpp:286-375).  HIP_MEMORY only.
+
+#include "viennacl/forwards.h"
+#include "viennacl/vector.hpp"
+
+namespace viennacl
+{
+namespace linalg
+{
+namespace hip
+{
+inline int spmv(vclhip_csr const& d, const double* x, uint32_t xs, uint32_t xi, double a, double* y, uint32_t ys,
+                uint32_t yi, double b)
+{
+  return vclhip_csr_spmv_f64(&d, x, xs, xi, a, y, ys, yi, b, backend::hip::current_stream());
+}
+inline int spmv(vclhip_csr const& d, const float* x, uint32_t xs, uint32_t xi, float a, float* y, uint32_t ys,
+                uint32_t yi, float b)
+{
+  return vclhip_csr_spmv_f32(&d, x, xs, xi, a, y, ys, yi, b, backend::hip::current_stream());
+}
+inline int spmv(vclhip_sell const& d, const double* x, uint32_t xs, uint32_t xi, double a, double* y, uint32_t ys,
+                uint32_t yi, double b)
+{
+  return vclhip_sell_spmv_f64(&d, x, xs, xi, a, y, ys, yi, b, backend::hip::current_stream());
+}
+inline int spmv(vclhip_sell const& d, const float* x, uint32_t xs, uint32_t xi, float a, float* y, uint32_t ys,
+                uint32_t yi, float b)
+{
+  return vclhip_sell_spmv_f32(&d, x, xs, xi, a, y, ys, yi, b, backend::hip::current_stream());
+}
+}  // namespace hip
+
+/** y = alpha * A * x + beta * y (linalg/sparse_matrix_operations.hpp:90-121).  x and y must not alias. */
+template <typename MatrixT, typename T>
+void prod_impl(MatrixT const& A, vector_base<T> const& x, T alpha, vector_base<T>& y, T beta)
+{
+  if (x.size() != A.size2() || y.size() != A.size1())
+    throw std::invalid_argument("ViennaCL (HIP): prod_impl size mismatch");
+  x.require_hip();
+  y.require_hip();
+  if (A.size1() == 0) return;
+  backend::hip::check(hip::spmv(A.descriptor(), x.device_ptr(), uint32_t(x.start()), uint32_t(x.stride()), alpha,
+                                y.device_ptr(), uint32_t(y.start()), uint32_t(y.stride()), beta),
+                      "prod_impl");
+}
+
+/** prod(A, x) as a lazy expression (linalg/prod.hpp:350-360). */
+template <typename MatrixT, typename T>
+vector_expression<MatrixT, vector_base<T>, op_prod> prod(MatrixT const& A, vector_base<T> const& x)
+{
+  return vector_expression<MatrixT, vector_base<T>, op_prod>(A, x);
+}
+
+namespace detail
+{
+/** Per-row information (linalg/sparse_matrix_operations.hpp:48-74): norms or the diagonal. */
+template <typename T, unsigned int A>
+void row_info(compressed_matrix<T, A> const& mat, vector_base<T>& vec, row_info_types info)
+{
+  if (vec.size() == 0) vec.init(mat.size1(), viennacl::context(HIP_MEMORY));
+  if (!vec.contiguous()) throw memory_exception("row_info needs a contiguous vector");
+  vclhip_csr d = mat.descriptor();
+  if (std::is_same<T, double>::value)
+    backend::hip::check(vclhip_csr_row_info_f64(&d, reinterpret_cast<double*>(vec.device_ptr()), int(info),
+                                                backend::hip::current_stream()), "row_info");
+  else
+    backend::hip::check(vclhip_csr_row_info_f32(&d, reinterpret_cast<float*>(vec.device_ptr()), int(info),
+                                                backend::hip::current_stream()), "row_info");
+}
+}  // namespace detail
+}  // namespace linalg
+
+/** y = prod(A, x); aliasing of y and x goes through a temporary (compressed_matrix.hpp:1231-1246). */
+template <typename T, typename MatrixT>
+void vcl_eval(vector_base<T>& y, vector_expression<MatrixT, vector_base<T>, op_prod> const& e)
+{
+  MatrixT const& A = e.lhs();
+  vector_base<T> const& x = e.rhs();
+  if (y.size() == 0) y.init(A.size1(), viennacl::context(HIP_MEMORY));
+  if (y.handle() == x.handle()) {
+    vector_base<T> tmp(A.size1());
+    linalg::prod_impl(A, x, T(1), tmp, T(0));
+    y.assign_linear(tmp, T(1));
+    return;
+  }
+  linalg::prod_impl(A, x, T(1), y, T(0));
+}
+
+/** y += sign * prod(A, x) (compressed_matrix.hpp:1248-1318). */
+template <typename T, typename MatrixT>
+void vcl_inplace(vector_base<T>& y, vector_expression<MatrixT, vector_base<T>, op_prod> const& e, T sign)
+{
+  MatrixT const& A = e.lhs();
+  vector_base<T> const& x = e.rhs();
+  if (y.handle() == x.handle()) {
+    vector_base<T> tmp(A.size1());
+    linalg::prod_impl(A, x, T(1), tmp, T(0));
+    y.require_hip();
+    backend::hip::check(detail::avbv(y, y, T(1), tmp, sign), "y += A x");
+    return;
+  }
+  linalg::prod_impl(A, x, sign, y, T(1));
+}
+
+}  // namespace viennacl
+
+#endif
