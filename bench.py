@@ -145,6 +145,59 @@ def timed_solve(vh, kind, A, b, n, precond=0, reps=2, **kw):
             "true_relres": float(rel), "converged": bool(res.converged)}
 
 
+def mixed_vs_fp64(vh, grid, reps=2):
+    """FP64 CG vs mixed-precision CG to ||b - A x|| / ||b|| <= 1e-10 on the 3-D variable-coefficient
+    diffusion operator (k = exp(g), g ~ N(0,1) per cell, harmonic-mean faces; seed 1 for k, 0 for b)."""
+    t = time.perf_counter()
+    n = grid ** 3
+    k = np.exp(vh.random_normal(1, n))
+    rp, col, val = vh.diffusion(grid, grid, grid, k)
+    nnz = int(rp[-1])
+    log(f"generated diffusion {grid}^3: n={n} nnz={nnz} ({time.perf_counter() - t:.1f} s)")
+    A = vh.CsrMatrix(rp, col, val)
+    b = vh.DeviceArray.from_numpy(vh.random_uniform(0, n))
+    out = {"workload": f"3-D variable-coefficient diffusion {grid}^3 (n={n}, nnz={nnz}), CSR, tol 1e-10, "
+                       f"b ~ U[-1,1]", "n": n, "nnz": nnz}
+
+    def run(name, fn, x):
+        best, res = None, None
+        for _ in range(reps):
+            vh.call("vclhip_device_synchronize")
+            t0 = time.perf_counter()
+            res = fn(x)
+            vh.call("vclhip_device_synchronize")
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        rec = {"seconds": round(best, 6), "iters": int(res.iters), "error": float(res.error),
+               "true_relres": float(true_relres(vh, A, b, x, n)), "converged": bool(res.converged)}
+        if "mixed" in name:
+            rec["fp64_corrections"] = int(res.restarts)
+        out[name] = rec
+        log(name, rec)
+        return rec
+
+    x64 = vh.DeviceArray.zeros(n, np.float64)
+    f64 = run("cg_fp64", lambda x: vh.cg(A, b, x, tol=1e-10, max_iters=20000), x64)
+    x64h = x64.to_numpy()
+    xm = vh.DeviceArray.zeros(n, np.float64)
+    for name, kw in (("mixed_reliable", dict(mixed_mode=1)), ("mixed_reference_schedule", dict(mixed_mode=0))):
+        rec = run(name, lambda x: vh.mixed_cg(A, b, x, tol=1e-10, max_iters=20000, **kw), xm)
+        xh = xm.to_numpy()
+        rec["rel_error_vs_fp64_solution"] = float(np.linalg.norm(xh - x64h) / np.linalg.norm(x64h))
+        rec["time_ratio_vs_fp64"] = round(rec["seconds"] / f64["seconds"], 4)
+    xj = vh.DeviceArray.zeros(n, np.float64)
+    j64 = run("jacobi_cg_fp64", lambda x: vh.cg(A, b, x, precond=1, tol=1e-10, max_iters=20000), xj)
+    xjh = xj.to_numpy()
+    rec = run("jacobi_mixed_reliable", lambda x: vh.mixed_cg(A, b, x, precond=1, tol=1e-10, max_iters=20000,
+                                                              mixed_mode=1), xm)
+    rec["rel_error_vs_fp64_solution"] = float(np.linalg.norm(xm.to_numpy() - xjh) / np.linalg.norm(xjh))
+    rec["time_ratio_vs_fp64"] = round(rec["seconds"] / j64["seconds"], 4)
+    for a in (x64, xm, xj, b):
+        a.free()
+    A.free()
+    return out
+
+
 def cpu_baseline(rp, col, val, x, n, nnz, seconds):
     """The reference host backend (OpenMP) timed on this machine's cores over a bounded sample."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -178,6 +231,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solvers", action="store_true")
+    ap.add_argument("--mixed-grid", type=int, default=160,
+                    help="grid of the FP64-vs-mixed CG comparison (0 skips it)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only warm CSR/SELL SpMV and the streaming kernel (for counter collection)")
     args = ap.parse_args()
@@ -264,6 +319,8 @@ def main():
         for k, v in sol.items():
             log(k, v)
         out["solvers_lap3d_128"] = sol
+        if args.mixed_grid:
+            out["mixed_vs_fp64"] = mixed_vs_fp64(vh, args.mixed_grid)
 
     if not args.no_cpu:
         log("CPU baseline (reference host backend) ...")

@@ -143,6 +143,12 @@ int vclhip_sell_spmv_f32(const vclhip_sell* A, const float* x_dev, uint32_t x_st
  * info 0 = inf-norm, 1 = 1-norm, 2 = 2-norm, 3 = diagonal (0 when absent). */
 int vclhip_csr_row_info_f64(const vclhip_csr* A, double* out_dev, int info, void* stream);
 int vclhip_csr_row_info_f32(const vclhip_csr* A, float* out_dev, int info, void* stream);
+/* Gradient of x = A^{-1} b with respect to A's values on A's pattern, given the adjoint solution
+ * lambda = A^{-T} dL/dx: dA[k] = -lambda[row(k)] * x[col(k)] (dL/db is lambda itself).  One kernel
+ * over the row-block plan.  No reference counterpart (the differentiable solve is new here). */
+int vclhip_csr_adjoint_grad_f64(const vclhip_csr* A, const double* lambda, const double* x, double* dA,
+                                void* stream);
+int vclhip_csr_adjoint_grad_f32(const vclhip_csr* A, const float* lambda, const float* x, float* dA, void* stream);
 
 /* ----------------------------------------------------------- vector ops ---- */
 /* inner_prod_impl / inner_prod_cpu (linalg/vector_operations.hpp:500-590;
@@ -280,6 +286,60 @@ int vclhip_bicgstab_solve_csr_f32(const vclhip_csr* A, const float* b_dev, float
 int vclhip_bicgstab_solve_sell_f32(const vclhip_sell* A, const float* b_dev, float* x_dev, int precond,
                                    const vclhip_solver_params* params, vclhip_solver_result* result,
                                    void* stream);
+
+/* Mixed-precision CG: replaces linalg::solve(A, b, mixed_precision_cg_tag)
+ * (mixed_precision_cg.hpp:95-193).  A holds FP64 values; the inner iteration runs SpMV, the Jacobi
+ * scaling and the vector updates in FP32 with inner products accumulated in FP64, and FP64 is used
+ * only for x += x32, r = b - A x and the convergence norm (true residual ||b - A x|| / ||b||).
+ * values32: FP32 copy of A's values (nnz floats for CSR, the stored elements for SELL), or NULL to
+ * convert inside the call as the reference does (mixed_precision_cg.hpp:127-137).
+ * params->mixed_mode 0: the reference's schedule -- FP32 classic CG restarted from the FP64 residual
+ * when the FP32 <r,r> has fallen below inner_tol times its value at the last restart (squared
+ * norms, mixed_precision_cg.hpp:161); no preconditioner.  1: reliable updates -- FP32 pipelined
+ * CG whose residual is replaced by the FP64 one under the same inner_tol test while the search
+ * direction is kept (fewer iterations than restarting; precond may be Jacobi).
+ * result->iters counts FP32 iterations, result->restarts the FP64 corrections, result->error the
+ * last FP64 relative residual. */
+int vclhip_mixed_cg_solve_csr(const vclhip_csr* A, const float* values32, const double* b_dev, double* x_dev,
+                              int precond, const vclhip_solver_params* params, vclhip_solver_result* result,
+                              void* stream);
+int vclhip_mixed_cg_solve_sell(const vclhip_sell* A, const float* elements32, const double* b_dev, double* x_dev,
+                               int precond, const vclhip_solver_params* params, vclhip_solver_result* result,
+                               void* stream);
+
+/* ---------------------------------------------------------------- batched CG */
+/* Batched CG: independent SPD systems, the diagonal blocks of one block-diagonal CSR matrix with
+ * global column indices; system i is rows [sys_ptr[i], sys_ptr[i+1]).  No counterpart in the
+ * reference (its nearest is one solve(A_i, b_i, cg_tag) per system, cg.hpp:128-187, whose
+ * recurrences each system follows).  One persistent launch: each workgroup takes systems from a
+ * work queue (largest first) and runs CG on one system at a time with p and A p in LDS, so a
+ * system has at most 4096 rows (VCLHIP_ERROR_TOO_LARGE otherwise).  A column outside a system's
+ * block sets its converged flag to 2. */
+typedef struct vclhip_batch_plan vclhip_batch_plan;
+/* Launch order and lanes-per-row choice of a batch (host arrays sys_ptr, num_systems+1 entries,
+ * and optionally row_ptr for the untuned heuristic).  Allocates device memory. */
+int vclhip_batch_plan_create(uint32_t num_systems, const uint32_t* sys_ptr_host, const uint32_t* row_ptr_host,
+                             vclhip_batch_plan** plan);
+int vclhip_batch_plan_destroy(vclhip_batch_plan* plan);
+/* Autotuner: for every size class (floor(log2 rows)) times 1, 2, 4, 8 and 16 lanes per row on up to
+ * 512 of the batch's own systems (12 iterations, 3 repetitions) and keeps the fastest, for the
+ * value type given (single_precision = 0: double, 1: float).  A is the batch matrix (values of
+ * that type). */
+int vclhip_batch_plan_autotune(vclhip_batch_plan* plan, const vclhip_csr* A, const uint32_t* sys_ptr_dev,
+                               int single_precision, void* stream);
+/* The tuned log2(lanes per row) per size class (16 entries, -1 = untuned). */
+int vclhip_batch_plan_lanes(const vclhip_batch_plan* plan, int single_precision, int32_t* lanes_log2_by_class);
+/* Solves every system: per-system relative tolerance params->tol (and abs_tol) on the recursive
+ * residual, params->max_iters; iters/errors/converged are device arrays of num_systems entries
+ * (errors: sqrt(<r,r>/||b_i||^2)).  params->use_graph with a non-null stream captures the launch as
+ * a hipGraph on first use and replays it while the arguments stay the same.  Asynchronous on the
+ * stream. */
+int vclhip_batched_cg_solve_f64(vclhip_batch_plan* plan, const vclhip_csr* A, const uint32_t* sys_ptr_dev,
+                                const double* b_dev, double* x_dev, const vclhip_solver_params* params,
+                                uint32_t* iters, double* errors, uint32_t* converged, void* stream);
+int vclhip_batched_cg_solve_f32(vclhip_batch_plan* plan, const vclhip_csr* A, const uint32_t* sys_ptr_dev,
+                                const float* b_dev, float* x_dev, const vclhip_solver_params* params,
+                                uint32_t* iters, double* errors, uint32_t* converged, void* stream);
 
 /* ---------------------------------------------------------------- tools ---- */
 /* Project RNG: std::mt19937_64(seed), u = (draw >> 11) * 2^-53; U[a,b] = a + (b-a) u; normals by

@@ -56,7 +56,9 @@ _ORC_SIGS = {
     "orc_bicgstab_pipelined_sell": (_Res, [P(_Sell), vp, vp, f64, f64, u32, vp]),
     "orc_bicgstab_classic_csr": (_Res, [P(_Csr), vp, vp, vp, f64, f64, u32, u32, vp]),
     "orc_mixed_cg_csr": (_Res, [P(_Csr), vp, vp, f64, u32, f32, P(u32)]),
+    "orc_mixed_cg2_csr": (_Res, [P(_Csr), vp, vp, vp, f64, u32, f32, i32, i32, P(u32), vp]),
     "orc_batched_cg_csr": (None, [u32, vp, P(_Csr), vp, vp, f64, u32, vp, vp]),
+    "orc_batched_cg2_csr": (None, [u32, vp, P(_Csr), vp, vp, f64, u32, i32, vp, vp]),
     "orc_adjoint_grad_csr": (None, [u32, vp, vp, vp, vp, vp]),
 }
 
@@ -280,14 +282,29 @@ def mixed_cg(A, b, tol, max_iters, inner_tol=1e-2):
     return x, r.iters, r.error, rest.value
 
 
-def batched_cg(sys_ptr, A, b, tol, max_iters):
+def mixed_cg2(A, b, tol, max_iters, inner_tol=1e-2, mode=1, acc64=True, diag=None, history=False):
+    """Mixed-precision CG of the HIP path (mode 0: reference schedule, 1: reliable updates).
+    Returns x, inner iterations, final FP64 relative residual, FP64 corrections, history."""
+    b = c64(b)
+    x = np.zeros_like(b)
+    d = c64(diag) if diag is not None else None
+    h = np.zeros(max(max_iters, 1)) if history else None
+    upd = u32()
+    r = orc().orc_mixed_cg2_csr(C.byref(A.desc), p(b), p(x), p(d), tol, max_iters, inner_tol, int(mode),
+                                int(bool(acc64)), C.byref(upd), p(h))
+    return x, r.iters, r.error, upd.value, (h[: r.iters] if history else None)
+
+
+def batched_cg(sys_ptr, A, b, tol, max_iters, single=False):
+    """Per-system pipelined CG of a block-diagonal CSR (single: FP32 vectors, FP64 accumulation)."""
     sys_ptr = cu32(sys_ptr)
     b = c64(b)
     x = np.zeros_like(b)
     ns = sys_ptr.size - 1
     iters = np.zeros(ns, np.uint32)
     errs = np.zeros(ns)
-    orc().orc_batched_cg_csr(ns, p(sys_ptr), C.byref(A.desc), p(b), p(x), tol, max_iters, p(iters), p(errs))
+    orc().orc_batched_cg2_csr(ns, p(sys_ptr), C.byref(A.desc), p(b), p(x), tol, max_iters, int(bool(single)),
+                              p(iters), p(errs))
     return x, iters, errs
 
 

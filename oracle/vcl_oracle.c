@@ -776,8 +776,23 @@ orc_result orc_bicgstab_classic_csr(const orc_csr64* A, const double* b, double*
 }
 
 /* mixed_precision_cg.hpp:95-187 */
-orc_result orc_mixed_cg_csr(const orc_csr64* A, const double* b, double* x, double tol, uint32_t max_iters,
-                            float inner_tol, uint32_t* restarts)
+/* Inner product of FP32 vectors accumulated in double (sequential). */
+static double dot32_acc64(size_t n, const float* x, const float* y)
+{
+  double t = 0;
+  for (size_t i = 0; i < n; ++i) t += (double)(x[i] * y[i]);
+  return t;
+}
+
+static float dot32(size_t n, const float* x, const float* y, int acc64)
+{
+  return acc64 ? (float)dot32_acc64(n, x, y) : orc_dot_f32(n, x, y);
+}
+
+/* mixed_precision_cg.hpp:95-187 (mode 0) and the reliable-update variant (mode 1). */
+orc_result orc_mixed_cg2_csr(const orc_csr64* A, const double* b, double* x, const double* diag, double tol,
+                             uint32_t max_iters, float inner_tol, int mode, int acc64, uint32_t* updates,
+                             double* history)
 {
   orc_result res = {0, 0.0};
   size_t n = A->n_rows;
@@ -789,59 +804,216 @@ orc_result orc_mixed_cg_csr(const orc_csr64* A, const double* b, double* x, doub
   double ip_rr = orc_dot_f64(n, b, b);
   double new_ip_rr = 0;
   double norm_rhs_squared = ip_rr;
+  if (updates) *updates = 0;
   if (norm_rhs_squared <= 0) {
     free(residual);
-    if (restarts) *restarts = 0;
     return res;
   }
   float* r_lo = (float*)malloc(n * sizeof(float));
   float* x_lo = (float*)calloc(n, sizeof(float));
   float* p_lo = (float*)malloc(n * sizeof(float));
   float* t_lo = (float*)malloc(n * sizeof(float));
+  float* d_lo = diag ? (float*)malloc(n * sizeof(float)) : NULL;
   float* val_lo = (float*)malloc((size_t)nnz * sizeof(float));
-  float inner_ip_rr = (float)ip_rr;
-  float new_inner_ip_rr = 0;
-  float initial_inner_rhs_norm_squared = (float)ip_rr;
-  for (size_t k = 0; k < n; ++k) p_lo[k] = (float)b[k];
-  memcpy(r_lo, p_lo, n * sizeof(float));
   for (size_t k = 0; k < nnz; ++k) val_lo[k] = (float)A->val[k];
+  if (d_lo)
+    for (size_t k = 0; k < n; ++k) d_lo[k] = (float)diag[k];
 
-  for (uint32_t i = 0; i < max_iters; ++i) {
-    res.iters = i + 1;
+  if (mode == 0) {
+    float inner_ip_rr = (float)ip_rr;
+    float new_inner_ip_rr = 0;
+    float initial_inner_rhs_norm_squared = (float)ip_rr;
+    for (size_t k = 0; k < n; ++k) p_lo[k] = (float)b[k];
+    memcpy(r_lo, p_lo, n * sizeof(float));
+    for (uint32_t i = 0; i < max_iters; ++i) {
+      res.iters = i + 1;
+      orc_csr_spmv_f32((uint32_t)n, A->row_ptr, A->col, val_lo, p_lo, 0, 1, 1.0f, t_lo, 0, 1, 0.0f);
+      float alpha = inner_ip_rr / dot32(n, t_lo, p_lo, acc64);
+      for (size_t k = 0; k < n; ++k) x_lo[k] = x_lo[k] + p_lo[k] * alpha;
+      for (size_t k = 0; k < n; ++k) r_lo[k] = r_lo[k] + t_lo[k] * (-alpha);
+      new_inner_ip_rr = dot32(n, r_lo, r_lo, acc64);
+      if (history) history[i] = new_inner_ip_rr;
+      float beta = new_inner_ip_rr / inner_ip_rr;
+      inner_ip_rr = new_inner_ip_rr;
+      for (size_t k = 0; k < n; ++k) p_lo[k] = r_lo[k] + p_lo[k] * beta;
+
+      if (new_inner_ip_rr < inner_tol * initial_inner_rhs_norm_squared || i == max_iters - 1) {
+        ++nrest;
+        for (size_t k = 0; k < n; ++k) residual[k] = (double)x_lo[k];
+        for (size_t k = 0; k < n; ++k) x[k] = x[k] + residual[k];
+        orc_csr_spmv_f64((uint32_t)n, A->row_ptr, A->col, A->val, x, 0, 1, 1.0, residual, 0, 1, 0.0);
+        for (size_t k = 0; k < n; ++k) residual[k] = b[k] - residual[k];
+        new_ip_rr = orc_dot_f64(n, residual, residual);
+        if (new_ip_rr / norm_rhs_squared < tol * tol)
+          break;
+        for (size_t k = 0; k < n; ++k) p_lo[k] = (float)residual[k];
+        memset(x_lo, 0, n * sizeof(float));
+        memcpy(r_lo, p_lo, n * sizeof(float));
+        initial_inner_rhs_norm_squared = (float)new_ip_rr;
+        inner_ip_rr = (float)new_ip_rr;
+      }
+    }
+  } else {
+    /* Pipelined CG in FP32 (the order of the HIP kernels: update, product, scalars). */
+    const float* dd = d_lo;
+    for (size_t k = 0; k < n; ++k) {
+      r_lo[k] = (float)b[k];
+      p_lo[k] = dd ? r_lo[k] / dd[k] : r_lo[k];
+    }
+    double rz = 0, rr = 0;
+    for (size_t k = 0; k < n; ++k) {
+      rz += (double)(r_lo[k] * p_lo[k]);
+      rr += (double)(r_lo[k] * r_lo[k]);
+    }
+    double rr_ref = rr;
     orc_csr_spmv_f32((uint32_t)n, A->row_ptr, A->col, val_lo, p_lo, 0, 1, 1.0f, t_lo, 0, 1, 0.0f);
-    float alpha = inner_ip_rr / orc_dot_f32(n, t_lo, p_lo);
-    for (size_t k = 0; k < n; ++k) x_lo[k] = x_lo[k] + p_lo[k] * alpha;
-    for (size_t k = 0; k < n; ++k) r_lo[k] = r_lo[k] + t_lo[k] * (-alpha);
-    new_inner_ip_rr = orc_dot_f32(n, r_lo, r_lo);
-    float beta = new_inner_ip_rr / inner_ip_rr;
-    inner_ip_rr = new_inner_ip_rr;
-    for (size_t k = 0; k < n; ++k) p_lo[k] = r_lo[k] + p_lo[k] * beta;
-
-    if (new_inner_ip_rr < inner_tol * initial_inner_rhs_norm_squared || i == max_iters - 1) {
-      ++nrest;
-      for (size_t k = 0; k < n; ++k) residual[k] = (double)x_lo[k];
-      for (size_t k = 0; k < n; ++k) x[k] = x[k] + residual[k];
-      orc_csr_spmv_f64((uint32_t)n, A->row_ptr, A->col, A->val, x, 0, 1, 1.0, residual, 0, 1, 0.0);
-      for (size_t k = 0; k < n; ++k) residual[k] = b[k] - residual[k];
-      new_ip_rr = orc_dot_f64(n, residual, residual);
-      if (new_ip_rr / norm_rhs_squared < tol * tol)
-        break;
-      for (size_t k = 0; k < n; ++k) p_lo[k] = (float)residual[k];
-      memset(x_lo, 0, n * sizeof(float));
-      memcpy(r_lo, p_lo, n * sizeof(float));
-      initial_inner_rhs_norm_squared = (float)new_ip_rr;
-      inner_ip_rr = (float)new_ip_rr;
+    double ApAp = 0, pAp = 0;
+    for (size_t k = 0; k < n; ++k) {
+      float s = t_lo[k];
+      ApAp += (double)(dd ? s * (s / dd[k]) : s * s);
+      pAp += (double)(p_lo[k] * s);
+    }
+    double alpha = rz / pAp;
+    double beta = (alpha * alpha * ApAp - rz) / rz;
+    for (uint32_t i = 0; i < max_iters; ++i) {
+      res.iters = i + 1;
+      const float fa = (float)alpha, fb = (float)beta;
+      rz = 0;
+      rr = 0;
+      for (size_t k = 0; k < n; ++k) {
+        x_lo[k] += fa * p_lo[k];
+        r_lo[k] -= fa * t_lo[k];
+        float z = dd ? r_lo[k] / dd[k] : r_lo[k];
+        p_lo[k] = z + fb * p_lo[k];
+        rz += (double)(r_lo[k] * z);
+        rr += (double)(r_lo[k] * r_lo[k]);
+      }
+      orc_csr_spmv_f32((uint32_t)n, A->row_ptr, A->col, val_lo, p_lo, 0, 1, 1.0f, t_lo, 0, 1, 0.0f);
+      ApAp = 0;
+      pAp = 0;
+      for (size_t k = 0; k < n; ++k) {
+        float s = t_lo[k];
+        ApAp += (double)(dd ? s * (s / dd[k]) : s * s);
+        pAp += (double)(p_lo[k] * s);
+      }
+      if (history) history[i] = rr;
+      if (rr < (double)inner_tol * rr_ref || rr / norm_rhs_squared < tol * tol || i == max_iters - 1) {
+        ++nrest;
+        for (size_t k = 0; k < n; ++k) {
+          x[k] = x[k] + (double)x_lo[k];
+          x_lo[k] = 0.0f;
+        }
+        orc_csr_spmv_f64((uint32_t)n, A->row_ptr, A->col, A->val, x, 0, 1, 1.0, residual, 0, 1, 0.0);
+        new_ip_rr = 0;
+        for (size_t k = 0; k < n; ++k) {
+          residual[k] = b[k] - residual[k];
+          new_ip_rr += residual[k] * residual[k];
+        }
+        if (new_ip_rr / norm_rhs_squared < tol * tol || i == max_iters - 1)
+          break;
+        rz = 0;
+        rr = 0;
+        for (size_t k = 0; k < n; ++k) {
+          float rn = (float)residual[k];
+          float zo = dd ? r_lo[k] / dd[k] : r_lo[k];
+          float zn = dd ? rn / dd[k] : rn;
+          p_lo[k] = p_lo[k] + (zn - zo);
+          r_lo[k] = rn;
+          rz += (double)(rn * zn);
+          rr += (double)(rn * rn);
+        }
+        rr_ref = rr;
+        orc_csr_spmv_f32((uint32_t)n, A->row_ptr, A->col, val_lo, p_lo, 0, 1, 1.0f, t_lo, 0, 1, 0.0f);
+        ApAp = 0;
+        pAp = 0;
+        for (size_t k = 0; k < n; ++k) {
+          float s = t_lo[k];
+          ApAp += (double)(dd ? s * (s / dd[k]) : s * s);
+          pAp += (double)(p_lo[k] * s);
+        }
+      }
+      alpha = rz / pAp;
+      beta = (alpha * alpha * ApAp - rz) / rz;
     }
   }
   res.error = sqrt(new_ip_rr / norm_rhs_squared);
-  if (restarts) *restarts = nrest;
-  free(residual); free(r_lo); free(x_lo); free(p_lo); free(t_lo); free(val_lo);
+  if (updates) *updates = nrest;
+  free(residual); free(r_lo); free(x_lo); free(p_lo); free(t_lo); free(val_lo); free(d_lo);
   return res;
 }
 
+orc_result orc_mixed_cg_csr(const orc_csr64* A, const double* b, double* x, double tol, uint32_t max_iters,
+                            float inner_tol, uint32_t* restarts)
+{
+  return orc_mixed_cg2_csr(A, b, x, NULL, tol, max_iters, inner_tol, 0, 0, restarts, NULL);
+}
+
 /* Independent classic CG per diagonal block (cg.hpp:257-322 with no preconditioner). */
+/* Pipelined CG (cg.hpp:128-187) on FP32 vectors, inner products accumulated in double. */
+static orc_result cg_pipelined_f32(uint32_t n, const uint32_t* rp, const uint32_t* cl, const float* val,
+                                   const float* b, float* x, double tol, uint32_t max_iters)
+{
+  orc_result res = {0, 0.0};
+  float* r = (float*)malloc((n ? n : 1) * sizeof(float));
+  float* p = (float*)malloc((n ? n : 1) * sizeof(float));
+  float* Ap = (float*)malloc((n ? n : 1) * sizeof(float));
+  double bb = 0, ApAp = 0, pAp = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    x[i] = 0.0f;
+    r[i] = b[i];
+    p[i] = b[i];
+    bb += (double)(b[i] * b[i]);
+  }
+  orc_csr_spmv_f32(n, rp, cl, val, p, 0, 1, 1.0f, Ap, 0, 1, 0.0f);
+  for (uint32_t i = 0; i < n; ++i) {
+    ApAp += (double)(Ap[i] * Ap[i]);
+    pAp += (double)(p[i] * Ap[i]);
+  }
+  double nrm = sqrt(bb);
+  double nrs = nrm * nrm;
+  double rho = nrs;
+  if (nrs > 0) {
+    double alpha = nrs / pAp;
+    double nap = sqrt(ApAp);
+    double beta = (alpha * alpha * nap * nap - nrs) / nrs;
+    for (uint32_t it = 0; it < max_iters; ++it) {
+      float ta = (float)alpha, tb = (float)beta;
+      double rr = 0;
+      for (uint32_t i = 0; i < n; ++i) {
+        x[i] += ta * p[i];
+        r[i] -= ta * Ap[i];
+        p[i] = r[i] + tb * p[i];
+        rr += (double)(r[i] * r[i]);
+      }
+      orc_csr_spmv_f32(n, rp, cl, val, p, 0, 1, 1.0f, Ap, 0, 1, 0.0f);
+      ApAp = 0;
+      pAp = 0;
+      for (uint32_t i = 0; i < n; ++i) {
+        ApAp += (double)(Ap[i] * Ap[i]);
+        pAp += (double)(p[i] * Ap[i]);
+      }
+      res.iters = it + 1;
+      rho = rr;
+      if (fabs(rho / nrs) < tol * tol) break;
+      alpha = rho / pAp;
+      beta = (alpha * alpha * ApAp - rho) / rho;
+    }
+  }
+  res.error = nrs > 0 ? sqrt(fabs(rho) / nrs) : 0.0;
+  free(r); free(p); free(Ap);
+  return res;
+}
+
+/* Independent pipelined CG (cg.hpp:128-187) per diagonal block; single != 0 runs it on FP32 copies
+ * of the values and b (inner products accumulated in double) and returns x rounded back. */
 void orc_batched_cg_csr(uint32_t num_systems, const uint32_t* sys_ptr, const orc_csr64* A, const double* b,
                         double* x, double tol, uint32_t max_iters, uint32_t* iters, double* errors)
+{
+  orc_batched_cg2_csr(num_systems, sys_ptr, A, b, x, tol, max_iters, 0, iters, errors);
+}
+
+void orc_batched_cg2_csr(uint32_t num_systems, const uint32_t* sys_ptr, const orc_csr64* A, const double* b,
+                         double* x, double tol, uint32_t max_iters, int single, uint32_t* iters, double* errors)
 {
   for (uint32_t s = 0; s < num_systems; ++s) {
     uint32_t r0 = sys_ptr[s], r1 = sys_ptr[s + 1];
@@ -852,8 +1024,20 @@ void orc_batched_cg_csr(uint32_t num_systems, const uint32_t* sys_ptr, const orc
     uint32_t* cl = (uint32_t*)malloc((nnz ? nnz : 1) * sizeof(uint32_t));
     for (uint32_t k = 0; k <= n; ++k) rp[k] = A->row_ptr[r0 + k] - base;
     for (uint32_t k = 0; k < nnz; ++k) cl[k] = A->col[base + k] - r0;
-    orc_csr64 sub = {n, rp, cl, A->val + base};
-    orc_result res = orc_cg_classic_csr(&sub, b + r0, x + r0, NULL, tol, 0.0, max_iters, NULL);
+    orc_result res;
+    if (single) {
+      float* v32 = (float*)malloc((nnz ? nnz : 1) * sizeof(float));
+      float* b32 = (float*)malloc((n ? n : 1) * sizeof(float));
+      float* x32 = (float*)malloc((n ? n : 1) * sizeof(float));
+      for (uint32_t k = 0; k < nnz; ++k) v32[k] = (float)A->val[base + k];
+      for (uint32_t k = 0; k < n; ++k) b32[k] = (float)b[r0 + k];
+      res = cg_pipelined_f32(n, rp, cl, v32, b32, x32, tol, max_iters);
+      for (uint32_t k = 0; k < n; ++k) x[r0 + k] = (double)x32[k];
+      free(v32); free(b32); free(x32);
+    } else {
+      orc_csr64 sub = {n, rp, cl, A->val + base};
+      res = orc_cg_pipelined_csr(&sub, b + r0, x + r0, tol, 0.0, max_iters, NULL);
+    }
     iters[s] = res.iters;
     errors[s] = res.error;
     free(rp);

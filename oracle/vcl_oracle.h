@@ -146,12 +146,31 @@ orc_result orc_bicgstab_classic_csr(const orc_csr64* A, const double* b, double*
 /* mixed_precision_cg.hpp:95-187: FP32 classic CG with FP64 defect-correction restarts. */
 orc_result orc_mixed_cg_csr(const orc_csr64* A, const double* b, double* x, double tol, uint32_t max_iters,
                             float inner_tol, uint32_t* restarts);
+/* Mixed-precision CG of the HIP path.
+ *   mode 0: the schedule of mixed_precision_cg.hpp:95-187 (FP32 classic CG, restart of the Krylov
+ *           space with an FP64 defect correction when <r,r> has fallen by inner_tol); acc64 != 0
+ *           accumulates the FP32 inner products in double (the HIP kernels do), acc64 == 0 is
+ *           orc_mixed_cg_csr.  diag must be NULL.
+ *   mode 1: reliable updates.  FP32 pipelined CG (cg.hpp:128-187 recurrences, Jacobi by
+ *           diag != NULL as <r, D^-1 r>) with FP64-accumulated inner products; when the FP32
+ *           recursive ||r||^2 has fallen by inner_tol since the last update, or says the tolerance is
+ *           met, x += x32 and r = b - A x are formed in FP64, ||b - A x|| is tested, r32 is replaced
+ *           by the true residual and p is corrected by D^-1 (r_true - r32); the search direction is
+ *           kept.  No reference counterpart.
+ * history (optional, max_iters entries): FP32 recursive <r,r> per iteration.
+ * updates receives the number of FP64 corrections. */
+orc_result orc_mixed_cg2_csr(const orc_csr64* A, const double* b, double* x, const double* diag, double tol,
+                             uint32_t max_iters, float inner_tol, int mode, int acc64, uint32_t* updates,
+                             double* history);
 
-/* Batched CG: independent classic CG (cg.hpp:257-322, no preconditioner) per system i
- * on rows [sys_ptr[i], sys_ptr[i+1]) of a block-diagonal CSR with global column indices.
- * iters/errors hold one entry per system. */
+/* Batched CG: independent pipelined CG (cg.hpp:128-187) per system i on rows
+ * [sys_ptr[i], sys_ptr[i+1]) of a block-diagonal CSR with global column indices.
+ * iters/errors hold one entry per system.  No reference counterpart. */
 void orc_batched_cg_csr(uint32_t num_systems, const uint32_t* sys_ptr, const orc_csr64* A, const double* b,
                         double* x, double tol, uint32_t max_iters, uint32_t* iters, double* errors);
+/* single != 0: the same on FP32 copies of the values and b, inner products accumulated in double. */
+void orc_batched_cg2_csr(uint32_t num_systems, const uint32_t* sys_ptr, const orc_csr64* A, const double* b,
+                         double* x, double tol, uint32_t max_iters, int single, uint32_t* iters, double* errors);
 
 /* Adjoint of x = A^{-1} b on the CSR pattern: dA[k] = -lambda[row(k)] * x[col(k)]. */
 void orc_adjoint_grad_csr(uint32_t n_rows, const uint32_t* row_ptr, const uint32_t* col,

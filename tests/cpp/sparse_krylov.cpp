@@ -15,6 +15,7 @@
 #include "viennacl/linalg/cg.hpp"
 #include "viennacl/linalg/inner_prod.hpp"
 #include "viennacl/linalg/jacobi_precond.hpp"
+#include "viennacl/linalg/mixed_precision_cg.hpp"
 #include "viennacl/linalg/norm_2.hpp"
 #include "viennacl/linalg/prod.hpp"
 #include "viennacl/sliced_ell_matrix.hpp"
@@ -284,6 +285,63 @@ void test_solvers(double tol_solver, double tol_check)
   }
 }
 
+void test_mixed_precision()
+{
+  // 2-D variable-coefficient problem: diagonal entries vary, so Jacobi changes the iteration
+  viennacl::compressed_matrix<double> A;
+  viennacl::tools::generate_fdm_laplace(A, 60, 50);
+  const unsigned n = 60 * 50;
+  StlMatrix<double> hA(n);
+  viennacl::copy(A, hA);
+  for (unsigned i = 0; i < n; ++i)
+    for (auto& kv : hA[i]) kv.second *= (1.0 + 0.5 * std::sin(0.01 * i)) * (1.0 + 0.5 * std::sin(0.01 * kv.first));
+  for (unsigned i = 0; i < n; ++i) {
+    double s = 0;
+    for (auto const& kv : hA[i]) if (kv.first != i) s += std::fabs(kv.second);
+    hA[i][i] = s + 0.01;
+  }
+  viennacl::copy(hA, A);
+  viennacl::sliced_ell_matrix<double> S;
+  viennacl::copy(hA, S);
+  std::mt19937_64 g(0);
+  std::uniform_real_distribution<double> u(-1, 1);
+  std::vector<double> b(n);
+  for (auto& v : b) v = u(g);
+  viennacl::vector<double> vb(n);
+  viennacl::copy(b, vb);
+
+  using namespace viennacl::linalg;
+  mixed_precision_cg_tag tag(1e-10, 5000);
+  viennacl::vector<double> x = solve(A, vb, tag);
+  double rr = true_relres(hA, b, to_host(x));
+  CHECK(rr < 1e-10 && tag.error() < 1e-10 && tag.corrections() >= 2, "mixed cg: relres %g iters %u corr %u", rr,
+        tag.iters(), tag.corrections());
+  mixed_precision_cg_tag tag1(1e-10, 5000);
+  tag1.reliable_updates(true);
+  viennacl::vector<double> x1 = solve(S, vb, tag1);
+  double rr1 = true_relres(hA, b, to_host(x1));
+  CHECK(rr1 < 1e-10 && tag1.iters() < tag.iters(), "reliable mixed cg: relres %g iters %u vs %u", rr1, tag1.iters(),
+        tag.iters());
+  jacobi_precond<viennacl::compressed_matrix<double> > J(A, jacobi_tag());
+  mixed_precision_cg_tag tag2(1e-10, 5000);
+  tag2.reliable_updates(true);
+  viennacl::vector<double> x2 = solve(A, vb, tag2, J);
+  CHECK(true_relres(hA, b, to_host(x2)) < 1e-10, "jacobi mixed cg");
+  bool threw = false;
+  try {
+    solve(A, vb, tag, J);
+  } catch (std::invalid_argument const&) {
+    threw = true;
+  }
+  CHECK(threw, "reference schedule with a preconditioner must throw");
+  cg_tag t64(1e-12, 5000);
+  viennacl::vector<double> x64 = solve(A, vb, t64);
+  std::vector<double> h64 = to_host(x64), h1 = to_host(x1);
+  double num = 0, den = 0;
+  for (unsigned i = 0; i < n; ++i) { num += (h1[i] - h64[i]) * (h1[i] - h64[i]); den += h64[i] * h64[i]; }
+  CHECK(std::sqrt(num / den) < 1e-7, "mixed vs FP64 solution %g", std::sqrt(num / den));
+}
+
 int main()
 {
   int ndev = 0;
@@ -295,6 +353,7 @@ int main()
   test_spmv_and_vectors<float>(1e-5);
   test_solvers<double>(1e-10, 1e-9);
   test_solvers<float>(1e-5, 1e-4);
+  test_mixed_precision();
   if (failures) {
     std::printf("%d FAILED\n", failures);
     return 1;

@@ -113,7 +113,7 @@ __device__ __forceinline__ void for_packs(uint32_t begin, uint32_t end, F&& f)
     for (uint32_t i = packed_end + threadIdx.x; i < end; i += kBlock) f(i, std::integral_constant<int, 1>());
 }
 
-// W-wide loads and stores (W = 1, 2 for double; 1, 2, 4 for float).
+// W-wide loads and stores (W = 1, 2, 4 for double as 16-byte halves; 1, 2, 4 for float).
 template <int W, typename T>
 struct Pack {
   T v[W];
@@ -127,6 +127,13 @@ __device__ __forceinline__ Pack<W, T> ldp(const T* __restrict__ p, uint32_t i)
     const double2 q = *reinterpret_cast<const double2*>(p + i);
     r.v[0] = q.x;
     r.v[1] = q.y;
+  } else if constexpr (W == 4 && sizeof(T) == 8) {
+    const double2 q0 = *reinterpret_cast<const double2*>(p + i);
+    const double2 q1 = *reinterpret_cast<const double2*>(p + i + 2);
+    r.v[0] = q0.x;
+    r.v[1] = q0.y;
+    r.v[2] = q1.x;
+    r.v[3] = q1.y;
   } else if constexpr (W == 2 && sizeof(T) == 4) {
     const float2 q = *reinterpret_cast<const float2*>(p + i);
     r.v[0] = q.x;
@@ -149,6 +156,9 @@ __device__ __forceinline__ void stp(T* __restrict__ p, uint32_t i, const Pack<W,
 {
   if constexpr (W == 2 && sizeof(T) == 8) {
     *reinterpret_cast<double2*>(p + i) = make_double2(r.v[0], r.v[1]);
+  } else if constexpr (W == 4 && sizeof(T) == 8) {
+    *reinterpret_cast<double2*>(p + i) = make_double2(r.v[0], r.v[1]);
+    *reinterpret_cast<double2*>(p + i + 2) = make_double2(r.v[2], r.v[3]);
   } else if constexpr (W == 2 && sizeof(T) == 4) {
     *reinterpret_cast<float2*>(p + i) = make_float2(r.v[0], r.v[1]);
   } else if constexpr (W == 4 && sizeof(T) == 4) {

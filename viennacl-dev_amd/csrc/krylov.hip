@@ -23,6 +23,12 @@ namespace vclhip {
 enum : uint32_t {
   kFlagDone = 1u,       // every kernel of the solve returns at once
   kFlagFinishing = 2u,  // BiCGStab: the final vector update is pending, everything else stops
+  // Mixed-precision CG.  An iteration that asks for an FP64 correction sets Pause (the remaining
+  // iterations of the chunk return at once) and clears NoPause (the correction kernels at the end of
+  // the chunk run).  The correction clears TailIdle while it refreshes the FP32 state.
+  kFlagPause = 4u,
+  kFlagNoPause = 8u,
+  kFlagTailIdle = 16u,
 };
 
 constexpr uint32_t kSlots = 8;  // partial-sum slots of kMaxPartials doubles each
@@ -42,6 +48,12 @@ struct KState {
   uint32_t final_mode;        // BiCGStab: 0 full update, 1 converged (x only), 2 last iteration (full)
   uint32_t rr_from_partials;  // BiCGStab: <r,r0*> comes from the previous vector update's partials
   uint32_t breakdown;
+  // mixed-precision CG
+  double rr2;       // FP32 recursive <r,r>
+  double rr_ref;    // <r,r> after the last FP64 correction (reliable update)
+  float inner, initial;  // reference schedule: FP32 <r,r> and its value at the last restart
+  float inner_tol;
+  uint32_t updates;      // FP64 corrections done
 };
 
 __host__ __device__ __forceinline__ double* slot_ptr(double* part, uint32_t slot) { return part + slot * kMaxPartials; }
@@ -773,13 +785,19 @@ struct EventPair {
   }
 };
 
+struct NoTail {
+  int operator()() const { return VCLHIP_SUCCESS; }
+};
+
 // Enqueues iterations in chunks and polls the state of the previous chunk.  launch_iter(i) enqueues
 // iteration i (0-based); needs_hook(i) says whether iteration i needs extra work before it (a
-// restart), which keeps it out of a captured graph; hook(i) enqueues that work.
-template <class LaunchIter, class NeedsHook, class Hook>
+// restart), which keeps it out of a captured graph; hook(i) enqueues that work.  tail() enqueues
+// work that closes every chunk (the mixed-precision defect correction, whose kernels return at
+// once unless an iteration of the chunk asked for it).
+template <class LaunchIter, class NeedsHook, class Hook, class Tail = NoTail>
 static int run_iterations(hipStream_t s, KState* st_dev, uint32_t first_iter, uint32_t max_iters,
                           const vclhip_solver_params* P, const void* x_dev, LaunchIter&& launch_iter,
-                          NeedsHook&& needs_hook, Hook&& hook, KState& final_state, bool bicg)
+                          NeedsHook&& needs_hook, Hook&& hook, KState& final_state, bool bicg, Tail tail = Tail())
 {
   PinnedState pin;
   VCLHIP_HIP(hipHostMalloc(reinterpret_cast<void**>(&pin.h), 2 * sizeof(KState), hipHostMallocDefault));
@@ -817,6 +835,7 @@ static int run_iterations(hipStream_t s, KState* st_dev, uint32_t first_iter, ui
         if (status) break;
         int st2 = VCLHIP_SUCCESS;
         for (uint32_t k = 0; k < cnt && !st2; ++k) st2 = launch_iter(launched + k);
+        if (!st2) st2 = tail();
         status = hip_status(hipStreamEndCapture(s, &graph), "hipStreamEndCapture");
         if (st2) status = st2;
         if (status) break;
@@ -830,6 +849,7 @@ static int run_iterations(hipStream_t s, KState* st_dev, uint32_t first_iter, ui
         if (needs_hook(launched + k)) status = hook(launched + k);
         if (!status) status = launch_iter(launched + k);
       }
+      if (!status) status = tail();
       if (status) break;
     }
     launched += cnt;
@@ -1099,6 +1119,560 @@ static int bicgstab_solve(const M* Ain, const T* b, T* x, int precond, const vcl
   return copy_history(P, hist, fin.iter, s);
 }
 
+// ------------------------------------------------------------------ mixed-precision CG
+
+// FP64 defect: r64 = b - A x64 and <r64,r64> (mixed_precision_cg.hpp:164-168).
+struct EpiResid {
+  using acc_t = double;
+  static constexpr int K = 1;
+  const double* __restrict__ b;
+  double* __restrict__ r;
+  __device__ __forceinline__ void operator()(uint32_t row, double s, acc_t* acc) const
+  {
+    const double v = b[row] - s;
+    r[row] = v;
+    acc[0] += v * v;
+  }
+};
+
+// Partial-sum slots of the mixed solver (each kernel's inner products are adjacent slots):
+// 0 <r,z>, 1 <r,r>, 2 <Ap, D^-1 Ap>, 3 <p,Ap>; 4-6 <r,z>, <r,r>, <b,b> of the start; 7 <r64,r64>.
+constexpr uint32_t kSlotRZ = 0, kSlotRR = 1, kSlotApAp = 2, kSlotPAp = 3, kSlotInit = 4, kSlotR64 = 7;
+
+// x64 = 0; x32 = 0; r32 = (float) b; p32 = D^-1 r32; partials <r,z>, <r,r>, <b,b>.
+template <bool Jac, int W>
+__global__ __launch_bounds__(kBlock) void mixed_init_kernel(uint32_t n, const double* __restrict__ b,
+                                                            double* __restrict__ x, float* __restrict__ x32,
+                                                            float* __restrict__ r32, float* __restrict__ p32,
+                                                            const float* __restrict__ d, double* __restrict__ part)
+{
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  double acc[3] = {0, 0, 0};
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    const Pack<V, double> vb = ldp<V>(b, i);
+    Pack<V, double> zero64;
+    Pack<V, float> r, z, zero32;
+    Pack<V, float> vd;
+    if constexpr (Jac) vd = ldp<V>(d, i);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      zero64.v[k] = 0;
+      zero32.v[k] = 0;
+      r.v[k] = static_cast<float>(vb.v[k]);
+      z.v[k] = Jac ? r.v[k] / vd.v[k] : r.v[k];
+      acc[0] += static_cast<double>(r.v[k] * z.v[k]);
+      acc[1] += static_cast<double>(r.v[k] * r.v[k]);
+      acc[2] += vb.v[k] * vb.v[k];
+    }
+    stp<V>(x, i, zero64);
+    stp<V>(x32, i, zero32);
+    stp<V>(r32, i, r);
+    stp<V>(p32, i, z);
+  });
+  block_store_partials<3, double>(acc, slot_ptr(part, kSlotInit), kMaxPartials, lw);
+}
+
+// Scalars after the start (mode 1 also has Ap = A p of the first direction).
+template <bool Jac>
+__global__ __launch_bounds__(kBlock) void mixed_setup_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                             uint32_t g_vec, uint32_t g_prod, int mode)
+{
+  const double rz = block_sum_partials(slot_ptr(part, kSlotInit), g_vec);
+  const double rr = Jac ? block_sum_partials(slot_ptr(part, kSlotInit + 1), g_vec) : rz;
+  const double bb = block_sum_partials(slot_ptr(part, kSlotInit + 2), g_vec);
+  double ApAp = 0, pAp = 0;
+  if (mode == 1) {
+    ApAp = block_sum_partials(slot_ptr(part, kSlotApAp), g_prod);
+    pAp = block_sum_partials(slot_ptr(part, kSlotPAp), g_prod);
+  }
+  if (threadIdx.x != 0) return;
+  st->nrs = bb;
+  st->iter = 0;
+  st->updates = 0;
+  st->res = 0;
+  st->flags = kFlagNoPause | kFlagTailIdle;
+  if (bb <= 0) {  // x = 0 solves b = 0 (mixed_precision_cg.hpp:112-113)
+    st->flags |= kFlagDone;
+    st->converged = 1;
+    return;
+  }
+  if (mode == 0) {
+    st->inner = static_cast<float>(bb);
+    st->initial = static_cast<float>(bb);
+  } else {
+    st->rr = rz;
+    st->rr2 = rr;
+    st->rr_ref = rr;
+    const double alpha = rz / pAp;
+    st->alpha = alpha;
+    st->beta = (alpha * alpha * ApAp - rz) / rz;
+  }
+  if (st->max_iters == 0) st->flags |= kFlagDone;
+}
+
+// ---- mode 0: the reference's schedule (FP32 classic CG, restart on inner_tol)
+
+// alpha = inner / <t,p>  in FP32 (mixed_precision_cg.hpp:148).
+__global__ __launch_bounds__(kBlock) void mixed0_alpha_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                              uint32_t g_prod)
+{
+  if (st->flags & (kFlagDone | kFlagPause)) return;
+  const double pAp = block_sum_partials(slot_ptr(part, kSlotPAp), g_prod);
+  if (threadIdx.x != 0) return;
+  st->alpha = static_cast<double>(st->inner / static_cast<float>(pAp));
+  st->flags |= kFlagTailIdle;  // a restart that preceded this iteration is complete
+}
+
+// x32 += alpha p; r32 -= alpha t; partial <r,r>  (mixed_precision_cg.hpp:149-152)
+template <int W>
+__global__ __launch_bounds__(kBlock) void mixed0_xr_kernel(uint32_t n, float* __restrict__ x, float* __restrict__ r,
+                                                           const float* __restrict__ p, const float* __restrict__ t,
+                                                           const KState* __restrict__ st, double* __restrict__ part)
+{
+  if (st->flags & (kFlagDone | kFlagPause)) return;
+  const float alpha = static_cast<float>(st->alpha);
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  double acc[1] = {0};
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    Pack<V, float> vx = ldp<V>(x, i);
+    Pack<V, float> vr = ldp<V>(r, i);
+    const Pack<V, float> vp = ldp<V>(p, i);
+    const Pack<V, float> vt = ldp<V>(t, i);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      vx.v[k] = vx.v[k] + vp.v[k] * alpha;
+      vr.v[k] = vr.v[k] + vt.v[k] * (-alpha);
+      acc[0] += static_cast<double>(vr.v[k] * vr.v[k]);
+    }
+    stp<V>(x, i, vx);
+    stp<V>(r, i, vr);
+  });
+  block_store_partials<1, double>(acc, slot_ptr(part, kSlotRZ), kMaxPartials, lw);
+}
+
+// beta = new/inner in FP32; restart test (mixed_precision_cg.hpp:154-161)
+__global__ __launch_bounds__(kBlock) void mixed0_beta_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                             uint32_t g_vec)
+{
+  if (st->flags & (kFlagDone | kFlagPause)) return;
+  const double s = block_sum_partials(slot_ptr(part, kSlotRZ), g_vec);
+  if (threadIdx.x != 0) return;
+  const uint32_t iter = st->iter + 1;
+  st->iter = iter;
+  const float nw = static_cast<float>(s);
+  record_history(st, iter, static_cast<double>(nw));
+  st->beta = static_cast<double>(nw / st->inner);
+  st->inner = nw;
+  if (nw < st->inner_tol * st->initial || iter >= st->max_iters)
+    st->flags = (st->flags | kFlagPause) & ~kFlagNoPause;
+}
+
+// p32 = r32 + beta p32
+template <int W>
+__global__ __launch_bounds__(kBlock) void mixed0_p_kernel(uint32_t n, float* __restrict__ p, const float* __restrict__ r,
+                                                          const KState* __restrict__ st)
+{
+  if (st->flags & (kFlagDone | kFlagPause)) return;
+  const float beta = static_cast<float>(st->beta);
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    Pack<V, float> vp = ldp<V>(p, i);
+    const Pack<V, float> vr = ldp<V>(r, i);
+#pragma unroll
+    for (int k = 0; k < V; ++k) vp.v[k] = vr.v[k] + vp.v[k] * beta;
+    stp<V>(p, i, vp);
+  });
+}
+
+// ---- mode 1: FP32 pipelined CG with reliable updates
+
+// x32 += alpha p; r32 -= alpha Ap; z = D^-1 r; p = z + beta p; partials <r,z> (and <r,r> with Jacobi)
+template <bool Jac, int W>
+__global__ __launch_bounds__(kBlock) void mixed1_update_kernel(uint32_t n, float* __restrict__ x,
+                                                               float* __restrict__ p, float* __restrict__ r,
+                                                               const float* __restrict__ Ap,
+                                                               const float* __restrict__ d,
+                                                               const KState* __restrict__ st,
+                                                               double* __restrict__ part)
+{
+  if (st->flags & (kFlagDone | kFlagPause)) return;
+  const float alpha = static_cast<float>(st->alpha);
+  const float beta = static_cast<float>(st->beta);
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  double acc[2] = {0, 0};
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    Pack<V, float> vp = ldp<V>(p, i);
+    Pack<V, float> vr = ldp<V>(r, i);
+    Pack<V, float> vx = ldp<V>(x, i);
+    const Pack<V, float> va = ldp<V>(Ap, i);
+    Pack<V, float> vd;
+    if constexpr (Jac) vd = ldp<V>(d, i);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      vx.v[k] += alpha * vp.v[k];
+      vr.v[k] -= alpha * va.v[k];
+      float z = vr.v[k];
+      if constexpr (Jac) z = vr.v[k] / vd.v[k];
+      vp.v[k] = z + beta * vp.v[k];
+      acc[0] += static_cast<double>(vr.v[k] * z);
+      if constexpr (Jac) acc[1] += static_cast<double>(vr.v[k] * vr.v[k]);
+    }
+    stp<V>(x, i, vx);
+    stp<V>(p, i, vp);
+    stp<V>(r, i, vr);
+  });
+  if constexpr (Jac) {
+    block_store_partials<2, double>(acc, slot_ptr(part, kSlotRZ), kMaxPartials, lw);
+  } else {
+    const double a0[1] = {acc[0]};
+    block_store_partials<1, double>(a0, slot_ptr(part, kSlotRZ), kMaxPartials, lw);
+  }
+}
+
+// Scalars of one iteration; asks for an FP64 correction when the FP32 <r,r> has fallen by
+// inner_tol since the last one, when it says the tolerance is met, or at the iteration limit.
+template <bool Jac>
+__global__ __launch_bounds__(kBlock) void mixed1_iter_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                             uint32_t g_vec, uint32_t g_prod)
+{
+  if (st->flags & (kFlagDone | kFlagPause)) return;
+  const double rz = block_sum_partials(slot_ptr(part, kSlotRZ), g_vec);
+  const double rr = Jac ? block_sum_partials(slot_ptr(part, kSlotRR), g_vec) : rz;
+  const double ApAp = block_sum_partials(slot_ptr(part, kSlotApAp), g_prod);
+  const double pAp = block_sum_partials(slot_ptr(part, kSlotPAp), g_prod);
+  if (threadIdx.x != 0) return;
+  const uint32_t iter = st->iter + 1;
+  st->iter = iter;
+  st->rr = rz;
+  st->rr2 = rr;
+  record_history(st, iter, rr);
+  if (rr < static_cast<double>(st->inner_tol) * st->rr_ref || rr / st->nrs < st->tol * st->tol ||
+      iter >= st->max_iters) {
+    st->flags = (st->flags | kFlagPause) & ~kFlagNoPause;
+    return;
+  }
+  const double alpha = rz / pAp;
+  st->alpha = alpha;
+  st->beta = (alpha * alpha * ApAp - rz) / rz;
+}
+
+// ---- FP64 correction (both modes)
+
+// x64 += x32; x32 = 0  (mixed_precision_cg.hpp:163-164)
+template <int W>
+__global__ __launch_bounds__(kBlock) void mixed_fold_kernel(uint32_t n, double* __restrict__ x,
+                                                            float* __restrict__ x32, const KState* __restrict__ st)
+{
+  if (st->flags & (kFlagDone | kFlagNoPause)) return;
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    Pack<V, double> vx = ldp<V>(x, i);
+    const Pack<V, float> vs = ldp<V>(x32, i);
+    Pack<V, float> zero;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      vx.v[k] = vx.v[k] + static_cast<double>(vs.v[k]);
+      zero.v[k] = 0;
+    }
+    stp<V>(x, i, vx);
+    stp<V>(x32, i, zero);
+  });
+}
+
+// Tests ||b - A x||^2 / ||b||^2 < tol^2 in FP64 (mixed_precision_cg.hpp:167-169).
+__global__ __launch_bounds__(kBlock) void mixed_check_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                             uint32_t g_prod, int mode)
+{
+  if (st->flags & (kFlagDone | kFlagNoPause)) return;
+  const double rr64 = block_sum_partials(slot_ptr(part, kSlotR64), g_prod);
+  if (threadIdx.x != 0) return;
+  st->updates += 1;
+  st->res = rr64;
+  if (rr64 / st->nrs < st->tol * st->tol) {
+    st->flags |= kFlagDone;
+    st->converged = 1;
+    return;
+  }
+  if (st->iter >= st->max_iters) {
+    st->flags |= kFlagDone;
+    return;
+  }
+  if (mode == 0) {
+    st->initial = static_cast<float>(rr64);
+    st->inner = static_cast<float>(rr64);
+  }
+  st->flags = (st->flags & ~(kFlagPause | kFlagTailIdle)) | kFlagNoPause;
+}
+
+// mode 0: p32 = r32 = (float) r64 (mixed_precision_cg.hpp:171-174)
+template <int W>
+__global__ __launch_bounds__(kBlock) void mixed0_restart_kernel(uint32_t n, const double* __restrict__ r64,
+                                                                float* __restrict__ r32, float* __restrict__ p32,
+                                                                const KState* __restrict__ st)
+{
+  if (st->flags & (kFlagDone | kFlagTailIdle)) return;
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    const Pack<V, double> v = ldp<V>(r64, i);
+    Pack<V, float> f;
+#pragma unroll
+    for (int k = 0; k < V; ++k) f.v[k] = static_cast<float>(v.v[k]);
+    stp<V>(r32, i, f);
+    stp<V>(p32, i, f);
+  });
+}
+
+// mode 1: r32 = (float) r64; p32 += D^-1 (r_new - r_old) so that p = z + beta p_prev still holds;
+// partials <r,z>, <r,r>.
+template <bool Jac, int W>
+__global__ __launch_bounds__(kBlock) void mixed1_replace_kernel(uint32_t n, const double* __restrict__ r64,
+                                                                float* __restrict__ r32, float* __restrict__ p32,
+                                                                const float* __restrict__ d,
+                                                                const KState* __restrict__ st,
+                                                                double* __restrict__ part)
+{
+  if (st->flags & (kFlagDone | kFlagTailIdle)) return;
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  double acc[2] = {0, 0};
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    const Pack<V, double> v = ldp<V>(r64, i);
+    Pack<V, float> vr = ldp<V>(r32, i);
+    Pack<V, float> vp = ldp<V>(p32, i);
+    Pack<V, float> vd;
+    if constexpr (Jac) vd = ldp<V>(d, i);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      const float rn = static_cast<float>(v.v[k]);
+      const float zo = Jac ? vr.v[k] / vd.v[k] : vr.v[k];
+      const float zn = Jac ? rn / vd.v[k] : rn;
+      vp.v[k] = vp.v[k] + (zn - zo);
+      vr.v[k] = rn;
+      acc[0] += static_cast<double>(rn * zn);
+      acc[1] += static_cast<double>(rn * rn);
+    }
+    stp<V>(r32, i, vr);
+    stp<V>(p32, i, vp);
+  });
+  block_store_partials<2, double>(acc, slot_ptr(part, kSlotRZ), kMaxPartials, lw);
+}
+
+// mode 1: scalars of the refreshed state; ends the correction.
+__global__ __launch_bounds__(kBlock) void mixed1_resume_kernel(KState* __restrict__ st, const double* __restrict__ part,
+                                                               uint32_t g_vec, uint32_t g_prod)
+{
+  if (st->flags & (kFlagDone | kFlagTailIdle)) return;
+  const double rz = block_sum_partials(slot_ptr(part, kSlotRZ), g_vec);
+  const double rr = block_sum_partials(slot_ptr(part, kSlotRR), g_vec);
+  const double ApAp = block_sum_partials(slot_ptr(part, kSlotApAp), g_prod);
+  const double pAp = block_sum_partials(slot_ptr(part, kSlotPAp), g_prod);
+  if (threadIdx.x != 0) return;
+  st->rr = rz;
+  st->rr2 = rr;
+  st->rr_ref = rr;
+  const double alpha = rz / pAp;
+  st->alpha = alpha;
+  st->beta = (alpha * alpha * ApAp - rz) / rz;
+  st->flags |= kFlagTailIdle;
+}
+
+__global__ __launch_bounds__(kBlock) void convert_values_kernel(size_t n, const double* __restrict__ v,
+                                                                float* __restrict__ f)
+{
+  for (size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * kBlock)
+    f[i] = static_cast<float>(v[i]);
+}
+
+inline const double* values_of(const vclhip_csr* A) { return static_cast<const double*>(A->values); }
+inline const double* values_of(const vclhip_sell* A) { return static_cast<const double*>(A->elements); }
+// Number of stored values: nnz for CSR; block_start[last] + columns_per_block[last] * C for SELL
+// (read back from the device).
+inline int value_count(const vclhip_csr* A, size_t* count)
+{
+  *count = A->nnz;
+  return VCLHIP_SUCCESS;
+}
+inline int value_count(const vclhip_sell* A, size_t* count)
+{
+  *count = 0;
+  if (!A || !A->block_start || !A->columns_per_block || A->num_slices == 0) return VCLHIP_SUCCESS;
+  uint32_t start = 0, width = 0;
+  VCLHIP_HIP(hipMemcpy(&start, A->block_start + (A->num_slices - 1), sizeof(uint32_t), hipMemcpyDeviceToHost));
+  VCLHIP_HIP(hipMemcpy(&width, A->columns_per_block + (A->num_slices - 1), sizeof(uint32_t), hipMemcpyDeviceToHost));
+  *count = static_cast<size_t>(start) + static_cast<size_t>(width) * A->C;
+  return VCLHIP_SUCCESS;
+}
+inline vclhip_csr with_values(const vclhip_csr* A, const float* v)
+{
+  vclhip_csr c = *A;
+  c.values = v;
+  return c;
+}
+inline vclhip_sell with_values(const vclhip_sell* A, const float* v)
+{
+  vclhip_sell c = *A;
+  c.elements = v;
+  return c;
+}
+
+// mixed_precision_cg.hpp:95-187 on the device.  values32: FP32 copy of the matrix values, or null
+// (converted into a workspace here, as the reference converts at every solve).
+template <class M>
+static int mixed_cg_solve(const M* A, const float* values32, const double* b, double* x, int precond,
+                          const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  if (!P || !R || !b || !x) return invalid("null argument");
+  if (precond != VCLHIP_PRECOND_NONE && precond != VCLHIP_PRECOND_JACOBI) return invalid("precond");
+  const int mode = P->mixed_mode;
+  if (mode != 0 && mode != 1) return invalid("mixed_mode must be 0 or 1");
+  if (mode == 0 && precond != VCLHIP_PRECOND_NONE)
+    return invalid("mixed_mode 0 (the reference's restart schedule) has no preconditioner");
+  DevOp<double> op64;
+  VCLHIP_TRY(make_op(A, op64));
+  SolveStream ss(stream, P->use_graph != 0);
+  VCLHIP_TRY(ss.status());
+  hipStream_t s = ss.get();
+  const uint32_t n = op64.n;
+  const bool jac = precond == VCLHIP_PRECOND_JACOBI;
+  Workspace ws(s);
+  float* v32 = const_cast<float*>(values32);
+  if (!v32) {
+    size_t value_count32 = 0;
+    VCLHIP_TRY(value_count(A, &value_count32));
+    VCLHIP_TRY(ws.alloc(&v32, value_count32 ? value_count32 : 1));
+    if (value_count32) {
+      const uint32_t g = static_cast<uint32_t>(std::min<size_t>((value_count32 + kBlock - 1) / kBlock, 8192));
+      hipLaunchKernelGGL(convert_values_kernel, dim3(g), dim3(kBlock), 0, s, value_count32, values_of(A), v32);
+      VCLHIP_TRY(check_launch("convert_values_kernel"));
+    }
+  }
+  const M A32 = with_values(A, v32);
+  DevOp<float> op32;
+  VCLHIP_TRY(make_op(&A32, op32));
+
+  float *x32, *r32, *p32, *Ap32, *d32 = nullptr;
+  double *r64, *part, *hist = nullptr;
+  KState* st;
+  VCLHIP_TRY(ws.alloc(&x32, n));
+  VCLHIP_TRY(ws.alloc(&r32, n));
+  VCLHIP_TRY(ws.alloc(&p32, n));
+  VCLHIP_TRY(ws.alloc(&Ap32, n));
+  VCLHIP_TRY(ws.alloc(&r64, n));
+  VCLHIP_TRY(ws.alloc(&part, kSlots * kMaxPartials));
+  VCLHIP_TRY(ws.alloc(&st, 1));
+  if (P->history_host && P->history_len && P->max_iters) VCLHIP_TRY(ws.alloc(&hist, std::min(P->history_len, P->max_iters)));
+  if (jac) {
+    uint32_t* zc;
+    VCLHIP_TRY(ws.alloc(&d32, n));
+    VCLHIP_TRY(ws.alloc(&zc, 1));
+    VCLHIP_TRY(jacobi_diag(op32, reinterpret_cast<const vclhip_csr*>(&A32), d32, zc, s));
+  }
+  KState h;
+  init_state(h, P, hist);
+  h.inner_tol = P->inner_tol;
+  VCLHIP_TRY(upload_state(st, h, s));
+
+  const uint32_t gv = vec_grid_n(n);
+  const bool v4 = aligned16({b, x});
+  if (jac)
+    VCLHIP_LAUNCH_W(v4, (mixed_init_kernel<true, 4>), (mixed_init_kernel<true, 1>), gv, s, n, b, x, x32, r32, p32, d32,
+                    part);
+  else
+    VCLHIP_LAUNCH_W(v4, (mixed_init_kernel<false, 4>), (mixed_init_kernel<false, 1>), gv, s, n, b, x, x32, r32, p32,
+                    d32, part);
+  VCLHIP_TRY(check_launch("mixed_init_kernel"));
+  if (mode == 1) {
+    if (jac)
+      op32.launch(s, p32, EpiCg<float, true>{Ap32, p32, d32}, slot_ptr(part, kSlotApAp), nullptr, 0u);
+    else
+      op32.launch(s, p32, EpiCg<float, false>{Ap32, p32, d32}, slot_ptr(part, kSlotApAp), nullptr, 0u);
+    VCLHIP_TRY(check_launch("mixed prod"));
+  }
+  if (jac)
+    hipLaunchKernelGGL(mixed_setup_kernel<true>, dim3(1), dim3(kBlock), 0, s, st, part, gv, op32.grid, mode);
+  else
+    hipLaunchKernelGGL(mixed_setup_kernel<false>, dim3(1), dim3(kBlock), 0, s, st, part, gv, op32.grid, mode);
+  VCLHIP_TRY(check_launch("mixed_setup_kernel"));
+
+  uint32_t* flags = &st->flags;
+  const uint32_t run_mask = kFlagDone | kFlagPause;
+  auto launch_iter = [&](uint32_t) -> int {
+    if (mode == 0) {
+      op32.launch(s, p32, EpiCg<float, false>{Ap32, p32, nullptr}, slot_ptr(part, kSlotApAp), flags, run_mask);
+      hipLaunchKernelGGL(mixed0_alpha_kernel, dim3(1), dim3(kBlock), 0, s, st, part, op32.grid);
+      hipLaunchKernelGGL(mixed0_xr_kernel<4>, dim3(gv), dim3(kBlock), 0, s, n, x32, r32, p32, Ap32, st, part);
+      hipLaunchKernelGGL(mixed0_beta_kernel, dim3(1), dim3(kBlock), 0, s, st, part, gv);
+      hipLaunchKernelGGL(mixed0_p_kernel<4>, dim3(gv), dim3(kBlock), 0, s, n, p32, r32, st);
+      return check_launch("mixed CG iteration");
+    }
+    if (jac) {
+      hipLaunchKernelGGL((mixed1_update_kernel<true, 4>), dim3(gv), dim3(kBlock), 0, s, n, x32, p32, r32, Ap32, d32, st,
+                         part);
+      op32.launch(s, p32, EpiCg<float, true>{Ap32, p32, d32}, slot_ptr(part, kSlotApAp), flags, run_mask);
+      hipLaunchKernelGGL(mixed1_iter_kernel<true>, dim3(1), dim3(kBlock), 0, s, st, part, gv, op32.grid);
+    } else {
+      hipLaunchKernelGGL((mixed1_update_kernel<false, 4>), dim3(gv), dim3(kBlock), 0, s, n, x32, p32, r32, Ap32, d32,
+                         st, part);
+      op32.launch(s, p32, EpiCg<float, false>{Ap32, p32, d32}, slot_ptr(part, kSlotApAp), flags, run_mask);
+      hipLaunchKernelGGL(mixed1_iter_kernel<false>, dim3(1), dim3(kBlock), 0, s, st, part, gv, op32.grid);
+    }
+    return check_launch("mixed CG iteration");
+  };
+  auto tail = [&]() -> int {
+    if (v4)
+      hipLaunchKernelGGL(mixed_fold_kernel<4>, dim3(gv), dim3(kBlock), 0, s, n, x, x32, st);
+    else
+      hipLaunchKernelGGL(mixed_fold_kernel<1>, dim3(gv), dim3(kBlock), 0, s, n, x, x32, st);
+    op64.launch(s, x, EpiResid{b, r64}, slot_ptr(part, kSlotR64), flags, kFlagDone | kFlagNoPause);
+    hipLaunchKernelGGL(mixed_check_kernel, dim3(1), dim3(kBlock), 0, s, st, part, op64.grid, mode);
+    if (mode == 0) {
+      hipLaunchKernelGGL(mixed0_restart_kernel<4>, dim3(gv), dim3(kBlock), 0, s, n, r64, r32, p32, st);
+      return check_launch("mixed CG restart");
+    }
+    if (jac) {
+      hipLaunchKernelGGL((mixed1_replace_kernel<true, 4>), dim3(gv), dim3(kBlock), 0, s, n, r64, r32, p32, d32, st, part);
+      op32.launch(s, p32, EpiCg<float, true>{Ap32, p32, d32}, slot_ptr(part, kSlotApAp), flags,
+                  kFlagDone | kFlagTailIdle);
+    } else {
+      hipLaunchKernelGGL((mixed1_replace_kernel<false, 4>), dim3(gv), dim3(kBlock), 0, s, n, r64, r32, p32, d32, st,
+                         part);
+      op32.launch(s, p32, EpiCg<float, false>{Ap32, p32, d32}, slot_ptr(part, kSlotApAp), flags,
+                  kFlagDone | kFlagTailIdle);
+    }
+    hipLaunchKernelGGL(mixed1_resume_kernel, dim3(1), dim3(kBlock), 0, s, st, part, gv, op32.grid);
+    return check_launch("mixed CG correction");
+  };
+  KState fin;
+  VCLHIP_TRY(run_iterations(
+      s, st, 0u, P->max_iters, P, x, launch_iter, [](uint32_t) { return false; }, [](uint32_t) { return 0; }, fin,
+      false, tail));
+  R->iters = fin.iter;
+  R->error = fin.nrs > 0 ? std::sqrt(fin.res / fin.nrs) : 0.0;
+  R->restarts = fin.updates;
+  R->converged = static_cast<int>(fin.converged);
+  return copy_history(P, hist, fin.iter, s);
+}
+
 // ------------------------------------------------------------------ pipelined ops with the reference's buffer
 
 // Sums slot q over `count` partials into buf[dest[q]] and zeroes buf[dest[q]+1 .. dest[q]+chunk).
@@ -1287,6 +1861,18 @@ int vclhip_bicgstab_solve_sell_f32(const vclhip_sell* A, const float* b, float* 
                                    const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
 {
   return bicgstab_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_mixed_cg_solve_csr(const vclhip_csr* A, const float* values32, const double* b, double* x, int precond,
+                              const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return mixed_cg_solve(A, values32, b, x, precond, P, R, stream);
+}
+
+int vclhip_mixed_cg_solve_sell(const vclhip_sell* A, const float* elements32, const double* b, double* x,
+                               int precond, const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return mixed_cg_solve(A, elements32, b, x, precond, P, R, stream);
 }
 
 int vclhip_pipelined_cg_vector_update_f64(uint32_t n, double* x, double alpha, double* p, double* r, const double* Ap,

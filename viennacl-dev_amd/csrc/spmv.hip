@@ -179,6 +179,61 @@ static int row_info(const vclhip_csr* A, T* out, int info, void* stream)
   return check_launch("row_info_kernel");
 }
 
+// Gradient of a linear solve with respect to the matrix values, on the pattern of A:
+// dA[k] = -lambda[row(k)] * x[col(k)].  Walks the row-block plan; a workgroup stages the block's
+// row pointers in LDS and every lane finds the row of its entry by bisection, so the col reads and
+// dA writes are coalesced whatever the row lengths.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void csr_adjoint_grad_kernel(CsrDev<T> A, const T* __restrict__ lambda,
+                                                                  const T* __restrict__ x, T* __restrict__ dA)
+{
+  __shared__ uint32_t srp[kCsrWG + 1];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lw = xcd_logical_block();
+  uint32_t b0, b1;
+  chunk_range(A.num_blocks, lw, gridDim.x, b0, b1);
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint32_t d0 = A.blocks[b];
+    const uint32_t r0 = d0 & (kPlanRowLimit - 1);
+    const uint32_t mode = d0 >> 28;
+    const uint32_t r1 = A.blocks[b + 1] & (kPlanRowLimit - 1);
+    const uint32_t e0 = A.row_ptr[r0];
+    const uint32_t e1 = A.row_ptr[r1];
+    if (mode == kModeLongRow) {
+      const T lam = lambda[r0];
+      for (uint32_t e = e0 + tid; e < e1; e += kBlock) dA[e] = -lam * x[A.col[e]];
+      continue;
+    }
+    const uint32_t rows = min(r1 - r0, static_cast<uint32_t>(kCsrWG));
+    for (uint32_t i = tid; i <= rows; i += kBlock) srp[i] = A.row_ptr[r0 + i];
+    __syncthreads();
+    for (uint32_t e = e0 + tid; e < e1; e += kBlock) {
+      uint32_t lo = 0, hi = rows;  // srp[lo] <= e < srp[hi]
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (srp[mid] <= e)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      dA[e] = -lambda[r0 + lo] * x[A.col[e]];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+static int csr_adjoint_grad(const vclhip_csr* A, const T* lambda, const T* x, T* dA, void* stream)
+{
+  VCLHIP_TRY(check_csr<T>(A));
+  if (A->n_rows == 0 || A->nnz == 0) return VCLHIP_SUCCESS;
+  if (!lambda || !x || !dA) return invalid("null vector");
+  const uint32_t grid = chunk_grid(A->num_blocks, spmv_gmax());
+  hipLaunchKernelGGL(csr_adjoint_grad_kernel<T>, dim3(grid), dim3(kBlock), 0, static_cast<hipStream_t>(stream),
+                     csr_dev<T>(A), lambda, x, dA);
+  return check_launch("csr_adjoint_grad_kernel");
+}
+
 }  // namespace vclhip
 
 extern "C" {
@@ -215,6 +270,16 @@ int vclhip_csr_row_info_f64(const vclhip_csr* A, double* out, int info, void* st
 int vclhip_csr_row_info_f32(const vclhip_csr* A, float* out, int info, void* stream)
 {
   return vclhip::row_info<float>(A, out, info, stream);
+}
+
+int vclhip_csr_adjoint_grad_f64(const vclhip_csr* A, const double* lambda, const double* x, double* dA, void* stream)
+{
+  return vclhip::csr_adjoint_grad<double>(A, lambda, x, dA, stream);
+}
+
+int vclhip_csr_adjoint_grad_f32(const vclhip_csr* A, const float* lambda, const float* x, float* dA, void* stream)
+{
+  return vclhip::csr_adjoint_grad<float>(A, lambda, x, dA, stream);
 }
 
 }  // extern "C"

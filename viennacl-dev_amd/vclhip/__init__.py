@@ -12,7 +12,7 @@ from . import _abi
 from ._abi import (Csr, Sell, SolverParams, SolverResult, MONITOR, VclHipError, call, lib,
                    PRECOND_NONE, PRECOND_JACOBI, LIB_PATH, declared_functions)
 
-__all__ = ["DeviceArray", "Stream", "CsrMatrix", "SellMatrix", "lib", "call", "VclHipError",
+__all__ = ["DeviceArray", "Stream", "CsrMatrix", "SellMatrix", "BatchPlan", "lib", "call", "VclHipError",
            "device_count", "PRECOND_NONE", "PRECOND_JACOBI", "SolverParams", "SolverResult"]
 
 
@@ -155,6 +155,10 @@ class CsrMatrix:
     def num_blocks(self):
         return self.blocks_host.size - 1
 
+    def free(self):
+        for a in (self.d_row_ptr, self.d_col, self.d_val, self.d_blocks):
+            a.free()
+
     def spmv(self, x, y, alpha=1.0, beta=0.0, xs=0, xi=1, ys=0, yi=1, stream=None):
         fn = "vclhip_csr_spmv_f64" if self.dtype == np.float64 else "vclhip_csr_spmv_f32"
         call(fn, C.byref(self.desc), x.ptr, xs, xi, alpha, y.ptr, ys, yi, beta, stream)
@@ -222,6 +226,67 @@ def cg(A, b, x, precond=PRECOND_NONE, stream=None, **kw):
 def bicgstab(A, b, x, precond=PRECOND_NONE, stream=None, **kw):
     kw.setdefault("max_iters", 400)
     return _solve("bicgstab", A, b, x, precond, solver_params(**kw), stream)
+
+
+def mixed_cg(A, b, x, precond=PRECOND_NONE, values32=None, stream=None, **kw):
+    """solve(A, b, mixed_precision_cg_tag): FP32 inner CG, FP64 corrections (mixed_mode 0 = the
+    reference's restart schedule, 1 = reliable updates).  A holds FP64 values; values32 is an
+    optional DeviceArray with their FP32 copy.  result.restarts counts the FP64 corrections."""
+    params = solver_params(**kw)
+    res = SolverResult()
+    mat = "csr" if isinstance(A, CsrMatrix) else "sell"
+    call(f"vclhip_mixed_cg_solve_{mat}", C.byref(A.desc), values32.ptr if values32 is not None else None, b.ptr,
+         x.ptr, precond, C.byref(params), C.byref(res), stream)
+    return res
+
+
+class BatchPlan:
+    """Launch order and lanes-per-row table of a batch of systems (vclhip_batch_plan)."""
+
+    def __init__(self, sys_ptr, row_ptr=None):
+        self.sys_ptr = np.ascontiguousarray(sys_ptr, dtype=np.uint32)
+        self.num_systems = self.sys_ptr.size - 1
+        rp = None if row_ptr is None else np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        h = C.c_void_p()
+        call("vclhip_batch_plan_create", self.num_systems, _ptr(self.sys_ptr), _ptr(rp) if rp is not None else None,
+             C.byref(h))
+        self.handle = h
+        self.d_sys_ptr = DeviceArray.from_numpy(self.sys_ptr)
+
+    def autotune(self, A, single=False, stream=None):
+        call("vclhip_batch_plan_autotune", self.handle, C.byref(A.desc), self.d_sys_ptr.ptr, int(bool(single)), stream)
+
+    def lanes(self, single=False):
+        out = np.zeros(16, np.int32)
+        call("vclhip_batch_plan_lanes", self.handle, int(bool(single)), _ptr(out))
+        return out
+
+    def solve(self, A, b, x, stream=None, **kw):
+        """Batched CG of every system (device arrays b, x); returns per-system iters, errors, converged."""
+        ns = self.num_systems
+        it = DeviceArray.zeros(ns, np.uint32)
+        er = DeviceArray.zeros(ns, np.float64)
+        cv = DeviceArray.zeros(ns, np.uint32)
+        self.solve_into(A, b, x, it, er, cv, stream, **kw)
+        call("vclhip_device_synchronize")
+        return it.to_numpy(), er.to_numpy(), cv.to_numpy()
+
+    def solve_into(self, A, b, x, iters, errors, converged, stream=None, **kw):
+        params = solver_params(**kw)
+        fn = "vclhip_batched_cg_solve_f32" if A.dtype == np.float32 else "vclhip_batched_cg_solve_f64"
+        call(fn, self.handle, C.byref(A.desc), self.d_sys_ptr.ptr, b.ptr, x.ptr, C.byref(params), iters.ptr,
+             errors.ptr, converged.ptr, stream)
+
+    def close(self):
+        if self.handle:
+            call("vclhip_batch_plan_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------------ generators (host)

@@ -89,6 +89,8 @@ _SIGS = {
     "vclhip_sell_spmv_f32": (i32, [P(Sell), vp, u32, u32, f32, vp, u32, u32, f32, vp]),
     "vclhip_csr_row_info_f64": (i32, [P(Csr), vp, i32, vp]),
     "vclhip_csr_row_info_f32": (i32, [P(Csr), vp, i32, vp]),
+    "vclhip_csr_adjoint_grad_f64": (i32, [P(Csr), vp, vp, vp, vp]),
+    "vclhip_csr_adjoint_grad_f32": (i32, [P(Csr), vp, vp, vp, vp]),
     "vclhip_dot_f64": (i32, [u32, vp, u32, u32, vp, u32, u32, P(f64), vp]),
     "vclhip_dot_f32": (i32, [u32, vp, u32, u32, vp, u32, u32, P(f32), vp]),
     "vclhip_dot_f32_acc64": (i32, [u32, vp, vp, P(f64), vp]),
@@ -120,6 +122,14 @@ _SIGS = {
     "vclhip_cg_solve_sell_f32": (i32, [P(Sell), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_bicgstab_solve_csr_f32": (i32, [P(Csr), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_bicgstab_solve_sell_f32": (i32, [P(Sell), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_mixed_cg_solve_csr": (i32, [P(Csr), vp, vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_mixed_cg_solve_sell": (i32, [P(Sell), vp, vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_batch_plan_create": (i32, [u32, vp, vp, P(vp)]),
+    "vclhip_batch_plan_destroy": (i32, [vp]),
+    "vclhip_batch_plan_autotune": (i32, [vp, P(Csr), vp, i32, vp]),
+    "vclhip_batch_plan_lanes": (i32, [vp, i32, vp]),
+    "vclhip_batched_cg_solve_f64": (i32, [vp, P(Csr), vp, vp, vp, P(SolverParams), vp, vp, vp, vp]),
+    "vclhip_batched_cg_solve_f32": (i32, [vp, P(Csr), vp, vp, vp, P(SolverParams), vp, vp, vp, vp]),
     "vclhip_random_uniform": (i32, [u64, u64, f64, f64, vp]),
     "vclhip_random_normal": (i32, [u64, u64, f64, f64, vp]),
     "vclhip_gen_fdm_laplace_2d": (i32, [u32, u32, vp, vp, vp, P(u32)]),
