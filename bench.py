@@ -17,6 +17,7 @@ import ctypes as C
 import json
 import os
 import pathlib
+import subprocess
 import sys
 import time
 
@@ -180,11 +181,19 @@ def mixed_vs_fp64(vh, grid, reps=2):
     f64 = run("cg_fp64", lambda x: vh.cg(A, b, x, tol=1e-10, max_iters=20000), x64)
     x64h = x64.to_numpy()
     xm = vh.DeviceArray.zeros(n, np.float64)
-    for name, kw in (("mixed_reliable", dict(mixed_mode=1)), ("mixed_reference_schedule", dict(mixed_mode=0))):
+    for name, kw in (("mixed_reliable", dict(mixed_mode=1)),
+                     ("mixed_reliable_csr_inner", dict(mixed_mode=1, inner_format=1)),
+                     ("mixed_reference_schedule", dict(mixed_mode=0))):
         rec = run(name, lambda x: vh.mixed_cg(A, b, x, tol=1e-10, max_iters=20000, **kw), xm)
         xh = xm.to_numpy()
         rec["rel_error_vs_fp64_solution"] = float(np.linalg.norm(xh - x64h) / np.linalg.norm(x64h))
         rec["time_ratio_vs_fp64"] = round(rec["seconds"] / f64["seconds"], 4)
+    S = vh.SellMatrix(rp, col, val, 64)
+    s64 = run("cg_fp64_sell64", lambda x: vh.cg(S, b, x, tol=1e-10, max_iters=20000), xm)
+    rec = run("mixed_reliable_sell64", lambda x: vh.mixed_cg(S, b, x, tol=1e-10, max_iters=20000, mixed_mode=1), xm)
+    rec["rel_error_vs_fp64_solution"] = float(np.linalg.norm(xm.to_numpy() - x64h) / np.linalg.norm(x64h))
+    rec["time_ratio_vs_fp64"] = round(rec["seconds"] / f64["seconds"], 4)
+    rec["time_ratio_vs_fp64_sell64"] = round(rec["seconds"] / s64["seconds"], 4)
     xj = vh.DeviceArray.zeros(n, np.float64)
     j64 = run("jacobi_cg_fp64", lambda x: vh.cg(A, b, x, precond=1, tol=1e-10, max_iters=20000), xj)
     xjh = xj.to_numpy()
@@ -195,6 +204,129 @@ def mixed_vs_fp64(vh, grid, reps=2):
     for a in (x64, xm, xj, b):
         a.free()
     A.free()
+    return out
+
+
+def batched_cg_bench(vh, num_systems, reps=3):
+    """4096 graph-Laplacian systems (n_i ~ U{64..4096}, average degree 8, weights U[0.1,1], shift 0.01),
+    per-system tol 1e-8, autotuned lanes per row, one persistent launch replayed as a hipGraph."""
+    t = time.perf_counter()
+    sys_ptr, rp, col, val = vh.graph_laplacian_batch(num_systems, 64, 4096, 8.0, 0.1, 1.0, 0.01, 1)
+    n, nnz = rp.size - 1, int(rp[-1])
+    log(f"generated batch of {num_systems} systems: rows={n} nnz={nnz} ({time.perf_counter() - t:.1f} s)")
+    b = vh.random_uniform(0, n)
+    out = {"workload": f"{num_systems} graph-Laplacian systems, n_i ~ U{{64..4096}} (total rows {n}, nnz {nnz}), "
+                       f"CSR, per-system tol 1e-8, b ~ U[-1,1]", "rows": n, "nnz": nnz}
+    plan = vh.BatchPlan(sys_ptr, rp)
+    stream = vh.Stream()
+    for name, dtype in (("fp64", np.float64), ("fp32", np.float32)):
+        A = vh.CsrMatrix(rp, col, val, dtype=dtype)
+        t0 = time.perf_counter()
+        plan.autotune(A, single=(dtype == np.float32), stream=stream.handle)
+        tune_s = time.perf_counter() - t0
+        db = vh.DeviceArray.from_numpy(b.astype(dtype))
+        dx = vh.DeviceArray.zeros(n, dtype)
+        it = vh.DeviceArray.zeros(num_systems, np.uint32)
+        er = vh.DeviceArray.zeros(num_systems, np.float64)
+        cv = vh.DeviceArray.zeros(num_systems, np.uint32)
+        ev = Events(vh, 2)
+        best = None
+        for r in range(reps + 1):
+            ev.record(0, stream.handle)
+            plan.solve_into(A, db, dx, it, er, cv, stream.handle, tol=1e-8, max_iters=20000, use_graph=True)
+            ev.record(1, stream.handle)
+            stream.synchronize()
+            ms = ev.ms(0, 1)
+            if r > 0:
+                best = ms if best is None else min(best, ms)
+        ev.close()
+        iters, conv = it.to_numpy(), cv.to_numpy()
+        # per-system FP64 true residual with the FP64 matrix
+        A64 = A if dtype == np.float64 else vh.CsrMatrix(rp, col, val)
+        x64 = vh.DeviceArray.from_numpy(dx.to_numpy().astype(np.float64))
+        bb = b.astype(dtype).astype(np.float64)
+        y = vh.DeviceArray.zeros(n, np.float64)
+        A64.spmv(x64, y)
+        r = bb - y.to_numpy()
+        seg = np.add.reduceat(r * r, sys_ptr[:-1].astype(np.int64))
+        segb = np.add.reduceat(bb * bb, sys_ptr[:-1].astype(np.int64))
+        rel = np.sqrt(seg / segb)
+        vb = 4 if dtype == np.float32 else 8
+        # bytes per system iteration: values + columns + row pointers (the vectors stay on chip)
+        rows_i = np.diff(sys_ptr).astype(np.float64)
+        nnz_i = np.diff(rp[sys_ptr].astype(np.int64)).astype(np.float64)
+        moved = float(np.sum(iters * ((vb + 4) * nnz_i + 4 * (rows_i + 1))))
+        out[name] = {"ms": round(best, 4), "systems_per_s": round(num_systems / (best * 1e-3), 1),
+                     "iters_total": int(iters.sum()), "iters_max": int(iters.max()), "iters_mean": float(iters.mean()),
+                     "converged": int((conv == 1).sum()), "max_true_relres": float(rel.max()),
+                     "median_true_relres": float(np.median(rel)), "matrix_gbs": round(moved / (best * 1e-3) / 1e9, 1),
+                     "autotune_s": round(tune_s, 3), "lanes_log2_by_size_class": plan.lanes(dtype == np.float32).tolist()}
+        log("batched", name, out[name])
+        for a in (db, dx, it, er, cv, x64, y):
+            a.free()
+        A.free()
+        if A64 is not A:
+            A64.free()
+    plan.close()
+    return out
+
+
+def adjoint_bench(vh, grid, n_fd=64):
+    """Differentiable solve on the 2-D variable-coefficient diffusion operator (theta = exp(N(0,0.5))),
+    L = 0.5 ||x - x_t||^2: forward CG + adjoint CG + fused gradient kernel, chained to dL/dtheta by
+    torch; central finite differences (h = 1e-6) on n_fd random entries."""
+    import torch
+    from vclhip import torch_solve as ts
+    nx = ny = grid
+    n = nx * ny
+    theta = np.exp(0.5 * vh.random_normal(1, n))
+    b = vh.random_uniform(0, n)
+    xt = vh.random_uniform(2, n)
+    rp, col, pos, masks = ts.diffusion_pattern_2d(nx, ny)
+    pat = ts.CsrPattern(rp, col)
+    dev = torch.device("cuda")
+    bt = torch.tensor(b, dtype=torch.float64, device=dev)
+    xtt = torch.tensor(xt, dtype=torch.float64, device=dev)
+
+    def loss(th, tol):
+        vals = ts.diffusion_values_2d(th, nx, ny, pos, masks)
+        x = ts.sparse_solve(vals, bt, pat, tol=tol, max_iters=50000)
+        return 0.5 * torch.sum((x - xtt) ** 2)
+
+    best = None
+    g = None
+    for r in range(3):
+        th = torch.tensor(theta.reshape(ny, nx), dtype=torch.float64, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L = loss(th, 1e-10)
+        L.backward()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if r > 0:
+            best = dt if best is None else min(best, dt)
+        g = th.grad.detach().cpu().numpy().ravel()
+    rng = np.random.default_rng(0)
+    idx = rng.choice(n, n_fd, replace=False)
+    h = 1e-6
+    th_ref = torch.tensor(theta.reshape(ny, nx), dtype=torch.float64, device=dev, requires_grad=True)
+    loss(th_ref, 1e-13).backward()
+    gref = th_ref.grad.detach().cpu().numpy().ravel()
+    errs = []
+    with torch.no_grad():
+        for i in idx:
+            tp, tm = theta.copy(), theta.copy()
+            tp[i] += h
+            tm[i] -= h
+            fd = (loss(torch.tensor(tp.reshape(ny, nx), device=dev), 1e-13).item()
+                  - loss(torch.tensor(tm.reshape(ny, nx), device=dev), 1e-13).item()) / (2 * h)
+            errs.append(abs(fd - gref[i]) / max(abs(gref[i]), 1e-12))
+    out = {"workload": f"2-D variable-coefficient diffusion {grid}^2 (n={n}, nnz={int(rp[-1])}), FP64 CG forward "
+                       f"and adjoint to 1e-10, dL/dA on the CSR pattern chained to dL/dtheta",
+           "forward_backward_s": round(best, 6), "fd_entries": n_fd, "fd_h": h,
+           "fd_max_rel_err": float(max(errs)), "fd_median_rel_err": float(np.median(errs)),
+           "grad_norm": float(np.linalg.norm(g))}
+    log("adjoint", out)
     return out
 
 
@@ -230,9 +362,14 @@ def main():
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-solvers", action="store_true")
+    ap.add_argument("--no-solvers", action="store_true", help="skip the 128^3 CG/BiCGStab solves")
     ap.add_argument("--mixed-grid", type=int, default=160,
                     help="grid of the FP64-vs-mixed CG comparison (0 skips it)")
+    ap.add_argument("--batch", type=int, default=4096, help="systems of the batched CG run (0 skips it)")
+    ap.add_argument("--adjoint-grid", type=int, default=512, help="grid of the differentiable solve (0 skips it)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the differentiable-solve measurement (a child process: torch is imported "
+                         "before libvclhip so that the process holds one HIP runtime)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only warm CSR/SELL SpMV and the streaming kernel (for counter collection)")
     args = ap.parse_args()
@@ -240,6 +377,13 @@ def main():
         print("bench.py measures one GPU only: the sharded (row-partitioned) path is not built; run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.adjoint_only:
+        import torch  # noqa: F401  (before libvclhip: one HIP runtime in the process)
+        import vclhip as vh
+        vh.call("vclhip_set_device", 0)
+        print(json.dumps(adjoint_bench(vh, args.adjoint_grid)), flush=True)
+        return
 
     import vclhip as vh
     assert vh.device_count() > 0, "no HIP device"
@@ -319,8 +463,18 @@ def main():
         for k, v in sol.items():
             log(k, v)
         out["solvers_lap3d_128"] = sol
-        if args.mixed_grid:
-            out["mixed_vs_fp64"] = mixed_vs_fp64(vh, args.mixed_grid)
+    if args.mixed_grid:
+        out["mixed_vs_fp64"] = mixed_vs_fp64(vh, args.mixed_grid)
+    if args.batch:
+        out["batched_cg"] = batched_cg_bench(vh, args.batch)
+    if args.adjoint_grid:
+        log("differentiable solve (child process) ...")
+        p = subprocess.run([sys.executable, str(pathlib.Path(__file__).resolve()), "--adjoint-only",
+                            "--adjoint-grid", str(args.adjoint_grid)], capture_output=True, text=True, timeout=900)
+        sys.stderr.write(p.stderr[-2000:])
+        if p.returncode != 0:
+            raise RuntimeError(f"adjoint measurement failed with status {p.returncode}")
+        out["adjoint"] = json.loads(p.stdout.strip().splitlines()[-1])
 
     if not args.no_cpu:
         log("CPU baseline (reference host backend) ...")

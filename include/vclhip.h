@@ -241,7 +241,14 @@ typedef struct vclhip_solver_params {
    * element type (double or float). */
   int (*monitor)(const void* x_dev, double rel_residual, void* user);
   void* monitor_user;
+  int inner_format;                  /* mixed CG on a CSR matrix: storage of the FP32 inner copy,
+                                        VCLHIP_INNER_AUTO (SELL-64 when it stores at most 25% padding,
+                                        else CSR), VCLHIP_INNER_CSR or VCLHIP_INNER_SELL64 */
 } vclhip_solver_params;
+
+#define VCLHIP_INNER_AUTO 0
+#define VCLHIP_INNER_CSR 1
+#define VCLHIP_INNER_SELL64 2
 
 typedef struct vclhip_solver_result {
   uint32_t iters;     /* tag.iters() */

@@ -1,0 +1,98 @@
+"""Checks of the differentiable solve, run in a process of their own by test_gpu_autograd.py.
+
+torch is imported before libvclhip is loaded, so that the process has one HIP runtime (torch's).
+Each check prints one JSON line and exits nonzero on failure.
+"""
+import json
+import pathlib
+import sys
+
+import torch  # noqa: F401  (first: its HIP runtime is the one libvclhip binds to)
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
+for sub in ("viennacl-dev_amd", "oracle"):
+    sys.path.insert(0, str(ROOT / sub))
+
+import numpy as np  # noqa: E402
+
+import pyoracle as oracle  # noqa: E402
+import vclhip as vh  # noqa: E402
+from vclhip import torch_solve as ts  # noqa: E402
+
+
+def gradients(nx=48, ny=40, n_fd=16):
+    theta = np.exp(0.5 * vh.random_normal(1, nx * ny))
+    b = vh.random_uniform(0, nx * ny)
+    xt = vh.random_uniform(2, nx * ny)
+    rp, col, pos, masks = ts.diffusion_pattern_2d(nx, ny)
+    pat = ts.CsrPattern(rp, col)
+    dev = torch.device("cuda")
+    th = torch.tensor(theta.reshape(ny, nx), dtype=torch.float64, device=dev, requires_grad=True)
+    bt = torch.tensor(b, dtype=torch.float64, device=dev, requires_grad=True)
+    xtt = torch.tensor(xt, dtype=torch.float64, device=dev)
+
+    def loss_of(theta_t, tol=1e-13):
+        vals = ts.diffusion_values_2d(theta_t, nx, ny, pos, masks)
+        x = ts.sparse_solve(vals, bt, pat, tol=tol, max_iters=20000)
+        return 0.5 * torch.sum((x - xtt) ** 2)
+
+    L = loss_of(th)
+    L.backward()
+    g = th.grad.detach().cpu().numpy().ravel()
+    gb = bt.grad.detach().cpu().numpy()
+
+    # FP64 host gradient: oracle CG for x and lambda, oracle adjoint product, chained on the CPU
+    oracle.set_threads(1)
+    vals_h = ts.diffusion_values_2d(torch.tensor(theta.reshape(ny, nx)), nx, ny, pos, masks).numpy()
+    Ah = oracle.CsrHost(rp, col, vals_h)
+    xh, _, _, _ = oracle.cg_pipelined(Ah, b, 1e-13, 20000)
+    lamh, _, _, _ = oracle.cg_pipelined(Ah, xh - xt, 1e-13, 20000)
+    dAh = oracle.adjoint_grad(rp, col, lamh, xh)
+    th_c = torch.tensor(theta.reshape(ny, nx), requires_grad=True)
+    ts.diffusion_values_2d(th_c, nx, ny, pos, masks).backward(torch.tensor(dAh))
+    gh = th_c.grad.numpy().ravel()
+    host_rel = float(np.linalg.norm(g - gh) / np.linalg.norm(gh))
+    db_rel = float(np.linalg.norm(gb - lamh) / np.linalg.norm(lamh))
+
+    rng = np.random.default_rng(11)
+    idx = rng.choice(nx * ny, n_fd, replace=False)
+    h = 1e-6
+    fd_bad = []
+    worst = 0.0
+    for i in idx:
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        with torch.no_grad():
+            Lp = loss_of(torch.tensor(tp.reshape(ny, nx), device=dev)).item()
+            Lm = loss_of(torch.tensor(tm.reshape(ny, nx), device=dev)).item()
+        fd = (Lp - Lm) / (2 * h)
+        err = abs(fd - g[i])
+        scale = max(abs(g[i]), 1e-3 * np.abs(g).max())
+        worst = max(worst, err / scale)
+        if err > 1e-5 * scale:
+            fd_bad.append([int(i), fd, float(g[i])])
+    ok = host_rel < 1e-8 and db_rel < 1e-8 and not fd_bad
+    return ok, {"host_rel": host_rel, "db_rel": db_rel, "fd_worst_rel": worst, "fd_bad": fd_bad}
+
+
+def input_checks():
+    rp, col, pos, masks = ts.diffusion_pattern_2d(8, 6)
+    pat = ts.CsrPattern(rp, col)
+    vals = torch.ones(int(rp[-1]), dtype=torch.float64, device="cuda")
+    raised = []
+    for v, b in ((vals.float(), torch.ones(48, dtype=torch.float64, device="cuda")),
+                 (vals, torch.ones(47, dtype=torch.float64, device="cuda"))):
+        try:
+            ts.sparse_solve(v, b, pat)
+            raised.append(False)
+        except ValueError:
+            raised.append(True)
+    return all(raised), {"raised": raised}
+
+
+if __name__ == "__main__":
+    check = {"gradients": gradients, "input_checks": input_checks}[sys.argv[1]]
+    ok, info = check()
+    print(json.dumps({"ok": ok, **info}), flush=True)
+    sys.exit(0 if ok else 1)

@@ -80,15 +80,37 @@ def test_reliable_update_vs_oracle(dev, vh, oracle, fmt, jacobi):
 def test_values32_argument_and_graph_are_identical(dev, vh):
     rp, col, val = vh.laplace_3d(30, 26, 22)
     b = vh.random_uniform(0, rp.size - 1)
-    x0, r0 = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=2000, mixed_mode=1)
+    # the caller's FP32 values run the CSR inner operator, as inner_format = CSR does
+    x0, r0 = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=2000, mixed_mode=1, inner_format=1)
     v32 = vh.DeviceArray.from_numpy(val.astype(np.float32))
     x1, r1 = run_mixed(vh, rp, col, val, b, values32=v32, tol=1e-10, max_iters=2000, mixed_mode=1)
-    x2, r2 = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=2000, mixed_mode=1, use_graph=True, chunk=8)
-    assert r0.iters == r1.iters == r2.iters and r0.restarts == r1.restarts == r2.restarts
-    assert np.array_equal(x0, x1) and np.array_equal(x0, x2)
-    x3, r3 = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=2000, mixed_mode=0, use_graph=True)
-    x4, r4 = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=2000, mixed_mode=0)
-    assert r3.iters == r4.iters and np.array_equal(x3, x4)
+    assert r0.iters == r1.iters and r0.restarts == r1.restarts and np.array_equal(x0, x1)
+    # default (SELL-64 inner copy for this stencil), eager and graph-captured
+    x2, r2 = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=2000, mixed_mode=1)
+    x3, r3 = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=2000, mixed_mode=1, use_graph=True, chunk=8)
+    assert r2.iters == r3.iters and r2.restarts == r3.restarts and np.array_equal(x2, x3)
+    assert iters_close(r2.iters, r0.iters)
+    x4, r4 = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=2000, mixed_mode=0, use_graph=True)
+    x5, r5 = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=2000, mixed_mode=0)
+    assert r4.iters == r5.iters and np.array_equal(x4, x5)
+
+
+@pytest.mark.parametrize("fmt", [1, 2])
+def test_inner_formats_vs_oracle(dev, vh, oracle, fmt):
+    """CSR and SELL-64 inner copies both follow the oracle (one lane per row: same sums)."""
+    k = np.exp(vh.random_normal(5, 20 * 22 * 24))
+    rp, col, val = vh.diffusion(20, 22, 24, k)
+    b = vh.random_uniform(0, rp.size - 1)
+    x, res = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=3000, mixed_mode=1, inner_format=fmt)
+    _, it_o, _, upd_o, _ = oracle.mixed_cg2(oracle.CsrHost(rp, col, val), b, 1e-10, 3000, mode=1)
+    assert res.converged and iters_close(res.iters, it_o) and abs(res.restarts - upd_o) <= 1
+    assert true_relres(rp, col, val, x, b) < 1e-10
+    # a random-graph Laplacian (row lengths vary) pads too much for the automatic SELL choice; both
+    # formats, forced, solve it
+    _, rp2, col2, val2 = vh.graph_laplacian_batch(1, 3000, 3000, 8.0, 0.1, 1.0, 0.01, 3)
+    b2 = vh.random_uniform(1, 3000)
+    x2, r2 = run_mixed(vh, rp2, col2, val2, b2, tol=1e-10, max_iters=3000, mixed_mode=1, inner_format=fmt)
+    assert r2.converged and true_relres(rp2, col2, val2, x2, b2) < 1e-10
 
 
 def test_edge_cases(dev, vh):

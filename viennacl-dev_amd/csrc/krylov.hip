@@ -877,6 +877,18 @@ static int run_iterations(hipStream_t s, KState* st_dev, uint32_t first_iter, ui
     }
     pending = slot;
     slot ^= 1;
+    if (!finished && launched >= max_iters) {
+      // Iterations that returned at once while an FP64 correction was pending did not count: when
+      // the device has done fewer than max_iters, enqueue the rest.
+      status = hip_status(hipStreamSynchronize(s), "solver sync");
+      if (status) break;
+      KState now;
+      status = hip_status(hipMemcpy(&now, st_dev, sizeof(KState), hipMemcpyDeviceToHost), "state D2H");
+      if (status) break;
+      if ((now.flags & end_mask) || now.iter >= max_iters) break;
+      launched = now.iter;
+      pending = -1;
+    }
   }
   cleanup_graph();
   int st_sync = hip_status(hipStreamSynchronize(s), "solver sync");
@@ -1503,6 +1515,121 @@ __global__ __launch_bounds__(kBlock) void convert_values_kernel(size_t n, const 
     f[i] = static_cast<float>(v[i]);
 }
 
+// ---- FP32 SELL-64 copy of a CSR matrix, built on the device (the inner operator of the mixed CG)
+
+constexpr uint32_t kInnerC = kWave;  // slice height: one wavefront per slice
+
+// width[s] = longest row of slice s (one wavefront per slice).
+__global__ __launch_bounds__(kBlock) void sell_width_kernel(const uint32_t* __restrict__ rp, uint32_t n,
+                                                            uint32_t slices, uint32_t* __restrict__ width)
+{
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t len = row < n ? rp[row + 1] - rp[row] : 0u;
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) len = max(len, static_cast<uint32_t>(__shfl_xor(static_cast<int>(len), o, kWave)));
+  const uint32_t s = row / kInnerC;
+  if ((threadIdx.x & (kWave - 1)) == 0 && s < slices) width[s] = len;
+}
+
+// start[s] = C * (width[0] + ... + width[s-1]); total = C * sum of all widths.  One workgroup.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void sell_scan_kernel(const uint32_t* __restrict__ width, uint32_t slices,
+                                                                 uint32_t* __restrict__ start,
+                                                                 unsigned long long* __restrict__ total)
+{
+  __shared__ unsigned long long part[kScanThreads];
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (slices + kScanThreads - 1) / kScanThreads;
+  const uint32_t lo = min(t * per, slices), hi = min(lo + per, slices);
+  unsigned long long sum = 0;
+  for (uint32_t i = lo; i < hi; ++i) sum += static_cast<unsigned long long>(width[i]) * kInnerC;
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < kScanThreads; o <<= 1) {  // inclusive Hillis-Steele scan of the thread sums
+    const unsigned long long v = t >= static_cast<uint32_t>(o) ? part[t - o] : 0ull;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  unsigned long long run = part[t] - sum;
+  for (uint32_t i = lo; i < hi; ++i) {
+    start[i] = static_cast<uint32_t>(run);
+    run += static_cast<unsigned long long>(width[i]) * kInnerC;
+  }
+  if (t == kScanThreads - 1) *total = part[t];
+}
+
+// Fills the slices: entry k of row r at start[s] + k*C + lane, FP32 values; padding is value 0,
+// column 0 (sliced_ell_matrix.hpp:180-200).
+__global__ __launch_bounds__(kBlock) void sell_fill_kernel(const uint32_t* __restrict__ rp,
+                                                           const uint32_t* __restrict__ col,
+                                                           const double* __restrict__ val, uint32_t n,
+                                                           uint32_t slices, const uint32_t* __restrict__ width,
+                                                           const uint32_t* __restrict__ start,
+                                                           uint32_t* __restrict__ ci, float* __restrict__ el)
+{
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t s = row / kInnerC;
+  if (s >= slices) return;
+  const uint32_t lane = row - s * kInnerC;
+  const uint32_t w = width[s];
+  const uint32_t a = row < n ? rp[row] : 0u;
+  const uint32_t len = row < n ? rp[row + 1] - a : 0u;
+  const size_t base = static_cast<size_t>(start[s]) + lane;
+  for (uint32_t k = 0; k < w; ++k) {
+    const size_t off = base + static_cast<size_t>(k) * kInnerC;
+    if (k < len) {
+      ci[off] = col[a + k];
+      el[off] = static_cast<float>(val[a + k]);
+    } else {
+      ci[off] = 0;
+      el[off] = 0.0f;
+    }
+  }
+}
+
+// Builds the SELL-64 FP32 copy of A into workspace memory when it stores at most `max_fill` times
+// nnz entries (or always when forced); *built tells whether it did.
+static int build_inner_sell(const vclhip_csr* A, Workspace& ws, hipStream_t s, double max_fill, bool force,
+                            vclhip_sell* out, bool* built)
+{
+  *built = false;
+  const uint32_t n = A->n_rows;
+  const uint32_t slices = (n + kInnerC - 1) / kInnerC;
+  uint32_t *width, *start;
+  unsigned long long* total;
+  VCLHIP_TRY(ws.alloc(&width, slices));
+  VCLHIP_TRY(ws.alloc(&start, slices));
+  VCLHIP_TRY(ws.alloc(&total, 1));
+  const uint32_t g = (slices * kInnerC + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(sell_width_kernel, dim3(g), dim3(kBlock), 0, s, A->row_ptr, n, slices, width);
+  VCLHIP_TRY(check_launch("sell_width_kernel"));
+  hipLaunchKernelGGL(sell_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, width, slices, start, total);
+  VCLHIP_TRY(check_launch("sell_scan_kernel"));
+  unsigned long long stored = 0;
+  VCLHIP_HIP(hipMemcpyAsync(&stored, total, sizeof(stored), hipMemcpyDeviceToHost, s));
+  VCLHIP_HIP(hipStreamSynchronize(s));
+  if (stored >= (1ull << 32)) return VCLHIP_SUCCESS;  // block_start is 32-bit
+  if (!force && static_cast<double>(stored) > max_fill * static_cast<double>(A->nnz)) return VCLHIP_SUCCESS;
+  uint32_t* ci;
+  float* el;
+  VCLHIP_TRY(ws.alloc(&ci, stored ? stored : 1));
+  VCLHIP_TRY(ws.alloc(&el, stored ? stored : 1));
+  hipLaunchKernelGGL(sell_fill_kernel, dim3(g), dim3(kBlock), 0, s, A->row_ptr, A->col_idx,
+                     static_cast<const double*>(A->values), n, slices, width, start, ci, el);
+  VCLHIP_TRY(check_launch("sell_fill_kernel"));
+  out->n_rows = n;
+  out->n_cols = A->n_cols;
+  out->C = kInnerC;
+  out->num_slices = slices;
+  out->columns_per_block = width;
+  out->column_indices = ci;
+  out->block_start = start;
+  out->elements = el;
+  *built = true;
+  return VCLHIP_SUCCESS;
+}
+
 inline const double* values_of(const vclhip_csr* A) { return static_cast<const double*>(A->values); }
 inline const double* values_of(const vclhip_sell* A) { return static_cast<const double*>(A->elements); }
 // Number of stored values: nnz for CSR; block_start[last] + columns_per_block[last] * C for SELL
@@ -1555,20 +1682,37 @@ static int mixed_cg_solve(const M* A, const float* values32, const double* b, do
   const uint32_t n = op64.n;
   const bool jac = precond == VCLHIP_PRECOND_JACOBI;
   Workspace ws(s);
-  float* v32 = const_cast<float*>(values32);
-  if (!v32) {
-    size_t value_count32 = 0;
-    VCLHIP_TRY(value_count(A, &value_count32));
-    VCLHIP_TRY(ws.alloc(&v32, value_count32 ? value_count32 : 1));
-    if (value_count32) {
-      const uint32_t g = static_cast<uint32_t>(std::min<size_t>((value_count32 + kBlock - 1) / kBlock, 8192));
-      hipLaunchKernelGGL(convert_values_kernel, dim3(g), dim3(kBlock), 0, s, value_count32, values_of(A), v32);
-      VCLHIP_TRY(check_launch("convert_values_kernel"));
+  // FP32 inner operator: the caller's FP32 values, or a copy converted here -- as SELL-64 when A is
+  // CSR and the slices pad little (one wavefront per slice, no row pointers), else in A's format.
+  DevOp<float> op32;
+  vclhip_sell inner_sell{};
+  bool inner_is_sell = false;
+  if constexpr (std::is_same<M, vclhip_csr>::value) {
+    if (!values32 && P->inner_format != VCLHIP_INNER_CSR) {
+      if (P->inner_format != VCLHIP_INNER_AUTO && P->inner_format != VCLHIP_INNER_SELL64)
+        return invalid("inner_format");
+      VCLHIP_TRY(build_inner_sell(A, ws, s, 1.25, P->inner_format == VCLHIP_INNER_SELL64, &inner_sell,
+                                  &inner_is_sell));
     }
   }
-  const M A32 = with_values(A, v32);
-  DevOp<float> op32;
-  VCLHIP_TRY(make_op(&A32, op32));
+  M A32{};
+  if (inner_is_sell) {
+    VCLHIP_TRY(make_op(&inner_sell, op32));
+  } else {
+    float* v32 = const_cast<float*>(values32);
+    if (!v32) {
+      size_t value_count32 = 0;
+      VCLHIP_TRY(value_count(A, &value_count32));
+      VCLHIP_TRY(ws.alloc(&v32, value_count32 ? value_count32 : 1));
+      if (value_count32) {
+        const uint32_t g = static_cast<uint32_t>(std::min<size_t>((value_count32 + kBlock - 1) / kBlock, 8192));
+        hipLaunchKernelGGL(convert_values_kernel, dim3(g), dim3(kBlock), 0, s, value_count32, values_of(A), v32);
+        VCLHIP_TRY(check_launch("convert_values_kernel"));
+      }
+    }
+    A32 = with_values(A, v32);
+    VCLHIP_TRY(make_op(&A32, op32));
+  }
 
   float *x32, *r32, *p32, *Ap32, *d32 = nullptr;
   double *r64, *part, *hist = nullptr;
@@ -1585,7 +1729,7 @@ static int mixed_cg_solve(const M* A, const float* values32, const double* b, do
     uint32_t* zc;
     VCLHIP_TRY(ws.alloc(&d32, n));
     VCLHIP_TRY(ws.alloc(&zc, 1));
-    VCLHIP_TRY(jacobi_diag(op32, reinterpret_cast<const vclhip_csr*>(&A32), d32, zc, s));
+    VCLHIP_TRY(jacobi_diag(op32, reinterpret_cast<const vclhip_csr*>(&A32), d32, zc, s));  // SELL: op32.sm
   }
   KState h;
   init_state(h, P, hist);

@@ -190,7 +190,7 @@ class SellMatrix:
 
 
 def solver_params(tol=1e-8, max_iters=300, abs_tol=0.0, max_iters_before_restart=200, inner_tol=1e-2,
-                  chunk=16, use_graph=False, mixed_mode=0, variant=0, history=None, monitor=None):
+                  chunk=16, use_graph=False, mixed_mode=0, variant=0, history=None, monitor=None, inner_format=0):
     p = SolverParams()
     lib().vclhip_solver_params_default(C.byref(p))
     p.variant = variant
@@ -202,6 +202,7 @@ def solver_params(tol=1e-8, max_iters=300, abs_tol=0.0, max_iters_before_restart
     p.chunk = chunk
     p.use_graph = 1 if use_graph else 0
     p.mixed_mode = mixed_mode
+    p.inner_format = inner_format
     if history is not None:
         p.history_host = history.ctypes.data_as(C.POINTER(C.c_double))
         p.history_len = history.size

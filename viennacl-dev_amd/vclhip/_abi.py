@@ -50,7 +50,7 @@ class SolverParams(C.Structure):
     _fields_ = [("tol", f64), ("abs_tol", f64), ("max_iters", u32),
                 ("max_iters_before_restart", u32), ("inner_tol", f32), ("chunk", u32),
                 ("use_graph", i32), ("mixed_mode", i32), ("variant", i32), ("history_host", P(f64)),
-                ("history_len", u32), ("monitor", MONITOR), ("monitor_user", vp)]
+                ("history_len", u32), ("monitor", MONITOR), ("monitor_user", vp), ("inner_format", i32)]
 
 
 class SolverResult(C.Structure):
