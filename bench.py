@@ -75,28 +75,29 @@ def evict_caches(vh, evict, stream):
 
 
 def time_launches(vh, launch, stream, steps, warmup, evict=None):
-    """Per-launch HIP-event times (cold when `evict` is given) and the warm back-to-back mean."""
+    """Kernel durations of `launch` (one SpMV or streaming kernel), each measured by events that the
+    kernel itself records at its start and end (vclhip_time_next_launch).  Returns the per-launch
+    cold durations (each after a 1 GiB read that evicts the caches; None without `evict`), the mean
+    warm (back-to-back) duration, and the wall time of the cold loop."""
     for _ in range(warmup):
         launch()
     stream.synchronize()
-    ev = Events(vh, 2 * steps + 2)
+    ev = Events(vh, 2 * steps)
     t0 = time.perf_counter()
     cold = None
     if evict is not None:
         for k in range(steps):
             evict_caches(vh, evict, stream)
-            ev.record(2 * k, stream.handle)
+            vh.call("vclhip_time_next_launch", ev.ev[2 * k], ev.ev[2 * k + 1])
             launch()
-            ev.record(2 * k + 1, stream.handle)
         stream.synchronize()
         cold = np.array([ev.ms(2 * k, 2 * k + 1) for k in range(steps)])
     wall = time.perf_counter() - t0
-    ev.record(2 * steps, stream.handle)
-    for _ in range(steps):
+    for k in range(steps):
+        vh.call("vclhip_time_next_launch", ev.ev[2 * k], ev.ev[2 * k + 1])
         launch()
-    ev.record(2 * steps + 1, stream.handle)
     stream.synchronize()
-    warm = ev.ms(2 * steps, 2 * steps + 1) / steps
+    warm = float(np.mean([ev.ms(2 * k, 2 * k + 1) for k in range(steps)]))
     ev.close()
     return cold, warm, wall
 

@@ -73,6 +73,11 @@ int vclhip_event_create(void** event);
 int vclhip_event_destroy(void* event);
 int vclhip_event_record(void* event, void* stream);
 int vclhip_event_elapsed_ms(void* start, void* stop, float* ms);
+/* Measurement hook (no counterpart in the reference): the next SpMV (vclhip_csr_spmv_*,
+ * vclhip_sell_spmv_*) or vclhip_axpby_f64 launched by the calling thread records start_event and
+ * stop_event at the kernel's own start and end (hipExtLaunchKernelGGL), so that one kernel can be
+ * timed without the latency of separate event packets.  The request applies to one launch. */
+int vclhip_time_next_launch(void* start_event, void* stop_event);
 
 /* ------------------------------------------------------------ sparse formats ---- */
 /* CSR matrix on the device (compressed_matrix<T>: handle1 = row_ptr, handle2 = col, handle = values,
@@ -243,12 +248,15 @@ typedef struct vclhip_solver_params {
   void* monitor_user;
   int inner_format;                  /* mixed CG on a CSR matrix: storage of the FP32 inner copy,
                                         VCLHIP_INNER_AUTO (SELL-64 when it stores at most 25% padding,
-                                        else CSR), VCLHIP_INNER_CSR or VCLHIP_INNER_SELL64 */
+                                        with 16-bit column offsets when every |col - row| <= 32767;
+                                        else CSR), VCLHIP_INNER_CSR, VCLHIP_INNER_SELL64 (32-bit
+                                        columns) or VCLHIP_INNER_SELL64_D16 */
 } vclhip_solver_params;
 
 #define VCLHIP_INNER_AUTO 0
 #define VCLHIP_INNER_CSR 1
 #define VCLHIP_INNER_SELL64 2
+#define VCLHIP_INNER_SELL64_D16 3
 
 typedef struct vclhip_solver_result {
   uint32_t iters;     /* tag.iters() */

@@ -95,9 +95,9 @@ def test_values32_argument_and_graph_are_identical(dev, vh):
     assert r4.iters == r5.iters and np.array_equal(x4, x5)
 
 
-@pytest.mark.parametrize("fmt", [1, 2])
+@pytest.mark.parametrize("fmt", [1, 2, 3])
 def test_inner_formats_vs_oracle(dev, vh, oracle, fmt):
-    """CSR and SELL-64 inner copies both follow the oracle (one lane per row: same sums)."""
+    """CSR, SELL-64 and SELL-64 with 16-bit column offsets as the inner copy all follow the oracle."""
     k = np.exp(vh.random_normal(5, 20 * 22 * 24))
     rp, col, val = vh.diffusion(20, 22, 24, k)
     b = vh.random_uniform(0, rp.size - 1)
@@ -111,6 +111,16 @@ def test_inner_formats_vs_oracle(dev, vh, oracle, fmt):
     b2 = vh.random_uniform(1, 3000)
     x2, r2 = run_mixed(vh, rp2, col2, val2, b2, tol=1e-10, max_iters=3000, mixed_mode=1, inner_format=fmt)
     assert r2.converged and true_relres(rp2, col2, val2, x2, b2) < 1e-10
+
+
+def test_sell16_refuses_far_columns(dev, vh):
+    # a 200 x 200 x 3 grid: the z-neighbours are 40000 rows away, beyond a 16-bit offset
+    rp, col, val = vh.laplace_3d(200, 200, 3)
+    b = vh.random_uniform(0, rp.size - 1)
+    with pytest.raises(vh.VclHipError):
+        run_mixed(vh, rp, col, val, b, tol=1e-6, max_iters=50, mixed_mode=1, inner_format=3)
+    x, res = run_mixed(vh, rp, col, val, b, tol=1e-6, max_iters=3000, mixed_mode=1)  # automatic: 32-bit columns
+    assert res.converged and true_relres(rp, col, val, x, b) < 1e-6
 
 
 def test_edge_cases(dev, vh):

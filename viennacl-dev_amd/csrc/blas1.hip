@@ -208,8 +208,12 @@ int vclhip_axpby_f64(uint32_t n, double a, const double* x, double b, const doub
 {
   if (n == 0) return VCLHIP_SUCCESS;
   if (!x || !y || !z) return invalid("null vector");
-  VCLHIP_LAUNCH_W(aligned16({x, y, z}), (axpby_kernel<double, 2>), (axpby_kernel<double, 1>), vec_grid(n),
-                  static_cast<hipStream_t>(stream), n, a, x, b, y, z);
+  if (aligned16({x, y, z}))
+    launch_timed(axpby_kernel<double, 2>, dim3(vec_grid(n)), dim3(kBlock), static_cast<hipStream_t>(stream), n, a, x,
+                 b, y, z);
+  else
+    launch_timed(axpby_kernel<double, 1>, dim3(vec_grid(n)), dim3(kBlock), static_cast<hipStream_t>(stream), n, a, x,
+                 b, y, z);
   return check_launch("axpby_kernel");
 }
 

@@ -1,6 +1,7 @@
 // internal.hpp -- shared host-side helpers of libvclhip (error reporting, argument checks).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -47,5 +48,24 @@ inline int check_launch(const char* kernel)
 
 // Number of compute units of the current device (cached per device).
 int device_cu_count();
+
+// Measurement hook (vclhip_time_next_launch): events that the next SpMV or streaming kernel
+// launched by this thread records at the kernel's own start and end.  Taking them clears them.
+struct LaunchTiming {
+  hipEvent_t start = nullptr;
+  hipEvent_t stop = nullptr;
+};
+LaunchTiming take_launch_timing();
+
+// Launches `kernel`, through hipExtLaunchKernelGGL when a timing request is pending.
+template <typename F, typename... Args>
+inline void launch_timed(F kernel, dim3 grid, dim3 block, hipStream_t s, Args... args)
+{
+  const LaunchTiming t = take_launch_timing();
+  if (t.start || t.stop)
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, t.start, t.stop, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
 
 }  // namespace vclhip

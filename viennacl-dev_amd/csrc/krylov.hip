@@ -609,11 +609,30 @@ struct DevOp {
   uint32_t n = 0;
   uint32_t grid = 8;
 
+  bool nt = false;  // nontemporal matrix loads (matrix larger than the Infinity Cache)
+
+  template <class Epi, bool Delta, bool NT>
+  void launch_sell(hipStream_t s, const T* x, const Epi& epi, double* part, const uint32_t* stop,
+                   uint32_t mask) const
+  {
+    hipLaunchKernelGGL((sell_kernel<T, true, Epi, Delta, NT>), dim3(grid), dim3(kBlock), 0, s, sm, x, 0u, 1u, epi,
+                       part, kMaxPartials, stop, mask);
+  }
+
   template <class Epi>
   void launch(hipStream_t s, const T* x, const Epi& epi, double* part, const uint32_t* stop, uint32_t mask) const
   {
-    if (sell)
-      hipLaunchKernelGGL((sell_kernel<T, true, Epi>), dim3(grid), dim3(kBlock), 0, s, sm, x, 0u, 1u, epi, part,
+    if (sell) {
+      if (sm.dcol && nt)
+        launch_sell<Epi, true, true>(s, x, epi, part, stop, mask);
+      else if (sm.dcol)
+        launch_sell<Epi, true, false>(s, x, epi, part, stop, mask);
+      else if (nt)
+        launch_sell<Epi, false, true>(s, x, epi, part, stop, mask);
+      else
+        launch_sell<Epi, false, false>(s, x, epi, part, stop, mask);
+    } else if (nt)
+      hipLaunchKernelGGL((csr_kernel<T, true, Epi, true>), dim3(grid), dim3(kBlock), 0, s, csr, x, 0u, 1u, epi, part,
                          kMaxPartials, stop, mask);
     else
       hipLaunchKernelGGL((csr_kernel<T, true, Epi>), dim3(grid), dim3(kBlock), 0, s, csr, x, 0u, 1u, epi, part,
@@ -636,13 +655,14 @@ static int make_op(const vclhip_csr* A, DevOp<T>& op)
   op.csr.n_rows = A->n_rows;
   op.n = A->n_rows;
   op.grid = chunk_grid(A->num_blocks, csr_gmax());
+  op.nt = stream_loads(static_cast<size_t>(A->nnz) * (sizeof(T) + 4));
   return VCLHIP_SUCCESS;
 }
 
 template <typename T>
 static int make_op(const vclhip_sell* A, DevOp<T>& op)
 {
-  if (!A || !A->columns_per_block || !A->column_indices || !A->block_start || !A->elements)
+  if (!A || !A->columns_per_block || !A->block_start || !A->elements)
     return invalid("SELL matrix arrays");
   if (A->n_rows == 0 || A->n_rows != A->n_cols || A->C == 0) return invalid("solvers need a nonempty square matrix");
   if (A->num_slices != (A->n_rows + A->C - 1) / A->C) return invalid("num_slices");
@@ -736,7 +756,8 @@ __global__ __launch_bounds__(kBlock) void sell_diag_kernel(SellDev<T> A, T* __re
   T v = 0;
   for (uint32_t k = 0; k < A.cpb[s]; ++k) {
     const size_t off = base + static_cast<size_t>(k) * A.C;
-    if (A.col[off] == row && (A.val[off] < 0 || A.val[off] > 0)) { v = A.val[off]; break; }
+    const bool on_diag = A.dcol ? A.dcol[off] == 0 : A.col[off] == row;
+    if (on_diag && (A.val[off] < 0 || A.val[off] > 0)) { v = A.val[off]; break; }
   }
   d[row] = v;
 }
@@ -1519,16 +1540,32 @@ __global__ __launch_bounds__(kBlock) void convert_values_kernel(size_t n, const 
 
 constexpr uint32_t kInnerC = kWave;  // slice height: one wavefront per slice
 
-// width[s] = longest row of slice s (one wavefront per slice).
-__global__ __launch_bounds__(kBlock) void sell_width_kernel(const uint32_t* __restrict__ rp, uint32_t n,
-                                                            uint32_t slices, uint32_t* __restrict__ width)
+// width[s] = longest row of slice s (one wavefront per slice); *max_offset = max |col - row|.
+__global__ __launch_bounds__(kBlock) void sell_width_kernel(const uint32_t* __restrict__ rp,
+                                                            const uint32_t* __restrict__ col, uint32_t n,
+                                                            uint32_t slices, uint32_t* __restrict__ width,
+                                                            uint32_t* __restrict__ max_offset)
 {
   const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t len = row < n ? rp[row + 1] - rp[row] : 0u;
+  uint32_t len = 0, dev = 0;
+  if (row < n) {
+    const uint32_t a = rp[row];
+    len = rp[row + 1] - a;
+    for (uint32_t k = 0; k < len; ++k) {
+      const uint32_t c = col[a + k];
+      dev = max(dev, c > row ? c - row : row - c);
+    }
+  }
 #pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) len = max(len, static_cast<uint32_t>(__shfl_xor(static_cast<int>(len), o, kWave)));
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    len = max(len, static_cast<uint32_t>(__shfl_xor(static_cast<int>(len), o, kWave)));
+    dev = max(dev, static_cast<uint32_t>(__shfl_xor(static_cast<int>(dev), o, kWave)));
+  }
   const uint32_t s = row / kInnerC;
-  if ((threadIdx.x & (kWave - 1)) == 0 && s < slices) width[s] = len;
+  if ((threadIdx.x & (kWave - 1)) == 0 && s < slices) {
+    width[s] = len;
+    atomicMax(max_offset, dev);
+  }
 }
 
 // start[s] = C * (width[0] + ... + width[s-1]); total = C * sum of all widths.  One workgroup.
@@ -1561,12 +1598,14 @@ __global__ __launch_bounds__(kScanThreads) void sell_scan_kernel(const uint32_t*
 
 // Fills the slices: entry k of row r at start[s] + k*C + lane, FP32 values; padding is value 0,
 // column 0 (sliced_ell_matrix.hpp:180-200).
+// ci or dci (16-bit column - row) receives the columns; the other pointer is null.
 __global__ __launch_bounds__(kBlock) void sell_fill_kernel(const uint32_t* __restrict__ rp,
                                                            const uint32_t* __restrict__ col,
                                                            const double* __restrict__ val, uint32_t n,
                                                            uint32_t slices, const uint32_t* __restrict__ width,
                                                            const uint32_t* __restrict__ start,
-                                                           uint32_t* __restrict__ ci, float* __restrict__ el)
+                                                           uint32_t* __restrict__ ci, int16_t* __restrict__ dci,
+                                                           float* __restrict__ el)
 {
   const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t s = row / kInnerC;
@@ -1578,45 +1617,58 @@ __global__ __launch_bounds__(kBlock) void sell_fill_kernel(const uint32_t* __res
   const size_t base = static_cast<size_t>(start[s]) + lane;
   for (uint32_t k = 0; k < w; ++k) {
     const size_t off = base + static_cast<size_t>(k) * kInnerC;
-    if (k < len) {
-      ci[off] = col[a + k];
-      el[off] = static_cast<float>(val[a + k]);
-    } else {
-      ci[off] = 0;
-      el[off] = 0.0f;
-    }
+    const uint32_t c = k < len ? col[a + k] : (dci ? row : 0u);
+    if (dci)
+      dci[off] = static_cast<int16_t>(static_cast<int32_t>(c) - static_cast<int32_t>(row));
+    else
+      ci[off] = c;
+    el[off] = k < len ? static_cast<float>(val[a + k]) : 0.0f;
   }
 }
 
 // Builds the SELL-64 FP32 copy of A into workspace memory when it stores at most `max_fill` times
-// nnz entries (or always when forced); *built tells whether it did.
+// nnz entries (or always when forced).  The columns are stored as 16-bit offsets from the row when
+// every |col - row| fits and `allow16` holds (*dcol receives them), else as 32-bit columns.
+// *built tells whether a SELL copy was made.
 static int build_inner_sell(const vclhip_csr* A, Workspace& ws, hipStream_t s, double max_fill, bool force,
-                            vclhip_sell* out, bool* built)
+                            bool allow16, vclhip_sell* out, const int16_t** dcol, size_t* stored_out, bool* built)
 {
   *built = false;
+  *dcol = nullptr;
+  *stored_out = 0;
   const uint32_t n = A->n_rows;
   const uint32_t slices = (n + kInnerC - 1) / kInnerC;
-  uint32_t *width, *start;
+  uint32_t *width, *start, *maxoff;
   unsigned long long* total;
   VCLHIP_TRY(ws.alloc(&width, slices));
   VCLHIP_TRY(ws.alloc(&start, slices));
   VCLHIP_TRY(ws.alloc(&total, 1));
+  VCLHIP_TRY(ws.alloc(&maxoff, 1));
+  VCLHIP_HIP(hipMemsetAsync(maxoff, 0, sizeof(uint32_t), s));
   const uint32_t g = (slices * kInnerC + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(sell_width_kernel, dim3(g), dim3(kBlock), 0, s, A->row_ptr, n, slices, width);
+  hipLaunchKernelGGL(sell_width_kernel, dim3(g), dim3(kBlock), 0, s, A->row_ptr, A->col_idx, n, slices, width,
+                     maxoff);
   VCLHIP_TRY(check_launch("sell_width_kernel"));
   hipLaunchKernelGGL(sell_scan_kernel, dim3(1), dim3(kScanThreads), 0, s, width, slices, start, total);
   VCLHIP_TRY(check_launch("sell_scan_kernel"));
   unsigned long long stored = 0;
+  uint32_t max_offset = 0;
   VCLHIP_HIP(hipMemcpyAsync(&stored, total, sizeof(stored), hipMemcpyDeviceToHost, s));
+  VCLHIP_HIP(hipMemcpyAsync(&max_offset, maxoff, sizeof(max_offset), hipMemcpyDeviceToHost, s));
   VCLHIP_HIP(hipStreamSynchronize(s));
   if (stored >= (1ull << 32)) return VCLHIP_SUCCESS;  // block_start is 32-bit
   if (!force && static_cast<double>(stored) > max_fill * static_cast<double>(A->nnz)) return VCLHIP_SUCCESS;
-  uint32_t* ci;
+  const bool use16 = allow16 && max_offset <= 32767u;
+  uint32_t* ci = nullptr;
+  int16_t* dci = nullptr;
   float* el;
-  VCLHIP_TRY(ws.alloc(&ci, stored ? stored : 1));
+  if (use16)
+    VCLHIP_TRY(ws.alloc(&dci, stored ? stored : 1));
+  else
+    VCLHIP_TRY(ws.alloc(&ci, stored ? stored : 1));
   VCLHIP_TRY(ws.alloc(&el, stored ? stored : 1));
   hipLaunchKernelGGL(sell_fill_kernel, dim3(g), dim3(kBlock), 0, s, A->row_ptr, A->col_idx,
-                     static_cast<const double*>(A->values), n, slices, width, start, ci, el);
+                     static_cast<const double*>(A->values), n, slices, width, start, ci, dci, el);
   VCLHIP_TRY(check_launch("sell_fill_kernel"));
   out->n_rows = n;
   out->n_cols = A->n_cols;
@@ -1626,6 +1678,8 @@ static int build_inner_sell(const vclhip_csr* A, Workspace& ws, hipStream_t s, d
   out->column_indices = ci;
   out->block_start = start;
   out->elements = el;
+  *dcol = dci;
+  *stored_out = static_cast<size_t>(stored);
   *built = true;
   return VCLHIP_SUCCESS;
 }
@@ -1686,18 +1740,28 @@ static int mixed_cg_solve(const M* A, const float* values32, const double* b, do
   // CSR and the slices pad little (one wavefront per slice, no row pointers), else in A's format.
   DevOp<float> op32;
   vclhip_sell inner_sell{};
+  const int16_t* inner_dcol = nullptr;
+  size_t inner_stored = 0;
   bool inner_is_sell = false;
+  if (P->inner_format < VCLHIP_INNER_AUTO || P->inner_format > VCLHIP_INNER_SELL64_D16)
+    return invalid("inner_format");
   if constexpr (std::is_same<M, vclhip_csr>::value) {
     if (!values32 && P->inner_format != VCLHIP_INNER_CSR) {
-      if (P->inner_format != VCLHIP_INNER_AUTO && P->inner_format != VCLHIP_INNER_SELL64)
-        return invalid("inner_format");
-      VCLHIP_TRY(build_inner_sell(A, ws, s, 1.25, P->inner_format == VCLHIP_INNER_SELL64, &inner_sell,
+      const bool forced = P->inner_format != VCLHIP_INNER_AUTO;
+      const bool allow16 = P->inner_format == VCLHIP_INNER_AUTO || P->inner_format == VCLHIP_INNER_SELL64_D16;
+      VCLHIP_TRY(build_inner_sell(A, ws, s, 1.25, forced, allow16, &inner_sell, &inner_dcol, &inner_stored,
                                   &inner_is_sell));
+      if (P->inner_format == VCLHIP_INNER_SELL64_D16 && !inner_dcol) {
+        set_error("inner_format SELL64_D16: a column is more than 32767 from its row");
+        return VCLHIP_ERROR_NOT_SUPPORTED;
+      }
     }
   }
   M A32{};
   if (inner_is_sell) {
     VCLHIP_TRY(make_op(&inner_sell, op32));
+    op32.sm.dcol = inner_dcol;
+    op32.nt = stream_loads(inner_stored * (sizeof(float) + (inner_dcol ? 2 : 4)));
   } else {
     float* v32 = const_cast<float*>(values32);
     if (!v32) {

@@ -21,6 +21,15 @@ void set_error(const char* fmt, ...)
 
 const char* last_error() { return g_last_error; }
 
+static thread_local LaunchTiming g_timing;
+
+LaunchTiming take_launch_timing()
+{
+  const LaunchTiming t = g_timing;
+  g_timing = LaunchTiming();
+  return t;
+}
+
 int device_cu_count()
 {
   static std::mutex mu;
@@ -200,6 +209,13 @@ int vclhip_stream_synchronize(void* stream)
 int vclhip_device_synchronize(void)
 {
   VCLHIP_HIP(hipDeviceSynchronize());
+  return VCLHIP_SUCCESS;
+}
+
+int vclhip_time_next_launch(void* start_event, void* stop_event)
+{
+  vclhip::g_timing.start = static_cast<hipEvent_t>(start_event);
+  vclhip::g_timing.stop = static_cast<hipEvent_t>(stop_event);
   return VCLHIP_SUCCESS;
 }
 

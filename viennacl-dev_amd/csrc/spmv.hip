@@ -16,6 +16,13 @@ static uint32_t env_u32(const char* name, uint32_t dflt, uint32_t lo, uint32_t h
   return static_cast<uint32_t>(v);
 }
 
+bool stream_loads(size_t bytes)
+{
+  static const uint32_t mode = env_u32("VCLHIP_STREAM_LOADS", 2, 0, 2);
+  if (mode < 2) return mode == 1;
+  return bytes > (size_t(256) << 20);
+}
+
 uint32_t csr_gmax()
 {
   static const uint32_t g = env_u32("VCLHIP_CSR_GMAX", 2048, 8, kMaxPartials);
@@ -101,14 +108,21 @@ static int csr_spmv(const vclhip_csr* A, const T* x, uint32_t xs, uint32_t xi, T
   if (x == y) return invalid("x and y must not alias");
   const uint32_t grid = chunk_grid(A->num_blocks, spmv_gmax());
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const bool nt = stream_loads(static_cast<size_t>(A->nnz) * (sizeof(T) + 4));
+  T* const np = nullptr;
+  const uint32_t* const ns = nullptr;
   if (xs == 0 && xi == 1 && ys == 0 && yi == 1) {
     EpiAxpby<T, true> epi{y, 0, 1, alpha, beta};
-    hipLaunchKernelGGL((csr_kernel<T, true, EpiAxpby<T, true> >), dim3(grid), dim3(kBlock), 0, s, csr_dev<T>(A), x,
-                       0u, 1u, epi, nullptr, 0u, nullptr, 0u);
+    if (nt)
+      launch_timed(csr_kernel<T, true, EpiAxpby<T, true>, true>, dim3(grid), dim3(kBlock), s, csr_dev<T>(A), x, 0u,
+                   1u, epi, np, 0u, ns, 0u);
+    else
+      launch_timed(csr_kernel<T, true, EpiAxpby<T, true>, false>, dim3(grid), dim3(kBlock), s, csr_dev<T>(A), x, 0u,
+                   1u, epi, np, 0u, ns, 0u);
   } else {
     EpiAxpby<T, false> epi{y, ys, yi, alpha, beta};
-    hipLaunchKernelGGL((csr_kernel<T, false, EpiAxpby<T, false> >), dim3(grid), dim3(kBlock), 0, s, csr_dev<T>(A),
-                       x, xs, xi, epi, nullptr, 0u, nullptr, 0u);
+    launch_timed(csr_kernel<T, false, EpiAxpby<T, false>, false>, dim3(grid), dim3(kBlock), s, csr_dev<T>(A), x, xs,
+                 xi, epi, np, 0u, ns, 0u);
   }
   return check_launch("csr_kernel");
 }
@@ -124,14 +138,22 @@ static int sell_spmv(const vclhip_sell* A, const T* x, uint32_t xs, uint32_t xi,
   const uint32_t tiles = (A->n_rows + kBlock - 1) / kBlock;
   const uint32_t grid = chunk_grid(tiles, spmv_gmax());
   hipStream_t s = static_cast<hipStream_t>(stream);
+  // the stored size is not in the descriptor: nontemporal loads only when asked for
+  const bool nt = stream_loads(0);
+  T* const np = nullptr;
+  const uint32_t* const ns = nullptr;
   if (xs == 0 && xi == 1 && ys == 0 && yi == 1) {
     EpiAxpby<T, true> epi{y, 0, 1, alpha, beta};
-    hipLaunchKernelGGL((sell_kernel<T, true, EpiAxpby<T, true> >), dim3(grid), dim3(kBlock), 0, s, sell_dev<T>(A), x,
-                       0u, 1u, epi, nullptr, 0u, nullptr, 0u);
+    if (nt)
+      launch_timed(sell_kernel<T, true, EpiAxpby<T, true>, false, true>, dim3(grid), dim3(kBlock), s, sell_dev<T>(A),
+                   x, 0u, 1u, epi, np, 0u, ns, 0u);
+    else
+      launch_timed(sell_kernel<T, true, EpiAxpby<T, true>, false, false>, dim3(grid), dim3(kBlock), s,
+                   sell_dev<T>(A), x, 0u, 1u, epi, np, 0u, ns, 0u);
   } else {
     EpiAxpby<T, false> epi{y, ys, yi, alpha, beta};
-    hipLaunchKernelGGL((sell_kernel<T, false, EpiAxpby<T, false> >), dim3(grid), dim3(kBlock), 0, s,
-                       sell_dev<T>(A), x, xs, xi, epi, nullptr, 0u, nullptr, 0u);
+    launch_timed(sell_kernel<T, false, EpiAxpby<T, false>, false, false>, dim3(grid), dim3(kBlock), s,
+                 sell_dev<T>(A), x, xs, xi, epi, np, 0u, ns, 0u);
   }
   return check_launch("sell_kernel");
 }

@@ -39,7 +39,18 @@ struct SellDev {
   const T* __restrict__ val;
   uint32_t n_rows;
   uint32_t C;
+  const int16_t* __restrict__ dcol = nullptr;  // Delta kernels: column - row as 16 bits (col unused)
 };
+
+// Loads of the matrix streams: nontemporal (not kept in the caches) when NT holds.
+template <bool NT, typename U>
+__device__ __forceinline__ U ld_stream(const U* p)
+{
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
 
 template <typename T, bool Contig>
 __device__ __forceinline__ T load_x(const T* __restrict__ x, uint32_t c, uint32_t xs, uint32_t xi)
@@ -67,7 +78,7 @@ struct EpiAxpby {
 };
 
 // CSR SpMV over the row-block plan.  Each logical workgroup walks a contiguous chunk of blocks.
-template <typename T, bool Contig, class Epi>
+template <typename T, bool Contig, class Epi, bool NT = false>
 __global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __restrict__ x, uint32_t xs, uint32_t xi,
                                                      Epi epi, typename Epi::acc_t* __restrict__ part,
                                                      uint32_t pstride, const uint32_t* __restrict__ stop,
@@ -106,18 +117,33 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __res
       continue;
     }
 
-    // Phase 1: coalesced loads of the block's entries; products into LDS.
+    // Phase 1: coalesced loads of the block's entries; products into LDS.  Every lane issues all its
+    // loads before using any (indices past the block are clamped to its last entry, their products
+    // are not stored), so a wavefront has U value/column loads and then U gathers in flight.  With
+    // NT the matrix streams are read nontemporally (for matrices larger than the Infinity Cache,
+    // whose lines would not be reused anyway).
     const uint32_t cnt = e1 - e0;
     const uint32_t nrows = r1 - r0;
+    if (tid < nrows) srp[tid + 1] = A.row_ptr[r0 + tid + 1] - e0;
+    if (cnt > 0) {
+      constexpr int U = kCsrNnzCap / kBlock;
+      uint32_t c[U];
+      T v[U];
 #pragma unroll
-    for (uint32_t k = 0; k < kCsrNnzCap / kBlock; ++k) {
-      const uint32_t i = tid + k * kBlock;
-      if (i < cnt) {
-        const uint32_t e = e0 + i;
-        stage[i] = A.val[e] * load_x<T, Contig>(x, A.col[e], xs, xi);
+      for (int k = 0; k < U; ++k) {
+        const uint32_t e = e0 + min(tid + k * kBlock, cnt - 1);
+        c[k] = ld_stream<NT>(A.col + e);
+        v[k] = ld_stream<NT>(A.val + e);
+      }
+      T xv[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) xv[k] = load_x<T, Contig>(x, c[k], xs, xi);
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const uint32_t i = tid + k * kBlock;
+        if (i < cnt) stage[i] = v[k] * xv[k];
       }
     }
-    if (tid < nrows) srp[tid + 1] = A.row_ptr[r0 + tid + 1] - e0;
     if (tid == 0) srp[0] = 0;
     __syncthreads();
 
@@ -141,7 +167,9 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __res
 
 // SELL-C SpMV, one lane per row; with C = 64 a wavefront is exactly one slice, so all lanes of a
 // wavefront run the same number of columns and every load is one contiguous 64-lane access.
-template <typename T, bool Contig, class Epi>
+// Delta: the columns are stored as 16-bit offsets from the row (the mixed solver's inner copy),
+// which cuts the index stream from 4 to 2 bytes per stored entry.
+template <typename T, bool Contig, class Epi, bool Delta = false, bool NT = false>
 __global__ __launch_bounds__(kBlock) void sell_kernel(SellDev<T> A, const T* __restrict__ x, uint32_t xs,
                                                       uint32_t xi, Epi epi, typename Epi::acc_t* __restrict__ part,
                                                       uint32_t pstride, const uint32_t* __restrict__ stop,
@@ -169,14 +197,26 @@ __global__ __launch_bounds__(kBlock) void sell_kernel(SellDev<T> A, const T* __r
       const uint32_t ncols = A.cpb[s];
       const size_t base = static_cast<size_t>(A.start[s]) + lane;
       const T* __restrict__ ev = A.val + base;
-      const uint32_t* __restrict__ cv = A.col + base;
       T sum = 0;
+      if constexpr (Delta) {
+        const int16_t* __restrict__ dv = A.dcol + base;
 #pragma unroll 4
-      for (uint32_t k = 0; k < ncols; ++k) {
-        const size_t off = static_cast<size_t>(k) * A.C;
-        const T v = ev[off];
-        const T xv = load_x<T, Contig>(x, cv[off], xs, xi);
-        sum += (v > 0 || v < 0) ? xv * v : T(0);  // host_based/sparse_matrix_operations.hpp:1831-1833
+        for (uint32_t k = 0; k < ncols; ++k) {
+          const size_t off = static_cast<size_t>(k) * A.C;
+          const T v = ld_stream<NT>(ev + off);
+          const uint32_t c = static_cast<uint32_t>(static_cast<int32_t>(row) + ld_stream<NT>(dv + off));
+          const T xv = load_x<T, Contig>(x, c, xs, xi);
+          sum += (v > 0 || v < 0) ? xv * v : T(0);
+        }
+      } else {
+        const uint32_t* __restrict__ cv = A.col + base;
+#pragma unroll 4
+        for (uint32_t k = 0; k < ncols; ++k) {
+          const size_t off = static_cast<size_t>(k) * A.C;
+          const T v = ld_stream<NT>(ev + off);
+          const T xv = load_x<T, Contig>(x, ld_stream<NT>(cv + off), xs, xi);
+          sum += (v > 0 || v < 0) ? xv * v : T(0);  // host_based/sparse_matrix_operations.hpp:1831-1833
+        }
       }
       epi(row, sum, acc);
     }
@@ -196,6 +236,9 @@ inline uint32_t chunk_grid(uint32_t units, uint32_t gmax)
 
 // Tunables read once from the environment (VCLHIP_CSR_GMAX, VCLHIP_SELL_GMAX, VCLHIP_VEC_GMAX).
 uint32_t csr_gmax();
+// Nontemporal matrix loads for a matrix of `bytes`: VCLHIP_STREAM_LOADS = 0 never, 1 always; by
+// default when the matrix is larger than the 256 MiB Infinity Cache.
+bool stream_loads(size_t bytes);
 uint32_t sell_gmax();
 uint32_t vec_gmax();
 uint32_t spmv_gmax();

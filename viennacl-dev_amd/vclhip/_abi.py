@@ -76,6 +76,7 @@ _SIGS = {
     "vclhip_stream_destroy": (i32, [vp]),
     "vclhip_stream_synchronize": (i32, [vp]),
     "vclhip_device_synchronize": (i32, []),
+    "vclhip_time_next_launch": (i32, [vp, vp]),
     "vclhip_event_create": (i32, [P(vp)]),
     "vclhip_event_destroy": (i32, [vp]),
     "vclhip_event_record": (i32, [vp, vp]),
