@@ -1,0 +1,126 @@
+"""The benchmark configurations at their full sizes (C1-C4 of the survey), through the C ABI.
+
+C1 is checked against the reference itself (oracle/_ref, the reference's host backend built from
+its sources; the restated oracle where that build is absent).  At the larger sizes the checks are
+the ones that do not depend on size: the FP64 true residual ||b - A x|| / ||b||, iteration counts
+against the oracle's (same recurrences), the mixed solve's distance to the FP64 solution, and
+every system of the batch.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def iters_close(got, want):
+    return abs(int(got) - int(want)) <= max(2, int(np.ceil(0.02 * want)))
+
+
+def device_relres(vh, A, b, x):
+    """||b - A x|| / ||b|| in FP64 with the device SpMV (for the large configurations)."""
+    import ctypes as C
+    n = b.size
+    r = vh.DeviceArray.from_numpy(b)
+    A.spmv(vh.DeviceArray.from_numpy(x), r, -1.0, 1.0)
+    nr = C.c_double()
+    vh.call("vclhip_norm2_f64", n, r.ptr, 0, 1, C.byref(nr), None)
+    return nr.value / np.linalg.norm(b)
+
+
+def test_c1_cg_fdm_256_vs_reference(dev, vh, oracle):
+    """generate_fdm_laplace(256, 256), b ~ U[-1,1] seed 0, cg_tag(1e-8, 2000)."""
+    rp, col, val = vh.fdm_laplace_2d(256, 256)
+    b = vh.random_uniform(0, rp.size - 1)
+    A = vh.CsrMatrix(rp, col, val)
+    db = vh.DeviceArray.from_numpy(b)
+    dx = vh.DeviceArray.zeros(b.size, np.float64)
+    res = vh.cg(A, db, dx, tol=1e-8, max_iters=2000)
+    if oracle.ref_available():
+        oracle.set_threads(1)
+        xr, it_r, err_r = oracle.ref_cg(rp, col, val, b, 1e-8, 2000)
+    else:
+        xr, it_r, err_r, _ = oracle.cg_pipelined(oracle.CsrHost(rp, col, val), b, 1e-8, 2000)
+    assert res.converged and iters_close(res.iters, it_r), (res.iters, it_r)
+    assert res.error < 1e-8 and abs(res.error - err_r) < 0.05 * err_r
+    x = dx.to_numpy()
+    r = b - oracle.csr_spmv(rp, col, val, x)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < 1e-8
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-6
+
+
+@pytest.mark.parametrize("fmt", ["csr", "sell"])
+def test_c2_cg_lap3d_128(dev, vh, oracle, fmt):
+    """3-D Laplacian 128^3, tol 1e-10: iterations against the oracle's pipelined CG (8 threads)."""
+    rp, col, val = vh.laplace_3d(128)
+    n = rp.size - 1
+    b = vh.random_uniform(0, n)
+    A = vh.CsrMatrix(rp, col, val) if fmt == "csr" else vh.SellMatrix(rp, col, val, 64)
+    db = vh.DeviceArray.from_numpy(b)
+    dx = vh.DeviceArray.zeros(n, np.float64)
+    res = vh.cg(A, db, dx, tol=1e-10, max_iters=5000)
+    oracle.set_threads(8)
+    try:
+        _, it_o, err_o, _ = oracle.cg_pipelined(oracle.CsrHost(rp, col, val), b, 1e-10, 5000)
+    finally:
+        oracle.set_threads(1)
+    assert res.converged and iters_close(res.iters, it_o), (res.iters, it_o)
+    Acsr = vh.CsrMatrix(rp, col, val)
+    assert device_relres(vh, Acsr, b, dx.to_numpy()) < 1e-10
+
+
+def test_c2_powerlaw_bicgstab_full_size(dev, vh):
+    """Power-law nonsymmetric matrix, n = 2,097,152 (row lengths P(k) ~ k^-2 on 1..512), BiCGStab
+    to 1e-10 without and with Jacobi: FP64 true residual at the tolerance."""
+    n = 2097152
+    rp, col, val = vh.powerlaw(n, 512, 2.0, 1.1, 1)
+    b = vh.random_uniform(0, n)
+    A = vh.CsrMatrix(rp, col, val)
+    db = vh.DeviceArray.from_numpy(b)
+    for pc in (0, 1):
+        dx = vh.DeviceArray.zeros(n, np.float64)
+        res = vh.bicgstab(A, db, dx, precond=pc, tol=1e-10, max_iters=2000)
+        assert res.converged, pc
+        assert device_relres(vh, A, b, dx.to_numpy()) < (2e-10 if pc == 0 else 1e-9), pc
+
+
+def test_c3_mixed_vs_fp64_160(dev, vh):
+    """160^3 variable-coefficient diffusion (k = exp(N(0,1)), harmonic faces), tol 1e-10: the mixed
+    solve reaches the FP64-verified tolerance, close to the FP64 CG solution, with at most 12% more
+    iterations."""
+    g = 160
+    k = np.exp(vh.random_normal(1, g ** 3))
+    rp, col, val = vh.diffusion(g, g, g, k)
+    n = rp.size - 1
+    b = vh.random_uniform(0, n)
+    A = vh.CsrMatrix(rp, col, val)
+    db = vh.DeviceArray.from_numpy(b)
+    x64 = vh.DeviceArray.zeros(n, np.float64)
+    r64 = vh.cg(A, db, x64, tol=1e-10, max_iters=20000)
+    xm = vh.DeviceArray.zeros(n, np.float64)
+    rm = vh.mixed_cg(A, db, xm, tol=1e-10, max_iters=20000, mixed_mode=1)
+    assert r64.converged and rm.converged
+    assert device_relres(vh, A, b, xm.to_numpy()) < 1e-10
+    assert rm.iters <= 1.12 * r64.iters
+    a, c = xm.to_numpy(), x64.to_numpy()
+    assert np.linalg.norm(a - c) / np.linalg.norm(c) < 1e-6
+
+
+def test_c4_batched_4096(dev, vh):
+    """4096 graph-Laplacian systems, n_i ~ U{64..4096}, tol 1e-8: every system converged with its
+    FP64 true residual at the tolerance."""
+    sys_ptr, rp, col, val = vh.graph_laplacian_batch(4096, 64, 4096, 8.0, 0.1, 1.0, 0.01, 1)
+    n = rp.size - 1
+    b = vh.random_uniform(0, n)
+    A = vh.CsrMatrix(rp, col, val)
+    plan = vh.BatchPlan(sys_ptr, rp)
+    plan.autotune(A)
+    db = vh.DeviceArray.from_numpy(b)
+    dx = vh.DeviceArray.zeros(n, np.float64)
+    it, err, conv = plan.solve(A, db, dx, tol=1e-8, max_iters=5000)
+    assert (conv == 1).all() and (err < 1e-8).all()
+    y = vh.DeviceArray.zeros(n, np.float64)
+    A.spmv(dx, y)
+    r = b - y.to_numpy()
+    starts = sys_ptr[:-1].astype(np.int64)
+    rel = np.sqrt(np.add.reduceat(r * r, starts) / np.add.reduceat(b * b, starts))
+    assert rel.max() < 1.05e-8, rel.max()
