@@ -147,6 +147,36 @@ def timed_solve(vh, kind, A, b, n, precond=0, reps=2, **kw):
             "true_relres": float(rel), "converged": bool(res.converged)}
 
 
+def powerlaw_bench(vh, stream, steps):
+    """Random nonsymmetric matrix, n = 2,097,152, row lengths P(k) ~ k^-2 on 1..512 (seed 1),
+    diagonal 1.1 x row abs-sum: cold CSR SpMV rate and BiCGStab (plain, Jacobi) to 1e-10."""
+    n = 2097152
+    t = time.perf_counter()
+    rp, col, val = vh.powerlaw(n, 512, 2.0, 1.1, 1)
+    nnz = int(rp[-1])
+    log(f"generated power-law matrix: n={n} nnz={nnz} ({time.perf_counter() - t:.1f} s)")
+    A = vh.CsrMatrix(rp, col, val)
+    x = vh.DeviceArray.from_numpy(vh.random_uniform(1, n))
+    y = vh.DeviceArray.zeros(n, np.float64)
+    evict = vh.DeviceArray(EVICT_DOUBLES, np.float64)
+    vh.call("vclhip_fill_zero", evict.ptr, C.c_size_t(evict.nbytes), None)
+    cold, warm, _ = time_launches(vh, lambda: A.spmv(x, y, stream=stream.handle), stream, steps, 3, evict)
+    evict.free()
+    by = csr_bytes(n, nnz)
+    out = {"workload": f"power-law nonsymmetric matrix n={n} nnz={nnz}, CSR, tol 1e-10", "n": n, "nnz": nnz,
+           "spmv_cold_us": round(float(cold.mean()) * 1e3, 2), "spmv_cold_gbs": round(float(by / (cold.mean() * 1e-3) / 1e9), 1),
+           "spmv_warm_gbs": round(by / (warm * 1e-3) / 1e9, 1), "algorithmic_bytes_per_spmv": by}
+    b = vh.DeviceArray.from_numpy(vh.random_uniform(0, n))
+    # without a preconditioner BiCGStab stagnates on this badly scaled matrix; 400 iterations show it
+    out["bicgstab_fp64_csr_400"] = timed_solve(vh, "bicgstab", A, b, n, reps=1, tol=1e-10, max_iters=400)
+    out["jacobi_bicgstab_fp64_csr"] = timed_solve(vh, "bicgstab", A, b, n, precond=1, tol=1e-10, max_iters=5000)
+    log("powerlaw", out)
+    for a in (x, y, b):
+        a.free()
+    A.free()
+    return out
+
+
 def mixed_vs_fp64(vh, grid, reps=2):
     """FP64 CG vs mixed-precision CG to ||b - A x|| / ||b|| <= 1e-10 on the 3-D variable-coefficient
     diffusion operator (k = exp(g), g ~ N(0,1) per cell, harmonic-mean faces; seed 1 for k, 0 for b)."""
@@ -461,9 +491,14 @@ def main():
         sol["bicgstab_fp64_csr"] = timed_solve(vh, "bicgstab", A, b, n, tol=1e-10, max_iters=5000)
         sol["jacobi_bicgstab_fp64_csr"] = timed_solve(vh, "bicgstab", A, b, n, precond=1, tol=1e-10,
                                                       max_iters=5000)
+        sol["jacobi_cg_fp64_sell64"] = timed_solve(vh, "cg", S, b, n, precond=1, tol=1e-10, max_iters=5000)
+        sol["bicgstab_fp64_sell64"] = timed_solve(vh, "bicgstab", S, b, n, tol=1e-10, max_iters=5000)
+        sol["jacobi_bicgstab_fp64_sell64"] = timed_solve(vh, "bicgstab", S, b, n, precond=1, tol=1e-10,
+                                                         max_iters=5000)
         for k, v in sol.items():
             log(k, v)
         out["solvers_lap3d_128"] = sol
+        out["powerlaw_2M"] = powerlaw_bench(vh, stream, args.steps)
     if args.mixed_grid:
         out["mixed_vs_fp64"] = mixed_vs_fp64(vh, args.mixed_grid)
     if args.batch:

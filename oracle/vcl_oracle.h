@@ -17,8 +17,10 @@
  *     pinned bit-for-bit against the reference build (oracle/_ref) and the
  *     golden fixtures under tests/golden/, plus closed forms (Laplacian
  *     eigenpairs, row sums) and dense numpy solves in tests/test_oracle_math.py.
- *   - orc_batched_cg / orc_adjoint_grad: no reference counterpart; pinned by
- *     dense numpy solves and central finite differences.
+ *   - orc_mixed_cg2_csr mode 1 (reliable updates), orc_batched_cg2_csr (pipelined CG per
+ *     block, FP64 or FP32), orc_adjoint_grad_csr: no reference counterpart; pinned by the FP64
+ *     true residual, dense numpy solves and central finite differences (tests/test_oracle_math.py).
+ *     orc_mixed_cg2_csr mode 0 with acc64 = 0 is orc_mixed_cg_csr (pinned against the reference).
  *
  * Index type is uint32_t as in compressed_matrix<T> (compressed_matrix.hpp:57-58).
  */

@@ -32,21 +32,25 @@ def test_status_strings_and_errors_without_device(vh):
 
 
 def check_plan(rp, blocks):
-    from math import log2
+    """Blocks of <= 256 rows and <= 2048 entries (mode 1 when a row is longer than 32, else 0); a row
+    longer than 2048 is a block of its own (mode 7); blocks are maximal (greedy)."""
     n = rp.size - 1
     assert blocks[0] & ((1 << 28) - 1) == 0 and blocks[-1] == n
     rows = blocks & ((1 << 28) - 1)
     modes = blocks >> 28
     assert np.all(np.diff(rows.astype(np.int64)) > 0)
+    lens_all = np.diff(rp.astype(np.int64))
     for b in range(blocks.size - 1):
         r0, r1, m = rows[b], rows[b + 1], modes[b]
-        lens = np.diff(rp[r0:r1 + 1].astype(np.int64))
+        lens = lens_all[r0:r1]
         if m == 7:
             assert r1 == r0 + 1 and lens[0] > 2048
-        else:
-            assert lens.sum() <= 2048 and (r1 - r0) << m <= 256 and m <= 6
-            want = max(1, -(-int(lens.max()) // 8)) if lens.size else 1
-            assert (1 << m) >= min(want, 64)
+            continue
+        assert m in (0, 1) and (m == 1) == bool((lens > 32).any())
+        assert lens.sum() <= 2048 and r1 - r0 <= 256 and (lens <= 2048).all()
+        if r1 < n:  # greedy: the next row did not fit
+            nxt = lens_all[r1]
+            assert r1 - r0 == 256 or nxt > 2048 or lens.sum() + nxt > 2048
 
 
 @pytest.mark.parametrize("kind", ["lap3d", "powerlaw", "long_rows", "empty"])
@@ -63,9 +67,13 @@ def test_block_plan_invariants(vh, kind):
     blocks = vh.plan_blocks(rp)
     check_plan(rp, blocks)
     if kind == "lap3d":
-        # 7-point rows: one lane per row, 256 rows per block
+        # 7-point rows: short rows only, 256 rows per block
         assert np.all(blocks[:-1] >> 28 == 0)
         assert np.median(np.diff(blocks & ((1 << 28) - 1))) == 256
+    if kind == "powerlaw":
+        # long rows no longer shrink the blocks: most blocks are full (256 rows or 2048 entries)
+        sizes = np.diff(blocks & ((1 << 28) - 1))
+        assert np.median(sizes) >= 200
 
 
 def test_rng_is_the_documented_mt19937_64(vh):

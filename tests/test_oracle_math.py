@@ -135,6 +135,52 @@ def test_mixed_cg_true_residual(oracle, vh):
     assert rel < 1e-10 and abs(rel - err) < 1e-12 and restarts >= 2
 
 
+@pytest.mark.parametrize("jacobi", [False, True])
+def test_mixed_reliable_update_pins(oracle, vh, jacobi):
+    """Mixed CG with reliable updates (no reference counterpart): the FP64 true residual is below the
+    tolerance and is the error it reports, x matches a dense solve, the FP32 iteration count stays
+    within 15% of FP64 CG's, and the history is the FP32 <r,r> that the updates reset."""
+    k = np.exp(vh.random_normal(3, 9 * 8 * 7))
+    rp, col, val = vh.diffusion(9, 8, 7, k)
+    A = dense(rp, col, val)
+    b = vh.random_uniform(0, rp.size - 1)
+    diag = oracle.row_info(rp, col, val, 3) if jacobi else None
+    x, it, err, upd, hist = oracle.mixed_cg2(oracle.CsrHost(rp, col, val), b, 1e-10, 3000, mode=1, diag=diag,
+                                             history=True)
+    rel = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    assert rel < 1e-10 and abs(rel - err) < 1e-12 and upd >= 2
+    assert np.linalg.norm(x - np.linalg.solve(A, b)) / np.linalg.norm(x) < 1e-8
+    _, it64, _, _ = oracle.cg_pipelined(oracle.CsrHost(rp, col, val), b, 1e-10, 3000)
+    if not jacobi:
+        assert it <= 1.15 * it64
+    assert hist.size == it and (hist > 0).all()
+
+
+def test_mixed_reference_schedule_accumulation(oracle, vh):
+    """mode 0 with FP32-accumulated inner products is the reference restatement itself; FP64
+    accumulation (the HIP kernels') moves the iteration count only slightly."""
+    rp, col, val = vh.laplace_3d(12, 11, 10)
+    b = vh.random_uniform(0, rp.size - 1)
+    A = oracle.CsrHost(rp, col, val)
+    x0, it0, err0, _ = oracle.mixed_cg(A, b, 1e-10, 3000)
+    x1, it1, err1, _, _ = oracle.mixed_cg2(A, b, 1e-10, 3000, mode=0, acc64=False)
+    assert it0 == it1 and err0 == err1 and np.array_equal(x0, x1)
+    _, it2, err2, _, _ = oracle.mixed_cg2(A, b, 1e-10, 3000, mode=0, acc64=True)
+    assert abs(it2 - it0) <= max(2, 0.02 * it0) and err2 < 1e-10
+
+
+def test_batched_fp32_against_dense(oracle, vh):
+    sys_ptr, rp, col, val = vh.graph_laplacian_batch(4, 20, 60, seed=5)
+    b = vh.random_uniform(0, rp.size - 1)
+    x, iters, errs = oracle.batched_cg(sys_ptr, oracle.CsrHost(rp, col, val), b, 1e-5, 500, single=True)
+    A = dense(rp, col, val)
+    for s in range(4):
+        r0, r1 = sys_ptr[s], sys_ptr[s + 1]
+        xs = np.linalg.solve(A[r0:r1, r0:r1], b[r0:r1])
+        assert np.linalg.norm(x[r0:r1] - xs) / np.linalg.norm(xs) < 1e-4
+        assert errs[s] < 1e-5
+
+
 def test_batched_oracle_per_system(oracle, vh):
     sys_ptr, rp, col, val = vh.graph_laplacian_batch(5, 10, 40, seed=2)
     b = vh.random_uniform(0, rp.size - 1)

@@ -147,17 +147,39 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __res
     if (tid == 0) srp[0] = 0;
     __syncthreads();
 
-    // Phase 2: 2^mode lanes per row sum the staged products.
-    const uint32_t L = 1u << mode;
-    const uint32_t g = tid >> mode;
-    const uint32_t lane = tid & (L - 1);
-    if (g < nrows) {
-      const uint32_t a = srp[g];
-      const uint32_t z = srp[g + 1];
-      T s = 0;
-      for (uint32_t k = a + lane; k < z; k += L) s += stage[k];
-      s = group_sum(s, L);
-      if (lane == 0) epi(r0 + g, s, acc);
+    // Phase 2a: each row of at most kShortRow entries is summed by one lane, in order.
+    if (tid < nrows) {
+      const uint32_t a = srp[tid];
+      const uint32_t z = srp[tid + 1];
+      if (z - a <= kShortRow) {
+        T s = 0;
+        for (uint32_t k = a; k < z; ++k) s += stage[k];
+        epi(r0 + tid, s, acc);
+      }
+    }
+    // Phase 2b: each longer row is summed by one wavefront (strided partial sums, xor tree); the
+    // four wavefronts take the block's long rows in turn.
+    if (mode == kModeMixed) {
+      const uint32_t wave = tid / kWave;
+      const uint32_t lane = tid & (kWave - 1);
+      uint32_t turn = 0;
+      for (uint32_t base = 0; base < nrows; base += kWave) {
+        const uint32_t i = base + lane;
+        const bool is_long = i < nrows && srp[i + 1] - srp[i] > kShortRow;
+        unsigned long long m = __ballot(is_long);
+        while (m) {
+          const uint32_t bit = static_cast<uint32_t>(__builtin_ctzll(m));
+          m &= m - 1;
+          if ((turn++ & (kWavesPerBlock - 1)) != wave) continue;
+          const uint32_t row = base + bit;
+          const uint32_t a = srp[row];
+          const uint32_t z = srp[row + 1];
+          T s = 0;
+          for (uint32_t k = a + lane; k < z; k += kWave) s += stage[k];
+          s = wave_sum(s);
+          if (lane == 0) epi(r0 + row, s, acc);
+        }
+      }
     }
     __syncthreads();
   }
