@@ -356,6 +356,25 @@ int vclhip_batched_cg_solve_f32(vclhip_batch_plan* plan, const vclhip_csr* A, co
                                 const float* b_dev, float* x_dev, const vclhip_solver_params* params,
                                 uint32_t* iters, double* errors, uint32_t* converged, void* stream);
 
+/* ---------------------------------------------------------------- MatrixMarket files */
+/* io::read_matrix_market_file (io/matrix_market.hpp:102-370) into CSR host arrays.  Coordinate
+ * files with a real, pattern or complex field (pattern entries are 1, complex entries keep the real
+ * part) and a general or symmetric structure (a symmetric file also sets (col, row)); '%' comment
+ * and blank lines are skipped; indices are shifted by index_base and checked; reading stops after
+ * the entry count of the size line; an entry given twice keeps its last value.  Columns come out
+ * sorted within each row.  *lines_read receives the number of lines read (the reference reader's
+ * return value).  The matrix stays allocated until vclhip_mm_free; vclhip_mm_csr copies it into
+ * row_ptr (n_rows + 1), col and val (nnz each). */
+typedef struct vclhip_mm_matrix vclhip_mm_matrix;
+int vclhip_mm_read(const char* path, int64_t index_base, vclhip_mm_matrix** matrix, uint32_t* n_rows,
+                   uint32_t* n_cols, uint32_t* nnz, int* symmetric, int64_t* lines_read);
+int vclhip_mm_csr(const vclhip_mm_matrix* matrix, uint32_t* row_ptr, uint32_t* col, double* val);
+int vclhip_mm_free(vclhip_mm_matrix* matrix);
+/* io::write_matrix_market_file (io/matrix_market.hpp:375-433): a "coordinate real general" file,
+ * one "row col value" line per entry in row order, values in the default iostream format (%g). */
+int vclhip_mm_write(const char* path, int64_t index_base, uint32_t n_rows, uint32_t n_cols, const uint32_t* row_ptr,
+                    const uint32_t* col, const double* val);
+
 /* ---------------------------------------------------------------- tools ---- */
 /* Project RNG: std::mt19937_64(seed), u = (draw >> 11) * 2^-53; U[a,b] = a + (b-a) u; normals by
  * Box-Muller on consecutive pairs (r = sqrt(-2 ln(1-u1)), z0 = r cos(2 pi u2), z1 = r sin(2 pi u2)).

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "viennacl/compressed_matrix.hpp"
+#include "viennacl/io/matrix_market.hpp"
 #include "viennacl/linalg/bicgstab.hpp"
 #include "viennacl/linalg/cg.hpp"
 #include "viennacl/linalg/inner_prod.hpp"
@@ -342,6 +343,41 @@ void test_mixed_precision()
   CHECK(std::sqrt(num / den) < 1e-7, "mixed vs FP64 solution %g", std::sqrt(num / den));
 }
 
+void test_matrix_market()
+{
+  // write an STL matrix, read it back into the STL type and into compressed_matrix
+  // (io/matrix_market.hpp; the writer keeps 6 significant digits, as the reference's does)
+  std::mt19937_64 g(11);
+  StlMatrix<double> A = random_matrix<double>(700, g);
+  const char* file = "/tmp/vclhip_sparse_krylov_test.mtx";
+  viennacl::io::write_matrix_market_file(A, file);
+  StlMatrix<double> B;
+  long lines = viennacl::io::read_matrix_market_file(B, file);
+  size_t nnz = 0;
+  for (auto const& r : A) nnz += r.size();
+  CHECK(lines == long(nnz) + 2 && B.size() == A.size(), "mm read lines %ld", lines);
+  bool same_pattern = true;
+  double max_rel = 0;
+  for (size_t i = 0; i < A.size() && same_pattern; ++i) {
+    same_pattern = A[i].size() == B[i].size();
+    for (auto const& kv : A[i]) {
+      auto it = B[i].find(kv.first);
+      if (it == B[i].end()) { same_pattern = false; break; }
+      max_rel = std::max(max_rel, std::fabs(it->second - kv.second) / std::fabs(kv.second));
+    }
+  }
+  CHECK(same_pattern && max_rel < 5e-6, "mm round trip: pattern %d, max rel %g", int(same_pattern), max_rel);
+  viennacl::compressed_matrix<double> vA;
+  CHECK(viennacl::io::read_matrix_market_file(vA, file) > 0 && vA.nnz() == nnz, "mm into compressed_matrix");
+  std::vector<double> x(700, 1.0);
+  viennacl::vector<double> vx(700), vy(700);
+  viennacl::copy(x, vx);
+  vy = viennacl::linalg::prod(vA, vx);
+  CHECK(rel_inf(to_host(vy), stl_prod(B, x)) < 1e-13, "SpMV of the file's matrix");
+  CHECK(viennacl::io::read_matrix_market_file(B, "does_not_exist.mtx") == EXIT_FAILURE, "missing file status");
+  std::remove(file);
+}
+
 int main()
 {
   int ndev = 0;
@@ -354,6 +390,7 @@ int main()
   test_solvers<double>(1e-10, 1e-9);
   test_solvers<float>(1e-5, 1e-4);
   test_mixed_precision();
+  test_matrix_market();
   if (failures) {
     std::printf("%d FAILED\n", failures);
     return 1;

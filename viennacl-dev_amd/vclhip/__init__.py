@@ -290,6 +290,34 @@ class BatchPlan:
             pass
 
 
+# ------------------------------------------------------------------ MatrixMarket files (host)
+
+def read_matrix_market(path, index_base=1):
+    """io::read_matrix_market_file: returns (row_ptr, col, val, n_cols, symmetric, lines_read)."""
+    h = C.c_void_p()
+    nr, nc, nz, sym, lines = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_int(), C.c_int64()
+    call("vclhip_mm_read", str(path).encode(), index_base, C.byref(h), C.byref(nr), C.byref(nc), C.byref(nz),
+         C.byref(sym), C.byref(lines))
+    try:
+        rp = np.zeros(nr.value + 1, np.uint32)
+        col = np.zeros(max(nz.value, 1), np.uint32)
+        val = np.zeros(max(nz.value, 1), np.float64)
+        call("vclhip_mm_csr", h, _ptr(rp), _ptr(col), _ptr(val))
+    finally:
+        call("vclhip_mm_free", h)
+    return rp, col[: nz.value], val[: nz.value], nc.value, bool(sym.value), lines.value
+
+
+def write_matrix_market(path, row_ptr, col, val, n_cols=None, index_base=1):
+    """io::write_matrix_market_file for CSR host arrays."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    col = np.ascontiguousarray(col, dtype=np.uint32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    n = rp.size - 1
+    call("vclhip_mm_write", str(path).encode(), index_base, n, n if n_cols is None else n_cols, _ptr(rp),
+         _ptr(col) if col.size else None, _ptr(val) if val.size else None)
+
+
 # ------------------------------------------------------------------ generators (host)
 
 def random_uniform(seed, n, a=-1.0, b=1.0):

@@ -131,6 +131,10 @@ _SIGS = {
     "vclhip_batch_plan_lanes": (i32, [vp, i32, vp]),
     "vclhip_batched_cg_solve_f64": (i32, [vp, P(Csr), vp, vp, vp, P(SolverParams), vp, vp, vp, vp]),
     "vclhip_batched_cg_solve_f32": (i32, [vp, P(Csr), vp, vp, vp, P(SolverParams), vp, vp, vp, vp]),
+    "vclhip_mm_read": (i32, [C.c_char_p, C.c_int64, P(vp), P(u32), P(u32), P(u32), P(i32), P(C.c_int64)]),
+    "vclhip_mm_csr": (i32, [vp, vp, vp, vp]),
+    "vclhip_mm_free": (i32, [vp]),
+    "vclhip_mm_write": (i32, [C.c_char_p, C.c_int64, u32, u32, vp, vp, vp]),
     "vclhip_random_uniform": (i32, [u64, u64, f64, f64, vp]),
     "vclhip_random_normal": (i32, [u64, u64, f64, f64, vp]),
     "vclhip_gen_fdm_laplace_2d": (i32, [u32, u32, vp, vp, vp, P(u32)]),
