@@ -138,13 +138,72 @@ def timed_solve(vh, kind, A, b, n, precond=0, reps=2, **kw):
     for _ in range(reps):
         vh.call("vclhip_device_synchronize")
         t0 = time.perf_counter()
-        res = (vh.cg if kind == "cg" else vh.bicgstab)(A, b, x, precond=precond, **kw)
+        if kind == "gmres":
+            res = vh.gmres(A, b, x, **kw)
+        else:
+            res = (vh.cg if kind == "cg" else vh.bicgstab)(A, b, x, precond=precond, **kw)
         vh.call("vclhip_device_synchronize")
         t = time.perf_counter() - t0
         best = t if best is None else min(best, t)
     rel = true_relres(vh, A, b, x, n)
-    return {"seconds": round(best, 6), "iters": int(res.iters), "error": float(res.error),
-            "true_relres": float(rel), "converged": bool(res.converged)}
+    out = {"seconds": round(best, 6), "iters": int(res.iters), "error": float(res.error),
+           "true_relres": float(rel), "converged": bool(res.converged)}
+    if kind == "gmres":
+        out["cycles"] = int(res.restarts)
+    return out
+
+
+def gmres_cycle_bytes(n, spmv_bytes, K, vec_bytes=8):
+    """Algorithmic bytes of one GMRES(K) cycle as the HIP solver runs it: K SpMVs (the first also
+    reads r), per step k >= 1 one pass over k + 1 basis vectors (<v_j, v_k>) and one reading k + 2
+    and writing one (v_k -= sum h_j v_j, ||v_k||^2, <r, v_k>), the update x += c0 r + sum c_j v_{j-1}
+    (x read and written, r and K - 1 basis vectors read) and the restart r = b - A x (SpMV plus b)."""
+    passes = 1 + sum(2 * k + 4 for k in range(1, K)) + (K + 2) + 1
+    return K * spmv_bytes + spmv_bytes + passes * vec_bytes * n
+
+
+def gmres_bench(vh, grid, reps=2):
+    """Nonsymmetric 7-point convection-diffusion (upwind, beta = (0.5, 0.3, 0.2)) on a grid^3 mesh,
+    GMRES(K) without preconditioner to 1e-8 (gmres_tag), CSR and SELL-64, FP64 and FP32."""
+    rp, col, val = vh.convection_diffusion(grid, grid, grid)
+    n, nnz = rp.size - 1, int(rp[-1])
+    A = vh.CsrMatrix(rp, col, val)
+    S = vh.SellMatrix(rp, col, val, 64)
+    b = vh.DeviceArray.from_numpy(vh.random_uniform(0, n))
+    out = {"workload": f"convection-diffusion {grid}^3 (n={n}, nnz={nnz}), GMRES(K) to 1e-8, no preconditioner",
+           "n": n, "nnz": nnz}
+    spmv = csr_bytes(n, nnz)
+    for name, M, K, kw in (("gmres20_fp64_csr", A, 20, {}), ("gmres20_fp64_csr_graph", A, 20, {"use_graph": True}),
+                           ("gmres20_fp64_sell64", S, 20, {}), ("gmres40_fp64_csr", A, 40, {}),
+                           ("gmres10_fp64_csr", A, 10, {})):
+        rec = timed_solve(vh, "gmres", M, b, n, reps=reps, tol=1e-8, max_iters=6000, krylov_dim=K, **kw)
+        moved = rec["cycles"] * gmres_cycle_bytes(n, spmv, K)
+        rec["ms_per_step"] = round(rec["seconds"] * 1e3 / max(rec["iters"], 1), 4)
+        rec["effective_gbs"] = round(moved / rec["seconds"] / 1e9, 1)
+        out[name] = rec
+        log("gmres", name, rec)
+    # FP32 to 1e-5 (FP32 matrix and basis, FP64-accumulated inner products)
+    A32 = vh.CsrMatrix(rp, col, val, dtype=np.float32)
+    b32 = vh.DeviceArray.from_numpy(b.to_numpy().astype(np.float32))
+    x32 = vh.DeviceArray.zeros(n, np.float32)
+    best = None
+    for _ in range(reps):
+        vh.call("vclhip_device_synchronize")
+        t0 = time.perf_counter()
+        res = vh.gmres(A32, b32, x32, tol=1e-5, max_iters=6000, krylov_dim=20)
+        vh.call("vclhip_device_synchronize")
+        t = time.perf_counter() - t0
+        best = t if best is None else min(best, t)
+    x64 = vh.DeviceArray.from_numpy(x32.to_numpy().astype(np.float64))
+    out["gmres20_fp32_csr_1e-5"] = {"seconds": round(best, 6), "iters": int(res.iters), "cycles": int(res.restarts),
+                                    "error": float(res.error), "converged": bool(res.converged),
+                                    "true_relres_fp64": float(true_relres(vh, A, b, x64, n))}
+    log("gmres", "gmres20_fp32_csr_1e-5", out["gmres20_fp32_csr_1e-5"])
+    for a in (b, b32, x32, x64):
+        a.free()
+    for M in (A, S, A32):
+        M.free()
+    return out
 
 
 def powerlaw_bench(vh, stream, steps):
@@ -396,6 +455,7 @@ def main():
     ap.add_argument("--no-solvers", action="store_true", help="skip the 128^3 CG/BiCGStab solves")
     ap.add_argument("--mixed-grid", type=int, default=160,
                     help="grid of the FP64-vs-mixed CG comparison (0 skips it)")
+    ap.add_argument("--gmres-grid", type=int, default=128, help="grid of the GMRES runs (0 skips them)")
     ap.add_argument("--batch", type=int, default=4096, help="systems of the batched CG run (0 skips it)")
     ap.add_argument("--adjoint-grid", type=int, default=512, help="grid of the differentiable solve (0 skips it)")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -501,6 +561,8 @@ def main():
         out["powerlaw_2M"] = powerlaw_bench(vh, stream, args.steps)
     if args.mixed_grid:
         out["mixed_vs_fp64"] = mixed_vs_fp64(vh, args.mixed_grid)
+    if args.gmres_grid:
+        out["gmres_convdiff"] = gmres_bench(vh, args.gmres_grid)
     if args.batch:
         out["batched_cg"] = batched_cg_bench(vh, args.batch)
     if args.adjoint_grid:

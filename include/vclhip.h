@@ -223,6 +223,36 @@ int vclhip_pipelined_bicgstab_prod_csr_f64(const vclhip_csr* A, const double* p_
 int vclhip_pipelined_bicgstab_prod_sell_f64(const vclhip_sell* A, const double* p_dev, double* Ap_dev,
                                             const double* r0star_dev, double* buf_dev, uint32_t chunk,
                                             uint32_t offset, void* stream);
+/* linalg::pipelined_gmres_prod (iterative_operations.hpp:393-418; host_based :931-937), which is
+ * pipelined_cg_prod: Ap = A p; chunk 1 <- <Ap,Ap>, chunk 2 <- <p,Ap> */
+int vclhip_pipelined_gmres_prod_csr_f64(const vclhip_csr* A, const double* p_dev, double* Ap_dev, double* buf_dev,
+                                        uint32_t chunk, void* stream);
+int vclhip_pipelined_gmres_prod_sell_f64(const vclhip_sell* A, const double* p_dev, double* Ap_dev,
+                                         double* buf_dev, uint32_t chunk, void* stream);
+/* The Krylov basis of the GMRES operations: vector i at basis_dev + i * ld (ld >= n, the
+ * reference's internal_size). */
+/* linalg::pipelined_gmres_normalize_vk (iterative_operations.hpp:248-277; host_based :734-779):
+ * norm = sqrt(sum buf[chunk .. 2 chunk)); R[offset_in_R] = norm; v_k /= norm;
+ * r_dot_vk_buf[chunk_offset .. + chunk) <- <residual, v_k> */
+int vclhip_pipelined_gmres_normalize_vk_f64(uint32_t n, double* v_k_dev, const double* residual_dev, double* R_dev,
+                                            size_t offset_in_R, const double* buf_dev, double* r_dot_vk_buf_dev,
+                                            uint32_t chunk, size_t chunk_offset, void* stream);
+/* linalg::pipelined_gmres_gram_schmidt_stage1 (iterative_operations.hpp:286-311; host_based
+ * :787-842): chunk j of vi_in_vk_buf <- <v_j, v_k> for j < k, over all n entries */
+int vclhip_pipelined_gmres_gram_schmidt_stage1_f64(const double* basis_dev, uint32_t n, size_t ld, uint32_t k,
+                                                   double* vi_in_vk_buf_dev, uint32_t chunk, void* stream);
+/* linalg::pipelined_gmres_gram_schmidt_stage2 (iterative_operations.hpp:321-349; host_based
+ * :851-893): h_j = sum of chunk j of vi_in_vk_buf; v_k -= sum_j h_j v_j;
+ * R[j + k krylov_dim] = h_j; buf[chunk .. 2 chunk) <- ||v_k||^2 */
+int vclhip_pipelined_gmres_gram_schmidt_stage2_f64(double* basis_dev, uint32_t n, size_t ld, uint32_t k,
+                                                   const double* vi_in_vk_buf_dev, double* R_dev,
+                                                   uint32_t krylov_dim, double* buf_dev, uint32_t chunk,
+                                                   void* stream);
+/* linalg::pipelined_gmres_update_result (iterative_operations.hpp:356-389; host_based :898-926):
+ * x += c[0] residual + sum_{j=1}^{k-1} c[j] v_{j-1}  (c[0] is applied also when k == 0) */
+int vclhip_pipelined_gmres_update_result_f64(double* x_dev, const double* residual_dev, const double* basis_dev,
+                                             uint32_t n, size_t ld, const double* coef_dev, uint32_t k,
+                                             void* stream);
 
 /* ------------------------------------------------------------- solvers ---- */
 /* Solver parameters: the fields of cg_tag (cg.hpp:48-87), bicgstab_tag (bicgstab.hpp:47-90) and
@@ -251,6 +281,7 @@ typedef struct vclhip_solver_params {
                                         with 16-bit column offsets when every |col - row| <= 32767;
                                         else CSR), VCLHIP_INNER_CSR, VCLHIP_INNER_SELL64 (32-bit
                                         columns) or VCLHIP_INNER_SELL64_D16 */
+  uint32_t krylov_dim;               /* GMRES: steps per restart cycle (gmres_tag default 20) */
 } vclhip_solver_params;
 
 #define VCLHIP_INNER_AUTO 0
@@ -321,6 +352,24 @@ int vclhip_mixed_cg_solve_csr(const vclhip_csr* A, const float* values32, const 
 int vclhip_mixed_cg_solve_sell(const vclhip_sell* A, const float* elements32, const double* b_dev, double* x_dev,
                                int precond, const vclhip_solver_params* params, vclhip_solver_result* result,
                                void* stream);
+
+/* solve(A, b, gmres_tag) without a preconditioner (gmres.hpp:374-383): the pipelined GMRES of
+ * gmres.hpp:182-370, restarted every params->krylov_dim steps (1 .. 1024), at most
+ * gmres_tag::max_restarts() + 1 cycles for params->max_iters (gmres.hpp:75-81); the convergence
+ * test rho_0 / ||b|| < tol on the recomputed residual at the start of a cycle, as the reference.
+ * result->iters counts Krylov steps as tag.iters(), result->error is tag.error() (the estimate of
+ * the last cycle), result->restarts the cycles run.  Inner products <v_j, v_k> run over all n
+ * entries as the reference's device kernels compute them.  params->history_host (optional)
+ * receives the estimate after every step; params->monitor is called once per cycle
+ * (gmres.hpp:366-367).  FP32: FP32 basis and vectors, inner products accumulated in FP64. */
+int vclhip_gmres_solve_csr_f64(const vclhip_csr* A, const double* b_dev, double* x_dev,
+                               const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+int vclhip_gmres_solve_sell_f64(const vclhip_sell* A, const double* b_dev, double* x_dev,
+                                const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+int vclhip_gmres_solve_csr_f32(const vclhip_csr* A, const float* b_dev, float* x_dev,
+                               const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+int vclhip_gmres_solve_sell_f32(const vclhip_sell* A, const float* b_dev, float* x_dev,
+                                const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
 
 /* ---------------------------------------------------------------- batched CG */
 /* Batched CG: independent SPD systems, the diagonal blocks of one block-diagonal CSR matrix with

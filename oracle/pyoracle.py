@@ -57,6 +57,11 @@ _ORC_SIGS = {
     "orc_bicgstab_classic_csr": (_Res, [P(_Csr), vp, vp, vp, f64, f64, u32, u32, vp]),
     "orc_mixed_cg_csr": (_Res, [P(_Csr), vp, vp, f64, u32, f32, P(u32)]),
     "orc_mixed_cg2_csr": (_Res, [P(_Csr), vp, vp, vp, f64, u32, f32, i32, i32, P(u32), vp]),
+    "orc_gmres_pipelined_csr": (_Res, [P(_Csr), vp, vp, f64, f64, u32, u32, i32, vp]),
+    "orc_gmres_normalize_vk": (None, [sz, vp, vp, vp, sz, vp, vp, sz, sz]),
+    "orc_gmres_stage1": (None, [vp, sz, sz, sz, vp, sz, i32]),
+    "orc_gmres_stage2": (None, [vp, sz, sz, sz, vp, vp, sz, vp, sz]),
+    "orc_gmres_update_result": (None, [vp, vp, vp, sz, sz, vp, sz]),
     "orc_batched_cg_csr": (None, [u32, vp, P(_Csr), vp, vp, f64, u32, vp, vp]),
     "orc_batched_cg2_csr": (None, [u32, vp, P(_Csr), vp, vp, f64, u32, i32, vp, vp]),
     "orc_adjoint_grad_csr": (None, [u32, vp, vp, vp, vp, vp]),
@@ -82,6 +87,11 @@ _REF_SIGS = {
     "ref_bicgstab_csr": (None, [u32, vp, vp, vp, vp, vp, f64, f64, u32, u32, i32, P(u32), P(f64)]),
     "ref_bicgstab_sell": (None, [u32, u32, vp, vp, vp, vp, vp, f64, f64, u32, P(u32), P(f64)]),
     "ref_mixed_cg_csr": (None, [u32, vp, vp, vp, vp, vp, f64, u32, f32, P(u32), P(f64)]),
+    "ref_gmres_csr": (None, [u32, vp, vp, vp, vp, vp, f64, f64, u32, u32, P(u32), P(f64)]),
+    "ref_pipelined_gmres_normalize_vk": (None, [u32, vp, vp, vp, u32, u32, vp, vp, u32, u32, u32]),
+    "ref_pipelined_gmres_stage1": (None, [vp, u32, u32, u32, u32, vp, u32]),
+    "ref_pipelined_gmres_stage2": (None, [vp, u32, u32, u32, u32, vp, vp, vp, u32]),
+    "ref_pipelined_gmres_update_result": (None, [vp, vp, vp, u32, u32, u32, vp, u32]),
     "ref_bench_csr_spmv": (f64, [u32, vp, vp, vp, vp, vp, u32]),
     "ref_bench_solve": (f64, [u32, vp, vp, vp, vp, vp, f64, u32, i32, i32, P(u32), P(f64)]),
     "ref_max_threads": (i32, []),
@@ -295,6 +305,37 @@ def mixed_cg2(A, b, tol, max_iters, inner_tol=1e-2, mode=1, acc64=True, diag=Non
     return x, r.iters, r.error, upd.value, (h[: r.iters] if history else None)
 
 
+def gmres(A, b, tol, max_iters, krylov_dim=20, gs_skip_tail=False, abs_tol=0.0, history=False):
+    """Pipelined GMRES (gmres.hpp:182-370); gs_skip_tail reproduces the reference's Gram-Schmidt range."""
+    b = c64(b)
+    x = np.zeros_like(b)
+    h = np.zeros(max(max_iters, 1)) if history else None
+    r = orc().orc_gmres_pipelined_csr(C.byref(A.desc), p(b), p(x), tol, abs_tol, max_iters, krylov_dim,
+                                      int(bool(gs_skip_tail)), p(h))
+    return x, r.iters, r.error, (h[: r.iters] if history else None)
+
+
+def gmres_normalize_vk(vk, residual, R, offset_in_R, buf, r_dot_vk, chunk, chunk_offset):
+    """host_based pipelined_gmres_normalize_vk (:734-779); updates vk, R, r_dot_vk in place."""
+    orc().orc_gmres_normalize_vk(vk.size, p(vk), p(residual), p(R), offset_in_R, p(buf), p(r_dot_vk), chunk,
+                                 chunk_offset)
+
+
+def gmres_stage1(basis, n, ld, k, vi_in_vk, chunk, skip_tail=False):
+    """host_based pipelined_gmres_gram_schmidt_stage1 (:787-842); skip_tail: the single-thread range."""
+    orc().orc_gmres_stage1(p(basis), n, ld, k, p(vi_in_vk), chunk, int(bool(skip_tail)))
+
+
+def gmres_stage2(basis, n, ld, k, vi_in_vk, R, krylov_dim, buf, chunk):
+    """host_based pipelined_gmres_gram_schmidt_stage2 (:851-893); updates basis, R, buf in place."""
+    orc().orc_gmres_stage2(p(basis), n, ld, k, p(vi_in_vk), p(R), krylov_dim, p(buf), chunk)
+
+
+def gmres_update_result(x, residual, basis, n, ld, coef, k):
+    """host_based pipelined_gmres_update_result (:898-926); updates x in place."""
+    orc().orc_gmres_update_result(p(x), p(residual), p(basis), n, ld, p(coef), k)
+
+
 def batched_cg(sys_ptr, A, b, tol, max_iters, single=False):
     """Per-system pipelined CG of a block-diagonal CSR (single: FP32 vectors, FP64 accumulation)."""
     sys_ptr = cu32(sys_ptr)
@@ -350,6 +391,15 @@ def ref_bicgstab_sell(rp, col, val, C_, b, tol, max_iters, abs_tol=0.0):
     it, err = u32(), f64()
     ref().ref_bicgstab_sell(rp.size - 1, C_, p(rp), p(col), p(val), p(b), p(x), tol, abs_tol, max_iters,
                             C.byref(it), C.byref(err))
+    return x, it.value, err.value
+
+
+def ref_gmres(rp, col, val, b, tol, max_iters, krylov_dim=20, abs_tol=0.0):
+    rp, col, val, b = cu32(rp), cu32(col), c64(val), c64(b)
+    x = np.zeros_like(b)
+    it, err = u32(), f64()
+    ref().ref_gmres_csr(rp.size - 1, p(rp), p(col), p(val), p(b), p(x), tol, abs_tol, max_iters, krylov_dim,
+                        C.byref(it), C.byref(err))
     return x, it.value, err.value
 
 

@@ -21,6 +21,7 @@
 #include "viennacl/linalg/cg.hpp"
 #include "viennacl/linalg/bicgstab.hpp"
 #include "viennacl/linalg/mixed_precision_cg.hpp"
+#include "viennacl/linalg/gmres.hpp"
 #include "viennacl/linalg/jacobi_precond.hpp"
 #include "viennacl/linalg/iterative_operations.hpp"
 #include "viennacl/tools/matrix_generation.hpp"
@@ -316,6 +317,63 @@ void ref_cg_csr(uint32_t n, const uint32_t* rp, const uint32_t* col, const doubl
   } else {
     vx = viennacl::linalg::solve(A, vb, tag);
   }
+  from_vcl(vx, x);
+  *iters = tag.iters();
+  *error = tag.error();
+}
+
+// linalg::pipelined_gmres_* (iterative_operations.hpp:247-418) on main-memory vectors.  The basis
+// is one vector of K*ld entries, vector i at offset i*ld (gmres.hpp:194).
+void ref_pipelined_gmres_normalize_vk(uint32_t n, double* vk, const double* residual, double* R, uint32_t R_size,
+                                      uint32_t offset_in_R, const double* buf, double* r_dot_vk,
+                                      uint32_t r_dot_vk_size, uint32_t chunk, uint32_t chunk_offset)
+{
+  viennacl::vector<double> vv(n), vr(n), vR(R_size), vb(3 * chunk), vd(r_dot_vk_size);
+  to_vcl(vk, vv); to_vcl(residual, vr); to_vcl(R, vR); to_vcl(buf, vb); to_vcl(r_dot_vk, vd);
+  viennacl::linalg::pipelined_gmres_normalize_vk(vv, vr, vR, offset_in_R, vb, vd, chunk, chunk_offset);
+  from_vcl(vv, vk); from_vcl(vR, R); from_vcl(vd, r_dot_vk);
+}
+
+void ref_pipelined_gmres_stage1(const double* basis, uint32_t n, uint32_t ld, uint32_t K, uint32_t k,
+                                double* vi_in_vk, uint32_t chunk)
+{
+  viennacl::vector<double> vB(static_cast<size_t>(K) * ld), vo(static_cast<size_t>(K) * chunk);
+  to_vcl(basis, vB); to_vcl(vi_in_vk, vo);
+  viennacl::linalg::pipelined_gmres_gram_schmidt_stage1(vB, n, ld, k, vo, chunk);
+  from_vcl(vo, vi_in_vk);
+}
+
+void ref_pipelined_gmres_stage2(double* basis, uint32_t n, uint32_t ld, uint32_t K, uint32_t k,
+                                const double* vi_in_vk, double* R, double* buf, uint32_t chunk)
+{
+  viennacl::vector<double> vB(static_cast<size_t>(K) * ld), vi(static_cast<size_t>(K) * chunk),
+      vR(static_cast<size_t>(K) * K), vb(3 * chunk);
+  to_vcl(basis, vB); to_vcl(vi_in_vk, vi); to_vcl(R, vR); to_vcl(buf, vb);
+  viennacl::linalg::pipelined_gmres_gram_schmidt_stage2(vB, n, ld, k, vi, vR, K, vb, chunk);
+  from_vcl(vB, basis); from_vcl(vR, R); from_vcl(vb, buf);
+}
+
+void ref_pipelined_gmres_update_result(double* x, const double* residual, const double* basis, uint32_t n,
+                                       uint32_t ld, uint32_t K, const double* coef, uint32_t k)
+{
+  viennacl::vector<double> vx(n), vr(n), vB(static_cast<size_t>(K) * ld), vc(K);
+  to_vcl(x, vx); to_vcl(residual, vr); to_vcl(basis, vB); to_vcl(coef, vc);
+  viennacl::linalg::pipelined_gmres_update_result(vx, vr, vB, n, ld, vc, k);
+  from_vcl(vx, x);
+}
+
+// gmres.hpp:372-378 (compressed_matrix, no preconditioner -> pipelined_solve)
+void ref_gmres_csr(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val, const double* b,
+                   double* x, double tol, double abs_tol, uint32_t max_iters, uint32_t krylov_dim, uint32_t* iters,
+                   double* error)
+{
+  viennacl::compressed_matrix<double> A;
+  fill_csr(A, n, rp, col, val);
+  viennacl::vector<double> vb(n);
+  to_vcl(b, vb);
+  viennacl::linalg::gmres_tag tag(tol, max_iters, krylov_dim);
+  tag.abs_tolerance(abs_tol);
+  viennacl::vector<double> vx = viennacl::linalg::solve(A, vb, tag);
   from_vcl(vx, x);
   *iters = tag.iters();
   *error = tag.error();

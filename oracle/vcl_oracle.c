@@ -949,6 +949,186 @@ orc_result orc_mixed_cg_csr(const orc_csr64* A, const double* b, double* x, doub
 }
 
 /* Independent classic CG per diagonal block (cg.hpp:257-322 with no preconditioner). */
+/* Pipelined GMRES (gmres.hpp:182-370, no preconditioner) with the host kernels of
+ * host_based/iterative_operations.hpp:734-937.  gs_skip_tail != 0 reproduces the reference's Gram-
+ * Schmidt stage 1 with one thread, whose range [0, (n-1)/T*T) leaves out the last entry
+ * (:820-822); 0 computes the full inner products.  iters counts the Krylov steps that entered the
+ * error estimator, as tag.iters() does. */
+orc_result orc_gmres_pipelined_csr(const orc_csr64* A, const double* b, double* x, double tol, double abs_tol,
+                                   uint32_t max_iters, uint32_t krylov_dim, int gs_skip_tail, double* history)
+{
+  orc_result res = {0, 0.0};
+  const size_t n = A->n_rows;
+  const size_t K = krylov_dim;
+  const size_t C = 128; /* buffer_size_per_vector */
+  uint32_t max_restarts = max_iters / krylov_dim;
+  if (max_restarts > 0 && max_restarts * krylov_dim == max_iters) max_restarts -= 1;
+  double* r = (double*)malloc(n * sizeof(double));
+  double* V = (double*)calloc(K * n, sizeof(double));
+  double* R = (double*)calloc(K * K, sizeof(double));
+  double* buf = (double*)calloc(3 * C, sizeof(double));
+  double* r_dot_vk = (double*)calloc(C * K, sizeof(double));
+  double* vi_in_vk = (double*)calloc(C * K, sizeof(double));
+  double* xi = (double*)calloc(K, sizeof(double));
+  double* eta = (double*)calloc(K, sizeof(double));
+  double* coef = (double*)calloc(K, sizeof(double));
+  double* h = (double*)calloc(K, sizeof(double));
+  memcpy(r, b, n * sizeof(double));
+  memset(x, 0, n * sizeof(double));
+  const double norm_rhs = orc_norm2_f64(n, r);
+  double rho_0 = norm_rhs, rho = 1.0;
+  const size_t gs_end = gs_skip_tail ? (n > 0 ? n - 1 : 0) : n;
+
+  for (uint32_t restart = 0; restart <= max_restarts; ++restart) {
+    if (restart > 0) {
+      orc_csr_spmv_f64((uint32_t)n, A->row_ptr, A->col, A->val, x, 0, 1, 1.0, r, 0, 1, 0.0);
+      for (size_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+      rho_0 = orc_norm2_f64(n, r);
+    }
+    if (rho_0 <= abs_tol) break;
+    for (size_t i = 0; i < n; ++i) r[i] = r[i] / rho_0;
+    rho = 1.0;
+    if (rho_0 / norm_rhs < tol || rho_0 < abs_tol) break;
+
+    size_t k;
+    for (k = 0; k < K; ++k) {
+      double* vk = V + k * n;
+      /* v_k = A v_{k-1} (v_{-1} = r); <v_k, v_k> into buf[C] (pipelined_gmres_prod = pipelined_cg_prod) */
+      orc_pipelined_prod_csr_f64((uint32_t)n, A->row_ptr, A->col, A->val, k == 0 ? r : V + (k - 1) * n, vk, NULL,
+                                 buf, C, 0);
+      if (k > 0) {
+        /* stage 1: <v_i, v_k>, i < k */
+        for (size_t j = 0; j < k; ++j) h[j] = 0.0;
+        for (size_t i = 0; i < gs_end; ++i) {
+          const double vki = vk[i];
+          for (size_t j = 0; j < k; ++j) h[j] += V[j * n + i] * vki;
+        }
+        for (size_t j = 0; j < k; ++j) vi_in_vk[j * C] = h[j];
+        /* stage 2: reduce, v_k -= sum_j <v_j,v_k> v_j, ||v_k||^2 */
+        for (size_t j = 0; j < k; ++j) {
+          double t = 0.0;
+          for (size_t c = 0; c < C; ++c) t += vi_in_vk[j * C + c];
+          h[j] = t;
+        }
+        double nsq = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+          double v = vk[i];
+          for (size_t j = 0; j < k; ++j) v -= h[j] * V[j * n + i];
+          nsq += v * v;
+          vk[i] = v;
+        }
+        for (size_t j = 0; j < k; ++j) R[j + k * K] = h[j];
+        buf[C] = nsq;
+      }
+      /* normalize: ||v_k|| from buf[C..2C), v_k /= ||v_k||, <r, v_k> */
+      double nv = 0.0;
+      for (size_t c = 0; c < C; ++c) nv += buf[c + C];
+      nv = sqrt(nv);
+      R[k * K + k] = nv;
+      double rv = 0.0;
+      for (size_t i = 0; i < n; ++i) {
+        const double v = vk[i] / nv;
+        rv += r[i] * v;
+        vk[i] = v;
+      }
+      r_dot_vk[k * C] = rv;
+    }
+    for (size_t i = 0; i < k; ++i) {
+      double t = 0.0;
+      for (size_t c = 0; c < C; ++c) t += r_dot_vk[i * C + c];
+      xi[i] = t;
+    }
+    const size_t full = k;
+    for (size_t i = 0; i < k; ++i)
+      if (fabs(R[i + i * k]) < tol * R[0]) {
+        k = i;
+        break;
+      }
+    for (size_t i = 0; i < k; ++i) {
+      res.iters += 1;
+      if (xi[i] >= rho || xi[i] <= -rho) {
+        if (history && res.iters <= max_iters) history[res.iters - 1] = fabs(rho * rho_0 / norm_rhs);
+        k = i;
+        break;
+      }
+      rho *= sin(acos(xi[i] / rho));
+      if (history && res.iters <= max_iters) history[res.iters - 1] = fabs(rho * rho_0 / norm_rhs);
+    }
+    for (size_t i = 0; i < k; ++i) eta[i] = xi[i];
+    for (long i2 = (long)k - 1; i2 > -1; --i2) {
+      const size_t i = (size_t)i2;
+      for (size_t j = i + 1; j < k; ++j) eta[i] -= R[i + j * full] * eta[j];
+      eta[i] /= R[i + i * full];
+    }
+    for (size_t i = 0; i < k; ++i) coef[i] = rho_0 * eta[i];
+    for (size_t i = 0; i < n; ++i) {
+      double v = x[i];
+      v += coef[0] * r[i];
+      for (size_t j = 1; j < k; ++j) v += coef[j] * V[(j - 1) * n + i];
+      x[i] = v;
+    }
+    res.error = fabs(rho * rho_0 / norm_rhs);
+  }
+  free(r); free(V); free(R); free(buf); free(r_dot_vk); free(vi_in_vk); free(xi); free(eta); free(coef); free(h);
+  return res;
+}
+
+void orc_gmres_normalize_vk(size_t n, double* v_k, const double* residual, double* R, size_t offset_in_R,
+                            const double* buf, double* r_dot_vk, size_t chunk, size_t chunk_offset)
+{
+  double norm_vk = 0;
+  for (size_t i = 0; i < chunk; ++i) norm_vk += buf[i + chunk];
+  norm_vk = sqrt(norm_vk);
+  R[offset_in_R] = norm_vk;
+  double rv = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const double v = v_k[i] / norm_vk;
+    rv += residual[i] * v;
+    v_k[i] = v;
+  }
+  r_dot_vk[chunk_offset] = rv;
+}
+
+void orc_gmres_stage1(const double* basis, size_t n, size_t ld, size_t k, double* vi_in_vk, size_t chunk,
+                      int skip_tail)
+{
+  const size_t end = skip_tail ? (n > 0 ? n - 1 : 0) : n;
+  for (size_t j = 0; j < k; ++j) {
+    double t = 0;
+    for (size_t i = 0; i < end; ++i) t += basis[i + j * ld] * basis[i + k * ld];
+    vi_in_vk[j * chunk] = t;
+  }
+}
+
+void orc_gmres_stage2(double* basis, size_t n, size_t ld, size_t k, const double* vi_in_vk, double* R,
+                      size_t krylov_dim, double* buf, size_t chunk)
+{
+  double* h = (double*)calloc(k ? k : 1, sizeof(double));
+  for (size_t i = 0; i < k; ++i)
+    for (size_t j = 0; j < chunk; ++j) h[i] += vi_in_vk[i * chunk + j];
+  double nsq = 0;
+  for (size_t i = 0; i < n; ++i) {
+    double v = basis[i + k * ld];
+    for (size_t j = 0; j < k; ++j) v -= h[j] * basis[i + j * ld];
+    nsq += v * v;
+    basis[i + k * ld] = v;
+  }
+  for (size_t i = 0; i < k; ++i) R[i + k * krylov_dim] = h[i];
+  buf[chunk] = nsq;
+  free(h);
+}
+
+void orc_gmres_update_result(double* x, const double* residual, const double* basis, size_t n, size_t ld,
+                             const double* coef, size_t k)
+{
+  for (size_t i = 0; i < n; ++i) {
+    double v = x[i];
+    v += coef[0] * residual[i];
+    for (size_t j = 1; j < k; ++j) v += coef[j] * basis[i + (j - 1) * ld];
+    x[i] = v;
+  }
+}
+
 /* Pipelined CG (cg.hpp:128-187) on FP32 vectors, inner products accumulated in double. */
 static orc_result cg_pipelined_f32(uint32_t n, const uint32_t* rp, const uint32_t* cl, const float* val,
                                    const float* b, float* x, double tol, uint32_t max_iters)

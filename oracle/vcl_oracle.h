@@ -165,6 +165,29 @@ orc_result orc_mixed_cg2_csr(const orc_csr64* A, const double* b, double* x, con
                              uint32_t max_iters, float inner_tol, int mode, int acc64, uint32_t* updates,
                              double* history);
 
+/* gmres.hpp:182-370: pipelined GMRES(krylov_dim) without preconditioner, classical Gram-Schmidt.
+ * gs_skip_tail = 1 reproduces the reference's single-thread Gram-Schmidt range, which leaves out
+ * the last vector entry (host_based/iterative_operations.hpp:820-822); 0 uses all entries.
+ * history (optional): the error estimate |rho * rho_0| / ||b|| after every counted step. */
+orc_result orc_gmres_pipelined_csr(const orc_csr64* A, const double* b, double* x, double tol, double abs_tol,
+                                   uint32_t max_iters, uint32_t krylov_dim, int gs_skip_tail, double* history);
+
+/* The pipelined GMRES operations of host_based/iterative_operations.hpp, single-threaded.  The
+ * basis holds vector i at basis + i * ld. */
+/* :734-779: norm = sqrt(sum buf[chunk..2chunk)); R[offset_in_R] = norm; v_k /= norm;
+ * r_dot_vk[chunk_offset] = <residual, v_k> */
+void orc_gmres_normalize_vk(size_t n, double* v_k, const double* residual, double* R, size_t offset_in_R,
+                            const double* buf, double* r_dot_vk, size_t chunk, size_t chunk_offset);
+/* :787-842: vi_in_vk[j*chunk] = <v_j, v_k>, j < k, over [0, n - 1) if skip_tail else [0, n) */
+void orc_gmres_stage1(const double* basis, size_t n, size_t ld, size_t k, double* vi_in_vk, size_t chunk,
+                      int skip_tail);
+/* :851-893: h_j = sum of chunk j; v_k -= sum_j h_j v_j; R[j + k*krylov_dim] = h_j; buf[chunk] = ||v_k||^2 */
+void orc_gmres_stage2(double* basis, size_t n, size_t ld, size_t k, const double* vi_in_vk, double* R,
+                      size_t krylov_dim, double* buf, size_t chunk);
+/* :898-926: x += coef[0] residual + sum_{j=1}^{k-1} coef[j] v_{j-1} */
+void orc_gmres_update_result(double* x, const double* residual, const double* basis, size_t n, size_t ld,
+                             const double* coef, size_t k);
+
 /* Batched CG: independent pipelined CG (cg.hpp:128-187) per system i on rows
  * [sys_ptr[i], sys_ptr[i+1]) of a block-diagonal CSR with global column indices.
  * iters/errors hold one entry per system.  No reference counterpart. */

@@ -1,7 +1,7 @@
 // sparse_krylov.cpp -- the drop-in C++ API on the HIP domain, exercised the way the reference's
 // tests/src/sparse.cpp and examples/tutorial/iterative.cpp use it: STL matrices copied into
 // compressed_matrix / sliced_ell_matrix, prod() against the serial STL product, vector
-// expressions against host loops, and solve() with cg_tag / bicgstab_tag, Jacobi and a
+// expressions against host loops, and solve() with cg_tag / bicgstab_tag / gmres_tag, Jacobi and a
 // user-written preconditioner, checked by the FP64 true residual.
 #include <cmath>
 #include <cstdio>
@@ -14,6 +14,7 @@
 #include "viennacl/io/matrix_market.hpp"
 #include "viennacl/linalg/bicgstab.hpp"
 #include "viennacl/linalg/cg.hpp"
+#include "viennacl/linalg/gmres.hpp"
 #include "viennacl/linalg/inner_prod.hpp"
 #include "viennacl/linalg/jacobi_precond.hpp"
 #include "viennacl/linalg/mixed_precision_cg.hpp"
@@ -286,6 +287,78 @@ void test_solvers(double tol_solver, double tol_check)
   }
 }
 
+template <typename T>
+void test_gmres(double tol_solver, double tol_check)
+{
+  // 2-D convection-diffusion, upwind (nonsymmetric), b ~ U[-1,1]
+  const unsigned px = 50, py = 40, n = px * py;
+  StlMatrix<T> hA(n);
+  for (unsigned j = 0; j < py; ++j)
+    for (unsigned i = 0; i < px; ++i) {
+      const unsigned r = j * px + i;
+      hA[r][r] = T(4.8);
+      if (i > 0) hA[r][r - 1] = T(-1.5);
+      if (i + 1 < px) hA[r][r + 1] = T(-1);
+      if (j > 0) hA[r][r - px] = T(-1.3);
+      if (j + 1 < py) hA[r][r + px] = T(-1);
+    }
+  viennacl::compressed_matrix<T> A;
+  viennacl::copy(hA, A);
+  viennacl::sliced_ell_matrix<T> S;
+  viennacl::copy(hA, S);
+  std::mt19937_64 g(3);
+  std::uniform_real_distribution<double> u(-1, 1);
+  std::vector<T> b(n);
+  for (auto& v : b) v = T(u(g));
+  viennacl::vector<T> vb(n);
+  viennacl::copy(b, vb);
+
+  using namespace viennacl::linalg;
+  {
+    gmres_tag tag(tol_solver, 2000, 20);
+    viennacl::vector<T> x = solve(A, vb, tag);
+    double rr = true_relres(hA, b, to_host(x));
+    CHECK(rr < tol_check && tag.iters() >= 20 && tag.iters() < 2000, "gmres csr: iters %u relres %g", tag.iters(), rr);
+    gmres_tag tag2(tol_solver, 2000, 20);
+    viennacl::vector<T> x2 = solve(S, vb, tag2);
+    CHECK(tag2.iters() == tag.iters(), "gmres sell iters %u vs csr %u", tag2.iters(), tag.iters());
+    CHECK(true_relres(hA, b, to_host(x2)) < tol_check, "gmres sell residual");
+  }
+  {
+    // preconditioned: the Householder GMRES on the HIP vector operations
+    jacobi_precond<viennacl::compressed_matrix<T> > J(A, jacobi_tag());
+    gmres_tag tag(tol_solver, 2000, 20);
+    viennacl::vector<T> x = solve(A, vb, tag, J);
+    double rr = true_relres(hA, b, to_host(x));
+    CHECK(rr < 10 * tol_check, "jacobi gmres: iters %u relres %g", tag.iters(), rr);
+    gmres_tag tag2(tol_solver, 2000, 20);
+    viennacl::vector<T> x2 = solve(A, vb, tag2, scaled_identity<T>{T(0.25)});
+    CHECK(true_relres(hA, b, to_host(x2)) < 10 * tol_check, "generic gmres residual");
+  }
+  {
+    // per-cycle monitor that stops after two cycles, and the initial guess
+    struct Count { int calls; };
+    Count c = {0};
+    gmres_solver<viennacl::vector<T> > s(gmres_tag(tol_solver, 2000, 15));
+    s.set_monitor([](viennacl::vector<T> const& xi, T, void* d) {
+      Count* cc = static_cast<Count*>(d);
+      return ++cc->calls >= 2 && xi.size() > 0;
+    }, &c);
+    s(A, vb);
+    CHECK(c.calls == 2 && s.tag().iters() == 30, "gmres monitor stop: calls %d iters %u", c.calls, s.tag().iters());
+    gmres_solver<viennacl::vector<T> > s2(gmres_tag(tol_solver, 2000, 20));
+    viennacl::vector<T> guess(n);
+    std::vector<T> hg(n, T(0.1));
+    viennacl::copy(hg, guess);
+    s2.set_initial_guess(guess);
+    CHECK(true_relres(hA, b, to_host(s2(A, vb))) < tol_check, "gmres_solver with guess");
+  }
+  {
+    std::vector<T> x = solve(hA, b, gmres_tag(tol_solver, 2000, 20), no_precond());
+    CHECK(true_relres(hA, b, x) < tol_check, "stl gmres solve");
+  }
+}
+
 void test_mixed_precision()
 {
   // 2-D variable-coefficient problem: diagonal entries vary, so Jacobi changes the iteration
@@ -389,6 +462,8 @@ int main()
   test_spmv_and_vectors<float>(1e-5);
   test_solvers<double>(1e-10, 1e-9);
   test_solvers<float>(1e-5, 1e-4);
+  test_gmres<double>(1e-10, 1e-9);
+  test_gmres<float>(1e-5, 1e-4);
   test_mixed_precision();
   test_matrix_market();
   if (failures) {

@@ -6,7 +6,8 @@ inputs and the reference's outputs, so the tests pin the oracle and the HIP path
 reference being present.  The reference's own SpMV test matrix (examples/testdata/mat65k.mtx) is
 not available, so these seeded problems stand in for it.
 
-Run from the repository root:  python tests/golden/make_golden.py
+Run from the repository root:  python tests/golden/make_golden.py  (add `gmres` to write only the
+GMRES fixtures)
 """
 import pathlib
 import sys
@@ -180,5 +181,62 @@ def main():
     solver_case("solve_powerlaw_2000", rp, col, val, b, bicgstab_x=xb, bicgstab_iters=itb, bicgstab_error=errb)
 
 
+def gmres_main():
+    """Pipelined GMRES (gmres.hpp:182-370) and its operations (iterative_operations.hpp:247-418)."""
+    po.set_threads(1)
+    R = po.ref()
+    p = po.p
+
+    # the four operations on a random basis; n = 1001 with ld = 1024, K = 6, k = 3
+    rng = np.random.default_rng(11)
+    n, ld, K, k, chunk = 1001, 1024, 6, 3, 128
+    basis = rng.standard_normal(K * ld)
+    residual = rng.standard_normal(n)
+    s1 = np.zeros(K * chunk)
+    R.ref_pipelined_gmres_stage1(p(basis), n, ld, K, k, p(s1), chunk)
+    vi = rng.standard_normal(K * chunk)
+    b2, R2, buf2 = basis.copy(), np.zeros(K * K), np.zeros(3 * chunk)
+    R.ref_pipelined_gmres_stage2(p(b2), n, ld, K, k, p(vi), p(R2), p(buf2), chunk)
+    vk = basis[k * ld:k * ld + n].copy()
+    R3, buf3, rdv = np.zeros(K * K), np.abs(rng.standard_normal(3 * chunk)), np.zeros(K * chunk)
+    buf3_in = buf3.copy()
+    R.ref_pipelined_gmres_normalize_vk(n, p(vk), p(residual), p(R3), K * K, k * K + k, p(buf3), p(rdv), K * chunk,
+                                       chunk, k * chunk)
+    x4, coef = rng.standard_normal(n), rng.standard_normal(K)
+    x4_in = x4.copy()
+    R.ref_pipelined_gmres_update_result(p(x4), p(residual), p(basis), n, ld, K, p(coef), K - 1)
+    save("gmres_ops_1001", n=n, ld=ld, K=K, k=k, chunk=chunk, basis=basis, residual=residual,
+         stage1_out=s1, vi_in_vk=vi, stage2_basis=b2, stage2_R=R2, stage2_buf=buf2,
+         normalize_buf=buf3_in, normalize_vk=vk, normalize_R=R3, normalize_rdv=rdv,
+         update_x0=x4_in, update_coef=coef, update_x=x4)
+
+    # solves: nonsymmetric convection-diffusion and the 2-D Laplacian.  Cases (tol, max_iters,
+    # krylov_dim) include a run that stops at max_iters and one with max_iters not a multiple of
+    # krylov_dim.
+    for name, (rp, col, val) in (("convdiff_20x15x3", vh.convection_diffusion(20, 15, 3)),
+                                 ("lap2d_40x40", vh.fdm_laplace_2d(40, 40))):
+        nn = rp.size - 1
+        b = vh.random_uniform(5, nn)
+        cases = np.array([(1e-8, 300, 20), (1e-10, 500, 30), (1e-6, 100, 10), (1e-12, 45, 20), (1e-8, 60, 1)])
+        xs, its, errs, env = [], [], [], []
+        for tol, mi, kd in cases:
+            x, it, err = po.ref_gmres(rp, col, val, b, tol, int(mi), int(kd))
+            xs.append(x)
+            its.append(it)
+            errs.append(err)
+            e = []
+            for t in range(1, 9):
+                po.set_threads(t)
+                e.append(po.ref_gmres(rp, col, val, b, tol, int(mi), int(kd))[1])
+            po.set_threads(1)
+            env.append((min(e), max(e)))
+        save(f"solve_gmres_{name}", row_ptr=rp, col=col, val=val, b=b, cases=cases, x=np.stack(xs),
+             iters=np.array(its, np.int64), error=np.array(errs), env=np.array(env, np.int64))
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["gmres"]:
+        gmres_main()
+    else:
+        main()
+        gmres_main()

@@ -125,6 +125,42 @@ def test_nonsymmetric_bicgstab_matches_reference(oracle):
     assert it == g["bicgstab_iters"] and err == g["bicgstab_error"] and np.array_equal(x, g["bicgstab_x"])
 
 
+def test_gmres_ops_match_reference(oracle):
+    """pipelined_gmres_* (host_based/iterative_operations.hpp:734-937), single-threaded."""
+    g = load("gmres_ops_1001")
+    n, ld, K, k, chunk = (int(g[q]) for q in ("n", "ld", "K", "k", "chunk"))
+    basis = np.ascontiguousarray(g["basis"])
+    s1 = np.zeros(K * chunk)
+    oracle.gmres_stage1(basis, n, ld, k, s1, chunk, skip_tail=True)
+    assert np.array_equal(s1, g["stage1_out"])
+    b2, R2, buf2 = basis.copy(), np.zeros(K * K), np.zeros(3 * chunk)
+    oracle.gmres_stage2(b2, n, ld, k, np.ascontiguousarray(g["vi_in_vk"]), R2, K, buf2, chunk)
+    assert np.array_equal(b2, g["stage2_basis"]) and np.array_equal(R2, g["stage2_R"])
+    assert np.array_equal(buf2, g["stage2_buf"])
+    vk = basis[k * ld:k * ld + n].copy()
+    R3, rdv = np.zeros(K * K), np.zeros(K * chunk)
+    oracle.gmres_normalize_vk(vk, np.ascontiguousarray(g["residual"]), R3, k * K + k,
+                              np.ascontiguousarray(g["normalize_buf"]), rdv, chunk, k * chunk)
+    assert np.array_equal(vk, g["normalize_vk"]) and np.array_equal(R3, g["normalize_R"])
+    assert np.array_equal(rdv, g["normalize_rdv"])
+    x = np.ascontiguousarray(g["update_x0"]).copy()
+    oracle.gmres_update_result(x, np.ascontiguousarray(g["residual"]), basis, n, ld,
+                               np.ascontiguousarray(g["update_coef"]), K - 1)
+    assert np.array_equal(x, g["update_x"])
+
+
+@pytest.mark.parametrize("case", ["convdiff_20x15x3", "lap2d_40x40"])
+def test_gmres_matches_reference(oracle, case):
+    """solve(A, b, gmres_tag) (gmres.hpp:182-370), single-threaded: x, tag.iters(), tag.error()."""
+    g = load(f"solve_gmres_{case}")
+    A = oracle.CsrHost(g["row_ptr"], g["col"], g["val"])
+    for i, (tol, mi, kd) in enumerate(g["cases"]):
+        x, it, err, _ = oracle.gmres(A, g["b"], tol, int(mi), int(kd), gs_skip_tail=True)
+        assert it == g["iters"][i] and err == g["error"][i] and np.array_equal(x, g["x"][i]), (tol, mi, kd)
+        lo, hi = g["env"][i]
+        assert lo <= it <= hi
+
+
 needs_ref = pytest.mark.skipif(not (pathlib.Path(__file__).resolve().parents[1] / "oracle/_ref/libvclref.so").exists(),
                                reason="reference build (oracle/_ref) not present")
 
@@ -153,6 +189,13 @@ def test_oracle_equals_reference_build_on_fresh_inputs(oracle, vh, seed):
     x, it, err, _ = oracle.bicgstab_classic(A, b, 1e-10, 3000, diag=d, max_before_restart=15)
     xr, itr, errr = oracle.ref_bicgstab(rp, col, val, b, 1e-10, 3000, jacobi=True, max_before_restart=15)
     assert (it, err) == (itr, errr) and np.array_equal(x, xr)
+    rp, col, val = vh.convection_diffusion(13, 9, 4, beta=(0.2 * seed, 0.1, 0.3))
+    b = vh.random_uniform(seed, rp.size - 1)
+    A = oracle.CsrHost(rp, col, val)
+    for tol, mi, kd in ((1e-9, 400, 25), (1e-7, 90, 7)):
+        x, it, err, _ = oracle.gmres(A, b, tol, mi, kd, gs_skip_tail=True)
+        xr, itr, errr = oracle.ref_gmres(rp, col, val, b, tol, mi, kd)
+        assert (it, err) == (itr, errr) and np.array_equal(x, xr)
 
 
 def test_blocked_reduction_orders(oracle, vh):

@@ -249,3 +249,90 @@ def test_planted_mistakes_are_caught(oracle, vh):
         pp = r - (rr_new / rr) * pp  # planted sign error
         rr = rr_new
     assert np.linalg.norm(b - A @ xx) / np.linalg.norm(b) > 1e-6
+
+
+def krylov_minimizer(A, b, K):
+    """argmin of ||b - A x|| over the Krylov space span{b, A b, ..., A^(K-1) b} (dense, via an
+    orthonormal Arnoldi basis)."""
+    n = b.size
+    Q = np.zeros((n, K))
+    Q[:, 0] = b / np.linalg.norm(b)
+    for j in range(1, K):
+        w = A @ Q[:, j - 1]
+        for _ in range(2):
+            w -= Q[:, :j] @ (Q[:, :j].T @ w)
+        Q[:, j] = w / np.linalg.norm(w)
+    y = np.linalg.lstsq(A @ Q, b, rcond=None)[0]
+    return Q @ y
+
+
+def test_gmres_cycle_is_the_krylov_minimizer(oracle, vh):
+    """One cycle of krylov_dim steps from x = 0 minimizes ||b - A x|| over K_krylov_dim(A, b), and
+    the cycle's estimate |rho rho_0| / ||b|| equals the true residual.  The Gram-Schmidt range that
+    leaves out the last entry (the reference's OpenMP kernel) misses the minimizer: the pin
+    distinguishes the two."""
+    rp, col, val = vh.convection_diffusion(9, 8, 3, beta=(0.9, 0.4, 0.2))
+    A = dense(rp, col, val)
+    b = vh.random_uniform(3, rp.size - 1)
+    Ah = oracle.CsrHost(rp, col, val)
+    for K in (1, 4, 8):
+        xs = krylov_minimizer(A, b, K)
+        x, it, err, _ = oracle.gmres(Ah, b, 1e-15, K, K)
+        assert it == K
+        assert np.linalg.norm(x - xs) <= 1e-9 * np.linalg.norm(xs)
+        true = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+        assert abs(err - true) <= 1e-8 * true
+    xk, _, _, _ = oracle.gmres(Ah, b, 1e-15, 8, 8, gs_skip_tail=True)
+    assert np.linalg.norm(xk - krylov_minimizer(A, b, 8)) > 1e-7 * np.linalg.norm(xk)
+
+
+def test_gmres_full_space_and_restarts(oracle, vh):
+    """krylov_dim >= n: the premature-convergence check stops the basis where it becomes dependent,
+    after at most n + 1 steps, and a restarted solve reaches the dense solution; restarted cycles
+    never increase the true residual."""
+    rng = np.random.default_rng(4)
+    n = 12
+    A = rng.standard_normal((n, n)) + 6 * np.eye(n)
+    rp = np.arange(0, n * n + 1, n, dtype=np.uint32)
+    col = np.tile(np.arange(n, dtype=np.uint32), n)
+    b = rng.standard_normal(n)
+    xs = np.linalg.solve(A, b)
+    x, it, err, _ = oracle.gmres(oracle.CsrHost(rp, col, A.ravel()), b, 1e-13, 20, 20)
+    assert it <= n + 1 and np.linalg.norm(x - xs) <= 1e-4 * np.linalg.norm(xs)
+    # classical Gram-Schmidt loses orthogonality over n steps; the restart recovers it
+    x, it, err, _ = oracle.gmres(oracle.CsrHost(rp, col, A.ravel()), b, 1e-13, 60, 20)
+    assert np.linalg.norm(x - xs) <= 1e-11 * np.linalg.norm(xs)
+    rp, col, val = vh.convection_diffusion(10, 9, 2)
+    Ad = dense(rp, col, val)
+    b = vh.random_uniform(6, rp.size - 1)
+    Ah = oracle.CsrHost(rp, col, val)
+    last = np.inf
+    for cycles in range(1, 6):
+        x, it, err, _ = oracle.gmres(Ah, b, 1e-15, 6 * cycles, 6)
+        assert it == 6 * cycles
+        true = np.linalg.norm(b - Ad @ x) / np.linalg.norm(b)
+        assert true <= last * (1 + 1e-12) and abs(err - true) <= 1e-6 * true
+        last = true
+
+
+def test_gmres_ops_against_numpy(oracle):
+    """The operations with all entries: stage 1 is the dense product V^T v_k, stage 2 the
+    projection v_k - V h, update x + c0 r + V c."""
+    rng = np.random.default_rng(8)
+    n, ld, K, k, chunk = 301, 320, 5, 4, 16
+    basis = rng.standard_normal(K * ld)
+    V = basis.reshape(K, ld)[:, :n]
+    out = np.zeros(K * chunk)
+    oracle.gmres_stage1(basis, n, ld, k, out, chunk)
+    assert np.allclose(out[::chunk][:k], V[:k] @ V[k], rtol=1e-13, atol=1e-12)
+    vi = np.zeros(K * chunk)
+    vi[::chunk][:k] = h = rng.standard_normal(k)
+    b2, R, buf = basis.copy(), np.zeros(K * K), np.zeros(3 * chunk)
+    oracle.gmres_stage2(b2, n, ld, k, vi, R, K, buf, chunk)
+    want = V[k] - h @ V[:k]
+    assert np.allclose(b2.reshape(K, ld)[k, :n], want, rtol=1e-13, atol=1e-13)
+    assert np.allclose(R.reshape(K, K)[k, :k], h) and np.isclose(buf[chunk], want @ want, rtol=1e-13)
+    x0, r, c = rng.standard_normal(n), rng.standard_normal(n), rng.standard_normal(K)
+    x = x0.copy()
+    oracle.gmres_update_result(x, r, basis, n, ld, c, K)
+    assert np.allclose(x, x0 + c[0] * r + c[1:] @ V[:K - 1], rtol=1e-13, atol=1e-13)

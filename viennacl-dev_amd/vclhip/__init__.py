@@ -188,9 +188,14 @@ class SellMatrix:
         fn = "vclhip_sell_spmv_f64" if self.dtype == np.float64 else "vclhip_sell_spmv_f32"
         call(fn, C.byref(self.desc), x.ptr, xs, xi, alpha, y.ptr, ys, yi, beta, stream)
 
+    def free(self):
+        for a in (self.d_cpb, self.d_ci, self.d_bs, self.d_el):
+            a.free()
+
 
 def solver_params(tol=1e-8, max_iters=300, abs_tol=0.0, max_iters_before_restart=200, inner_tol=1e-2,
-                  chunk=16, use_graph=False, mixed_mode=0, variant=0, history=None, monitor=None, inner_format=0):
+                  chunk=16, use_graph=False, mixed_mode=0, variant=0, history=None, monitor=None, inner_format=0,
+                  krylov_dim=20):
     p = SolverParams()
     lib().vclhip_solver_params_default(C.byref(p))
     p.variant = variant
@@ -203,6 +208,7 @@ def solver_params(tol=1e-8, max_iters=300, abs_tol=0.0, max_iters_before_restart
     p.use_graph = 1 if use_graph else 0
     p.mixed_mode = mixed_mode
     p.inner_format = inner_format
+    p.krylov_dim = krylov_dim
     if history is not None:
         p.history_host = history.ctypes.data_as(C.POINTER(C.c_double))
         p.history_len = history.size
@@ -238,6 +244,19 @@ def mixed_cg(A, b, x, precond=PRECOND_NONE, values32=None, stream=None, **kw):
     mat = "csr" if isinstance(A, CsrMatrix) else "sell"
     call(f"vclhip_mixed_cg_solve_{mat}", C.byref(A.desc), values32.ptr if values32 is not None else None, b.ptr,
          x.ptr, precond, C.byref(params), C.byref(res), stream)
+    return res
+
+
+def gmres(A, b, x, stream=None, **kw):
+    """solve(A, b, gmres_tag(tol, max_iters, krylov_dim)) without a preconditioner; returns
+    SolverResult (iters = Krylov steps, error = the last cycle's estimate, restarts = cycles run).
+    The gmres_tag defaults are tol=1e-10, max_iters=300, krylov_dim=20."""
+    kw.setdefault("tol", 1e-10)
+    params = solver_params(**kw)
+    res = SolverResult()
+    mat = "csr" if isinstance(A, CsrMatrix) else "sell"
+    prec = "f32" if A.dtype == np.float32 else "f64"
+    call(f"vclhip_gmres_solve_{mat}_{prec}", C.byref(A.desc), b.ptr, x.ptr, C.byref(params), C.byref(res), stream)
     return res
 
 
@@ -364,6 +383,20 @@ def diffusion(nx, ny, nz, k):
     nnz = C.c_uint32()
     call("vclhip_gen_diffusion", nx, ny, nz, _ptr(k), _ptr(rp), _ptr(col), _ptr(val), C.byref(nnz))
     return rp, col[: nnz.value].copy(), val[: nnz.value].copy()
+
+
+def convection_diffusion(nx, ny, nz, beta=(0.5, 0.3, 0.2)):
+    """Nonsymmetric test matrix: the 7-point Laplacian of laplace_3d with first-order upwind
+    convection of velocity beta (grid units).  Row i: (6 + sum beta) on the diagonal, -1 - beta_d
+    for the upstream neighbour along axis d and -1 for the downstream one; diagonally dominant."""
+    rp, col, val = laplace_3d(nx, ny, nz)
+    row = np.repeat(np.arange(rp.size - 1, dtype=np.int64), np.diff(rp.astype(np.int64)))
+    d = col.astype(np.int64) - row
+    val = val.copy()
+    for step, bd in ((1, beta[0]), (nx, beta[1]), (nx * ny, beta[2])):
+        val[d == -step] -= bd
+    val[d == 0] += float(sum(beta))
+    return rp, col, val
 
 
 def powerlaw(n, max_len=512, alpha=2.0, diag_factor=1.1, seed=1):

@@ -50,7 +50,8 @@ class SolverParams(C.Structure):
     _fields_ = [("tol", f64), ("abs_tol", f64), ("max_iters", u32),
                 ("max_iters_before_restart", u32), ("inner_tol", f32), ("chunk", u32),
                 ("use_graph", i32), ("mixed_mode", i32), ("variant", i32), ("history_host", P(f64)),
-                ("history_len", u32), ("monitor", MONITOR), ("monitor_user", vp), ("inner_format", i32)]
+                ("history_len", u32), ("monitor", MONITOR), ("monitor_user", vp), ("inner_format", i32),
+                ("krylov_dim", u32)]
 
 
 class SolverResult(C.Structure):
@@ -114,6 +115,12 @@ _SIGS = {
         (i32, [u32, vp, f64, vp, f64, vp, vp, vp, f64, vp, vp, vp, u32, vp]),
     "vclhip_pipelined_bicgstab_prod_csr_f64": (i32, [P(Csr), vp, vp, vp, vp, u32, u32, vp]),
     "vclhip_pipelined_bicgstab_prod_sell_f64": (i32, [P(Sell), vp, vp, vp, vp, u32, u32, vp]),
+    "vclhip_pipelined_gmres_prod_csr_f64": (i32, [P(Csr), vp, vp, vp, u32, vp]),
+    "vclhip_pipelined_gmres_prod_sell_f64": (i32, [P(Sell), vp, vp, vp, u32, vp]),
+    "vclhip_pipelined_gmres_normalize_vk_f64": (i32, [u32, vp, vp, vp, C.c_size_t, vp, vp, u32, C.c_size_t, vp]),
+    "vclhip_pipelined_gmres_gram_schmidt_stage1_f64": (i32, [vp, u32, C.c_size_t, u32, vp, u32, vp]),
+    "vclhip_pipelined_gmres_gram_schmidt_stage2_f64": (i32, [vp, u32, C.c_size_t, u32, vp, vp, u32, vp, u32, vp]),
+    "vclhip_pipelined_gmres_update_result_f64": (i32, [vp, vp, vp, u32, C.c_size_t, vp, u32, vp]),
     "vclhip_solver_params_default": (None, [P(SolverParams)]),
     "vclhip_cg_solve_csr_f64": (i32, [P(Csr), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_cg_solve_sell_f64": (i32, [P(Sell), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
@@ -124,6 +131,10 @@ _SIGS = {
     "vclhip_bicgstab_solve_csr_f32": (i32, [P(Csr), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_bicgstab_solve_sell_f32": (i32, [P(Sell), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_mixed_cg_solve_csr": (i32, [P(Csr), vp, vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_gmres_solve_csr_f64": (i32, [P(Csr), vp, vp, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_gmres_solve_sell_f64": (i32, [P(Sell), vp, vp, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_gmres_solve_csr_f32": (i32, [P(Csr), vp, vp, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_gmres_solve_sell_f32": (i32, [P(Sell), vp, vp, P(SolverParams), P(SolverResult), vp]),
     "vclhip_mixed_cg_solve_sell": (i32, [P(Sell), vp, vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_batch_plan_create": (i32, [u32, vp, vp, P(vp)]),
     "vclhip_batch_plan_destroy": (i32, [vp]),
