@@ -236,6 +236,48 @@ def powerlaw_bench(vh, stream, steps):
     return out
 
 
+def formats_bench(vh, stream, steps):
+    """Cold and warm SpMV of coordinate_matrix, ell_matrix and hyb_matrix beside CSR: the 128^3
+    Laplacian and the n = 2M power-law matrix (where HYB moves the few long rows out of the ELL
+    part).  Algorithmic bytes: the stored arrays the kernel must read (ELL padding values included,
+    their columns are not read), x once and y written."""
+    out = {}
+    evict = vh.DeviceArray(EVICT_DOUBLES, np.float64)
+    vh.call("vclhip_fill_zero", evict.ptr, C.c_size_t(evict.nbytes), None)
+    for wname, gen in (("lap3d_128", lambda: vh.laplace_3d(128)),
+                       ("powerlaw_2M", lambda: vh.powerlaw(2097152, 512, 2.0, 1.1, 1))):
+        rp, col, val = gen()
+        n, nnz = rp.size - 1, int(rp[-1])
+        x = vh.DeviceArray.from_numpy(vh.random_uniform(1, n))
+        y = vh.DeviceArray.zeros(n, np.float64)
+        rec = {"n": n, "nnz": nnz}
+        mats = {"csr": (vh.CsrMatrix(rp, col, val), csr_bytes(n, nnz)),
+                "coo": (vh.CooMatrix(rp, col, val), nnz * 16 + 4 * (n + 1) + 16 * n)}
+        width = int(np.diff(rp.astype(np.int64)).max())
+        if n * width <= 6 * nnz:
+            E = vh.EllMatrix(rp, col, val)
+            mats["ell"] = (E, nnz * 12 + (E.ld * E.width - nnz) * 8 + 16 * n)
+        else:  # the power-law matrix's ELL would store 512 columns for every row
+            rec["ell_skipped"] = f"ELL width {width} would store {n * width / nnz:.0f}x the entries"
+        H = vh.HybMatrix(rp, col, val)
+        csr_real = int(H.host[2][-1])
+        ell_real = nnz - csr_real
+        mats["hyb"] = (H, ell_real * 12 + (H.ld * H.width - ell_real) * 8 + csr_real * 12 + 4 * (n + 1) + 16 * n)
+        for name, (M, by) in mats.items():
+            cold, warm, _ = time_launches(vh, lambda: M.spmv(x, y, stream=stream.handle), stream, steps, 3, evict)
+            rec[name] = {"cold_us": round(float(cold.mean()) * 1e3, 2), "warm_us": round(warm * 1e3, 2),
+                         "algorithmic_bytes": int(by), "cold_gbs": round(by / (cold.mean() * 1e-3) / 1e9, 1),
+                         "warm_gbs": round(by / (warm * 1e-3) / 1e9, 1)}
+            M.free()
+        rec["hyb_ell_width"], rec["hyb_csr_nnz"] = H.width, H.csr_nnz
+        out[wname] = rec
+        log("formats", wname, rec)
+        x.free()
+        y.free()
+    evict.free()
+    return out
+
+
 def mixed_vs_fp64(vh, grid, reps=2):
     """FP64 CG vs mixed-precision CG to ||b - A x|| / ||b|| <= 1e-10 on the 3-D variable-coefficient
     diffusion operator (k = exp(g), g ~ N(0,1) per cell, harmonic-mean faces; seed 1 for k, 0 for b)."""
@@ -456,6 +498,7 @@ def main():
     ap.add_argument("--mixed-grid", type=int, default=160,
                     help="grid of the FP64-vs-mixed CG comparison (0 skips it)")
     ap.add_argument("--gmres-grid", type=int, default=128, help="grid of the GMRES runs (0 skips them)")
+    ap.add_argument("--no-formats", action="store_true", help="skip the COO / ELL / HYB SpMV runs")
     ap.add_argument("--batch", type=int, default=4096, help="systems of the batched CG run (0 skips it)")
     ap.add_argument("--adjoint-grid", type=int, default=512, help="grid of the differentiable solve (0 skips it)")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -561,6 +604,8 @@ def main():
         out["powerlaw_2M"] = powerlaw_bench(vh, stream, args.steps)
     if args.mixed_grid:
         out["mixed_vs_fp64"] = mixed_vs_fp64(vh, args.mixed_grid)
+    if not args.no_formats:
+        out["formats_spmv"] = formats_bench(vh, stream, args.steps)
     if args.gmres_grid:
         out["gmres_convdiff"] = gmres_bench(vh, args.gmres_grid)
     if args.batch:

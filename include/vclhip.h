@@ -108,10 +108,65 @@ typedef struct vclhip_sell {
   const void* elements;
 } vclhip_sell;
 
+/* COO matrix on the device (coordinate_matrix<T>: handle12 = (row, col) pairs, handle = values;
+ * coordinate_matrix.hpp:46-99, 185-260).  The entries are ordered by row, as copy() writes them;
+ * row_ptr (n_rows + 1) and blocks are the row pointers of that order and the CSR kernel's row-block
+ * plan, both built on the host by vclhip_coo_plan (the reference keeps its own auxiliary
+ * group_boundaries for its device kernels, coordinate_matrix.hpp:68-91). */
+typedef struct vclhip_coo {
+  uint32_t n_rows;
+  uint32_t n_cols;
+  uint32_t nnz;
+  uint32_t num_blocks;
+  const uint32_t* coords;    /* 2 * nnz: row, col */
+  const void* elements;
+  const uint32_t* row_ptr;
+  const uint32_t* blocks;
+} vclhip_coo;
+
+/* ELL matrix on the device (ell_matrix<T>: handle2 = coords, handle = elements; ell_matrix.hpp:
+ * 123-171).  Column-major: entry k of row i is at [k * internal_size1 + i], k < internal_maxnnz;
+ * padding entries hold the value 0. */
+typedef struct vclhip_ell {
+  uint32_t n_rows;
+  uint32_t n_cols;
+  uint32_t internal_size1;
+  uint32_t internal_maxnnz;
+  const uint32_t* coords;
+  const void* elements;
+} vclhip_ell;
+
+/* HYB matrix on the device (hyb_matrix<T>: an ELL part of internal_ellnnz columns, handle2 =
+ * coords, handle = elements, and a CSR part for the rest of each row, handle3 = row pointers,
+ * handle4 = columns, handle5 = elements; hyb_matrix.hpp:136-240). */
+typedef struct vclhip_hyb {
+  uint32_t n_rows;
+  uint32_t n_cols;
+  uint32_t internal_size1;
+  uint32_t internal_ellnnz;
+  const uint32_t* ell_coords;
+  const void* ell_elements;
+  const uint32_t* csr_rows;
+  const uint32_t* csr_cols;
+  const void* csr_elements;
+} vclhip_hyb;
+
+/* A matrix of any of the formats, for the format-generic entry points below. */
+#define VCLHIP_FORMAT_CSR 0
+#define VCLHIP_FORMAT_SELL 1
+#define VCLHIP_FORMAT_COO 2
+#define VCLHIP_FORMAT_ELL 3
+#define VCLHIP_FORMAT_HYB 4
+typedef struct vclhip_matrix {
+  int format;        /* VCLHIP_FORMAT_* */
+  const void* desc;  /* the vclhip_csr / _sell / _coo / _ell / _hyb descriptor */
+} vclhip_matrix;
+
 /* Row-block plan for the CSR kernels (host function).  Replaces the reference's 1024-nnz row
  * blocks (compressed_matrix::generate_row_block_information, compressed_matrix.hpp:1152-1188) with
  * blocks sized for 256-lane workgroups: each entry packs the first row (bits 0-27) and a mode
- * (bits 28-31: log2 of lanes per row for LDS-staged blocks, 7 for one long row read directly).
+ * (bits 28-31: 0 = every row of at most 32 entries, 1 = some longer rows, 7 = one row of more than
+ * 2048 entries read directly).
  * `blocks_host` must hold n_rows + 1 entries; *num_blocks receives the count (entry num_blocks is
  * the end sentinel n_rows).  n_rows must be below 2^28. */
 int vclhip_csr_plan_blocks(const uint32_t* row_ptr_host, uint32_t n_rows, uint32_t* blocks_host,
@@ -125,6 +180,23 @@ int vclhip_sell_from_csr_f64(uint32_t n_rows, uint32_t C, const uint32_t* row_pt
                              const double* val, uint32_t* num_slices, uint64_t* stored,
                              uint32_t* columns_per_block, uint32_t* column_indices, uint32_t* block_start,
                              double* elements);
+/* COO row pointers and CSR-kernel plan (host function): row_ptr_host (n_rows + 1) and blocks_host
+ * (n_rows + 1) from the (row, col) pairs; fails unless the rows are nondecreasing and below
+ * n_rows. */
+int vclhip_coo_plan(const uint32_t* coords_host, uint32_t nnz, uint32_t n_rows, uint32_t* row_ptr_host,
+                    uint32_t* blocks_host, uint32_t* num_blocks);
+/* CSR -> ELL on the host as ell_matrix's copy() lays it out (ell_matrix.hpp:123-171):
+ * internal_size1 = n_rows rounded up to `align`; first call with NULL outputs for the width. */
+int vclhip_ell_from_csr_f64(uint32_t n_rows, uint32_t align, const uint32_t* row_ptr, const uint32_t* col,
+                            const double* val, uint32_t* internal_maxnnz, uint32_t* coords, double* elements);
+/* CSR -> HYB on the host as hyb_matrix's copy() splits it (hyb_matrix.hpp:136-240): the ELL width is
+ * the smallest row length that at least csr_threshold * n_rows rows do not exceed, rounded up to
+ * `align`; longer rows continue in the CSR part (one zero entry when it is empty).  First call with
+ * NULL outputs for the sizes. */
+int vclhip_hyb_from_csr_f64(uint32_t n_rows, double csr_threshold, uint32_t align, const uint32_t* row_ptr,
+                            const uint32_t* col, const double* val, uint32_t* internal_ellnnz, uint32_t* csr_nnz,
+                            uint32_t* ell_coords, double* ell_elements, uint32_t* csr_rows, uint32_t* csr_cols,
+                            double* csr_elements);
 
 /* ---------------------------------------------------------------- SpMV ---- */
 /* y = alpha*A*x + beta*y.  linalg::prod_impl(compressed_matrix, x, alpha, y, beta)
@@ -144,6 +216,35 @@ int vclhip_sell_spmv_f64(const vclhip_sell* A, const double* x_dev, uint32_t x_s
 int vclhip_sell_spmv_f32(const vclhip_sell* A, const float* x_dev, uint32_t x_start, uint32_t x_stride,
                          float alpha, float* y_dev, uint32_t y_start, uint32_t y_stride, float beta,
                          void* stream);
+/* linalg::prod_impl(coordinate_matrix, ...) (host_based/sparse_matrix_operations.hpp:1221-1246;
+ * cuda/sparse_matrix_operations.hpp:1067-1227): rows summed in entry order. */
+int vclhip_coo_spmv_f64(const vclhip_coo* A, const double* x_dev, uint32_t x_start, uint32_t x_stride,
+                        double alpha, double* y_dev, uint32_t y_start, uint32_t y_stride, double beta,
+                        void* stream);
+int vclhip_coo_spmv_f32(const vclhip_coo* A, const float* x_dev, uint32_t x_start, uint32_t x_stride,
+                        float alpha, float* y_dev, uint32_t y_start, uint32_t y_stride, float beta,
+                        void* stream);
+/* linalg::prod_impl(ell_matrix, ...) (host_based/sparse_matrix_operations.hpp:1502-1537): zero
+ * entries skipped as in the reference. */
+int vclhip_ell_spmv_f64(const vclhip_ell* A, const double* x_dev, uint32_t x_start, uint32_t x_stride,
+                        double alpha, double* y_dev, uint32_t y_start, uint32_t y_stride, double beta,
+                        void* stream);
+int vclhip_ell_spmv_f32(const vclhip_ell* A, const float* x_dev, uint32_t x_start, uint32_t x_stride,
+                        float alpha, float* y_dev, uint32_t y_start, uint32_t y_stride, float beta,
+                        void* stream);
+/* linalg::prod_impl(hyb_matrix, ...) (host_based/sparse_matrix_operations.hpp:1872-1929): the ELL
+ * part (zero entries skipped), then the row's CSR part. */
+int vclhip_hyb_spmv_f64(const vclhip_hyb* A, const double* x_dev, uint32_t x_start, uint32_t x_stride,
+                        double alpha, double* y_dev, uint32_t y_start, uint32_t y_stride, double beta,
+                        void* stream);
+int vclhip_hyb_spmv_f32(const vclhip_hyb* A, const float* x_dev, uint32_t x_start, uint32_t x_stride,
+                        float alpha, float* y_dev, uint32_t y_start, uint32_t y_stride, float beta,
+                        void* stream);
+/* Diagonal of a matrix of any format (row_info with SPARSE_ROW_DIAGONAL, the Jacobi
+ * preconditioner's setup, jacobi_precond.hpp:115-118; host_based :1156-1205 for COO): 0 where a
+ * row has no nonzero diagonal entry. */
+int vclhip_matrix_diag_f64(const vclhip_matrix* A, double* d_dev, void* stream);
+int vclhip_matrix_diag_f32(const vclhip_matrix* A, float* d_dev, void* stream);
 /* linalg::detail::row_info (linalg/sparse_matrix_operations.hpp:48-74; host_based :52-98):
  * info 0 = inf-norm, 1 = 1-norm, 2 = 2-norm, 3 = diagonal (0 when absent). */
 int vclhip_csr_row_info_f64(const vclhip_csr* A, double* out_dev, int info, void* stream);
@@ -179,6 +280,9 @@ int vclhip_axpby_f32(uint32_t n, float a, const float* x_dev, float b, const flo
 int vclhip_avbv_f64(uint32_t n, double* v1_dev, uint32_t s1, uint32_t i1,
                     const double* v2_dev, uint32_t s2, uint32_t i2, double alpha,
                     const double* v3_dev, uint32_t s3, uint32_t i3, double beta, void* stream);
+int vclhip_avbv_f32(uint32_t n, float* v1_dev, uint32_t s1, uint32_t i1, const float* v2_dev, uint32_t s2,
+                    uint32_t i2, float alpha, const float* v3_dev, uint32_t s3, uint32_t i3, float beta,
+                    void* stream);
 /* vec = alpha (vector_assign, linalg/vector_operations.hpp:208-240) */
 int vclhip_fill_f64(uint32_t n, double* v_dev, uint32_t start, uint32_t stride, double alpha, void* stream);
 int vclhip_fill_f32(uint32_t n, float* v_dev, uint32_t start, uint32_t stride, float alpha, void* stream);
@@ -188,6 +292,35 @@ int vclhip_element_div_f32(uint32_t n, const float* x_dev, const float* y_dev, f
 /* convert (linalg/vector_operations.hpp:50-70; host_based :67-83) */
 int vclhip_convert_f64_to_f32(uint32_t n, const double* src_dev, float* dst_dev, void* stream);
 int vclhip_convert_f32_to_f64(uint32_t n, const float* src_dev, double* dst_dev, void* stream);
+/* sum |x_i| (norm_1, linalg/vector_operations.hpp:673-700; host_based :708-733), to the host */
+int vclhip_asum_f64(uint32_t n, const double* x_dev, uint32_t x_start, uint32_t x_stride, double* result_host,
+                    void* stream);
+int vclhip_asum_f32(uint32_t n, const float* x_dev, uint32_t x_start, uint32_t x_stride, float* result_host,
+                    void* stream);
+/* index of the first largest |x_i| (index_norm_inf, host_based/vector_operations.hpp:891-950); when
+ * no entry is above zero the reference returns the vector's start offset, and so does this. */
+int vclhip_iamax_f64(uint32_t n, const double* x_dev, uint32_t x_start, uint32_t x_stride, uint32_t* index_host,
+                     void* stream);
+int vclhip_iamax_f32(uint32_t n, const float* x_dev, uint32_t x_start, uint32_t x_stride, uint32_t* index_host,
+                     void* stream);
+/* x = alpha x (av with vec1 = vec2, host_based/vector_operations.hpp:88-123) */
+int vclhip_scal_f64(uint32_t n, double* x_dev, uint32_t x_start, uint32_t x_stride, double alpha, void* stream);
+int vclhip_scal_f32(uint32_t n, float* x_dev, uint32_t x_start, uint32_t x_stride, float alpha, void* stream);
+/* y = x (vector assignment) */
+int vclhip_copy_f64(uint32_t n, const double* x_dev, uint32_t x_start, uint32_t x_stride, double* y_dev,
+                    uint32_t y_start, uint32_t y_stride, void* stream);
+int vclhip_copy_f32(uint32_t n, const float* x_dev, uint32_t x_start, uint32_t x_stride, float* y_dev,
+                    uint32_t y_start, uint32_t y_stride, void* stream);
+/* x <-> y (vector_swap, linalg/vector_operations.hpp:218-245; host_based :302-327) */
+int vclhip_swap_f64(uint32_t n, double* x_dev, uint32_t x_start, uint32_t x_stride, double* y_dev,
+                    uint32_t y_start, uint32_t y_stride, void* stream);
+int vclhip_swap_f32(uint32_t n, float* x_dev, uint32_t x_start, uint32_t x_stride, float* y_dev,
+                    uint32_t y_start, uint32_t y_stride, void* stream);
+/* x' = c x + s y, y' = c y - s x (plane_rotation, host_based/vector_operations.hpp:1080-1110) */
+int vclhip_rot_f64(uint32_t n, double* x_dev, uint32_t x_start, uint32_t x_stride, double* y_dev, uint32_t y_start,
+                   uint32_t y_stride, double c, double s, void* stream);
+int vclhip_rot_f32(uint32_t n, float* x_dev, uint32_t x_start, uint32_t x_stride, float* y_dev, uint32_t y_start,
+                   uint32_t y_stride, float c, float s, void* stream);
 
 /* -------------------------------------------------- pipelined operations ---- */
 /* Buffer convention of the reference (cg.hpp:144-172, bicgstab.hpp:114-189): inner products are
@@ -229,6 +362,14 @@ int vclhip_pipelined_gmres_prod_csr_f64(const vclhip_csr* A, const double* p_dev
                                         uint32_t chunk, void* stream);
 int vclhip_pipelined_gmres_prod_sell_f64(const vclhip_sell* A, const double* p_dev, double* Ap_dev,
                                          double* buf_dev, uint32_t chunk, void* stream);
+/* The same fused products for a matrix of any format (coordinate_matrix, ell_matrix,
+ * sliced_ell_matrix, hyb_matrix overloads: host_based/iterative_operations.hpp:113-365,
+ * :445-518, :629-701). */
+int vclhip_pipelined_cg_prod_f64(const vclhip_matrix* A, const double* p_dev, double* Ap_dev, double* buf_dev,
+                                 uint32_t chunk, void* stream);
+int vclhip_pipelined_bicgstab_prod_f64(const vclhip_matrix* A, const double* p_dev, double* Ap_dev,
+                                       const double* r0star_dev, double* buf_dev, uint32_t chunk, uint32_t offset,
+                                       void* stream);
 /* The Krylov basis of the GMRES operations: vector i at basis_dev + i * ld (ld >= n, the
  * reference's internal_size). */
 /* linalg::pipelined_gmres_normalize_vk (iterative_operations.hpp:248-277; host_based :734-779):
@@ -370,6 +511,23 @@ int vclhip_gmres_solve_csr_f32(const vclhip_csr* A, const float* b_dev, float* x
                                const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
 int vclhip_gmres_solve_sell_f32(const vclhip_sell* A, const float* b_dev, float* x_dev,
                                 const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+
+/* The solvers for a matrix of any format: solve(A, b, cg_tag / bicgstab_tag / gmres_tag) with
+ * A a coordinate_matrix, ell_matrix or hyb_matrix (cg.hpp:204-254, bicgstab.hpp:231-285 and
+ * gmres.hpp:384-432 send them to the same pipelined solvers), as well as CSR and SELL.  Same
+ * semantics as the per-format entry points above; Jacobi is available for every format. */
+int vclhip_cg_solve_f64(const vclhip_matrix* A, const double* b_dev, double* x_dev, int precond,
+                        const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+int vclhip_cg_solve_f32(const vclhip_matrix* A, const float* b_dev, float* x_dev, int precond,
+                        const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+int vclhip_bicgstab_solve_f64(const vclhip_matrix* A, const double* b_dev, double* x_dev, int precond,
+                              const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+int vclhip_bicgstab_solve_f32(const vclhip_matrix* A, const float* b_dev, float* x_dev, int precond,
+                              const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+int vclhip_gmres_solve_f64(const vclhip_matrix* A, const double* b_dev, double* x_dev,
+                           const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+int vclhip_gmres_solve_f32(const vclhip_matrix* A, const float* b_dev, float* x_dev,
+                           const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
 
 /* ---------------------------------------------------------------- batched CG */
 /* Batched CG: independent SPD systems, the diagonal blocks of one block-diagonal CSR matrix with

@@ -28,6 +28,11 @@ class _Res(C.Structure):
     _fields_ = [("iters", u32), ("error", f64)]
 
 
+class _Fmt(C.Structure):
+    _fields_ = [("format", i32), ("n_rows", u32), ("nnz", u32), ("ld", u32), ("width", u32), ("coords", vp),
+                ("elements", vp), ("csr_rows", vp), ("csr_cols", vp), ("csr_elements", vp)]
+
+
 _ORC_SIGS = {
     "orc_set_num_threads": (None, [i32]),
     "orc_get_num_threads": (i32, []),
@@ -58,6 +63,10 @@ _ORC_SIGS = {
     "orc_mixed_cg_csr": (_Res, [P(_Csr), vp, vp, f64, u32, f32, P(u32)]),
     "orc_mixed_cg2_csr": (_Res, [P(_Csr), vp, vp, vp, f64, u32, f32, i32, i32, P(u32), vp]),
     "orc_gmres_pipelined_csr": (_Res, [P(_Csr), vp, vp, f64, f64, u32, u32, i32, vp]),
+    "orc_fmt_spmv_f64": (None, [P(_Fmt), vp, f64, vp, f64]),
+    "orc_fmt_pipelined_prod_f64": (None, [P(_Fmt), vp, vp, vp, vp, sz, sz]),
+    "orc_cg_pipelined_fmt": (_Res, [P(_Fmt), vp, vp, f64, f64, u32, vp]),
+    "orc_bicgstab_pipelined_fmt": (_Res, [P(_Fmt), vp, vp, f64, f64, u32, vp]),
     "orc_gmres_normalize_vk": (None, [sz, vp, vp, vp, sz, vp, vp, sz, sz]),
     "orc_gmres_stage1": (None, [vp, sz, sz, sz, vp, sz, i32]),
     "orc_gmres_stage2": (None, [vp, sz, sz, sz, vp, vp, sz, vp, sz]),
@@ -88,6 +97,12 @@ _REF_SIGS = {
     "ref_bicgstab_sell": (None, [u32, u32, vp, vp, vp, vp, vp, f64, f64, u32, P(u32), P(f64)]),
     "ref_mixed_cg_csr": (None, [u32, vp, vp, vp, vp, vp, f64, u32, f32, P(u32), P(f64)]),
     "ref_gmres_csr": (None, [u32, vp, vp, vp, vp, vp, f64, f64, u32, u32, P(u32), P(f64)]),
+    "ref_coo_arrays": (None, [u32, vp, vp, vp, vp, vp]),
+    "ref_ell_arrays": (None, [u32, vp, vp, vp, P(u32), P(u32), vp, vp]),
+    "ref_hyb_arrays": (None, [u32, vp, vp, vp, P(u32), P(u32), P(u32), vp, vp, vp, vp, vp]),
+    "ref_fmt_spmv_f64": (None, [i32, u32, vp, vp, vp, vp, f64, vp, f64]),
+    "ref_fmt_pipelined_prod": (None, [i32, u32, vp, vp, vp, vp, vp, vp, vp, u32]),
+    "ref_fmt_solve": (None, [i32, i32, u32, vp, vp, vp, vp, vp, f64, u32, u32, P(u32), P(f64)]),
     "ref_pipelined_gmres_normalize_vk": (None, [u32, vp, vp, vp, u32, u32, vp, vp, u32, u32, u32]),
     "ref_pipelined_gmres_stage1": (None, [vp, u32, u32, u32, u32, vp, u32]),
     "ref_pipelined_gmres_stage2": (None, [vp, u32, u32, u32, u32, vp, vp, vp, u32]),
@@ -167,6 +182,89 @@ class SellHost:
         self.ci, self.el = self.ci[: stored.value], self.el[: stored.value]
         self.n, self.C = n, C_
         self.desc = _Sell(n, C_, p(self.cpb), p(self.ci), p(self.bs), p(self.el))
+
+
+FORMAT_COO, FORMAT_ELL, FORMAT_HYB = 2, 3, 4
+
+
+class FmtHost:
+    """coordinate_matrix / ell_matrix / hyb_matrix arrays kept alive with the oracle descriptor.
+
+    Built by the reference build's copy() (ref_*_arrays) when `arrays` is None, else from the given
+    dict of arrays (a fixture): coords/elements (COO), ld/width/coords/elements (ELL), and for HYB
+    also csr_rows/csr_cols/csr_elements."""
+
+    def __init__(self, fmt, row_ptr, col, val, arrays=None):
+        rp, cl, vl = cu32(row_ptr), cu32(col), c64(val)
+        n = rp.size - 1
+        self.format, self.n = fmt, n
+        if arrays is None:
+            arrays = ref_format_arrays(fmt, rp, cl, vl)
+        self.arrays = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) and v.ndim else v)
+                       for k, v in arrays.items()}
+        a = self.arrays
+        ld = int(a.get("ld", 0))
+        width = int(a.get("width", 0))
+        nnz = int(rp[-1]) if fmt == FORMAT_COO else 0
+        self.desc = _Fmt(fmt, n, nnz, ld, width, p(a["coords"]), p(a["elements"]),
+                         p(a["csr_rows"]) if fmt == FORMAT_HYB else None,
+                         p(a["csr_cols"]) if fmt == FORMAT_HYB else None,
+                         p(a["csr_elements"]) if fmt == FORMAT_HYB else None)
+
+
+def ref_format_arrays(fmt, rp, col, val):
+    """The arrays the reference's copy() writes for the format."""
+    rp, col, val = cu32(rp), cu32(col), c64(val)
+    n = rp.size - 1
+    R = ref()
+    if fmt == FORMAT_COO:
+        nnz = int(rp[-1])
+        coords = np.zeros(2 * max(nnz, 1), np.uint32)
+        el = np.zeros(max(nnz, 1))
+        R.ref_coo_arrays(n, p(rp), p(col), p(val), p(coords), p(el))
+        return {"coords": coords[:2 * nnz], "elements": el[:nnz]}
+    ld, w = u32(), u32()
+    if fmt == FORMAT_ELL:
+        R.ref_ell_arrays(n, p(rp), p(col), p(val), C.byref(ld), C.byref(w), None, None)
+        coords = np.zeros(max(ld.value * w.value, 1), np.uint32)
+        el = np.zeros(max(ld.value * w.value, 1))
+        R.ref_ell_arrays(n, p(rp), p(col), p(val), C.byref(ld), C.byref(w), p(coords), p(el))
+        return {"ld": ld.value, "width": w.value, "coords": coords, "elements": el}
+    cn = u32()
+    R.ref_hyb_arrays(n, p(rp), p(col), p(val), C.byref(ld), C.byref(w), C.byref(cn), None, None, None, None, None)
+    m = max(ld.value * w.value, 1)
+    ec, ee = np.zeros(m, np.uint32), np.zeros(m)
+    cr, cc, ce = np.zeros(n + 1, np.uint32), np.zeros(cn.value, np.uint32), np.zeros(cn.value)
+    R.ref_hyb_arrays(n, p(rp), p(col), p(val), C.byref(ld), C.byref(w), C.byref(cn), p(ec), p(ee), p(cr), p(cc),
+                     p(ce))
+    return {"ld": ld.value, "width": w.value, "coords": ec, "elements": ee, "csr_rows": cr, "csr_cols": cc,
+            "csr_elements": ce}
+
+
+def fmt_spmv(A, x, alpha=1.0, y=None, beta=0.0):
+    x = c64(x)
+    y = np.zeros(A.n) if y is None else c64(y).copy()
+    orc().orc_fmt_spmv_f64(C.byref(A.desc), p(x), alpha, p(y), beta)
+    return y
+
+
+def fmt_pipelined_prod(A, pv, r0=None, chunk=256, offset=0):
+    pv = c64(pv)
+    Ap = np.zeros(A.n)
+    buf = np.zeros(6 * chunk)
+    orc().orc_fmt_pipelined_prod_f64(C.byref(A.desc), p(pv), p(Ap), p(c64(r0)) if r0 is not None else None, p(buf),
+                                     chunk, offset)
+    return Ap, buf
+
+
+def ref_fmt_solve(fmt, kind, rp, col, val, b, tol, max_iters, krylov_dim=20):
+    """solve(A, b, cg_tag / bicgstab_tag / gmres_tag) (kind 0 / 1 / 2) with A in the format."""
+    rp, col, val, b = cu32(rp), cu32(col), c64(val), c64(b)
+    x = np.zeros_like(b)
+    it, err = u32(), f64()
+    ref().ref_fmt_solve(fmt, kind, rp.size - 1, p(rp), p(col), p(val), p(b), p(x), tol, max_iters, krylov_dim,
+                        C.byref(it), C.byref(err))
+    return x, it.value, err.value
 
 
 # ------------------------------------------------------------------ oracle wrappers
@@ -334,6 +432,20 @@ def gmres_stage2(basis, n, ld, k, vi_in_vk, R, krylov_dim, buf, chunk):
 def gmres_update_result(x, residual, basis, n, ld, coef, k):
     """host_based pipelined_gmres_update_result (:898-926); updates x in place."""
     orc().orc_gmres_update_result(p(x), p(residual), p(basis), n, ld, p(coef), k)
+
+
+def cg_pipelined_fmt(A, b, tol, max_iters, abs_tol=0.0):
+    b = c64(b)
+    x = np.zeros_like(b)
+    r = orc().orc_cg_pipelined_fmt(C.byref(A.desc), p(b), p(x), tol, abs_tol, max_iters, None)
+    return x, r.iters, r.error
+
+
+def bicgstab_pipelined_fmt(A, b, tol, max_iters, abs_tol=0.0):
+    b = c64(b)
+    x = np.zeros_like(b)
+    r = orc().orc_bicgstab_pipelined_fmt(C.byref(A.desc), p(b), p(x), tol, abs_tol, max_iters, None)
+    return x, r.iters, r.error
 
 
 def batched_cg(sys_ptr, A, b, tol, max_iters, single=False):

@@ -13,6 +13,9 @@
 #include <vector>
 
 #include "viennacl/compressed_matrix.hpp"
+#include "viennacl/coordinate_matrix.hpp"
+#include "viennacl/ell_matrix.hpp"
+#include "viennacl/hyb_matrix.hpp"
 #include "viennacl/sliced_ell_matrix.hpp"
 #include "viennacl/vector.hpp"
 #include "viennacl/linalg/prod.hpp"
@@ -77,6 +80,25 @@ struct map_matrix {
   void resize(std::size_t r, std::size_t, bool) { rows.assign(r, std::map<unsigned int, double>()); }
   ref operator()(std::size_t r, std::size_t c) { ref x = {this, r, c}; return x; }
 };
+
+template <class F>
+void with_format(int format, uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val, F f)
+{
+  stl_matrix m = to_stl(n, rp, col, val);
+  if (format == 2) {
+    viennacl::coordinate_matrix<double> A(n, n);
+    viennacl::copy(m, A);
+    f(A);
+  } else if (format == 3) {
+    viennacl::ell_matrix<double> A;
+    viennacl::copy(m, A);
+    f(A);
+  } else {
+    viennacl::hyb_matrix<double> A;
+    viennacl::copy(m, A);
+    f(A);
+  }
+}
 
 }  // namespace
 
@@ -360,6 +382,115 @@ void ref_pipelined_gmres_update_result(double* x, const double* residual, const 
   to_vcl(x, vx); to_vcl(residual, vr); to_vcl(basis, vB); to_vcl(coef, vc);
   viennacl::linalg::pipelined_gmres_update_result(vx, vr, vB, n, ld, vc, k);
   from_vcl(vx, x);
+}
+
+// ---- coordinate_matrix / ell_matrix / hyb_matrix, filled by copy() from the STL matrix of the CSR
+// arrays.  format: 2 = COO, 3 = ELL, 4 = HYB.
+
+// The arrays copy() writes (coordinate_matrix.hpp:46-99, ell_matrix.hpp:123-171, hyb_matrix.hpp:136-240).
+void ref_coo_arrays(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val, uint32_t* coords,
+                    double* elements)
+{
+  viennacl::coordinate_matrix<double> A(n, n);
+  viennacl::copy(to_stl(n, rp, col, val), A);
+  viennacl::backend::memory_read(A.handle12(), 0, 2 * A.nnz() * sizeof(uint32_t), coords);
+  viennacl::backend::memory_read(A.handle(), 0, A.nnz() * sizeof(double), elements);
+}
+
+void ref_ell_arrays(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val, uint32_t* ld,
+                    uint32_t* width, uint32_t* coords, double* elements)
+{
+  viennacl::ell_matrix<double> A;
+  viennacl::copy(to_stl(n, rp, col, val), A);
+  *ld = (uint32_t)A.internal_size1();
+  *width = (uint32_t)A.internal_maxnnz();
+  if (!coords) return;
+  viennacl::backend::memory_read(A.handle2(), 0, A.internal_nnz() * sizeof(uint32_t), coords);
+  viennacl::backend::memory_read(A.handle(), 0, A.internal_nnz() * sizeof(double), elements);
+}
+
+void ref_hyb_arrays(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val, uint32_t* ld,
+                    uint32_t* width, uint32_t* csr_nnz, uint32_t* ell_coords, double* ell_elements,
+                    uint32_t* csr_rows, uint32_t* csr_cols, double* csr_elements)
+{
+  viennacl::hyb_matrix<double> A;
+  viennacl::copy(to_stl(n, rp, col, val), A);
+  *ld = (uint32_t)A.internal_size1();
+  *width = (uint32_t)A.internal_ellnnz();
+  *csr_nnz = (uint32_t)A.csr_nnz();
+  if (!ell_coords) return;
+  const size_t ell = A.internal_size1() * A.internal_ellnnz();
+  viennacl::backend::memory_read(A.handle2(), 0, ell * sizeof(uint32_t), ell_coords);
+  viennacl::backend::memory_read(A.handle(), 0, ell * sizeof(double), ell_elements);
+  viennacl::backend::memory_read(A.handle3(), 0, (n + 1) * sizeof(uint32_t), csr_rows);
+  viennacl::backend::memory_read(A.handle4(), 0, A.csr_nnz() * sizeof(uint32_t), csr_cols);
+  viennacl::backend::memory_read(A.handle5(), 0, A.csr_nnz() * sizeof(double), csr_elements);
+}
+
+// linalg::prod_impl for the format (y = alpha A x + beta y)
+void ref_fmt_spmv_f64(int format, uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val,
+                      const double* x, double alpha, double* y, double beta)
+{
+  with_format(format, n, rp, col, val, [&](auto& A) {
+    viennacl::vector<double> vx(n), vy(n);
+    to_vcl(x, vx);
+    to_vcl(y, vy);
+    viennacl::linalg::prod_impl(A, vx, alpha, vy, beta);
+    from_vcl(vy, y);
+  });
+}
+
+// linalg::pipelined_cg_prod / pipelined_bicgstab_prod for the format (r0 == NULL: CG)
+void ref_fmt_pipelined_prod(int format, uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val,
+                            const double* p, double* Ap, const double* r0, double* buf, uint32_t offset)
+{
+  with_format(format, n, rp, col, val, [&](auto& A) {
+    viennacl::vector<double> vp(n), vAp(n);
+    to_vcl(p, vp);
+    to_vcl(Ap, vAp);
+    if (r0) {
+      viennacl::vector<double> vr0(n), vb(1536);
+      to_vcl(r0, vr0);
+      to_vcl(buf, vb);
+      viennacl::linalg::pipelined_bicgstab_prod(A, vp, vAp, vr0, vb, 256, offset);
+      from_vcl(vb, buf);
+    } else {
+      viennacl::vector<double> vb(768);
+      to_vcl(buf, vb);
+      viennacl::linalg::pipelined_cg_prod(A, vp, vAp, vb);
+      from_vcl(vb, buf);
+    }
+    from_vcl(vAp, Ap);
+  });
+}
+
+// solve(A, b, cg_tag / bicgstab_tag / gmres_tag) for the format (kind 0 / 1 / 2)
+void ref_fmt_solve(int format, int kind, uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val,
+                   const double* b, double* x, double tol, uint32_t max_iters, uint32_t krylov_dim, uint32_t* iters,
+                   double* error)
+{
+  with_format(format, n, rp, col, val, [&](auto& A) {
+    viennacl::vector<double> vb(n);
+    to_vcl(b, vb);
+    viennacl::vector<double> vx(n);
+    if (kind == 0) {
+      viennacl::linalg::cg_tag tag(tol, max_iters);
+      vx = viennacl::linalg::solve(A, vb, tag);
+      *iters = tag.iters();
+      *error = tag.error();
+    } else if (kind == 1) {
+      viennacl::linalg::bicgstab_tag tag(tol, max_iters);
+      vx = viennacl::linalg::solve(A, vb, tag);
+      *iters = tag.iters();
+      *error = tag.error();
+    } else {
+      viennacl::linalg::gmres_tag tag(tol, max_iters, krylov_dim);
+      vx = viennacl::linalg::solve(A, vb, tag);
+      *iters = tag.iters();
+      *error = tag.error();
+    }
+    from_vcl(vx, x);
+  });
 }
 
 // gmres.hpp:372-378 (compressed_matrix, no preconditioner -> pipelined_solve)

@@ -471,19 +471,97 @@ void orc_pipelined_bicgstab_vector_update_f64(size_t n, double* x, double alpha,
   buf[0] = inner_prod_r_r0star;
 }
 
+/* ------------------------------------------- coordinate / ELL / HYB matrices */
+
+/* host_based/sparse_matrix_operations.hpp:1221-1246 (COO), :1502-1537 (ELL), :1872-1929 (HYB);
+ * contiguous vectors. */
+void orc_fmt_spmv_f64(const orc_fmt64* A, const double* x, double alpha, double* y, double beta)
+{
+  const size_t n = A->n_rows;
+  if (A->format == ORC_FORMAT_COO) {
+    for (size_t i = 0; i < n; ++i) {
+      if (beta < 0 || beta > 0)
+        y[i] *= beta;
+      else
+        y[i] = 0;
+    }
+    for (size_t i = 0; i < A->nnz; ++i)
+      y[A->coords[2 * i]] += alpha * A->elements[i] * x[A->coords[2 * i + 1]];
+    return;
+  }
+  for (size_t row = 0; row < n; ++row) {
+    double sum = 0;
+    for (size_t k = 0; k < A->width; ++k) {
+      const size_t off = row + k * A->ld;
+      const double v = A->elements[off];
+      if (v > 0 || v < 0) sum += x[A->coords[off]] * v;
+    }
+    if (A->format == ORC_FORMAT_HYB)
+      for (size_t e = A->csr_rows[row]; e < A->csr_rows[row + 1]; ++e) sum += x[A->csr_cols[e]] * A->csr_elements[e];
+    if (beta < 0 || beta > 0)
+      y[row] = alpha * sum + beta * y[row];
+    else
+      y[row] = alpha * sum;
+  }
+}
+
+/* pipelined_prod_impl for coordinate_matrix / ell_matrix / hyb_matrix
+ * (host_based/iterative_operations.hpp:113-156, :166-210, :302-365), single-threaded order. */
+void orc_fmt_pipelined_prod_f64(const orc_fmt64* A, const double* p, double* Ap, const double* r0star,
+                                double* buf, size_t chunk, size_t offset)
+{
+  const size_t n = A->n_rows;
+  double ApAp = 0, pAp = 0, Apr0 = 0;
+  if (A->format == ORC_FORMAT_COO) {
+    for (size_t i = 0; i < n; ++i) Ap[i] = 0;
+    for (size_t i = 0; i < A->nnz; ++i) Ap[A->coords[2 * i]] += A->elements[i] * p[A->coords[2 * i + 1]];
+    for (size_t i = 0; i < n; ++i) {
+      const double a = Ap[i], q = p[i];
+      ApAp += a * a;
+      pAp += a * q;
+      Apr0 += r0star ? a * r0star[i] : 0.0;
+    }
+  } else {
+    for (size_t row = 0; row < n; ++row) {
+      const double pd = p[row];
+      double sum = 0;
+      for (size_t k = 0; k < A->width; ++k) {
+        const size_t off = row + k * A->ld;
+        const double v = A->elements[off];
+        if (v != 0) sum += p[A->coords[off]] * v;
+      }
+      if (A->format == ORC_FORMAT_HYB)
+        for (size_t e = A->csr_rows[row]; e < A->csr_rows[row + 1]; ++e) sum += p[A->csr_cols[e]] * A->csr_elements[e];
+      Ap[row] = sum;
+      ApAp += sum * sum;
+      pAp += pd * sum;
+      Apr0 += r0star ? sum * r0star[row] : 0.0;
+    }
+  }
+  buf[chunk] = ApAp;
+  buf[2 * chunk] = pAp;
+  if (r0star) buf[offset] = Apr0;
+}
+
 /* --------------------------------------------------------------- solvers */
 
 typedef struct orc_op {
-  int is_sell;
+  int is_sell;  /* 0 CSR, 1 SELL, 2 COO / ELL / HYB */
   const orc_csr64* csr;
   const orc_sell64* sell;
+  const orc_fmt64* fmt;
 } orc_op;
 
-static uint32_t op_rows(const orc_op* A) { return A->is_sell ? A->sell->n_rows : A->csr->n_rows; }
+static uint32_t op_rows(const orc_op* A)
+{
+  return A->is_sell == 2 ? A->fmt->n_rows : A->is_sell ? A->sell->n_rows : A->csr->n_rows;
+}
 
 static void op_prod(const orc_op* A, const double* x, double* y)
 {
-  if (A->is_sell)
+  if (A->is_sell == 2)
+    orc_fmt_spmv_f64(A->fmt, x, 1.0, y, 0.0);
+  else if (A->is_sell)
     orc_sell_spmv_f64(A->sell->n_rows, A->sell->C, A->sell->columns_per_block, A->sell->column_indices,
                       A->sell->block_start, A->sell->elements, x, 0, 1, 1.0, y, 0, 1, 0.0);
   else
@@ -493,7 +571,9 @@ static void op_prod(const orc_op* A, const double* x, double* y)
 static void op_pipelined_prod(const orc_op* A, const double* p, double* Ap, const double* r0star,
                               double* buf, size_t chunk, size_t offset)
 {
-  if (A->is_sell)
+  if (A->is_sell == 2)
+    orc_fmt_pipelined_prod_f64(A->fmt, p, Ap, r0star, buf, chunk, offset);
+  else if (A->is_sell)
     orc_pipelined_prod_sell_f64(A->sell->n_rows, A->sell->C, A->sell->columns_per_block, A->sell->column_indices,
                                 A->sell->block_start, A->sell->elements, p, Ap, r0star, buf, chunk, offset);
   else
@@ -557,14 +637,14 @@ static orc_result cg_pipelined(const orc_op* A, const double* b, double* x, doub
 orc_result orc_cg_pipelined_csr(const orc_csr64* A, const double* b, double* x, double tol, double abs_tol,
                                 uint32_t max_iters, double* history)
 {
-  orc_op op = {0, A, NULL};
+  orc_op op = {0, A, NULL, NULL};
   return cg_pipelined(&op, b, x, tol, abs_tol, max_iters, history);
 }
 
 orc_result orc_cg_pipelined_sell(const orc_sell64* A, const double* b, double* x, double tol, double abs_tol,
                                  uint32_t max_iters, double* history)
 {
-  orc_op op = {1, NULL, A};
+  orc_op op = {1, NULL, A, NULL};
   return cg_pipelined(&op, b, x, tol, abs_tol, max_iters, history);
 }
 
@@ -575,7 +655,7 @@ orc_result orc_cg_classic_csr(const orc_csr64* A, const double* b, double* x, co
                               double tol, double abs_tol, uint32_t max_iters, double* history)
 {
   orc_result res = {0, 0.0};
-  orc_op op = {0, A, NULL};
+  orc_op op = {0, A, NULL, NULL};
   size_t n = A->n_rows;
   double* r = (double*)malloc(n * sizeof(double));
   double* tmp = (double*)malloc(n * sizeof(double));
@@ -678,14 +758,28 @@ static orc_result bicgstab_pipelined(const orc_op* A, const double* b, double* x
 orc_result orc_bicgstab_pipelined_csr(const orc_csr64* A, const double* b, double* x, double tol, double abs_tol,
                                       uint32_t max_iters, double* history)
 {
-  orc_op op = {0, A, NULL};
+  orc_op op = {0, A, NULL, NULL};
   return bicgstab_pipelined(&op, b, x, tol, abs_tol, max_iters, history);
 }
 
 orc_result orc_bicgstab_pipelined_sell(const orc_sell64* A, const double* b, double* x, double tol, double abs_tol,
                                        uint32_t max_iters, double* history)
 {
-  orc_op op = {1, NULL, A};
+  orc_op op = {1, NULL, A, NULL};
+  return bicgstab_pipelined(&op, b, x, tol, abs_tol, max_iters, history);
+}
+
+orc_result orc_cg_pipelined_fmt(const orc_fmt64* A, const double* b, double* x, double tol, double abs_tol,
+                                uint32_t max_iters, double* history)
+{
+  orc_op op = {2, NULL, NULL, A};
+  return cg_pipelined(&op, b, x, tol, abs_tol, max_iters, history);
+}
+
+orc_result orc_bicgstab_pipelined_fmt(const orc_fmt64* A, const double* b, double* x, double tol, double abs_tol,
+                                      uint32_t max_iters, double* history)
+{
+  orc_op op = {2, NULL, NULL, A};
   return bicgstab_pipelined(&op, b, x, tol, abs_tol, max_iters, history);
 }
 
@@ -695,7 +789,7 @@ orc_result orc_bicgstab_classic_csr(const orc_csr64* A, const double* b, double*
                                     double* history)
 {
   orc_result res = {0, 0.0};
-  orc_op op = {0, A, NULL};
+  orc_op op = {0, A, NULL, NULL};
   size_t n = A->n_rows;
   double* r = (double*)malloc(n * sizeof(double));
   double* r0star = (double*)malloc(n * sizeof(double));

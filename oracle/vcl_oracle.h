@@ -122,6 +122,25 @@ typedef struct orc_sell64 {
   const double* elements;
 } orc_sell64;
 
+/* coordinate_matrix (format 2: coords = 2*nnz (row, col) pairs in row order), ell_matrix
+ * (format 3) and hyb_matrix (format 4: ELL part plus csr_rows / csr_cols / csr_elements); the ELL
+ * parts are column-major with leading dimension ld and width columns. */
+#define ORC_FORMAT_COO 2
+#define ORC_FORMAT_ELL 3
+#define ORC_FORMAT_HYB 4
+typedef struct orc_fmt64 {
+  int format;
+  uint32_t n_rows;
+  uint32_t nnz;
+  uint32_t ld;
+  uint32_t width;
+  const uint32_t* coords;
+  const double* elements;
+  const uint32_t* csr_rows;
+  const uint32_t* csr_cols;
+  const double* csr_elements;
+} orc_fmt64;
+
 typedef struct orc_result {
   uint32_t iters;
   double error;
@@ -164,6 +183,20 @@ orc_result orc_mixed_cg_csr(const orc_csr64* A, const double* b, double* x, doub
 orc_result orc_mixed_cg2_csr(const orc_csr64* A, const double* b, double* x, const double* diag, double tol,
                              uint32_t max_iters, float inner_tol, int mode, int acc64, uint32_t* updates,
                              double* history);
+
+/* prod_impl for the format (host_based/sparse_matrix_operations.hpp:1221-1246 COO, :1502-1537
+ * ELL, :1872-1929 HYB), contiguous vectors. */
+void orc_fmt_spmv_f64(const orc_fmt64* A, const double* x, double alpha, double* y, double beta);
+/* pipelined_cg_prod / pipelined_bicgstab_prod for the format (host_based/iterative_operations.hpp:
+ * 113-156, 166-210, 302-365). */
+void orc_fmt_pipelined_prod_f64(const orc_fmt64* A, const double* p, double* Ap, const double* r0star,
+                                double* buf, size_t chunk, size_t offset);
+/* cg.hpp:128-187 / bicgstab.hpp:97-215 with the format's SpMV and fused product
+ * (cg.hpp:204-254, bicgstab.hpp:231-285 send these formats to the pipelined solvers). */
+orc_result orc_cg_pipelined_fmt(const orc_fmt64* A, const double* b, double* x, double tol, double abs_tol,
+                                uint32_t max_iters, double* history);
+orc_result orc_bicgstab_pipelined_fmt(const orc_fmt64* A, const double* b, double* x, double tol, double abs_tol,
+                                      uint32_t max_iters, double* history);
 
 /* gmres.hpp:182-370: pipelined GMRES(krylov_dim) without preconditioner, classical Gram-Schmidt.
  * gs_skip_tail = 1 reproduces the reference's single-thread Gram-Schmidt range, which leaves out

@@ -11,6 +11,9 @@
 #include <vector>
 
 #include "viennacl/compressed_matrix.hpp"
+#include "viennacl/coordinate_matrix.hpp"
+#include "viennacl/ell_matrix.hpp"
+#include "viennacl/hyb_matrix.hpp"
 #include "viennacl/io/matrix_market.hpp"
 #include "viennacl/linalg/bicgstab.hpp"
 #include "viennacl/linalg/cg.hpp"
@@ -359,6 +362,81 @@ void test_gmres(double tol_solver, double tol_check)
   }
 }
 
+template <typename T, typename MatrixT>
+void check_format(const char* name, StlMatrix<T> const& hA, std::vector<T> const& b, viennacl::vector<T> const& vb,
+                  double eps, double tol_solver, double tol_check)
+{
+  using namespace viennacl::linalg;
+  const unsigned n = unsigned(hA.size());
+  MatrixT M;
+  viennacl::copy(hA, M);
+  CHECK(M.size1() == n && M.size2() == n, "%s sizes", name);
+  // prod against the STL product, y = A x and y += A x
+  std::vector<T> hx(n);
+  for (unsigned i = 0; i < n; ++i) hx[i] = T(std::sin(0.37 * i));
+  viennacl::vector<T> vx(n), vy(n);
+  viennacl::copy(hx, vx);
+  vy = viennacl::linalg::prod(M, vx);
+  std::vector<T> ref = stl_prod(hA, hx);
+  CHECK(rel_inf(to_host(vy), ref) < eps, "%s prod %g", name, rel_inf(to_host(vy), ref));
+  vy += viennacl::linalg::prod(M, vx);
+  std::vector<T> r2(ref);
+  for (auto& v : r2) v *= T(2);
+  CHECK(rel_inf(to_host(vy), r2) < eps, "%s prod += %g", name, rel_inf(to_host(vy), r2));
+  // back to the STL matrix
+  StlMatrix<T> back(n);
+  viennacl::copy(M, back);
+  bool same = true;
+  for (unsigned i = 0; i < n; ++i) same = same && back[i] == hA[i];
+  CHECK(same, "%s -> STL round trip", name);
+  // solvers
+  bicgstab_tag tag(tol_solver, 2000);
+  viennacl::vector<T> x = solve(M, vb, tag);
+  CHECK(true_relres(hA, b, to_host(x)) < tol_check, "%s bicgstab (iters %lu)", name, (unsigned long)tag.iters());
+  gmres_tag gtag(tol_solver, 2000, 20);
+  viennacl::vector<T> x2 = solve(M, vb, gtag);
+  CHECK(true_relres(hA, b, to_host(x2)) < tol_check, "%s gmres (iters %u)", name, gtag.iters());
+}
+
+template <typename T>
+void test_formats(double eps, double tol_solver, double tol_check)
+{
+  // nonsymmetric convection-diffusion with a few long rows, so that HYB has a CSR part
+  const unsigned px = 40, py = 30, n = px * py;
+  StlMatrix<T> hA(n);
+  for (unsigned j = 0; j < py; ++j)
+    for (unsigned i = 0; i < px; ++i) {
+      const unsigned r = j * px + i;
+      hA[r][r] = T(4.8);
+      if (i > 0) hA[r][r - 1] = T(-1.5);
+      if (i + 1 < px) hA[r][r + 1] = T(-1);
+      if (j > 0) hA[r][r - px] = T(-1.3);
+      if (j + 1 < py) hA[r][r + px] = T(-1);
+    }
+  for (unsigned r = 0; r < n; r += 97)
+    for (unsigned k = 1; k <= 12; ++k) hA[r][(r + 37 * k) % n] += T(-0.01);
+  std::mt19937_64 g(5);
+  std::uniform_real_distribution<double> u(-1, 1);
+  std::vector<T> b(n);
+  for (auto& v : b) v = T(u(g));
+  viennacl::vector<T> vb(n);
+  viennacl::copy(b, vb);
+  check_format<T, viennacl::coordinate_matrix<T> >("coordinate_matrix", hA, b, vb, eps, tol_solver, tol_check);
+  check_format<T, viennacl::ell_matrix<T> >("ell_matrix", hA, b, vb, eps, tol_solver, tol_check);
+  check_format<T, viennacl::hyb_matrix<T> >("hyb_matrix", hA, b, vb, eps, tol_solver, tol_check);
+  // Jacobi on coordinate_matrix (row_info of the COO diagonal, jacobi_precond.hpp:104-130)
+  viennacl::coordinate_matrix<T> C;
+  viennacl::copy(hA, C);
+  viennacl::linalg::jacobi_precond<viennacl::coordinate_matrix<T> > J(C, viennacl::linalg::jacobi_tag());
+  viennacl::linalg::bicgstab_tag tag(tol_solver, 2000);
+  viennacl::vector<T> x = viennacl::linalg::solve(C, vb, tag, J);
+  CHECK(true_relres(hA, b, to_host(x)) < 10 * tol_check, "jacobi bicgstab on coordinate_matrix");
+  viennacl::hyb_matrix<T> H;
+  viennacl::copy(hA, H);
+  CHECK(H.csr_nnz() > 1 && H.internal_ellnnz() == 5, "hyb split: ell width %lu, csr part %lu",
+        (unsigned long)H.internal_ellnnz(), (unsigned long)H.csr_nnz());
+}
+
 void test_mixed_precision()
 {
   // 2-D variable-coefficient problem: diagonal entries vary, so Jacobi changes the iteration
@@ -464,6 +542,8 @@ int main()
   test_solvers<float>(1e-5, 1e-4);
   test_gmres<double>(1e-10, 1e-9);
   test_gmres<float>(1e-5, 1e-4);
+  test_formats<double>(1e-13, 1e-10, 1e-9);
+  test_formats<float>(1e-5, 1e-5, 1e-4);
   test_mixed_precision();
   test_matrix_market();
   if (failures) {

@@ -6,8 +6,8 @@ inputs and the reference's outputs, so the tests pin the oracle and the HIP path
 reference being present.  The reference's own SpMV test matrix (examples/testdata/mat65k.mtx) is
 not available, so these seeded problems stand in for it.
 
-Run from the repository root:  python tests/golden/make_golden.py  (add `gmres` to write only the
-GMRES fixtures)
+Run from the repository root:  python tests/golden/make_golden.py  (add `gmres` or `formats` to write
+only those fixtures)
 """
 import pathlib
 import sys
@@ -234,9 +234,61 @@ def gmres_main():
              iters=np.array(its, np.int64), error=np.array(errs), env=np.array(env, np.int64))
 
 
+def formats_main():
+    """coordinate_matrix / ell_matrix / hyb_matrix: the arrays copy() writes, prod_impl, the fused
+    pipelined products and the solvers (iterative_operations.hpp:113-365; cg.hpp:204-254,
+    bicgstab.hpp:231-285, gmres.hpp:384-432)."""
+    po.set_threads(1)
+    R = po.ref()
+    p = po.p
+    rp, col, val = vh.powerlaw(700, max_len=60, seed=4)
+    # rows 0, 5 and 699 empty
+    keep = np.ones(col.size, bool)
+    for r in (0, 5, 699):
+        keep[rp[r]:rp[r + 1]] = False
+    lens = np.diff(rp.astype(np.int64))
+    lens[[0, 5, 699]] = 0
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint32)
+    col, val = col[keep], val[keep]
+    n = rp.size - 1
+    x, y0, r0 = vh.random_uniform(31, n), vh.random_uniform(32, n), vh.random_uniform(33, n)
+    out = {"row_ptr": rp, "col": col, "val": val, "x": x, "y0": y0, "r0": r0}
+    for fmt, tag in ((po.FORMAT_COO, "coo"), (po.FORMAT_ELL, "ell"), (po.FORMAT_HYB, "hyb")):
+        for k, v in po.ref_format_arrays(fmt, rp, col, val).items():
+            out[f"{tag}_{k}"] = v
+        for at, alpha, beta in (("a1b0", 1.0, 0.0), ("a2b05", 2.0, 0.5), ("am1b1", -1.0, 1.0)):
+            y = y0.copy()
+            R.ref_fmt_spmv_f64(fmt, n, p(rp), p(col), p(val), p(x), alpha, p(y), beta)
+            out[f"{tag}_y_{at}"] = y
+        Ap, buf = np.zeros(n), np.zeros(768)
+        R.ref_fmt_pipelined_prod(fmt, n, p(rp), p(col), p(val), p(x), p(Ap), None, p(buf), 0)
+        out[f"{tag}_cgprod_Ap"], out[f"{tag}_cgprod_buf"] = Ap, buf
+        Ap, buf = np.zeros(n), np.zeros(1536)
+        R.ref_fmt_pipelined_prod(fmt, n, p(rp), p(col), p(val), p(x), p(Ap), p(r0), p(buf), 768)
+        out[f"{tag}_bprod_Ap"], out[f"{tag}_bprod_buf"] = Ap, buf
+    save("formats_powerlaw_700", **out)
+
+    # solvers through linalg::solve for each format: CG on the 2-D Laplacian, BiCGStab and GMRES on
+    # convection-diffusion
+    for name, (rp, col, val), kind, tol, mi in (("lap2d_30x22", vh.fdm_laplace_2d(30, 22), 0, 1e-9, 2000),
+                                                ("convdiff_12x9x4", vh.convection_diffusion(12, 9, 4), 1, 1e-9, 2000),
+                                                ("convdiff_12x9x4_gmres", vh.convection_diffusion(12, 9, 4), 2, 1e-9,
+                                                 400)):
+        n = rp.size - 1
+        b = vh.random_uniform(7, n)
+        out = {"row_ptr": rp, "col": col, "val": val, "b": b, "kind": kind, "tol": tol, "max_iters": mi}
+        for fmt, tag in ((po.FORMAT_COO, "coo"), (po.FORMAT_ELL, "ell"), (po.FORMAT_HYB, "hyb")):
+            x, it, err = po.ref_fmt_solve(fmt, kind, rp, col, val, b, tol, mi, 20)
+            out[f"{tag}_x"], out[f"{tag}_iters"], out[f"{tag}_error"] = x, it, err
+        save(f"solve_formats_{name}", **out)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["gmres"]:
         gmres_main()
+    elif sys.argv[1:] == ["formats"]:
+        formats_main()
     else:
         main()
         gmres_main()
+        formats_main()

@@ -1,12 +1,13 @@
-"""Runs the C++ drop-in API test programs (tests/cpp/*.cpp, built by `make -C viennacl-dev_amd
-tests`) on the device.  Each program prints one FAIL line per failed check and exits nonzero."""
+"""Runs the C++ drop-in API test programs (tests/cpp/*.cpp) and the C programs of the libviennacl-style
+API (tests/cpp/*.c), built by `make -C viennacl-dev_amd tests`, on the device.  Each program prints one FAIL line per failed check and exits nonzero."""
 import pathlib
 import subprocess
 
 import pytest
 
 BIN = pathlib.Path(__file__).resolve().parent / "cpp" / "bin"
-SOURCES = sorted((pathlib.Path(__file__).resolve().parent / "cpp").glob("*.cpp"))
+SOURCES = sorted(list((pathlib.Path(__file__).resolve().parent / "cpp").glob("*.cpp")) +
+                 list((pathlib.Path(__file__).resolve().parent / "cpp").glob("*.c")))
 
 
 @pytest.mark.gpu

@@ -132,3 +132,25 @@ def test_graph_laplacian_batch(vh):
             assert np.all((c >= r0) & (c < r1))
             assert np.isclose(v.sum(), 0.01)
             assert np.all(v[c != i] < 0)
+
+
+def test_coo_plan_rows_and_order(vh):
+    """vclhip_coo_plan: row pointers of the row order and the CSR plan; unordered rows are refused."""
+    rp, col, val = vh.powerlaw(3000, max_len=300, seed=9)
+    n, nnz = rp.size - 1, int(rp[-1])
+    rows = np.repeat(np.arange(n, dtype=np.uint32), np.diff(rp.astype(np.int64)))
+    coords = np.empty(2 * nnz, np.uint32)
+    coords[0::2], coords[1::2] = rows, col
+    prp, blocks, nb = np.zeros(n + 1, np.uint32), np.zeros(n + 1, np.uint32), ctypes.c_uint32()
+    vh.call("vclhip_coo_plan", _abi_ptr(coords), nnz, n, _abi_ptr(prp), _abi_ptr(blocks), ctypes.byref(nb))
+    assert np.array_equal(prp, rp)
+    ref_blocks = vh.plan_blocks(rp)
+    assert np.array_equal(blocks[: nb.value + 1], ref_blocks)
+    bad = coords.copy()
+    bad[0], bad[2 * (nnz - 1)] = bad[2 * (nnz - 1)], bad[0]
+    with pytest.raises(vh.VclHipError):
+        vh.call("vclhip_coo_plan", _abi_ptr(bad), nnz, n, _abi_ptr(prp), _abi_ptr(blocks), ctypes.byref(nb))
+
+
+def _abi_ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)

@@ -161,6 +161,76 @@ def test_gmres_matches_reference(oracle, case):
         assert lo <= it <= hi
 
 
+FORMATS = (("coo", 2), ("ell", 3), ("hyb", 4))
+
+
+def fixture_arrays(g, tag):
+    return {k[len(tag) + 1:]: g[k] for k in g if k.startswith(tag + "_") and not k.startswith(tag + "_y_")
+            and "prod" not in k}
+
+
+@pytest.mark.parametrize("tag,fmt", FORMATS)
+def test_format_arrays_spmv_and_products_match_reference(oracle, vh, tag, fmt):
+    """copy() into coordinate_matrix / ell_matrix / hyb_matrix, prod_impl and the pipelined products
+    (host_based/sparse_matrix_operations.hpp:1221-1929, iterative_operations.hpp:113-365)."""
+    g = load("formats_powerlaw_700")
+    rp, col, val = g["row_ptr"], g["col"], g["val"]
+    arrays = fixture_arrays(g, tag)
+    # the library's host conversions build the reference's arrays
+    if fmt == 3:
+        ld, w, c, e = vh.ell_from_csr(rp, col, val)
+        assert (ld, w) == (int(arrays["ld"]), int(arrays["width"]))
+        assert np.array_equal(c, arrays["coords"]) and np.array_equal(e, arrays["elements"])
+    elif fmt == 4:
+        ld, w, ec, ee, cr, cc, ce = vh.hyb_from_csr(rp, col, val)
+        assert (ld, w) == (int(arrays["ld"]), int(arrays["width"]))
+        assert np.array_equal(ec, arrays["coords"]) and np.array_equal(ee, arrays["elements"])
+        assert np.array_equal(cr, arrays["csr_rows"]) and np.array_equal(cc, arrays["csr_cols"])
+        assert np.array_equal(ce, arrays["csr_elements"])
+    else:
+        rows = np.repeat(np.arange(rp.size - 1), np.diff(rp.astype(np.int64)))
+        assert np.array_equal(arrays["coords"][0::2], rows) and np.array_equal(arrays["coords"][1::2], col)
+    A = oracle.FmtHost(fmt, rp, col, val, arrays)
+    for at, alpha, beta in (("a1b0", 1.0, 0.0), ("a2b05", 2.0, 0.5), ("am1b1", -1.0, 1.0)):
+        assert np.array_equal(oracle.fmt_spmv(A, g["x"], alpha, g["y0"], beta), g[f"{tag}_y_{at}"]), at
+    Ap, buf = oracle.fmt_pipelined_prod(A, g["x"])
+    assert np.array_equal(Ap, g[f"{tag}_cgprod_Ap"]) and np.array_equal(buf[:768], g[f"{tag}_cgprod_buf"])
+    Ap, buf = oracle.fmt_pipelined_prod(A, g["x"], r0=g["r0"], offset=768)
+    assert np.array_equal(Ap, g[f"{tag}_bprod_Ap"]) and np.array_equal(buf, g[f"{tag}_bprod_buf"])
+
+
+def library_format_arrays(vh, fmt, rp, col, val):
+    """The format's arrays as the library's host conversions build them."""
+    if fmt == 2:
+        rows = np.repeat(np.arange(rp.size - 1), np.diff(rp.astype(np.int64))).astype(np.uint32)
+        coords = np.empty(2 * col.size, np.uint32)
+        coords[0::2], coords[1::2] = rows, col
+        return {"coords": coords, "elements": np.asarray(val, np.float64)}
+    if fmt == 3:
+        ld, w, c, e = vh.ell_from_csr(rp, col, val)
+        return {"ld": ld, "width": w, "coords": c, "elements": e}
+    ld, w, ec, ee, cr, cc, ce = vh.hyb_from_csr(rp, col, val)
+    return {"ld": ld, "width": w, "coords": ec, "elements": ee, "csr_rows": cr, "csr_cols": cc, "csr_elements": ce}
+
+
+@pytest.mark.parametrize("case", ["lap2d_30x22", "convdiff_12x9x4", "convdiff_12x9x4_gmres"])
+def test_format_solvers_match_reference(oracle, vh, case):
+    """solve() with coordinate_matrix / ell_matrix / hyb_matrix (cg.hpp:204-254, bicgstab.hpp:231-285,
+    gmres.hpp:384-432): the oracle's pipelined solvers on the format reproduce x, iterations and error.
+    GMRES on these formats is the CSR restatement (the products sum each row in the same order)."""
+    g = load(f"solve_formats_{case}")
+    rp, col, val, b = g["row_ptr"], g["col"], g["val"], g["b"]
+    kind, tol, mi = int(g["kind"]), float(g["tol"]), int(g["max_iters"])
+    for tag, fmt in FORMATS:
+        if kind == 2:
+            x, it, err, _ = oracle.gmres(oracle.CsrHost(rp, col, val), b, tol, mi, 20, gs_skip_tail=True)
+        else:
+            A = oracle.FmtHost(fmt, rp, col, val, library_format_arrays(vh, fmt, rp, col, val))
+            run = oracle.cg_pipelined_fmt if kind == 0 else oracle.bicgstab_pipelined_fmt
+            x, it, err = run(A, b, tol, mi)
+        assert it == g[f"{tag}_iters"] and err == g[f"{tag}_error"] and np.array_equal(x, g[f"{tag}_x"]), tag
+
+
 needs_ref = pytest.mark.skipif(not (pathlib.Path(__file__).resolve().parents[1] / "oracle/_ref/libvclref.so").exists(),
                                reason="reference build (oracle/_ref) not present")
 
@@ -217,3 +287,24 @@ def test_blocked_reduction_orders(oracle, vh):
     lo, hi = oracle.iteration_envelope(lambda: oracle.bicgstab_classic(A, g["b"], 1e-10, 3000, diag=d,
                                                                         max_before_restart=20))
     assert lo <= int(g["jbicgstab_iters"]) <= hi
+
+
+@needs_ref
+@pytest.mark.parametrize("tag,fmt", FORMATS)
+def test_formats_equal_reference_build_on_fresh_inputs(oracle, vh, tag, fmt):
+    rp, col, val = vh.convection_diffusion(11, 6, 5, beta=(0.7, 0.1, 0.4))
+    n = rp.size - 1
+    x, y0 = vh.random_uniform(41, n), vh.random_uniform(42, n)
+    ref_arrays = oracle.ref_format_arrays(fmt, rp, col, val)
+    mine = library_format_arrays(vh, fmt, rp, col, val)
+    for k in ref_arrays:
+        assert np.array_equal(np.asarray(ref_arrays[k]), np.asarray(mine[k])), k
+    A = oracle.FmtHost(fmt, rp, col, val, mine)
+    y = y0.copy()
+    oracle.ref().ref_fmt_spmv_f64(fmt, n, oracle.p(oracle.cu32(rp)), oracle.p(oracle.cu32(col)),
+                                  oracle.p(oracle.c64(val)), oracle.p(x), 1.5, oracle.p(y), -0.5)
+    assert np.array_equal(oracle.fmt_spmv(A, x, 1.5, y0, -0.5), y)
+    b = vh.random_uniform(43, n)
+    xo, io, eo = oracle.bicgstab_pipelined_fmt(A, b, 1e-10, 500)
+    xr, ir, er = oracle.ref_fmt_solve(fmt, 1, rp, col, val, b, 1e-10, 500)
+    assert (io, eo) == (ir, er) and np.array_equal(xo, xr)

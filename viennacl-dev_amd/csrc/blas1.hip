@@ -137,6 +137,203 @@ __global__ __launch_bounds__(kBlock) void convert_kernel(uint32_t n, const S* __
 
 static uint32_t vec_grid(uint32_t n) { return chunk_grid((n + 4 * kBlock - 1) / (4 * kBlock), vec_gmax()); }
 
+// sum |x_i| (norm_1_impl, host_based/vector_operations.hpp:708-733)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void asum_partial_kernel(uint32_t n, const T* __restrict__ x, uint32_t xs,
+                                                              uint32_t xi, T* __restrict__ part)
+{
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range(n, lw, gridDim.x, c0, c1);
+  T acc[1] = {0};
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) acc[0] += fabs(at(x, i, xs, xi));
+  block_store_partials<1, T>(acc, part, 0, lw);
+}
+
+// (value, index) of the largest |x_i|, the smallest index among equal values: the first maximum, as
+// the reference's strictly-greater scan finds it (index_norm_inf, host_based/vector_operations.hpp:
+// 891-950).  NaN entries never win.
+template <typename T>
+__device__ __forceinline__ void better(T& v, uint32_t& i, T v2, uint32_t i2)
+{
+  if (v2 > v || (v2 == v && i2 < i)) {
+    v = v2;
+    i = i2;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void block_argmax(T& v, uint32_t& i)
+{
+  __shared__ T sv[kWavesPerBlock];
+  __shared__ uint32_t si[kWavesPerBlock];
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const T v2 = __shfl_xor(v, o, kWave);
+    const uint32_t i2 = __shfl_xor(i, o, kWave);
+    better(v, i, v2, i2);
+  }
+  const int wave = threadIdx.x / kWave;
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    sv[wave] = v;
+    si[wave] = i;
+  }
+  __syncthreads();
+  v = sv[0];
+  i = si[0];
+#pragma unroll
+  for (int w = 1; w < kWavesPerBlock; ++w) better(v, i, sv[w], si[w]);
+  __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void iamax_partial_kernel(uint32_t n, const T* __restrict__ x, uint32_t xs,
+                                                               uint32_t xi, T* __restrict__ pv,
+                                                               uint32_t* __restrict__ pi)
+{
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range(n, lw, gridDim.x, c0, c1);
+  T v = 0;
+  uint32_t idx = 0xFFFFFFFFu;
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    const T a = fabs(at(x, i, xs, xi));
+    if (a > v) {
+      v = a;
+      idx = i;
+    }
+  }
+  block_argmax(v, idx);
+  if (threadIdx.x == 0) {
+    pv[lw] = v;
+    pi[lw] = idx;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void iamax_final_kernel(const T* __restrict__ pv, const uint32_t* __restrict__ pi,
+                                                             uint32_t count, uint32_t* __restrict__ out)
+{
+  T v = 0;
+  uint32_t idx = 0xFFFFFFFFu;
+  for (uint32_t k = threadIdx.x; k < count; k += kBlock) better(v, idx, pv[k], pi[k]);
+  block_argmax(v, idx);
+  if (threadIdx.x == 0) *out = idx;
+}
+
+// x = alpha x (av with vec1 = vec2, host_based/vector_operations.hpp:88-123)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void scal_kernel(uint32_t n, T* __restrict__ x, uint32_t xs, uint32_t xi, T a)
+{
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range(n, lw, gridDim.x, c0, c1);
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    const size_t k = static_cast<size_t>(i) * xi + xs;
+    x[k] = x[k] * a;
+  }
+}
+
+// y = x (kind 0), swap (kind 1, vector_swap :302-327), plane rotation x' = c x + s y, y' = c y - s x
+// (kind 2, plane_rotation :1080-1110)
+template <typename T, int Kind>
+__global__ __launch_bounds__(kBlock) void pair_kernel(uint32_t n, T* __restrict__ x, uint32_t xs, uint32_t xi,
+                                                      T* __restrict__ y, uint32_t ys, uint32_t yi, T c, T s)
+{
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range(n, lw, gridDim.x, c0, c1);
+  for (uint32_t i = c0 + threadIdx.x; i < c1; i += kBlock) {
+    const size_t kx = static_cast<size_t>(i) * xi + xs;
+    const size_t ky = static_cast<size_t>(i) * yi + ys;
+    const T a = x[kx];
+    if constexpr (Kind == 0) {
+      y[ky] = a;
+    } else if constexpr (Kind == 1) {
+      x[kx] = y[ky];
+      y[ky] = a;
+    } else {
+      const T b = y[ky];
+      x[kx] = c * a + s * b;
+      y[ky] = c * b - s * a;
+    }
+  }
+}
+
+template <typename T>
+static int asum_to_host(uint32_t n, const T* x, uint32_t xs, uint32_t xi, T* result, void* stream)
+{
+  if (!result) return invalid("result is NULL");
+  *result = 0;
+  if (n == 0) return VCLHIP_SUCCESS;
+  if (!x || xi == 0) return invalid("null vector or zero stride");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t grid = vec_grid(n);
+  void* ws = nullptr;
+  VCLHIP_HIP(hipMallocAsync(&ws, (grid + 1) * sizeof(T), s));
+  T* part = static_cast<T*>(ws);
+  hipLaunchKernelGGL(asum_partial_kernel<T>, dim3(grid), dim3(kBlock), 0, s, n, x, xs, xi, part);
+  int st = check_launch("asum_partial_kernel");
+  if (st == 0) {
+    hipLaunchKernelGGL(sum_partials_kernel<T>, dim3(1), dim3(kBlock), 0, s, part, grid, part + grid);
+    st = check_launch("sum_partials_kernel");
+  }
+  if (st == 0) st = hip_status(hipMemcpyAsync(result, part + grid, sizeof(T), hipMemcpyDeviceToHost, s), "asum D2H");
+  int st2 = hip_status(hipFreeAsync(ws, s), "hipFreeAsync");
+  int st3 = hip_status(hipStreamSynchronize(s), "asum sync");
+  return st ? st : (st2 ? st2 : st3);
+}
+
+template <typename T>
+static int iamax_to_host(uint32_t n, const T* x, uint32_t xs, uint32_t xi, uint32_t* index, void* stream)
+{
+  if (!index) return invalid("index is NULL");
+  *index = xs;
+  if (n == 0) return VCLHIP_SUCCESS;
+  if (!x || xi == 0) return invalid("null vector or zero stride");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t grid = vec_grid(n);
+  void* ws = nullptr;
+  VCLHIP_HIP(hipMallocAsync(&ws, grid * (sizeof(T) + sizeof(uint32_t)) + 64, s));
+  T* pv = static_cast<T*>(ws);
+  uint32_t* pi = reinterpret_cast<uint32_t*>(pv + grid);
+  uint32_t* out = pi + grid;
+  hipLaunchKernelGGL(iamax_partial_kernel<T>, dim3(grid), dim3(kBlock), 0, s, n, x, xs, xi, pv, pi);
+  int st = check_launch("iamax_partial_kernel");
+  if (st == 0) {
+    hipLaunchKernelGGL(iamax_final_kernel<T>, dim3(1), dim3(kBlock), 0, s, pv, pi, grid, out);
+    st = check_launch("iamax_final_kernel");
+  }
+  uint32_t r = 0xFFFFFFFFu;
+  if (st == 0) st = hip_status(hipMemcpyAsync(&r, out, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "iamax D2H");
+  int st2 = hip_status(hipFreeAsync(ws, s), "hipFreeAsync");
+  int st3 = hip_status(hipStreamSynchronize(s), "iamax sync");
+  if (st || st2 || st3) return st ? st : (st2 ? st2 : st3);
+  *index = (r == 0xFFFFFFFFu) ? xs : r;  // no entry above 0: the reference returns the start offset
+  return VCLHIP_SUCCESS;
+}
+
+template <typename T, int Kind>
+static int pair_op(uint32_t n, T* x, uint32_t xs, uint32_t xi, T* y, uint32_t ys, uint32_t yi, T c, T sn,
+                   void* stream)
+{
+  if (n == 0) return VCLHIP_SUCCESS;
+  if (!x || !y || xi == 0 || yi == 0) return invalid("null vector or zero stride");
+  hipLaunchKernelGGL((pair_kernel<T, Kind>), dim3(vec_grid(n)), dim3(kBlock), 0, static_cast<hipStream_t>(stream), n,
+                     x, xs, xi, y, ys, yi, c, sn);
+  return check_launch("pair_kernel");
+}
+
+template <typename T>
+static int scal_op(uint32_t n, T* x, uint32_t xs, uint32_t xi, T a, void* stream)
+{
+  if (n == 0) return VCLHIP_SUCCESS;
+  if (!x || xi == 0) return invalid("null vector or zero stride");
+  hipLaunchKernelGGL(scal_kernel<T>, dim3(vec_grid(n)), dim3(kBlock), 0, static_cast<hipStream_t>(stream), n, x, xs,
+                     xi, a);
+  return check_launch("scal_kernel");
+}
+
 // Sum of x*y over n entries into *result_host, through a stream-ordered workspace.
 template <typename T, typename A>
 static int dot_to_host(uint32_t n, const T* x, uint32_t xs, uint32_t xi, const T* y, uint32_t ys, uint32_t yi,
@@ -236,6 +433,16 @@ int vclhip_avbv_f64(uint32_t n, double* v1, uint32_t s1, uint32_t i1, const doub
   return check_launch("avbv_kernel");
 }
 
+int vclhip_avbv_f32(uint32_t n, float* v1, uint32_t s1, uint32_t i1, const float* v2, uint32_t s2, uint32_t i2,
+                    float alpha, const float* v3, uint32_t s3, uint32_t i3, float beta, void* stream)
+{
+  if (n == 0) return VCLHIP_SUCCESS;
+  if (!v1 || !v2 || !v3 || !i1 || !i2 || !i3) return invalid("null vector or zero stride");
+  hipLaunchKernelGGL(avbv_kernel<float>, dim3(vec_grid(n)), dim3(kBlock), 0, static_cast<hipStream_t>(stream), n, v1,
+                     s1, i1, v2, s2, i2, alpha, v3, s3, i3, beta);
+  return check_launch("avbv_kernel");
+}
+
 int vclhip_fill_f64(uint32_t n, double* v, uint32_t s, uint32_t inc, double a, void* stream)
 {
   if (n == 0) return VCLHIP_SUCCESS;
@@ -288,6 +495,71 @@ int vclhip_convert_f32_to_f64(uint32_t n, const float* src, double* dst, void* s
   VCLHIP_LAUNCH_W(aligned16({src, dst}), (convert_kernel<float, double, 4>), (convert_kernel<float, double, 1>),
                   vec_grid(n), static_cast<hipStream_t>(stream), n, src, dst);
   return check_launch("convert_kernel");
+}
+
+int vclhip_asum_f64(uint32_t n, const double* x, uint32_t xs, uint32_t xi, double* result, void* stream)
+{
+  return asum_to_host<double>(n, x, xs, xi, result, stream);
+}
+
+int vclhip_asum_f32(uint32_t n, const float* x, uint32_t xs, uint32_t xi, float* result, void* stream)
+{
+  return asum_to_host<float>(n, x, xs, xi, result, stream);
+}
+
+int vclhip_iamax_f64(uint32_t n, const double* x, uint32_t xs, uint32_t xi, uint32_t* index, void* stream)
+{
+  return iamax_to_host<double>(n, x, xs, xi, index, stream);
+}
+
+int vclhip_iamax_f32(uint32_t n, const float* x, uint32_t xs, uint32_t xi, uint32_t* index, void* stream)
+{
+  return iamax_to_host<float>(n, x, xs, xi, index, stream);
+}
+
+int vclhip_scal_f64(uint32_t n, double* x, uint32_t xs, uint32_t xi, double alpha, void* stream)
+{
+  return scal_op<double>(n, x, xs, xi, alpha, stream);
+}
+
+int vclhip_scal_f32(uint32_t n, float* x, uint32_t xs, uint32_t xi, float alpha, void* stream)
+{
+  return scal_op<float>(n, x, xs, xi, alpha, stream);
+}
+
+int vclhip_copy_f64(uint32_t n, const double* x, uint32_t xs, uint32_t xi, double* y, uint32_t ys, uint32_t yi,
+                    void* stream)
+{
+  return pair_op<double, 0>(n, const_cast<double*>(x), xs, xi, y, ys, yi, 0.0, 0.0, stream);
+}
+
+int vclhip_copy_f32(uint32_t n, const float* x, uint32_t xs, uint32_t xi, float* y, uint32_t ys, uint32_t yi,
+                    void* stream)
+{
+  return pair_op<float, 0>(n, const_cast<float*>(x), xs, xi, y, ys, yi, 0.0f, 0.0f, stream);
+}
+
+int vclhip_swap_f64(uint32_t n, double* x, uint32_t xs, uint32_t xi, double* y, uint32_t ys, uint32_t yi,
+                    void* stream)
+{
+  return pair_op<double, 1>(n, x, xs, xi, y, ys, yi, 0.0, 0.0, stream);
+}
+
+int vclhip_swap_f32(uint32_t n, float* x, uint32_t xs, uint32_t xi, float* y, uint32_t ys, uint32_t yi, void* stream)
+{
+  return pair_op<float, 1>(n, x, xs, xi, y, ys, yi, 0.0f, 0.0f, stream);
+}
+
+int vclhip_rot_f64(uint32_t n, double* x, uint32_t xs, uint32_t xi, double* y, uint32_t ys, uint32_t yi, double c,
+                   double s, void* stream)
+{
+  return pair_op<double, 2>(n, x, xs, xi, y, ys, yi, c, s, stream);
+}
+
+int vclhip_rot_f32(uint32_t n, float* x, uint32_t xs, uint32_t xi, float* y, uint32_t ys, uint32_t yi, float c,
+                   float s, void* stream)
+{
+  return pair_op<float, 2>(n, x, xs, xi, y, ys, yi, c, s, stream);
 }
 
 }  // extern "C"

@@ -724,6 +724,18 @@ int vclhip_gmres_solve_sell_f32(const vclhip_sell* A, const float* b, float* x, 
   return gmres_solve<float>(A, b, x, P, R, stream);
 }
 
+int vclhip_gmres_solve_f64(const vclhip_matrix* A, const double* b, double* x, const vclhip_solver_params* P,
+                           vclhip_solver_result* R, void* stream)
+{
+  return gmres_solve<double>(A, b, x, P, R, stream);
+}
+
+int vclhip_gmres_solve_f32(const vclhip_matrix* A, const float* b, float* x, const vclhip_solver_params* P,
+                           vclhip_solver_result* R, void* stream)
+{
+  return gmres_solve<float>(A, b, x, P, R, stream);
+}
+
 int vclhip_pipelined_gmres_normalize_vk_f64(uint32_t n, double* v_k, const double* residual, double* R,
                                             size_t offset_in_R, const double* inner_prod_buffer,
                                             double* r_dot_vk_buffer, uint32_t chunk, size_t chunk_offset,

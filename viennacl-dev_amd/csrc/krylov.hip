@@ -626,18 +626,74 @@ __global__ __launch_bounds__(kBlock) void sell_diag_kernel(SellDev<T> A, T* __re
   d[row] = v;
 }
 
+// COO (row_info, host_based/sparse_matrix_operations.hpp:1156-1205): the row's last entry with
+// column == row, whatever its value; 0 when there is none.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void coo_diag_kernel(CsrDev<T> A, T* __restrict__ d)
+{
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= A.n_rows) return;
+  T v = 0;
+  for (uint32_t e = A.row_ptr[row]; e < A.row_ptr[row + 1]; ++e)
+    if (A.col[2 * static_cast<size_t>(e)] == row) v = A.val[e];
+  d[row] = v;
+}
+
+// ELL / HYB: the first nonzero entry with column == row (ELL part, then the CSR part).
+template <typename T, bool Hyb>
+__global__ __launch_bounds__(kBlock) void ell_diag_kernel(EllDev<T> A, T* __restrict__ d)
+{
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= A.n_rows) return;
+  T v = 0;
+  bool found = false;
+  for (uint32_t k = 0; k < A.width && !found; ++k) {
+    const size_t off = static_cast<size_t>(k) * A.ld + row;
+    if (A.coords[off] == row && (A.val[off] < 0 || A.val[off] > 0)) {
+      v = A.val[off];
+      found = true;
+    }
+  }
+  if constexpr (Hyb)
+    for (uint32_t e = A.csr_rows[row]; e < A.csr_rows[row + 1] && !found; ++e)
+      if (A.csr_cols[e] == row && (A.csr_val[e] < 0 || A.csr_val[e] > 0)) {
+        v = A.csr_val[e];
+        found = true;
+      }
+  d[row] = v;
+}
+
 inline int csr_row_info(const vclhip_csr* A, double* d, hipStream_t s) { return vclhip_csr_row_info_f64(A, d, 3, s); }
 inline int csr_row_info(const vclhip_csr* A, float* d, hipStream_t s) { return vclhip_csr_row_info_f32(A, d, 3, s); }
 
+// d = diag(A) for any format (0 where a row has no nonzero diagonal entry).
 template <typename T>
-static int jacobi_diag(const DevOp<T>& op, const vclhip_csr* Acsr, T* d, uint32_t* zero_count, hipStream_t s)
+static int matrix_diag(const DevOp<T>& op, T* d, hipStream_t s)
 {
-  if (!op.sell) {
-    VCLHIP_TRY(csr_row_info(Acsr, d, s));
-  } else {
-    hipLaunchKernelGGL(sell_diag_kernel<T>, dim3((op.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, op.sm, d);
-    VCLHIP_TRY(check_launch("sell_diag_kernel"));
+  const dim3 grid((op.n + kBlock - 1) / kBlock);
+  switch (op.fmt) {
+    case OpFormat::Csr:
+      return csr_row_info(op.csr_desc, d, s);
+    case OpFormat::Sell:
+      hipLaunchKernelGGL(sell_diag_kernel<T>, grid, dim3(kBlock), 0, s, op.sm, d);
+      break;
+    case OpFormat::Coo:
+      hipLaunchKernelGGL(coo_diag_kernel<T>, grid, dim3(kBlock), 0, s, op.csr, d);
+      break;
+    case OpFormat::Ell:
+      hipLaunchKernelGGL((ell_diag_kernel<T, false>), grid, dim3(kBlock), 0, s, op.ell, d);
+      break;
+    case OpFormat::Hyb:
+      hipLaunchKernelGGL((ell_diag_kernel<T, true>), grid, dim3(kBlock), 0, s, op.ell, d);
+      break;
   }
+  return check_launch("diagonal");
+}
+
+template <typename T>
+static int jacobi_diag(const DevOp<T>& op, T* d, uint32_t* zero_count, hipStream_t s)
+{
+  VCLHIP_TRY(matrix_diag(op, d, s));
   VCLHIP_HIP(hipMemsetAsync(zero_count, 0, sizeof(uint32_t), s));
   hipLaunchKernelGGL(count_zero_kernel<T>, dim3((op.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, op.n, d, zero_count);
   VCLHIP_TRY(check_launch("count_zero_kernel"));
@@ -839,7 +895,7 @@ static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_so
     uint32_t* zc;
     VCLHIP_TRY(ws.alloc(&d, n));
     VCLHIP_TRY(ws.alloc(&zc, 1));
-    VCLHIP_TRY(jacobi_diag(op, reinterpret_cast<const vclhip_csr*>(Ain), d, zc, s));
+    VCLHIP_TRY(jacobi_diag(op, d, zc, s));
   }
   KState h;
   init_state(h, P, hist);
@@ -925,7 +981,7 @@ static int bicgstab_solve(const M* Ain, const T* b, T* x, int precond, const vcl
     uint32_t* zc;
     VCLHIP_TRY(ws.alloc(&d, n));
     VCLHIP_TRY(ws.alloc(&zc, 1));
-    VCLHIP_TRY(jacobi_diag(op, reinterpret_cast<const vclhip_csr*>(Ain), d, zc, s));
+    VCLHIP_TRY(jacobi_diag(op, d, zc, s));
   }
   KState h;
   init_state(h, P, hist);
@@ -1657,7 +1713,7 @@ static int mixed_cg_solve(const M* A, const float* values32, const double* b, do
     uint32_t* zc;
     VCLHIP_TRY(ws.alloc(&d32, n));
     VCLHIP_TRY(ws.alloc(&zc, 1));
-    VCLHIP_TRY(jacobi_diag(op32, reinterpret_cast<const vclhip_csr*>(&A32), d32, zc, s));  // SELL: op32.sm
+    VCLHIP_TRY(jacobi_diag(op32, d32, zc, s));
   }
   KState h;
   init_state(h, P, hist);
@@ -1934,6 +1990,59 @@ int vclhip_bicgstab_solve_sell_f32(const vclhip_sell* A, const float* b, float* 
                                    const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
 {
   return bicgstab_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_cg_solve_f64(const vclhip_matrix* A, const double* b, double* x, int precond,
+                        const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return cg_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_cg_solve_f32(const vclhip_matrix* A, const float* b, float* x, int precond,
+                        const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return cg_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_bicgstab_solve_f64(const vclhip_matrix* A, const double* b, double* x, int precond,
+                              const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return bicgstab_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_bicgstab_solve_f32(const vclhip_matrix* A, const float* b, float* x, int precond,
+                              const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  return bicgstab_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_pipelined_cg_prod_f64(const vclhip_matrix* A, const double* p, double* Ap, double* buf, uint32_t chunk,
+                                 void* stream)
+{
+  return pipelined_prod(A, p, Ap, nullptr, buf, chunk, 0u, stream);
+}
+
+int vclhip_pipelined_bicgstab_prod_f64(const vclhip_matrix* A, const double* p, double* Ap, const double* r0,
+                                       double* buf, uint32_t chunk, uint32_t offset, void* stream)
+{
+  if (!r0) return invalid("null argument");
+  return pipelined_prod(A, p, Ap, r0, buf, chunk, offset, stream);
+}
+
+int vclhip_matrix_diag_f64(const vclhip_matrix* A, double* d, void* stream)
+{
+  if (!d) return invalid("null argument");
+  DevOp<double> op;
+  VCLHIP_TRY(make_op(A, op));
+  return matrix_diag(op, d, static_cast<hipStream_t>(stream));
+}
+
+int vclhip_matrix_diag_f32(const vclhip_matrix* A, float* d, void* stream)
+{
+  if (!d) return invalid("null argument");
+  DevOp<float> op;
+  VCLHIP_TRY(make_op(A, op));
+  return matrix_diag(op, d, static_cast<hipStream_t>(stream));
 }
 
 int vclhip_mixed_cg_solve_csr(const vclhip_csr* A, const float* values32, const double* b, double* x, int precond,

@@ -13,11 +13,16 @@ namespace vclhip {
 
 // ------------------------------------------------------------------ matrix wrapper
 
+enum class OpFormat { Csr, Sell, Coo, Ell, Hyb };
+
 template <typename T>
 struct DevOp {
-  bool sell = false;
-  CsrDev<T> csr{};
+  OpFormat fmt = OpFormat::Csr;
+  bool sell = false;                   // fmt == Sell (the mixed solver's inner copy is SELL)
+  const vclhip_csr* csr_desc = nullptr;  // the caller's CSR descriptor (row_info)
+  CsrDev<T> csr{};                     // CSR, and COO through its row pointers (column stride 2)
   SellDev<T> sm{};
+  EllDev<T> ell{};                     // ELL and HYB
   uint32_t n = 0;
   uint32_t grid = 8;
 
@@ -31,24 +36,45 @@ struct DevOp {
                        part, kMaxPartials, stop, mask);
   }
 
+  template <class Epi, int CS>
+  void launch_csr(hipStream_t s, const T* x, const Epi& epi, double* part, const uint32_t* stop, uint32_t mask) const
+  {
+    if (nt)
+      hipLaunchKernelGGL((csr_kernel<T, true, Epi, true, CS>), dim3(grid), dim3(kBlock), 0, s, csr, x, 0u, 1u, epi,
+                         part, kMaxPartials, stop, mask);
+    else
+      hipLaunchKernelGGL((csr_kernel<T, true, Epi, false, CS>), dim3(grid), dim3(kBlock), 0, s, csr, x, 0u, 1u, epi,
+                         part, kMaxPartials, stop, mask);
+  }
+
   template <class Epi>
   void launch(hipStream_t s, const T* x, const Epi& epi, double* part, const uint32_t* stop, uint32_t mask) const
   {
-    if (sell) {
-      if (sm.dcol && nt)
-        launch_sell<Epi, true, true>(s, x, epi, part, stop, mask);
-      else if (sm.dcol)
-        launch_sell<Epi, true, false>(s, x, epi, part, stop, mask);
-      else if (nt)
-        launch_sell<Epi, false, true>(s, x, epi, part, stop, mask);
-      else
-        launch_sell<Epi, false, false>(s, x, epi, part, stop, mask);
-    } else if (nt)
-      hipLaunchKernelGGL((csr_kernel<T, true, Epi, true>), dim3(grid), dim3(kBlock), 0, s, csr, x, 0u, 1u, epi, part,
-                         kMaxPartials, stop, mask);
-    else
-      hipLaunchKernelGGL((csr_kernel<T, true, Epi>), dim3(grid), dim3(kBlock), 0, s, csr, x, 0u, 1u, epi, part,
-                         kMaxPartials, stop, mask);
+    switch (fmt) {
+      case OpFormat::Sell:
+        if (sm.dcol && nt)
+          launch_sell<Epi, true, true>(s, x, epi, part, stop, mask);
+        else if (sm.dcol)
+          launch_sell<Epi, true, false>(s, x, epi, part, stop, mask);
+        else if (nt)
+          launch_sell<Epi, false, true>(s, x, epi, part, stop, mask);
+        else
+          launch_sell<Epi, false, false>(s, x, epi, part, stop, mask);
+        break;
+      case OpFormat::Coo:
+        launch_csr<Epi, 2>(s, x, epi, part, stop, mask);
+        break;
+      case OpFormat::Ell:
+        hipLaunchKernelGGL((ell_kernel<T, true, Epi, false>), dim3(grid), dim3(kBlock), 0, s, ell, x, 0u, 1u, epi,
+                           part, kMaxPartials, stop, mask);
+        break;
+      case OpFormat::Hyb:
+        hipLaunchKernelGGL((ell_kernel<T, true, Epi, true>), dim3(grid), dim3(kBlock), 0, s, ell, x, 0u, 1u, epi,
+                           part, kMaxPartials, stop, mask);
+        break;
+      default:
+        launch_csr<Epi, 1>(s, x, epi, part, stop, mask);
+    }
   }
 };
 
@@ -58,7 +84,9 @@ static int make_op(const vclhip_csr* A, DevOp<T>& op)
   if (!A || !A->row_ptr || !A->blocks || !A->col_idx || !A->values) return invalid("CSR matrix arrays");
   if (A->n_rows == 0 || A->n_rows != A->n_cols) return invalid("solvers need a nonempty square matrix");
   if (A->n_rows >= kPlanRowLimit || A->num_blocks == 0 || A->num_blocks > A->n_rows) return invalid("CSR plan");
+  op.fmt = OpFormat::Csr;
   op.sell = false;
+  op.csr_desc = A;
   op.csr.row_ptr = A->row_ptr;
   op.csr.col = A->col_idx;
   op.csr.val = static_cast<const T*>(A->values);
@@ -78,6 +106,7 @@ static int make_op(const vclhip_sell* A, DevOp<T>& op)
     return invalid("SELL matrix arrays");
   if (A->n_rows == 0 || A->n_rows != A->n_cols || A->C == 0) return invalid("solvers need a nonempty square matrix");
   if (A->num_slices != (A->n_rows + A->C - 1) / A->C) return invalid("num_slices");
+  op.fmt = OpFormat::Sell;
   op.sell = true;
   op.sm.cpb = A->columns_per_block;
   op.sm.col = A->column_indices;
@@ -88,6 +117,79 @@ static int make_op(const vclhip_sell* A, DevOp<T>& op)
   op.n = A->n_rows;
   op.grid = chunk_grid((A->n_rows + kBlock - 1) / kBlock, sell_gmax());
   return VCLHIP_SUCCESS;
+}
+
+template <typename T>
+static int make_op(const vclhip_coo* A, DevOp<T>& op)
+{
+  if (!A || !A->row_ptr || !A->blocks || (A->nnz && (!A->coords || !A->elements))) return invalid("COO matrix arrays");
+  if (A->n_rows == 0 || A->n_rows != A->n_cols) return invalid("solvers need a nonempty square matrix");
+  if (A->n_rows >= kPlanRowLimit || A->num_blocks == 0 || A->num_blocks > A->n_rows) return invalid("COO plan");
+  op.fmt = OpFormat::Coo;
+  op.sell = false;
+  op.csr.row_ptr = A->row_ptr;
+  op.csr.col = A->coords ? A->coords + 1 : nullptr;
+  op.csr.val = static_cast<const T*>(A->elements);
+  op.csr.blocks = A->blocks;
+  op.csr.num_blocks = A->num_blocks;
+  op.csr.n_rows = A->n_rows;
+  op.n = A->n_rows;
+  op.grid = chunk_grid(A->num_blocks, csr_gmax());
+  op.nt = stream_loads(static_cast<size_t>(A->nnz) * (sizeof(T) + 8));
+  return VCLHIP_SUCCESS;
+}
+
+template <typename T>
+static int make_ell_op(uint32_t n_rows, uint32_t n_cols, uint32_t ld, uint32_t width, const uint32_t* coords,
+                       const void* elements, DevOp<T>& op)
+{
+  if (n_rows == 0 || n_rows != n_cols) return invalid("solvers need a nonempty square matrix");
+  if (ld < n_rows || (width && (!coords || !elements))) return invalid("ELL arrays");
+  op.sell = false;
+  op.ell.coords = coords;
+  op.ell.val = static_cast<const T*>(elements);
+  op.ell.n_rows = n_rows;
+  op.ell.ld = ld;
+  op.ell.width = width;
+  op.n = n_rows;
+  op.grid = chunk_grid((n_rows + kBlock - 1) / kBlock, sell_gmax());
+  return VCLHIP_SUCCESS;
+}
+
+template <typename T>
+static int make_op(const vclhip_ell* A, DevOp<T>& op)
+{
+  if (!A) return invalid("ELL matrix is NULL");
+  VCLHIP_TRY(make_ell_op(A->n_rows, A->n_cols, A->internal_size1, A->internal_maxnnz, A->coords, A->elements, op));
+  op.fmt = OpFormat::Ell;
+  return VCLHIP_SUCCESS;
+}
+
+template <typename T>
+static int make_op(const vclhip_hyb* A, DevOp<T>& op)
+{
+  if (!A || !A->csr_rows || !A->csr_cols || !A->csr_elements) return invalid("HYB matrix arrays");
+  VCLHIP_TRY(make_ell_op(A->n_rows, A->n_cols, A->internal_size1, A->internal_ellnnz, A->ell_coords,
+                         A->ell_elements, op));
+  op.fmt = OpFormat::Hyb;
+  op.ell.csr_rows = A->csr_rows;
+  op.ell.csr_cols = A->csr_cols;
+  op.ell.csr_val = static_cast<const T*>(A->csr_elements);
+  return VCLHIP_SUCCESS;
+}
+
+template <typename T>
+static int make_op(const vclhip_matrix* A, DevOp<T>& op)
+{
+  if (!A || !A->desc) return invalid("matrix is NULL");
+  switch (A->format) {
+    case VCLHIP_FORMAT_CSR: return make_op(static_cast<const vclhip_csr*>(A->desc), op);
+    case VCLHIP_FORMAT_SELL: return make_op(static_cast<const vclhip_sell*>(A->desc), op);
+    case VCLHIP_FORMAT_COO: return make_op(static_cast<const vclhip_coo*>(A->desc), op);
+    case VCLHIP_FORMAT_ELL: return make_op(static_cast<const vclhip_ell*>(A->desc), op);
+    case VCLHIP_FORMAT_HYB: return make_op(static_cast<const vclhip_hyb*>(A->desc), op);
+    default: return invalid("unknown matrix format");
+  }
 }
 
 inline uint32_t vec_grid_n(uint32_t n) { return chunk_grid((n + 4 * kBlock - 1) / (4 * kBlock), vec_gmax()); }

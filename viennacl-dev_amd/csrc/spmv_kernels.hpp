@@ -78,7 +78,9 @@ struct EpiAxpby {
 };
 
 // CSR SpMV over the row-block plan.  Each logical workgroup walks a contiguous chunk of blocks.
-template <typename T, bool Contig, class Epi, bool NT = false>
+// CS is the distance between consecutive column indices: 1 for CSR, 2 for COO, whose (row, col)
+// pairs are read through col = coords + 1 with the row pointers of its row order.
+template <typename T, bool Contig, class Epi, bool NT = false, int CS = 1>
 __global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __restrict__ x, uint32_t xs, uint32_t xi,
                                                      Epi epi, typename Epi::acc_t* __restrict__ part,
                                                      uint32_t pstride, const uint32_t* __restrict__ stop,
@@ -111,7 +113,8 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __res
     if (mode == kModeLongRow) {
       // one row longer than the staging buffer: strided partial sums, then a workgroup reduction
       T s = 0;
-      for (uint32_t e = e0 + tid; e < e1; e += kBlock) s += A.val[e] * load_x<T, Contig>(x, A.col[e], xs, xi);
+      for (uint32_t e = e0 + tid; e < e1; e += kBlock)
+        s += A.val[e] * load_x<T, Contig>(x, A.col[static_cast<size_t>(e) * CS], xs, xi);
       s = block_sum(s);
       if (tid == 0) epi(r0, s, acc);
       continue;
@@ -132,7 +135,7 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __res
 #pragma unroll
       for (int k = 0; k < U; ++k) {
         const uint32_t e = e0 + min(tid + k * kBlock, cnt - 1);
-        c[k] = ld_stream<NT>(A.col + e);
+        c[k] = ld_stream<NT>(A.col + static_cast<size_t>(e) * CS);
         v[k] = ld_stream<NT>(A.val + e);
       }
       T xv[U];
@@ -239,6 +242,64 @@ __global__ __launch_bounds__(kBlock) void sell_kernel(SellDev<T> A, const T* __r
           const T xv = load_x<T, Contig>(x, ld_stream<NT>(cv + off), xs, xi);
           sum += (v > 0 || v < 0) ? xv * v : T(0);  // host_based/sparse_matrix_operations.hpp:1831-1833
         }
+      }
+      epi(row, sum, acc);
+    }
+  }
+
+  if constexpr (Epi::K > 0) block_store_partials<Epi::K, acc_t>(acc, part, pstride, lw);
+}
+
+// ELL (one slice of height ld, column-major) and HYB (ELL part plus a CSR part per row).
+template <typename T>
+struct EllDev {
+  const uint32_t* __restrict__ coords;
+  const T* __restrict__ val;
+  uint32_t n_rows;
+  uint32_t ld;     // internal_size1
+  uint32_t width;  // internal_maxnnz / internal_ellnnz
+  const uint32_t* __restrict__ csr_rows = nullptr;  // HYB
+  const uint32_t* __restrict__ csr_cols = nullptr;
+  const T* __restrict__ csr_val = nullptr;
+};
+
+// One lane per row; the lanes of a wavefront read 64 consecutive entries of each ELL column.
+// Zero ELL entries are skipped (their x entry is not read), as the reference's host loop does
+// (host_based/sparse_matrix_operations.hpp:1516-1524); the HYB CSR part follows in entry order
+// (:1906-1912).
+template <typename T, bool Contig, class Epi, bool Hyb>
+__global__ __launch_bounds__(kBlock) void ell_kernel(EllDev<T> A, const T* __restrict__ x, uint32_t xs, uint32_t xi,
+                                                     Epi epi, typename Epi::acc_t* __restrict__ part,
+                                                     uint32_t pstride, const uint32_t* __restrict__ stop,
+                                                     uint32_t stop_mask)
+{
+  using acc_t = typename Epi::acc_t;
+  constexpr int KA = Epi::K > 0 ? Epi::K : 1;
+  if (stop && (*stop & stop_mask)) return;
+
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lw = xcd_logical_block();
+  const uint32_t tiles = (A.n_rows + kBlock - 1) / kBlock;
+  uint32_t t0, t1;
+  chunk_range(tiles, lw, gridDim.x, t0, t1);
+
+  acc_t acc[KA];
+#pragma unroll
+  for (int q = 0; q < KA; ++q) acc[q] = 0;
+
+  for (uint32_t t = t0; t < t1; ++t) {
+    const uint32_t row = t * kBlock + tid;
+    if (row < A.n_rows) {
+      T sum = 0;
+#pragma unroll 4
+      for (uint32_t k = 0; k < A.width; ++k) {
+        const size_t off = static_cast<size_t>(k) * A.ld + row;
+        const T v = A.val[off];
+        if (v > 0 || v < 0) sum += load_x<T, Contig>(x, A.coords[off], xs, xi) * v;
+      }
+      if constexpr (Hyb) {
+        const uint32_t e1 = A.csr_rows[row + 1];
+        for (uint32_t e = A.csr_rows[row]; e < e1; ++e) sum += load_x<T, Contig>(x, A.csr_cols[e], xs, xi) * A.csr_val[e];
       }
       epi(row, sum, acc);
     }

@@ -346,4 +346,101 @@ int vclhip_sell_from_csr_f64(uint32_t n_rows, uint32_t C, const uint32_t* row_pt
   return VCLHIP_SUCCESS;
 }
 
+int vclhip_coo_plan(const uint32_t* coords, uint32_t nnz, uint32_t n_rows, uint32_t* row_ptr, uint32_t* blocks,
+                    uint32_t* num_blocks)
+{
+  if ((nnz && !coords) || !row_ptr || !blocks || !num_blocks || n_rows == 0) return vclhip::invalid("coo plan arguments");
+  if (n_rows >= vclhip::kPlanRowLimit) return vclhip::invalid("n_rows must be below 2^28");
+  std::fill(row_ptr, row_ptr + n_rows + 1, 0u);
+  uint32_t prev = 0;
+  for (uint32_t e = 0; e < nnz; ++e) {
+    const uint32_t r = coords[2 * static_cast<size_t>(e)];
+    if (r >= n_rows) return vclhip::invalid("COO row index out of range");
+    if (r < prev) return vclhip::invalid("COO entries must be ordered by row");
+    prev = r;
+    ++row_ptr[r + 1];
+  }
+  for (uint32_t r = 0; r < n_rows; ++r) row_ptr[r + 1] += row_ptr[r];
+  *num_blocks = vclhip::plan_csr_blocks(row_ptr, n_rows, blocks);
+  return VCLHIP_SUCCESS;
+}
+
+int vclhip_ell_from_csr_f64(uint32_t n_rows, uint32_t align, const uint32_t* row_ptr, const uint32_t* col,
+                            const double* val, uint32_t* internal_maxnnz, uint32_t* coords, double* elements)
+{
+  if (!row_ptr || !internal_maxnnz || align == 0 || n_rows == 0) return vclhip::invalid("ell arguments");
+  uint32_t m = 0;
+  for (uint32_t r = 0; r < n_rows; ++r) m = std::max(m, row_ptr[r + 1] - row_ptr[r]);
+  const uint64_t ld = (static_cast<uint64_t>(n_rows) + align - 1) / align * align;
+  const uint64_t width = (static_cast<uint64_t>(m) + align - 1) / align * align;
+  if (ld * width > 0xFFFFFFFFull || ld > 0xFFFFFFFFull) return vclhip::invalid("ELL storage exceeds 32-bit sizes");
+  *internal_maxnnz = static_cast<uint32_t>(width);
+  if (!coords || !elements) return VCLHIP_SUCCESS;
+  if (row_ptr[n_rows] && (!col || !val)) return vclhip::invalid("ell arguments");
+  std::fill(coords, coords + ld * width, 0u);
+  std::fill(elements, elements + ld * width, 0.0);
+  for (uint32_t r = 0; r < n_rows; ++r)
+    for (uint32_t i = row_ptr[r], k = 0; i < row_ptr[r + 1]; ++i, ++k) {
+      coords[ld * k + r] = col[i];
+      elements[ld * k + r] = val[i];
+    }
+  return VCLHIP_SUCCESS;
+}
+
+int vclhip_hyb_from_csr_f64(uint32_t n_rows, double csr_threshold, uint32_t align, const uint32_t* row_ptr,
+                            const uint32_t* col, const double* val, uint32_t* internal_ellnnz, uint32_t* csr_nnz,
+                            uint32_t* ell_coords, double* ell_elements, uint32_t* csr_rows, uint32_t* csr_cols,
+                            double* csr_elements)
+{
+  if (!row_ptr || !internal_ellnnz || !csr_nnz || align == 0 || n_rows == 0) return vclhip::invalid("hyb arguments");
+  // ELL width: the first row length whose cumulative row count reaches csr_threshold * n_rows
+  uint32_t m = 0;
+  for (uint32_t r = 0; r < n_rows; ++r) m = std::max(m, row_ptr[r + 1] - row_ptr[r]);
+  std::vector<uint64_t> hist(static_cast<size_t>(m) + 1, 0);
+  for (uint32_t r = 0; r < n_rows; ++r) ++hist[row_ptr[r + 1] - row_ptr[r]];
+  uint32_t width = m;
+  uint64_t sum = 0;
+  for (uint32_t len = 0; len <= m; ++len) {
+    sum += hist[len];
+    if (static_cast<double>(sum) >= csr_threshold * static_cast<double>(n_rows)) {
+      width = len;
+      break;
+    }
+  }
+  const uint64_t ld = (static_cast<uint64_t>(n_rows) + align - 1) / align * align;
+  const uint64_t iw = (static_cast<uint64_t>(width) + align - 1) / align * align;
+  if (ld * iw > 0xFFFFFFFFull || ld > 0xFFFFFFFFull) return vclhip::invalid("HYB storage exceeds 32-bit sizes");
+  uint64_t rest = 0;
+  for (uint32_t r = 0; r < n_rows; ++r) {
+    const uint32_t len = row_ptr[r + 1] - row_ptr[r];
+    if (len > width) rest += len - width;
+  }
+  *internal_ellnnz = static_cast<uint32_t>(iw);
+  *csr_nnz = static_cast<uint32_t>(rest ? rest : 1);  // an empty CSR part keeps one zero entry
+  if (!ell_coords || !ell_elements || !csr_rows || !csr_cols || !csr_elements) return VCLHIP_SUCCESS;
+  if (row_ptr[n_rows] && (!col || !val)) return vclhip::invalid("hyb arguments");
+  std::fill(ell_coords, ell_coords + ld * iw, 0u);
+  std::fill(ell_elements, ell_elements + ld * iw, 0.0);
+  uint32_t c = 0;
+  for (uint32_t r = 0; r < n_rows; ++r) {
+    csr_rows[r] = c;
+    for (uint32_t i = row_ptr[r], k = 0; i < row_ptr[r + 1]; ++i, ++k) {
+      if (k < width) {
+        ell_coords[ld * k + r] = col[i];
+        ell_elements[ld * k + r] = val[i];
+      } else {
+        csr_cols[c] = col[i];
+        csr_elements[c] = val[i];
+        ++c;
+      }
+    }
+  }
+  csr_rows[n_rows] = c;
+  if (c == 0) {
+    csr_cols[0] = 0;
+    csr_elements[0] = 0.0;
+  }
+  return VCLHIP_SUCCESS;
+}
+
 }  // extern "C"

@@ -9,10 +9,12 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import (Csr, Sell, SolverParams, SolverResult, MONITOR, VclHipError, call, lib,
-                   PRECOND_NONE, PRECOND_JACOBI, LIB_PATH, declared_functions)
+from ._abi import (Csr, Sell, Coo, Ell, Hyb, Matrix, SolverParams, SolverResult, MONITOR, VclHipError, call, lib,
+                   PRECOND_NONE, PRECOND_JACOBI, LIB_PATH, declared_functions,
+                   FORMAT_CSR, FORMAT_SELL, FORMAT_COO, FORMAT_ELL, FORMAT_HYB)
 
-__all__ = ["DeviceArray", "Stream", "CsrMatrix", "SellMatrix", "BatchPlan", "lib", "call", "VclHipError",
+__all__ = ["DeviceArray", "Stream", "CsrMatrix", "SellMatrix", "CooMatrix", "EllMatrix", "HybMatrix", "BatchPlan",
+           "lib", "call", "VclHipError",
            "device_count", "PRECOND_NONE", "PRECOND_JACOBI", "SolverParams", "SolverResult"]
 
 
@@ -193,6 +195,129 @@ class SellMatrix:
             a.free()
 
 
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+class CooMatrix:
+    """coordinate_matrix on the device: (row, col) pairs in row order, values, and the row pointers
+    and CSR-kernel plan of that order (vclhip_coo_plan)."""
+
+    format = FORMAT_COO
+
+    def __init__(self, row_ptr, col, val, dtype=np.float64, n_cols=None, stream=None):
+        rp = _u32(row_ptr)
+        n = rp.size - 1
+        self.n, self.nnz, self.dtype = n, int(rp[-1]), np.dtype(dtype)
+        rows = np.repeat(np.arange(n, dtype=np.uint32), np.diff(rp.astype(np.int64)))
+        coords = np.empty(2 * self.nnz, np.uint32)
+        coords[0::2], coords[1::2] = rows, _u32(col)
+        self.coords_host = coords
+        prp = np.zeros(n + 1, np.uint32)
+        blocks = np.zeros(n + 1, np.uint32)
+        nb = C.c_uint32()
+        call("vclhip_coo_plan", _ptr(coords), self.nnz, n, _ptr(prp), _ptr(blocks), C.byref(nb))
+        self.d_coords = DeviceArray.from_numpy(coords if coords.size else np.zeros(2, np.uint32), stream)
+        vv = np.ascontiguousarray(val, dtype=dtype)
+        self.d_el = DeviceArray.from_numpy(vv if vv.size else np.zeros(1, dtype), stream)
+        self.d_rp = DeviceArray.from_numpy(prp, stream)
+        self.d_blocks = DeviceArray.from_numpy(blocks[: nb.value + 1], stream)
+        self.desc = Coo(n, n if n_cols is None else n_cols, self.nnz, nb.value, self.d_coords.ptr, self.d_el.ptr,
+                        self.d_rp.ptr, self.d_blocks.ptr)
+
+    def spmv(self, x, y, alpha=1.0, beta=0.0, xs=0, xi=1, ys=0, yi=1, stream=None):
+        fn = "vclhip_coo_spmv_f64" if self.dtype == np.float64 else "vclhip_coo_spmv_f32"
+        call(fn, C.byref(self.desc), x.ptr, xs, xi, alpha, y.ptr, ys, yi, beta, stream)
+
+    def free(self):
+        for a in (self.d_coords, self.d_el, self.d_rp, self.d_blocks):
+            a.free()
+
+
+def ell_from_csr(row_ptr, col, val, align=1):
+    """Host CSR -> the ell_matrix arrays (ld, width, coords, elements)."""
+    rp, cl, vl = _u32(row_ptr), _u32(col), np.ascontiguousarray(val, dtype=np.float64)
+    n = rp.size - 1
+    w = C.c_uint32()
+    call("vclhip_ell_from_csr_f64", n, align, _ptr(rp), _ptr(cl), _ptr(vl), C.byref(w), None, None)
+    ld = (n + align - 1) // align * align
+    coords = np.zeros(max(ld * w.value, 1), np.uint32)
+    el = np.zeros(max(ld * w.value, 1))
+    call("vclhip_ell_from_csr_f64", n, align, _ptr(rp), _ptr(cl), _ptr(vl), C.byref(w), _ptr(coords), _ptr(el))
+    return ld, w.value, coords, el
+
+
+def hyb_from_csr(row_ptr, col, val, csr_threshold=0.8, align=1):
+    """Host CSR -> the hyb_matrix arrays (ld, width, ell coords, ell elements, csr rows, cols, elements)."""
+    rp, cl, vl = _u32(row_ptr), _u32(col), np.ascontiguousarray(val, dtype=np.float64)
+    n = rp.size - 1
+    w, cn = C.c_uint32(), C.c_uint32()
+    args = (n, csr_threshold, align, _ptr(rp), _ptr(cl), _ptr(vl), C.byref(w), C.byref(cn))
+    call("vclhip_hyb_from_csr_f64", *args, None, None, None, None, None)
+    ld = (n + align - 1) // align * align
+    ec, ee = np.zeros(max(ld * w.value, 1), np.uint32), np.zeros(max(ld * w.value, 1))
+    cr, cc, ce = np.zeros(n + 1, np.uint32), np.zeros(cn.value, np.uint32), np.zeros(cn.value)
+    call("vclhip_hyb_from_csr_f64", *args, _ptr(ec), _ptr(ee), _ptr(cr), _ptr(cc), _ptr(ce))
+    return ld, w.value, ec, ee, cr, cc, ce
+
+
+class EllMatrix:
+    """ell_matrix on the device (column-major ELL, internal_size1 rows per column)."""
+
+    format = FORMAT_ELL
+
+    def __init__(self, row_ptr, col, val, dtype=np.float64, align=1, stream=None):
+        n = len(row_ptr) - 1
+        ld, w, coords, el = ell_from_csr(row_ptr, col, val, align)
+        self.n, self.dtype, self.ld, self.width = n, np.dtype(dtype), ld, w
+        self.host = (coords, el)
+        self.d_coords = DeviceArray.from_numpy(coords, stream)
+        self.d_el = DeviceArray.from_numpy(el.astype(dtype), stream)
+        self.desc = Ell(n, n, ld, w, self.d_coords.ptr, self.d_el.ptr)
+
+    def spmv(self, x, y, alpha=1.0, beta=0.0, xs=0, xi=1, ys=0, yi=1, stream=None):
+        fn = "vclhip_ell_spmv_f64" if self.dtype == np.float64 else "vclhip_ell_spmv_f32"
+        call(fn, C.byref(self.desc), x.ptr, xs, xi, alpha, y.ptr, ys, yi, beta, stream)
+
+    def free(self):
+        for a in (self.d_coords, self.d_el):
+            a.free()
+
+
+class HybMatrix:
+    """hyb_matrix on the device: ELL part plus a CSR part for the entries beyond the ELL width."""
+
+    format = FORMAT_HYB
+
+    def __init__(self, row_ptr, col, val, dtype=np.float64, csr_threshold=0.8, align=1, stream=None):
+        n = len(row_ptr) - 1
+        ld, w, ec, ee, cr, cc, ce = hyb_from_csr(row_ptr, col, val, csr_threshold, align)
+        self.n, self.dtype, self.ld, self.width, self.csr_nnz = n, np.dtype(dtype), ld, w, cc.size
+        self.host = (ec, ee, cr, cc, ce)
+        self.d = [DeviceArray.from_numpy(a, stream) for a in (ec, ee.astype(dtype), cr, cc, ce.astype(dtype))]
+        self.desc = Hyb(n, n, ld, w, *(a.ptr for a in self.d))
+
+    def spmv(self, x, y, alpha=1.0, beta=0.0, xs=0, xi=1, ys=0, yi=1, stream=None):
+        fn = "vclhip_hyb_spmv_f64" if self.dtype == np.float64 else "vclhip_hyb_spmv_f32"
+        call(fn, C.byref(self.desc), x.ptr, xs, xi, alpha, y.ptr, ys, yi, beta, stream)
+
+    def free(self):
+        for a in self.d:
+            a.free()
+
+
+def generic(A):
+    """The format-generic descriptor (vclhip_matrix) of a device matrix; keep the result alive."""
+    fmt = getattr(A, "format", FORMAT_SELL if isinstance(A, SellMatrix) else FORMAT_CSR)
+    return Matrix(fmt, C.cast(C.pointer(A.desc), C.c_void_p))
+
+
+def matrix_diag(A, d, stream=None):
+    m = generic(A)
+    fn = "vclhip_matrix_diag_f32" if A.dtype == np.float32 else "vclhip_matrix_diag_f64"
+    call(fn, C.byref(m), d.ptr, stream)
+
+
 def solver_params(tol=1e-8, max_iters=300, abs_tol=0.0, max_iters_before_restart=200, inner_tol=1e-2,
                   chunk=16, use_graph=False, mixed_mode=0, variant=0, history=None, monitor=None, inner_format=0,
                   krylov_dim=20):
@@ -219,9 +344,14 @@ def solver_params(tol=1e-8, max_iters=300, abs_tol=0.0, max_iters_before_restart
 
 def _solve(kind, A, b, x, precond, params, stream):
     res = SolverResult()
-    mat = "csr" if isinstance(A, CsrMatrix) else "sell"
-    call(f"vclhip_{kind}_solve_{mat}_f64", C.byref(A.desc), b.ptr, x.ptr, precond, C.byref(params), C.byref(res),
-         stream)
+    if isinstance(A, (CsrMatrix, SellMatrix)) and A.dtype == np.float64:
+        mat = "csr" if isinstance(A, CsrMatrix) else "sell"
+        call(f"vclhip_{kind}_solve_{mat}_f64", C.byref(A.desc), b.ptr, x.ptr, precond, C.byref(params), C.byref(res),
+             stream)
+    else:
+        m = generic(A)
+        prec = "f32" if A.dtype == np.float32 else "f64"
+        call(f"vclhip_{kind}_solve_{prec}", C.byref(m), b.ptr, x.ptr, precond, C.byref(params), C.byref(res), stream)
     return res
 
 
@@ -254,9 +384,13 @@ def gmres(A, b, x, stream=None, **kw):
     kw.setdefault("tol", 1e-10)
     params = solver_params(**kw)
     res = SolverResult()
-    mat = "csr" if isinstance(A, CsrMatrix) else "sell"
     prec = "f32" if A.dtype == np.float32 else "f64"
-    call(f"vclhip_gmres_solve_{mat}_{prec}", C.byref(A.desc), b.ptr, x.ptr, C.byref(params), C.byref(res), stream)
+    if isinstance(A, (CsrMatrix, SellMatrix)):
+        mat = "csr" if isinstance(A, CsrMatrix) else "sell"
+        call(f"vclhip_gmres_solve_{mat}_{prec}", C.byref(A.desc), b.ptr, x.ptr, C.byref(params), C.byref(res), stream)
+    else:
+        m = generic(A)
+        call(f"vclhip_gmres_solve_{prec}", C.byref(m), b.ptr, x.ptr, C.byref(params), C.byref(res), stream)
     return res
 
 

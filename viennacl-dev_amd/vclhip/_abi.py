@@ -43,6 +43,29 @@ class Sell(C.Structure):
                 ("elements", vp)]
 
 
+class Coo(C.Structure):
+    _fields_ = [("n_rows", u32), ("n_cols", u32), ("nnz", u32), ("num_blocks", u32),
+                ("coords", vp), ("elements", vp), ("row_ptr", vp), ("blocks", vp)]
+
+
+class Ell(C.Structure):
+    _fields_ = [("n_rows", u32), ("n_cols", u32), ("internal_size1", u32), ("internal_maxnnz", u32),
+                ("coords", vp), ("elements", vp)]
+
+
+class Hyb(C.Structure):
+    _fields_ = [("n_rows", u32), ("n_cols", u32), ("internal_size1", u32), ("internal_ellnnz", u32),
+                ("ell_coords", vp), ("ell_elements", vp), ("csr_rows", vp), ("csr_cols", vp),
+                ("csr_elements", vp)]
+
+
+FORMAT_CSR, FORMAT_SELL, FORMAT_COO, FORMAT_ELL, FORMAT_HYB = 0, 1, 2, 3, 4
+
+
+class Matrix(C.Structure):
+    _fields_ = [("format", i32), ("desc", vp)]
+
+
 MONITOR = C.CFUNCTYPE(C.c_int, vp, f64, vp)
 
 
@@ -101,12 +124,25 @@ _SIGS = {
     "vclhip_axpby_f64": (i32, [u32, f64, vp, f64, vp, vp, vp]),
     "vclhip_axpby_f32": (i32, [u32, f32, vp, f32, vp, vp, vp]),
     "vclhip_avbv_f64": (i32, [u32, vp, u32, u32, vp, u32, u32, f64, vp, u32, u32, f64, vp]),
+    "vclhip_avbv_f32": (i32, [u32, vp, u32, u32, vp, u32, u32, f32, vp, u32, u32, f32, vp]),
     "vclhip_fill_f64": (i32, [u32, vp, u32, u32, f64, vp]),
     "vclhip_fill_f32": (i32, [u32, vp, u32, u32, f32, vp]),
     "vclhip_element_div_f64": (i32, [u32, vp, vp, vp, vp]),
     "vclhip_element_div_f32": (i32, [u32, vp, vp, vp, vp]),
     "vclhip_convert_f64_to_f32": (i32, [u32, vp, vp, vp]),
     "vclhip_convert_f32_to_f64": (i32, [u32, vp, vp, vp]),
+    "vclhip_asum_f64": (i32, [u32, vp, u32, u32, P(f64), vp]),
+    "vclhip_asum_f32": (i32, [u32, vp, u32, u32, P(f32), vp]),
+    "vclhip_iamax_f64": (i32, [u32, vp, u32, u32, P(u32), vp]),
+    "vclhip_iamax_f32": (i32, [u32, vp, u32, u32, P(u32), vp]),
+    "vclhip_scal_f64": (i32, [u32, vp, u32, u32, f64, vp]),
+    "vclhip_scal_f32": (i32, [u32, vp, u32, u32, f32, vp]),
+    "vclhip_copy_f64": (i32, [u32, vp, u32, u32, vp, u32, u32, vp]),
+    "vclhip_copy_f32": (i32, [u32, vp, u32, u32, vp, u32, u32, vp]),
+    "vclhip_swap_f64": (i32, [u32, vp, u32, u32, vp, u32, u32, vp]),
+    "vclhip_swap_f32": (i32, [u32, vp, u32, u32, vp, u32, u32, vp]),
+    "vclhip_rot_f64": (i32, [u32, vp, u32, u32, vp, u32, u32, f64, f64, vp]),
+    "vclhip_rot_f32": (i32, [u32, vp, u32, u32, vp, u32, u32, f32, f32, vp]),
     "vclhip_pipelined_cg_vector_update_f64": (i32, [u32, vp, f64, vp, vp, vp, f64, vp, u32, vp]),
     "vclhip_pipelined_cg_prod_csr_f64": (i32, [P(Csr), vp, vp, vp, u32, vp]),
     "vclhip_pipelined_cg_prod_sell_f64": (i32, [P(Sell), vp, vp, vp, u32, vp]),
@@ -121,6 +157,25 @@ _SIGS = {
     "vclhip_pipelined_gmres_gram_schmidt_stage1_f64": (i32, [vp, u32, C.c_size_t, u32, vp, u32, vp]),
     "vclhip_pipelined_gmres_gram_schmidt_stage2_f64": (i32, [vp, u32, C.c_size_t, u32, vp, vp, u32, vp, u32, vp]),
     "vclhip_pipelined_gmres_update_result_f64": (i32, [vp, vp, vp, u32, C.c_size_t, vp, u32, vp]),
+    "vclhip_coo_plan": (i32, [vp, u32, u32, vp, vp, P(u32)]),
+    "vclhip_ell_from_csr_f64": (i32, [u32, u32, vp, vp, vp, P(u32), vp, vp]),
+    "vclhip_hyb_from_csr_f64": (i32, [u32, f64, u32, vp, vp, vp, P(u32), P(u32), vp, vp, vp, vp, vp]),
+    "vclhip_coo_spmv_f64": (i32, [P(Coo), vp, u32, u32, f64, vp, u32, u32, f64, vp]),
+    "vclhip_coo_spmv_f32": (i32, [P(Coo), vp, u32, u32, f32, vp, u32, u32, f32, vp]),
+    "vclhip_ell_spmv_f64": (i32, [P(Ell), vp, u32, u32, f64, vp, u32, u32, f64, vp]),
+    "vclhip_ell_spmv_f32": (i32, [P(Ell), vp, u32, u32, f32, vp, u32, u32, f32, vp]),
+    "vclhip_hyb_spmv_f64": (i32, [P(Hyb), vp, u32, u32, f64, vp, u32, u32, f64, vp]),
+    "vclhip_hyb_spmv_f32": (i32, [P(Hyb), vp, u32, u32, f32, vp, u32, u32, f32, vp]),
+    "vclhip_matrix_diag_f64": (i32, [P(Matrix), vp, vp]),
+    "vclhip_matrix_diag_f32": (i32, [P(Matrix), vp, vp]),
+    "vclhip_pipelined_cg_prod_f64": (i32, [P(Matrix), vp, vp, vp, u32, vp]),
+    "vclhip_pipelined_bicgstab_prod_f64": (i32, [P(Matrix), vp, vp, vp, vp, u32, u32, vp]),
+    "vclhip_cg_solve_f64": (i32, [P(Matrix), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_cg_solve_f32": (i32, [P(Matrix), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_bicgstab_solve_f64": (i32, [P(Matrix), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_bicgstab_solve_f32": (i32, [P(Matrix), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_gmres_solve_f64": (i32, [P(Matrix), vp, vp, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_gmres_solve_f32": (i32, [P(Matrix), vp, vp, P(SolverParams), P(SolverResult), vp]),
     "vclhip_solver_params_default": (None, [P(SolverParams)]),
     "vclhip_cg_solve_csr_f64": (i32, [P(Csr), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_cg_solve_sell_f64": (i32, [P(Sell), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),

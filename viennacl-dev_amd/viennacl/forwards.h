@@ -75,6 +75,9 @@ template <typename NumericT> class vector_base;
 template <typename NumericT, unsigned int AlignmentV = 1> class vector;
 template <typename NumericT, unsigned int AlignmentV = 1> class compressed_matrix;
 template <typename ScalarT, typename IndexT = unsigned int> class sliced_ell_matrix;
+template <typename NumericT, unsigned int AlignmentV = 128> class coordinate_matrix;
+template <typename NumericT, unsigned int AlignmentV = 1> class ell_matrix;
+template <typename NumericT, unsigned int AlignmentV = 1> class hyb_matrix;
 
 /** Vectors are padded to a multiple of this many entries (reference forwards.h:385). */
 static const vcl_size_t dense_padding_size = 128;

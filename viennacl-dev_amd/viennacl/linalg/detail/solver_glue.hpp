@@ -5,6 +5,9 @@
 // type and matrix format.
 
 #include "viennacl/compressed_matrix.hpp"
+#include "viennacl/coordinate_matrix.hpp"
+#include "viennacl/ell_matrix.hpp"
+#include "viennacl/hyb_matrix.hpp"
 #include "viennacl/sliced_ell_matrix.hpp"
 #include "viennacl/vector.hpp"
 
@@ -66,6 +69,36 @@ inline int bicgstab_solve(vclhip_sell const& A, const float* b, float* x, int pc
                           vclhip_solver_result& R)
 { return vclhip_bicgstab_solve_sell_f32(&A, b, x, pc, &P, &R, backend::hip::current_stream()); }
 
+// COO / ELL / HYB go through the format-generic drivers
+template <typename DescT>
+inline int cg_solve(DescT const& A, const double* b, double* x, int pc, vclhip_solver_params const& P,
+                    vclhip_solver_result& R)
+{
+  vclhip_matrix m = hip::generic(A);
+  return vclhip_cg_solve_f64(&m, b, x, pc, &P, &R, backend::hip::current_stream());
+}
+template <typename DescT>
+inline int cg_solve(DescT const& A, const float* b, float* x, int pc, vclhip_solver_params const& P,
+                    vclhip_solver_result& R)
+{
+  vclhip_matrix m = hip::generic(A);
+  return vclhip_cg_solve_f32(&m, b, x, pc, &P, &R, backend::hip::current_stream());
+}
+template <typename DescT>
+inline int bicgstab_solve(DescT const& A, const double* b, double* x, int pc, vclhip_solver_params const& P,
+                          vclhip_solver_result& R)
+{
+  vclhip_matrix m = hip::generic(A);
+  return vclhip_bicgstab_solve_f64(&m, b, x, pc, &P, &R, backend::hip::current_stream());
+}
+template <typename DescT>
+inline int bicgstab_solve(DescT const& A, const float* b, float* x, int pc, vclhip_solver_params const& P,
+                          vclhip_solver_result& R)
+{
+  vclhip_matrix m = hip::generic(A);
+  return vclhip_bicgstab_solve_f32(&m, b, x, pc, &P, &R, backend::hip::current_stream());
+}
+
 /** Contiguous copy of b (the drivers take unit-stride vectors). */
 template <typename T>
 vector<T> contiguous_copy(vector_base<T> const& b)
@@ -78,6 +111,9 @@ vector<T> contiguous_copy(vector_base<T> const& b)
 template <typename T> struct is_hip_sparse : std::false_type {};
 template <typename T, unsigned A> struct is_hip_sparse<compressed_matrix<T, A> > : std::true_type {};
 template <typename T, typename I> struct is_hip_sparse<sliced_ell_matrix<T, I> > : std::true_type {};
+template <typename T, unsigned A> struct is_hip_sparse<coordinate_matrix<T, A> > : std::true_type {};
+template <typename T, unsigned A> struct is_hip_sparse<ell_matrix<T, A> > : std::true_type {};
+template <typename T, unsigned A> struct is_hip_sparse<hyb_matrix<T, A> > : std::true_type {};
 
 }  // namespace detail
 }  // namespace linalg

@@ -67,6 +67,21 @@ inline int gmres_solve(vclhip_sell const& A, const float* b, float* x, vclhip_so
                        vclhip_solver_result& R)
 { return vclhip_gmres_solve_sell_f32(&A, b, x, &P, &R, backend::hip::current_stream()); }
 
+template <typename DescT>
+inline int gmres_solve(DescT const& A, const double* b, double* x, vclhip_solver_params const& P,
+                       vclhip_solver_result& R)
+{
+  vclhip_matrix m = hip::generic(A);
+  return vclhip_gmres_solve_f64(&m, b, x, &P, &R, backend::hip::current_stream());
+}
+template <typename DescT>
+inline int gmres_solve(DescT const& A, const float* b, float* x, vclhip_solver_params const& P,
+                       vclhip_solver_result& R)
+{
+  vclhip_matrix m = hip::generic(A);
+  return vclhip_gmres_solve_f32(&m, b, x, &P, &R, backend::hip::current_stream());
+}
+
 /** Device-resident pipelined GMRES (gmres.hpp:182-370); the monitor is called once per cycle. */
 template <typename MatrixT, typename T>
 vector<T> gmres_device(MatrixT const& A, vector_base<T> const& rhs, gmres_tag const& tag,

@@ -1,7 +1,7 @@
 #ifndef VIENNACL_LINALG_SPARSE_MATRIX_OPERATIONS_HPP
 #define VIENNACL_LINALG_SPARSE_MATRIX_OPERATIONS_HPP
-// sparse_matrix_operations.hpp -- prod_impl and row_info for CSR and SELL-C matrices, the
-// operations behind y = prod(A, x), y += prod(A, x), y -= prod(A, x)
+// sparse_matrix_operations.hpp -- prod_impl and row_info for the sparse formats (CSR, SELL-C, COO,
+// ELL, HYB), the operations behind y = prod(A, x), y += prod(A, x), y -= prod(A, x)
 // (reference linalg/sparse_matrix_operations.hpp:48-121 and the op_executors of
 // compressed_matrix.hpp:1231-1318 / sliced_ell_matrix.hpp:286-375).  HIP_MEMORY only.
 
@@ -33,6 +33,36 @@ inline int spmv(vclhip_sell const& d, const float* x, uint32_t xs, uint32_t xi, 
                 uint32_t yi, float b)
 {
   return vclhip_sell_spmv_f32(&d, x, xs, xi, a, y, ys, yi, b, backend::hip::current_stream());
+}
+#define VCLHIP_FORMAT_SPMV(DESC, FN)                                                                          \
+  inline int spmv(DESC const& d, const double* x, uint32_t xs, uint32_t xi, double a, double* y, uint32_t ys,  \
+                  uint32_t yi, double b)                                                                       \
+  {                                                                                                            \
+    return FN##_f64(&d, x, xs, xi, a, y, ys, yi, b, backend::hip::current_stream());                            \
+  }                                                                                                            \
+  inline int spmv(DESC const& d, const float* x, uint32_t xs, uint32_t xi, float a, float* y, uint32_t ys,     \
+                  uint32_t yi, float b)                                                                        \
+  {                                                                                                            \
+    return FN##_f32(&d, x, xs, xi, a, y, ys, yi, b, backend::hip::current_stream());                            \
+  }
+VCLHIP_FORMAT_SPMV(vclhip_coo, vclhip_coo_spmv)
+VCLHIP_FORMAT_SPMV(vclhip_ell, vclhip_ell_spmv)
+VCLHIP_FORMAT_SPMV(vclhip_hyb, vclhip_hyb_spmv)
+#undef VCLHIP_FORMAT_SPMV
+
+/** The format-generic descriptor of a matrix (vclhip_matrix) for the solvers and the diagonal. */
+inline int format_of(vclhip_csr const&) { return VCLHIP_FORMAT_CSR; }
+inline int format_of(vclhip_sell const&) { return VCLHIP_FORMAT_SELL; }
+inline int format_of(vclhip_coo const&) { return VCLHIP_FORMAT_COO; }
+inline int format_of(vclhip_ell const&) { return VCLHIP_FORMAT_ELL; }
+inline int format_of(vclhip_hyb const&) { return VCLHIP_FORMAT_HYB; }
+template <typename DescT>
+vclhip_matrix generic(DescT const& d)
+{
+  vclhip_matrix m;
+  m.format = format_of(d);
+  m.desc = &d;
+  return m;
 }
 }  // namespace hip
 
@@ -72,6 +102,25 @@ void row_info(compressed_matrix<T, A> const& mat, vector_base<T>& vec, row_info_
   else
     backend::hip::check(vclhip_csr_row_info_f32(&d, reinterpret_cast<float*>(vec.device_ptr()), int(info),
                                                 backend::hip::current_stream()), "row_info");
+}
+
+/** Diagonal of a COO, ELL or HYB matrix (row_info with SPARSE_ROW_DIAGONAL; COO:
+ *  host_based/sparse_matrix_operations.hpp:1156-1205).  Other row_info kinds exist for CSR only. */
+template <typename MatrixT, typename T>
+typename std::enable_if<!std::is_same<MatrixT, compressed_matrix<T, 1> >::value>::type
+row_info(MatrixT const& mat, vector_base<T>& vec, row_info_types info)
+{
+  if (info != SPARSE_ROW_DIAGONAL) throw std::invalid_argument("ViennaCL (HIP): row_info: only the diagonal for this format");
+  if (vec.size() == 0) vec.init(mat.size1(), viennacl::context(HIP_MEMORY));
+  if (!vec.contiguous()) throw memory_exception("row_info needs a contiguous vector");
+  auto d = mat.descriptor();
+  vclhip_matrix m = hip::generic(d);
+  if (std::is_same<T, double>::value)
+    backend::hip::check(vclhip_matrix_diag_f64(&m, reinterpret_cast<double*>(vec.device_ptr()),
+                                               backend::hip::current_stream()), "row_info");
+  else
+    backend::hip::check(vclhip_matrix_diag_f32(&m, reinterpret_cast<float*>(vec.device_ptr()),
+                                               backend::hip::current_stream()), "row_info");
 }
 }  // namespace detail
 }  // namespace linalg
