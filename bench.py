@@ -259,14 +259,19 @@ def formats_bench(vh, stream, steps):
             mats["ell"] = (E, nnz * 12 + (E.ld * E.width - nnz) * 8 + 16 * n)
         else:  # the power-law matrix's ELL would store 512 columns for every row
             rec["ell_skipped"] = f"ELL width {width} would store {n * width / nnz:.0f}x the entries"
+        # HYB with the plan of its CSR part (long CSR parts summed by a wavefront), and HYB summed one
+        # lane per row in the reference's order
         H = vh.HybMatrix(rp, col, val)
         csr_real = int(H.host[2][-1])
         ell_real = nnz - csr_real
-        mats["hyb"] = (H, ell_real * 12 + (H.ld * H.width - ell_real) * 8 + csr_real * 12 + 4 * (n + 1) + 16 * n)
+        hyb_bytes = ell_real * 12 + (H.ld * H.width - ell_real) * 8 + csr_real * 12 + 4 * (n + 1) + 16 * n
+        mats["hyb"] = (H, hyb_bytes)
+        mats["hyb_lane_per_row"] = (vh.HybMatrix(rp, col, val, plan=False), hyb_bytes)
         for name, (M, by) in mats.items():
             cold, warm, _ = time_launches(vh, lambda: M.spmv(x, y, stream=stream.handle), stream, steps, 3, evict)
-            rec[name] = {"cold_us": round(float(cold.mean()) * 1e3, 2), "warm_us": round(warm * 1e3, 2),
-                         "algorithmic_bytes": int(by), "cold_gbs": round(by / (cold.mean() * 1e-3) / 1e9, 1),
+            cold_ms = float(cold.mean())
+            rec[name] = {"cold_us": round(cold_ms * 1e3, 2), "warm_us": round(warm * 1e3, 2),
+                         "algorithmic_bytes": int(by), "cold_gbs": round(by / (cold_ms * 1e-3) / 1e9, 1),
                          "warm_gbs": round(by / (warm * 1e-3) / 1e9, 1)}
             M.free()
         rec["hyb_ell_width"], rec["hyb_csr_nnz"] = H.width, H.csr_nnz

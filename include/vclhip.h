@@ -138,7 +138,10 @@ typedef struct vclhip_ell {
 
 /* HYB matrix on the device (hyb_matrix<T>: an ELL part of internal_ellnnz columns, handle2 =
  * coords, handle = elements, and a CSR part for the rest of each row, handle3 = row pointers,
- * handle4 = columns, handle5 = elements; hyb_matrix.hpp:136-240). */
+ * handle4 = columns, handle5 = elements; hyb_matrix.hpp:136-240).  `blocks` is optional: the
+ * row-block plan of the CSR part (vclhip_csr_plan_blocks on csr_rows), with which long CSR parts
+ * are summed by a wavefront; with blocks = NULL one lane sums each row, ELL part then CSR part,
+ * in the reference's order. */
 typedef struct vclhip_hyb {
   uint32_t n_rows;
   uint32_t n_cols;
@@ -149,6 +152,8 @@ typedef struct vclhip_hyb {
   const uint32_t* csr_rows;
   const uint32_t* csr_cols;
   const void* csr_elements;
+  uint32_t num_blocks;
+  const uint32_t* blocks;
 } vclhip_hyb;
 
 /* A matrix of any of the formats, for the format-generic entry points below. */

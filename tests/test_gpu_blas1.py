@@ -78,3 +78,74 @@ def test_fill_zero_and_copies(dev, vh):
     dst = vh.DeviceArray.zeros(1000, np.float64)
     vh.call("vclhip_memcpy_d2d", dst.ptr, src.ptr, 8000, None)
     assert np.array_equal(dst.to_numpy(), np.arange(1000.0))
+
+
+@pytest.mark.parametrize("n", [1, 257, 100_003])
+def test_asum_iamax(dev, vh, n):
+    x = vh.random_uniform(11, 2 * n + 1) - 0.5
+    dx = dev_arr(vh, x)
+    r, rf, idx = C.c_double(), C.c_float(), C.c_uint32()
+    sub = x[1::2][:n]
+    vh.call("vclhip_asum_f64", n, dx.ptr, 1, 2, C.byref(r), None)
+    assert abs(r.value - np.sum(np.abs(sub))) <= 4 * n * 2.2e-16 * np.sum(np.abs(sub))
+    vh.call("vclhip_iamax_f64", n, dx.ptr, 1, 2, C.byref(idx), None)
+    assert idx.value == int(np.argmax(np.abs(sub)))  # numpy also returns the first maximum
+    xf = sub.astype(np.float32)
+    dxf = dev_arr(vh, xf)
+    vh.call("vclhip_asum_f32", n, dxf.ptr, 0, 1, C.byref(rf), None)
+    assert abs(rf.value - np.sum(np.abs(xf.astype(np.float64)))) <= 1e-5 * np.sum(np.abs(xf))
+    vh.call("vclhip_iamax_f32", n, dxf.ptr, 0, 1, C.byref(idx), None)
+    assert idx.value == int(np.argmax(np.abs(xf)))
+
+
+def test_iamax_ties_and_zero_vector(dev, vh):
+    n = 70_000
+    x = np.zeros(n)
+    x[[5, 40_000, 69_999]] = [-3.0, 3.0, 3.0]  # ties across workgroups: the first one wins
+    dx, idx = dev_arr(vh, x), C.c_uint32()
+    vh.call("vclhip_iamax_f64", n, dx.ptr, 0, 1, C.byref(idx), None)
+    assert idx.value == 5
+    z = dev_arr(vh, np.zeros(n))
+    vh.call("vclhip_iamax_f64", n - 7, z.ptr, 7, 1, C.byref(idx), None)
+    assert idx.value == 7  # no entry above zero: the vector's start offset
+
+
+def test_scal_copy_swap_rot(dev, vh):
+    n = 50_001
+    x = vh.random_uniform(12, 3 * n)
+    y = vh.random_uniform(13, 2 * n + 4)
+    dx, dy = dev_arr(vh, x), dev_arr(vh, y)
+    vh.call("vclhip_scal_f64", n, dx.ptr, 2, 3, -1.25, None)
+    want_x = x.copy()
+    want_x[2::3] *= -1.25
+    assert np.array_equal(dx.to_numpy(), want_x)
+    vh.call("vclhip_copy_f64", n, dx.ptr, 2, 3, dy.ptr, 4, 2, None)
+    want_y = y.copy()
+    want_y[4::2] = want_x[2::3]
+    assert np.array_equal(dy.to_numpy(), want_y)
+    dx, dy = dev_arr(vh, x), dev_arr(vh, y)
+    vh.call("vclhip_swap_f64", n, dx.ptr, 0, 3, dy.ptr, 4, 2, None)
+    sx, sy = x.copy(), y.copy()
+    sx[0::3], sy[4::2] = y[4::2], x[0::3]
+    assert np.array_equal(dx.to_numpy(), sx) and np.array_equal(dy.to_numpy(), sy)
+    c, s = 0.6, 0.8
+    vh.call("vclhip_rot_f64", n, dx.ptr, 0, 3, dy.ptr, 4, 2, c, s, None)
+    a, b = sx[0::3].copy(), sy[4::2].copy()
+    rx, ry = sx.copy(), sy.copy()
+    rx[0::3], ry[4::2] = c * a + s * b, c * b - s * a
+    assert np.allclose(dx.to_numpy(), rx, rtol=0, atol=1e-15) and np.allclose(dy.to_numpy(), ry, rtol=0, atol=1e-15)
+    # FP32 forms and the strided f32 avbv
+    xf, yf = x[:n].astype(np.float32), y[:n].astype(np.float32)
+    dxf, dyf = dev_arr(vh, xf), dev_arr(vh, yf)
+    vh.call("vclhip_rot_f32", n, dxf.ptr, 0, 1, dyf.ptr, 0, 1, 0.6, 0.8, None)
+    c32, s32 = np.float32(0.6), np.float32(0.8)
+    assert np.allclose(dxf.to_numpy(), c32 * xf + s32 * yf, rtol=1e-6, atol=1e-6)
+    assert np.allclose(dyf.to_numpy(), c32 * yf - s32 * xf, rtol=1e-6, atol=1e-6)
+    dxf = dev_arr(vh, xf)
+    vh.call("vclhip_scal_f32", n, dxf.ptr, 0, 1, 2.0, None)
+    assert np.array_equal(dxf.to_numpy(), xf * 2)
+    v1 = vh.DeviceArray.zeros(2 * 1000 + 1, np.float32)
+    vh.call("vclhip_avbv_f32", 1000, v1.ptr, 1, 2, dxf.ptr, 0, 1, 1.5, dyf.ptr, 0, 3, -0.5, None)
+    got = v1.to_numpy()[1::2]
+    want = (xf[:1000] * 2) * np.float32(1.5) + dyf.to_numpy()[0:3000:3] * np.float32(-0.5)
+    assert np.allclose(got, want, rtol=1e-6, atol=1e-6)

@@ -4,16 +4,18 @@
 // coords[2e + 1] (the kernel's column stride 2); its row sums are the reference's sequential
 // per-entry accumulation when alpha = 1 and beta = 0, and differ from it only in rounding
 // otherwise (the reference adds alpha * value * x entry by entry,
-// host_based/sparse_matrix_operations.hpp:1233-1245).  ELL and HYB run one lane per row over the
-// column-major ELL part, so that a wavefront reads 64 consecutive entries of each column, then the
-// row's CSR part (HYB).
+// host_based/sparse_matrix_operations.hpp:1233-1245).  ELL runs one lane per row over the
+// column-major ELL part, so that a wavefront reads 64 consecutive entries of each column.  HYB with
+// a row-block plan of its CSR part runs the CSR kernel over that part, each row starting from its
+// ELL sum (one lane per row again); without a plan, one lane walks the ELL part and then the CSR
+// part of its row.
 #include "internal.hpp"
 #include "spmv_kernels.hpp"
 
 namespace vclhip {
 namespace formats_detail {
 
-template <typename T, int CS, class Dev>
+template <typename T, int CS, class Dev, bool Hyb = false>
 int run_csr_like(const Dev& d, uint32_t num_blocks, size_t matrix_bytes, const T* x, uint32_t xs, uint32_t xi,
                  T alpha, T* y, uint32_t ys, uint32_t yi, T beta, void* stream, const char* name)
 {
@@ -24,14 +26,14 @@ int run_csr_like(const Dev& d, uint32_t num_blocks, size_t matrix_bytes, const T
   if (xs == 0 && xi == 1 && ys == 0 && yi == 1) {
     EpiAxpby<T, true> epi{y, 0, 1, alpha, beta};
     if (stream_loads(matrix_bytes))
-      launch_timed(csr_kernel<T, true, EpiAxpby<T, true>, true, CS>, dim3(grid), dim3(kBlock), s, d, x, 0u, 1u, epi,
+      launch_timed(csr_kernel<T, true, EpiAxpby<T, true>, true, CS, Hyb>, dim3(grid), dim3(kBlock), s, d, x, 0u, 1u, epi,
                    np, 0u, ns, 0u);
     else
-      launch_timed(csr_kernel<T, true, EpiAxpby<T, true>, false, CS>, dim3(grid), dim3(kBlock), s, d, x, 0u, 1u, epi,
+      launch_timed(csr_kernel<T, true, EpiAxpby<T, true>, false, CS, Hyb>, dim3(grid), dim3(kBlock), s, d, x, 0u, 1u, epi,
                    np, 0u, ns, 0u);
   } else {
     EpiAxpby<T, false> epi{y, ys, yi, alpha, beta};
-    launch_timed(csr_kernel<T, false, EpiAxpby<T, false>, false, CS>, dim3(grid), dim3(kBlock), s, d, x, xs, xi, epi,
+    launch_timed(csr_kernel<T, false, EpiAxpby<T, false>, false, CS, Hyb>, dim3(grid), dim3(kBlock), s, d, x, xs, xi, epi,
                  np, 0u, ns, 0u);
   }
   return check_launch(name);
@@ -114,6 +116,25 @@ int hyb_spmv(const vclhip_hyb* A, const T* x, uint32_t xs, uint32_t xi, T alpha,
   if ((A->internal_ellnnz && (!A->ell_coords || !A->ell_elements)) || !A->csr_rows || !A->csr_cols ||
       !A->csr_elements)
     return invalid("HYB arrays must be non-NULL");
+  if (A->blocks) {
+    if (A->n_rows >= kPlanRowLimit) return invalid("n_rows must be below 2^28");
+    if (A->num_blocks == 0 || A->num_blocks > A->n_rows) return invalid("num_blocks out of range");
+    VCLHIP_TRY(check_vectors(x, xi, y, yi));
+    HybDev<T> h;
+    h.row_ptr = A->csr_rows;
+    h.col = A->csr_cols;
+    h.val = static_cast<const T*>(A->csr_elements);
+    h.blocks = A->blocks;
+    h.num_blocks = A->num_blocks;
+    h.n_rows = A->n_rows;
+    h.ell_coords = A->ell_coords;
+    h.ell_val = static_cast<const T*>(A->ell_elements);
+    h.ell_ld = A->internal_size1;
+    h.ell_width = A->internal_ellnnz;
+    const size_t ell_bytes = static_cast<size_t>(A->internal_size1) * A->internal_ellnnz * (sizeof(T) + 4);
+    return run_csr_like<T, 1, HybDev<T>, true>(h, A->num_blocks, ell_bytes, x, xs, xi, alpha, y, ys, yi, beta,
+                                               stream, "hyb spmv");
+  }
   EllDev<T> d;
   d.coords = A->ell_coords;
   d.val = static_cast<const T*>(A->ell_elements);

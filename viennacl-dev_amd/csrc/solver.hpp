@@ -36,6 +36,9 @@ struct DevOp {
                        part, kMaxPartials, stop, mask);
   }
 
+  HybDev<T> hyb{};  // HYB with a plan for its CSR part
+  bool hyb_plan = false;
+
   template <class Epi, int CS>
   void launch_csr(hipStream_t s, const T* x, const Epi& epi, double* part, const uint32_t* stop, uint32_t mask) const
   {
@@ -69,8 +72,12 @@ struct DevOp {
                            part, kMaxPartials, stop, mask);
         break;
       case OpFormat::Hyb:
-        hipLaunchKernelGGL((ell_kernel<T, true, Epi, true>), dim3(grid), dim3(kBlock), 0, s, ell, x, 0u, 1u, epi,
-                           part, kMaxPartials, stop, mask);
+        if (hyb_plan)
+          hipLaunchKernelGGL((csr_kernel<T, true, Epi, false, 1, true>), dim3(grid), dim3(kBlock), 0, s, hyb, x, 0u,
+                             1u, epi, part, kMaxPartials, stop, mask);
+        else
+          hipLaunchKernelGGL((ell_kernel<T, true, Epi, true>), dim3(grid), dim3(kBlock), 0, s, ell, x, 0u, 1u, epi,
+                             part, kMaxPartials, stop, mask);
         break;
       default:
         launch_csr<Epi, 1>(s, x, epi, part, stop, mask);
@@ -175,6 +182,21 @@ static int make_op(const vclhip_hyb* A, DevOp<T>& op)
   op.ell.csr_rows = A->csr_rows;
   op.ell.csr_cols = A->csr_cols;
   op.ell.csr_val = static_cast<const T*>(A->csr_elements);
+  op.hyb_plan = A->blocks != nullptr;
+  if (op.hyb_plan) {
+    if (A->n_rows >= kPlanRowLimit || A->num_blocks == 0 || A->num_blocks > A->n_rows) return invalid("HYB plan");
+    op.hyb.row_ptr = A->csr_rows;
+    op.hyb.col = A->csr_cols;
+    op.hyb.val = static_cast<const T*>(A->csr_elements);
+    op.hyb.blocks = A->blocks;
+    op.hyb.num_blocks = A->num_blocks;
+    op.hyb.n_rows = A->n_rows;
+    op.hyb.ell_coords = A->ell_coords;
+    op.hyb.ell_val = static_cast<const T*>(A->ell_elements);
+    op.hyb.ell_ld = A->internal_size1;
+    op.hyb.ell_width = A->internal_ellnnz;
+    op.grid = chunk_grid(A->num_blocks, csr_gmax());
+  }
   return VCLHIP_SUCCESS;
 }
 

@@ -31,6 +31,17 @@ struct CsrDev {
   uint32_t n_rows;
 };
 
+// HYB through the CSR kernel: the CSR part with its row-block plan, plus the ELL part, whose
+// entries of a row are summed before the row's CSR entries (the reference's order,
+// host_based/sparse_matrix_operations.hpp:1893-1912).
+template <typename T>
+struct HybDev : CsrDev<T> {
+  const uint32_t* __restrict__ ell_coords;
+  const T* __restrict__ ell_val;
+  uint32_t ell_ld;     // internal_size1
+  uint32_t ell_width;  // internal_ellnnz
+};
+
 template <typename T>
 struct SellDev {
   const uint32_t* __restrict__ cpb;
@@ -59,6 +70,23 @@ __device__ __forceinline__ T load_x(const T* __restrict__ x, uint32_t c, uint32_
   return x[static_cast<size_t>(c) * xi + xs];
 }
 
+// Sum of the ELL entries of one row in column order; zero entries are skipped without reading x
+// (host_based/sparse_matrix_operations.hpp:1516-1524, :1897-1904).
+template <typename T, bool Contig>
+__device__ __forceinline__ T ell_row_sum(const uint32_t* __restrict__ coords, const T* __restrict__ val,
+                                         uint32_t ld, uint32_t width, uint32_t row, const T* __restrict__ x,
+                                         uint32_t xs, uint32_t xi)
+{
+  T sum = 0;
+#pragma unroll 4
+  for (uint32_t k = 0; k < width; ++k) {
+    const size_t off = static_cast<size_t>(k) * ld + row;
+    const T v = val[off];
+    if (v > 0 || v < 0) sum += load_x<T, Contig>(x, coords[off], xs, xi) * v;
+  }
+  return sum;
+}
+
 // y = alpha*s + beta*y with the reference's beta test (host_based/sparse_matrix_operations.hpp:177-183).
 template <typename T, bool Contig>
 struct EpiAxpby {
@@ -79,12 +107,13 @@ struct EpiAxpby {
 
 // CSR SpMV over the row-block plan.  Each logical workgroup walks a contiguous chunk of blocks.
 // CS is the distance between consecutive column indices: 1 for CSR, 2 for COO, whose (row, col)
-// pairs are read through col = coords + 1 with the row pointers of its row order.
-template <typename T, bool Contig, class Epi, bool NT = false, int CS = 1>
-__global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __restrict__ x, uint32_t xs, uint32_t xi,
-                                                     Epi epi, typename Epi::acc_t* __restrict__ part,
-                                                     uint32_t pstride, const uint32_t* __restrict__ stop,
-                                                     uint32_t stop_mask)
+// pairs are read through col = coords + 1 with the row pointers of its row order.  Hyb: the matrix
+// is a HybDev, the plan covers its CSR part, and each row sum starts from the row's ELL sum.
+template <typename T, bool Contig, class Epi, bool NT = false, int CS = 1, bool Hyb = false>
+__global__ __launch_bounds__(kBlock) void csr_kernel(std::conditional_t<Hyb, HybDev<T>, CsrDev<T>> A,
+                                                     const T* __restrict__ x, uint32_t xs, uint32_t xi, Epi epi,
+                                                     typename Epi::acc_t* __restrict__ part, uint32_t pstride,
+                                                     const uint32_t* __restrict__ stop, uint32_t stop_mask)
 {
   using acc_t = typename Epi::acc_t;
   constexpr int KA = Epi::K > 0 ? Epi::K : 1;
@@ -92,6 +121,7 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __res
 
   __shared__ T stage[kCsrNnzCap];
   __shared__ uint32_t srp[kCsrWG + 1];
+  __shared__ T spre[Hyb ? kCsrWG : 1];  // ELL sums of the block's rows, for the wavefront pass
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lw = xcd_logical_block();
@@ -116,7 +146,10 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __res
       for (uint32_t e = e0 + tid; e < e1; e += kBlock)
         s += A.val[e] * load_x<T, Contig>(x, A.col[static_cast<size_t>(e) * CS], xs, xi);
       s = block_sum(s);
-      if (tid == 0) epi(r0, s, acc);
+      if (tid == 0) {
+        if constexpr (Hyb) s = ell_row_sum<T, Contig>(A.ell_coords, A.ell_val, A.ell_ld, A.ell_width, r0, x, xs, xi) + s;
+        epi(r0, s, acc);
+      }
       continue;
     }
 
@@ -128,6 +161,13 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __res
     const uint32_t cnt = e1 - e0;
     const uint32_t nrows = r1 - r0;
     if (tid < nrows) srp[tid + 1] = A.row_ptr[r0 + tid + 1] - e0;
+    T pre = 0;  // Hyb: the ELL sum of row r0 + tid
+    if constexpr (Hyb) {
+      if (tid < nrows) {
+        pre = ell_row_sum<T, Contig>(A.ell_coords, A.ell_val, A.ell_ld, A.ell_width, r0 + tid, x, xs, xi);
+        if (mode == kModeMixed) spre[tid] = pre;
+      }
+    }
     if (cnt > 0) {
       constexpr int U = kCsrNnzCap / kBlock;
       uint32_t c[U];
@@ -155,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __res
       const uint32_t a = srp[tid];
       const uint32_t z = srp[tid + 1];
       if (z - a <= kShortRow) {
-        T s = 0;
+        T s = pre;
         for (uint32_t k = a; k < z; ++k) s += stage[k];
         epi(r0 + tid, s, acc);
       }
@@ -180,6 +220,7 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(CsrDev<T> A, const T* __res
           T s = 0;
           for (uint32_t k = a + lane; k < z; k += kWave) s += stage[k];
           s = wave_sum(s);
+          if constexpr (Hyb) s = spre[row] + s;
           if (lane == 0) epi(r0 + row, s, acc);
         }
       }
@@ -266,7 +307,9 @@ struct EllDev {
 // One lane per row; the lanes of a wavefront read 64 consecutive entries of each ELL column.
 // Zero ELL entries are skipped (their x entry is not read), as the reference's host loop does
 // (host_based/sparse_matrix_operations.hpp:1516-1524); the HYB CSR part follows in entry order
-// (:1906-1912).
+// (:1906-1912), so every row is summed in the reference's order.  HYB matrices with a plan for
+// their CSR part take csr_kernel<..., Hyb = true> instead, which spreads long CSR parts over a
+// wavefront.
 template <typename T, bool Contig, class Epi, bool Hyb>
 __global__ __launch_bounds__(kBlock) void ell_kernel(EllDev<T> A, const T* __restrict__ x, uint32_t xs, uint32_t xi,
                                                      Epi epi, typename Epi::acc_t* __restrict__ part,
@@ -290,13 +333,7 @@ __global__ __launch_bounds__(kBlock) void ell_kernel(EllDev<T> A, const T* __res
   for (uint32_t t = t0; t < t1; ++t) {
     const uint32_t row = t * kBlock + tid;
     if (row < A.n_rows) {
-      T sum = 0;
-#pragma unroll 4
-      for (uint32_t k = 0; k < A.width; ++k) {
-        const size_t off = static_cast<size_t>(k) * A.ld + row;
-        const T v = A.val[off];
-        if (v > 0 || v < 0) sum += load_x<T, Contig>(x, A.coords[off], xs, xi) * v;
-      }
+      T sum = ell_row_sum<T, Contig>(A.coords, A.val, A.ld, A.width, row, x, xs, xi);
       if constexpr (Hyb) {
         const uint32_t e1 = A.csr_rows[row + 1];
         for (uint32_t e = A.csr_rows[row]; e < e1; ++e) sum += load_x<T, Contig>(x, A.csr_cols[e], xs, xi) * A.csr_val[e];

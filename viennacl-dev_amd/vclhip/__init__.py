@@ -285,17 +285,30 @@ class EllMatrix:
 
 
 class HybMatrix:
-    """hyb_matrix on the device: ELL part plus a CSR part for the entries beyond the ELL width."""
+    """hyb_matrix on the device: ELL part plus a CSR part for the entries beyond the ELL width.
+
+    plan=True uploads the row-block plan of the CSR part, so that long CSR parts are summed by a
+    wavefront; plan=False sums every row in one lane, in the reference's order; plan="auto" (the
+    default) takes the plan when some row's CSR part is longer than 32 entries."""
 
     format = FORMAT_HYB
 
-    def __init__(self, row_ptr, col, val, dtype=np.float64, csr_threshold=0.8, align=1, stream=None):
+    def __init__(self, row_ptr, col, val, dtype=np.float64, csr_threshold=0.8, align=1, stream=None, plan="auto"):
         n = len(row_ptr) - 1
         ld, w, ec, ee, cr, cc, ce = hyb_from_csr(row_ptr, col, val, csr_threshold, align)
         self.n, self.dtype, self.ld, self.width, self.csr_nnz = n, np.dtype(dtype), ld, w, cc.size
         self.host = (ec, ee, cr, cc, ce)
-        self.d = [DeviceArray.from_numpy(a, stream) for a in (ec, ee.astype(dtype), cr, cc, ce.astype(dtype))]
-        self.desc = Hyb(n, n, ld, w, *(a.ptr for a in self.d))
+        arrays = [ec, ee.astype(dtype), cr, cc, ce.astype(dtype)]
+        if plan == "auto":
+            plan = bool(n) and int(np.diff(cr.astype(np.int64)).max()) > 32
+        self.planned = bool(plan)
+        nb = 0
+        if plan:
+            blocks = plan_blocks(cr)
+            nb = blocks.size - 1
+            arrays.append(blocks)
+        self.d = [DeviceArray.from_numpy(a, stream) for a in arrays]
+        self.desc = Hyb(n, n, ld, w, *(a.ptr for a in self.d[:5]), nb, self.d[5].ptr if plan else None)
 
     def spmv(self, x, y, alpha=1.0, beta=0.0, xs=0, xi=1, ys=0, yi=1, stream=None):
         fn = "vclhip_hyb_spmv_f64" if self.dtype == np.float64 else "vclhip_hyb_spmv_f32"

@@ -56,7 +56,7 @@ class Ell(C.Structure):
 class Hyb(C.Structure):
     _fields_ = [("n_rows", u32), ("n_cols", u32), ("internal_size1", u32), ("internal_ellnnz", u32),
                 ("ell_coords", vp), ("ell_elements", vp), ("csr_rows", vp), ("csr_cols", vp),
-                ("csr_elements", vp)]
+                ("csr_elements", vp), ("num_blocks", u32), ("blocks", vp)]
 
 
 FORMAT_CSR, FORMAT_SELL, FORMAT_COO, FORMAT_ELL, FORMAT_HYB = 0, 1, 2, 3, 4

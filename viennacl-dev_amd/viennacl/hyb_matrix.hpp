@@ -5,8 +5,11 @@
 // rows), handle3 / handle4 / handle5 = the CSR part (row pointers, columns, values) that holds the
 // entries of each row beyond the ELL width.  The ELL width is the smallest row length that at least
 // csr_threshold() of the rows do not exceed (default 0.8, hyb_matrix.hpp:44, :150-169), so a few
-// long rows do not widen the ELL part.
+// long rows do not widen the ELL part.  When some row's CSR part is longer than 32 entries,
+// blocks_handle() holds the row-block plan of the CSR part, with which the product sums such a
+// part by a wavefront; otherwise one lane sums each row, ELL part then CSR part.
 
+#include <algorithm>
 #include <map>
 #include <vector>
 
@@ -25,10 +28,11 @@ public:
   typedef backend::mem_handle handle_type;
   typedef vcl_size_t size_type;
 
-  hyb_matrix() : csr_threshold_(NumericT(0.8)), rows_(0), cols_(0), ellnnz_(0), csrnnz_(0) {}
-  explicit hyb_matrix(viennacl::context) : csr_threshold_(NumericT(0.8)), rows_(0), cols_(0), ellnnz_(0), csrnnz_(0) {}
+  hyb_matrix() : csr_threshold_(NumericT(0.8)), rows_(0), cols_(0), ellnnz_(0), csrnnz_(0), num_blocks_(0) {}
+  explicit hyb_matrix(viennacl::context)
+    : csr_threshold_(NumericT(0.8)), rows_(0), cols_(0), ellnnz_(0), csrnnz_(0), num_blocks_(0) {}
   hyb_matrix(size_type rows, size_type cols)
-    : csr_threshold_(NumericT(0.8)), rows_(rows), cols_(cols), ellnnz_(0), csrnnz_(0) {}
+    : csr_threshold_(NumericT(0.8)), rows_(rows), cols_(cols), ellnnz_(0), csrnnz_(0), num_blocks_(0) {}
 
   NumericT csr_threshold() const { return csr_threshold_; }
   void csr_threshold(NumericT thr) { csr_threshold_ = thr; }
@@ -46,6 +50,7 @@ public:
   handle_type const& handle3() const { return csr_rows_; }
   handle_type const& handle4() const { return csr_cols_; }
   handle_type const& handle5() const { return csr_elements_; }
+  handle_type const& blocks_handle() const { return blocks_; }
 
   vclhip_hyb descriptor() const
   {
@@ -59,6 +64,8 @@ public:
     d.csr_rows = csr_rows_.template hip_ptr<uint32_t>();
     d.csr_cols = csr_cols_.template hip_ptr<uint32_t>();
     d.csr_elements = csr_elements_.hip_handle();
+    d.num_blocks = num_blocks_;
+    d.blocks = num_blocks_ ? blocks_.template hip_ptr<uint32_t>() : nullptr;
     return d;
   }
 
@@ -83,12 +90,20 @@ public:
                                                 col.data(), dval.data(), &width, &cnnz, ec.data(), ee.data(),
                                                 cr.data(), cc.data(), ce.data()), "hyb_matrix build");
     std::vector<NumericT> eev(ee.begin(), ee.end()), cev(ce.begin(), ce.end());
+    uint32_t longest = 0;
+    for (size_type r = 0; r < rows_; ++r) longest = std::max(longest, cr[r + 1] - cr[r]);
+    std::vector<uint32_t> blocks(rows_ + 1);
+    num_blocks_ = 0;
+    if (longest > 32)
+      backend::hip::check(vclhip_csr_plan_blocks(cr.data(), uint32_t(rows_), blocks.data(), &num_blocks_),
+                          "hyb_matrix plan");
     viennacl::context ctx(HIP_MEMORY);
     backend::memory_create(ell_coords_, sizeof(uint32_t) * ec.size(), ctx, ec.data());
     backend::memory_create(ell_elements_, sizeof(NumericT) * eev.size(), ctx, eev.data());
     backend::memory_create(csr_rows_, sizeof(uint32_t) * cr.size(), ctx, cr.data());
     backend::memory_create(csr_cols_, sizeof(uint32_t) * cc.size(), ctx, cc.data());
     backend::memory_create(csr_elements_, sizeof(NumericT) * cev.size(), ctx, cev.data());
+    if (num_blocks_) backend::memory_create(blocks_, sizeof(uint32_t) * (num_blocks_ + 1), ctx, blocks.data());
   }
 
 private:
@@ -97,11 +112,13 @@ private:
   size_type cols_;
   size_type ellnnz_;
   size_type csrnnz_;
+  uint32_t num_blocks_;
   handle_type ell_coords_;
   handle_type ell_elements_;
   handle_type csr_rows_;
   handle_type csr_cols_;
   handle_type csr_elements_;
+  handle_type blocks_;
 };
 
 /** std::vector<std::map<index, value>> -> HYB (hyb_matrix.hpp:136-255). */
