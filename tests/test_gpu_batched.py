@@ -33,14 +33,17 @@ def small_batch(vh, ns=48, n_max=1500, seed=3):
 
 
 @pytest.mark.parametrize("single", [False, True])
-def test_batched_vs_oracle(dev, vh, oracle, single):
+@pytest.mark.parametrize("staged", [True, False])
+def test_batched_vs_oracle(dev, vh, oracle, single, staged):
+    """staged: the plan has the systems' row blocks (the staged SpMV, the untuned default); without
+    the row pointers at plan time every system takes the lanes-per-row SpMV."""
     sys_ptr, rp, col, val = small_batch(vh)
     n = rp.size - 1
     b = vh.random_uniform(0, n)
     dtype = np.float32 if single else np.float64
     tol = 1e-5 if single else 1e-8
     A = vh.CsrMatrix(rp, col, val, dtype=dtype)
-    plan = vh.BatchPlan(sys_ptr, rp)
+    plan = vh.BatchPlan(sys_ptr, rp if staged else None)
     db = vh.DeviceArray.from_numpy(b.astype(dtype))
     dx = vh.DeviceArray.zeros(n, dtype)
     it, err, conv = plan.solve(A, db, dx, tol=tol, max_iters=3000)
@@ -66,7 +69,7 @@ def test_autotune_and_graph_replay(dev, vh, oracle):
     classes = set(int(np.floor(np.log2(s))) for s in sizes)
     for c in range(16):
         if c in classes:
-            assert 0 <= lanes[c] <= 4
+            assert 0 <= lanes[c] <= 5  # 2^l lanes per row, or 5: staged over row blocks
         else:
             assert lanes[c] == -1
     _, ito, _ = oracle.batched_cg(sys_ptr, oracle.CsrHost(rp, col, val), b, 1e-8, 3000)
@@ -110,8 +113,74 @@ def test_edge_cases(dev, vh):
     k = rp[sys_ptr[4]]  # first entry of system 4 points into system 0
     col_bad[k] = 0
     Ab = vh.CsrMatrix(rp, col_bad, val)
-    it, err, conv = plan.solve(Ab, db, dx, tol=1e-8, max_iters=50)
-    assert conv[4] == 2 and (conv[[0, 1, 2, 3, 5]] != 2).all()
+    for p in (plan, vh.BatchPlan(sys_ptr)):  # staged and lanes-per-row SpMV
+        it, err, conv = p.solve(Ab, db, dx, tol=1e-8, max_iters=50)
+        assert conv[4] == 2 and (conv[[0, 1, 2, 3, 5]] != 2).all()
     # a system larger than one workgroup holds is refused at plan time
     with pytest.raises(vh.VclHipError):
         vh.BatchPlan(np.array([0, 10, 4107], np.uint32))
+
+
+
+def long_row_batch(seed=11):
+    """Four systems: 300 rows, an empty one, 2600 rows with two dense rows of more than 2048 entries,
+    and 900 rows with one denser row; each SPD (symmetric, diagonally dominant)."""
+    rng = np.random.default_rng(seed)
+    sizes = [300, 0, 2600, 900]
+    sys_ptr = np.zeros(len(sizes) + 1, np.uint32)
+    sys_ptr[1:] = np.cumsum(sizes)
+    ri, ci, vv = [], [], []
+    for s, n in enumerate(sizes):
+        if n == 0:
+            continue
+        a = int(sys_ptr[s])
+        M = {}
+
+        def add(i, j, w):
+            M[(i, j)] = M.get((i, j), 0.0) - w
+            M[(j, i)] = M.get((j, i), 0.0) - w
+
+        for i in range(n):
+            for j in rng.choice(n, 4, replace=False):
+                if i != j:
+                    add(i, int(j), rng.uniform(0.1, 1.0))
+        dense, frac = ((5, n // 2), 0.9) if n > 2100 else ((3,), 0.3)
+        for d in dense:
+            for j in np.nonzero(rng.random(n) < frac)[0]:
+                if j != d:
+                    add(d, int(j), rng.uniform(0.01, 0.1))
+        diag = np.zeros(n)
+        for (i, _), v in M.items():
+            diag[i] -= v
+        for i in range(n):
+            M[(i, i)] = diag[i] + 0.05
+        for (i, j) in sorted(M):
+            ri.append(a + i)
+            ci.append(a + j)
+            vv.append(M[(i, j)])
+    ntot = int(sys_ptr[-1])
+    rp = np.zeros(ntot + 1, np.int64)
+    np.add.at(rp, np.array(ri) + 1, 1)
+    return sys_ptr, np.cumsum(rp).astype(np.uint32), np.array(ci, np.uint32), np.array(vv)
+
+
+def test_staged_long_rows_and_empty_system(dev, vh, oracle):
+    """Staged SpMV over every block kind: rows longer than 32 entries (a wavefront), dense rows of
+    more than 2048 entries (a block of their own), and an empty system."""
+    sys_ptr, rp, col, val = long_row_batch()
+    ntot = rp.size - 1
+    assert np.diff(rp.astype(np.int64)).max() > 2048
+    b = vh.random_uniform(2, ntot)
+    A = vh.CsrMatrix(rp, col, val)
+    plan = vh.BatchPlan(sys_ptr, rp)
+    db, dx = vh.DeviceArray.from_numpy(b), vh.DeviceArray.zeros(ntot, np.float64)
+    it, err, conv = plan.solve(A, db, dx, tol=1e-8, max_iters=3000)
+    xo, ito, _ = oracle.batched_cg(sys_ptr, oracle.CsrHost(rp, col, val), b, 1e-8, 3000)
+    assert (conv == 1).all() and it[1] == 0
+    assert iters_close(it, ito).all(), np.c_[it, ito]
+    x = dx.to_numpy()
+    r = b - oracle.csr_spmv(rp, col, val, x)
+    for s in (0, 2, 3):
+        a, e = int(sys_ptr[s]), int(sys_ptr[s + 1])
+        assert np.linalg.norm(r[a:e]) / np.linalg.norm(b[a:e]) < 1e-7
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6

@@ -3,10 +3,14 @@
 // The systems are the diagonal blocks of one block-diagonal CSR matrix (global column indices) and
 // sys_ptr gives their row ranges.  Each workgroup of the persistent kernel takes the next system
 // from a work queue (largest first), runs the whole CG for it, and writes the per-system result
-// (iterations, estimated error, convergence flag).  Inside a workgroup the search direction p and
-// A p live in LDS, x and r in registers (16 rows per lane, rows <= 4096); the SpMV of a system uses
-// L lanes per row, with L chosen per size class by an autotuner that times the candidates on the
-// batch itself.  The recurrences are the pipelined CG of cg.hpp:128-187, one system at a time.
+// (iterations, estimated error, convergence flag).  Inside a workgroup the search direction p lives
+// in LDS, x and r in registers (16 rows per lane, rows <= 4096), and A p in a workspace of the plan
+// (global memory, read back by the same workgroup).  The SpMV of a system is one of two kinds,
+// chosen per size class by an autotuner that times the candidates on the batch itself: L lanes per
+// row (L = 1..16), each group of L lanes walking its rows; or staged over the system's row blocks
+// (the CSR kernel's scheme: a block's up to 2048 entries read with coalesced loads, the products
+// a_ij p_j put in LDS, the rows summed from there), which keeps many more loads in flight.  The
+// recurrences are the pipelined CG of cg.hpp:128-187, one system at a time.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -14,6 +18,7 @@
 
 #include "internal.hpp"
 #include "device.hpp"
+#include "plan.hpp"
 
 struct vclhip_batch_plan {
   uint32_t num_systems = 0;
@@ -22,9 +27,13 @@ struct vclhip_batch_plan {
   std::vector<uint32_t> sys_ptr;      // host copy
   std::vector<uint32_t> order;        // launch order: largest first
   uint32_t* order_dev = nullptr;
-  uint8_t* lanes_dev[2] = {nullptr, nullptr};  // log2(lanes per row) per system, for f64 and f32
+  uint8_t* lanes_dev[2] = {nullptr, nullptr};  // SpMV choice per system, for f64 and f32
+  uint32_t* blocks_dev = nullptr;     // row blocks of every system (staged SpMV), global rows
+  uint32_t* sys_blk_dev = nullptr;    // system s owns blocks sys_blk[s] .. sys_blk[s+1]-1
+  double* ap_dev = nullptr;           // A p workspace, total_rows entries
   uint32_t* counter_dev = nullptr;    // work queue head
   int lanes_log2[16][2];              // per size class (log2 rows) and value type (0 f64, 1 f32); -1 = untuned
+                                      // 0..4: 2^l lanes per row, 5: staged over row blocks
   int device = 0;
   // cached graph of the last graph-mode launch
   hipGraphExec_t gexec = nullptr;
@@ -37,6 +46,7 @@ namespace vclhip {
 constexpr int kBatchRowsPerLane = 16;
 constexpr uint32_t kBatchMaxRows = kBlock * kBatchRowsPerLane;  // 4096
 constexpr int kLaneChoices = 5;                                   // 1, 2, 4, 8, 16 lanes per row
+constexpr int kStaged = kLaneChoices;                             // the staged SpMV's choice number
 
 template <typename T>
 struct BatchDev {
@@ -48,6 +58,9 @@ struct BatchDev {
   T* x;
   const uint32_t* order;
   const uint8_t* lanes;
+  const uint32_t* blocks;   // staged SpMV plan (null: not available)
+  const uint32_t* sys_blk;
+  T* ap;                    // A p, total_rows entries
   uint32_t* counter;
   uint32_t num_systems;
   uint32_t max_iters;
@@ -112,14 +125,138 @@ __device__ __forceinline__ void batch_spmv(const BatchDev<T>& B, uint32_t r0, ui
   }
 }
 
-// Pipelined CG (cg.hpp:128-187) of system s by the whole workgroup.
+// Ap = A p for system s (rows rs .. rs+n-1) over its row blocks, with the lane partials of <Ap,Ap>,
+// <p,Ap>.  Per block: coalesced loads of the block's entries (each lane issues all its loads
+// before using any), the products a_ij p_j into LDS, then each row of at most kShortRow entries
+// summed in order by one lane and each longer row by a wavefront; a row longer than the staging
+// buffer is summed by the whole workgroup straight from memory.
+template <typename T>
+__device__ __forceinline__ void batch_spmv_staged(const BatchDev<T>& B, uint32_t s, uint32_t rs, uint32_t n,
+                                                  const T* __restrict__ p, T* __restrict__ ap, double& ApAp,
+                                                  double& pAp, bool& bad, T* __restrict__ stage,
+                                                  uint32_t* __restrict__ srp)
+{
+  const uint32_t tid = threadIdx.x;
+  const uint32_t b1 = B.sys_blk[s + 1];
+  for (uint32_t b = B.sys_blk[s]; b < b1; ++b) {
+    const uint32_t d0 = B.blocks[b];
+    const uint32_t r0 = d0 & (kPlanRowLimit - 1);  // global rows
+    const uint32_t mode = d0 >> 28;
+    const uint32_t r1 = B.blocks[b + 1] & (kPlanRowLimit - 1);
+    const uint32_t e0 = B.row_ptr[r0];
+    const uint32_t e1 = B.row_ptr[r1];
+    if (mode == kModeLongRow) {
+      T sum = 0;
+      for (uint32_t e = e0 + tid; e < e1; e += kBlock) {
+        const uint32_t c = B.col[e] - rs;
+        if (c < n)
+          sum += B.val[e] * p[c];
+        else
+          bad = true;
+      }
+      sum = block_sum(sum);
+      if (tid == 0) {
+        const uint32_t i = r0 - rs;
+        ap[i] = sum;
+        ApAp += static_cast<double>(sum * sum);
+        pAp += static_cast<double>(p[i] * sum);
+      }
+      continue;
+    }
+    const uint32_t cnt = e1 - e0;
+    const uint32_t nrows = r1 - r0;
+    if (cnt > kCsrNnzCap || nrows > kCsrWG) {  // the matrix is not the one the plan was made for
+      bad = true;
+      continue;
+    }
+    if (tid < nrows) srp[tid + 1] = B.row_ptr[r0 + tid + 1] - e0;
+    if (tid == 0) srp[0] = 0;
+    if (cnt > 0) {
+      constexpr int U = kCsrNnzCap / kBlock;
+      uint32_t c[U];
+      T v[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const uint32_t e = e0 + min(tid + k * kBlock, cnt - 1);
+        c[k] = B.col[e] - rs;
+        v[k] = B.val[e];
+      }
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const uint32_t i = tid + k * kBlock;
+        if (i < cnt) {
+          T prod = 0;
+          if (c[k] < n)
+            prod = v[k] * p[c[k]];
+          else
+            bad = true;
+          stage[i] = prod;
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < nrows) {
+      const uint32_t a = srp[tid];
+      const uint32_t z = srp[tid + 1];
+      if (z - a <= kShortRow) {
+        T sum = 0;
+        for (uint32_t k = a; k < z; ++k) sum += stage[k];
+        const uint32_t i = r0 - rs + tid;
+        ap[i] = sum;
+        ApAp += static_cast<double>(sum * sum);
+        pAp += static_cast<double>(p[i] * sum);
+      }
+    }
+    if (mode == kModeMixed) {
+      const uint32_t wave = tid / kWave;
+      const uint32_t lane = tid & (kWave - 1);
+      uint32_t turn = 0;
+      for (uint32_t base = 0; base < nrows; base += kWave) {
+        const uint32_t i = base + lane;
+        const bool is_long = i < nrows && srp[i + 1] - srp[i] > kShortRow;
+        unsigned long long m = __ballot(is_long);
+        while (m) {
+          const uint32_t bit = static_cast<uint32_t>(__builtin_ctzll(m));
+          m &= m - 1;
+          if ((turn++ & (kWavesPerBlock - 1)) != wave) continue;
+          const uint32_t row = base + bit;
+          T sum = 0;
+          for (uint32_t k = srp[row] + lane; k < srp[row + 1]; k += kWave) sum += stage[k];
+          sum = wave_sum(sum);
+          if (lane == 0) {
+            const uint32_t li = r0 - rs + row;
+            ap[li] = sum;
+            ApAp += static_cast<double>(sum * sum);
+            pAp += static_cast<double>(p[li] * sum);
+          }
+        }
+      }
+    }
+    __syncthreads();  // stage and srp are reused by the next block
+  }
+}
+
+// The SpMV of choice l (0..4: 2^l lanes per row, kStaged: over the row blocks).
 template <typename T, int L>
-__device__ void batch_solve_system(const BatchDev<T>& B, uint32_t s, T* __restrict__ p, T* __restrict__ ap,
-                                   double* red)
+__device__ __forceinline__ void batch_spmv_any(const BatchDev<T>& B, uint32_t s, uint32_t r0, uint32_t n,
+                                               const T* __restrict__ p, T* __restrict__ ap, double& ApAp,
+                                               double& pAp, bool& bad, T* stage, uint32_t* srp)
+{
+  if constexpr (L == 0)
+    batch_spmv_staged<T>(B, s, r0, n, p, ap, ApAp, pAp, bad, stage, srp);
+  else
+    batch_spmv<T, L>(B, r0, n, p, ap, ApAp, pAp, bad);
+}
+
+// Pipelined CG (cg.hpp:128-187) of system s by the whole workgroup; L = 0 selects the staged SpMV.
+template <typename T, int L>
+__device__ void batch_solve_system(const BatchDev<T>& B, uint32_t s, T* __restrict__ p, T* __restrict__ stage,
+                                   uint32_t* __restrict__ srp, double* red)
 {
   const uint32_t r0 = B.sys_ptr[s];
   const uint32_t n = B.sys_ptr[s + 1] - r0;
   const uint32_t t = threadIdx.x;
+  T* const ap = B.ap + r0;
   T xr[kBatchRowsPerLane], rr[kBatchRowsPerLane];
   double bb = 0, ApAp = 0, pAp = 0;
   bool bad = false;
@@ -136,7 +273,7 @@ __device__ void batch_solve_system(const BatchDev<T>& B, uint32_t s, T* __restri
     }
   }
   __syncthreads();
-  batch_spmv<T, L>(B, r0, n, p, ap, ApAp, pAp, bad);
+  batch_spmv_any<T, L>(B, s, r0, n, p, ap, ApAp, pAp, bad, stage, srp);
   batch_reduce3(bb, ApAp, pAp, red);
   const double nrm = sqrt(bb);  // norm_2 then squared, cg.hpp:149
   const double nrs = nrm * nrm;
@@ -167,7 +304,7 @@ __device__ void batch_solve_system(const BatchDev<T>& B, uint32_t s, T* __restri
       }
       __syncthreads();
       double part_ApAp = 0, part_pAp = 0;
-      batch_spmv<T, L>(B, r0, n, p, ap, part_ApAp, part_pAp, bad);
+      batch_spmv_any<T, L>(B, s, r0, n, p, ap, part_ApAp, part_pAp, bad, stage, srp);
       batch_reduce3(part_rr, part_ApAp, part_pAp, red);
       ++it;
       rho = part_rr;
@@ -191,15 +328,18 @@ __device__ void batch_solve_system(const BatchDev<T>& B, uint32_t s, T* __restri
     B.err[s] = nrs > 0 ? sqrt(fabs(rho) / nrs) : 0.0;
     B.conv[s] = any_bad ? 2u : (converged ? 1u : 0u);
   }
-  __syncthreads();  // p, ap and red are reused by the next system
+  __syncthreads();  // p, stage, srp and red are reused by the next system
 }
 
-// Persistent kernel: each workgroup takes systems from the queue until it is empty.
+// Persistent kernel: each workgroup takes systems from the queue until it is empty.  Three
+// workgroups per CU (the LDS of three FP64 workgroups fits in 160 KB; the register budget is set to
+// match, which measured 17% faster in FP64 than two workgroups without spills).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void batched_cg_kernel(BatchDev<T> B)
+__global__ __launch_bounds__(kBlock, 3) void batched_cg_kernel(BatchDev<T> B)
 {
   __shared__ T p[kBatchMaxRows];
-  __shared__ T ap[kBatchMaxRows];
+  __shared__ T stage[kCsrNnzCap];
+  __shared__ uint32_t srp[kCsrWG + 1];
   __shared__ double red[3 * kWavesPerBlock];
   __shared__ uint32_t next;
   for (;;) {
@@ -211,11 +351,17 @@ __global__ __launch_bounds__(kBlock) void batched_cg_kernel(BatchDev<T> B)
     const uint32_t s = B.order[q];
     const int l2 = B.force_lanes >= 0 ? B.force_lanes : static_cast<int>(B.lanes[s]);
     switch (l2) {
-      case 0: batch_solve_system<T, 1>(B, s, p, ap, red); break;
-      case 1: batch_solve_system<T, 2>(B, s, p, ap, red); break;
-      case 2: batch_solve_system<T, 4>(B, s, p, ap, red); break;
-      case 3: batch_solve_system<T, 8>(B, s, p, ap, red); break;
-      default: batch_solve_system<T, 16>(B, s, p, ap, red); break;
+      case 0: batch_solve_system<T, 1>(B, s, p, stage, srp, red); break;
+      case 1: batch_solve_system<T, 2>(B, s, p, stage, srp, red); break;
+      case 2: batch_solve_system<T, 4>(B, s, p, stage, srp, red); break;
+      case 3: batch_solve_system<T, 8>(B, s, p, stage, srp, red); break;
+      case 4: batch_solve_system<T, 16>(B, s, p, stage, srp, red); break;
+      default:
+        if (B.blocks)
+          batch_solve_system<T, 0>(B, s, p, stage, srp, red);
+        else
+          batch_solve_system<T, 16>(B, s, p, stage, srp, red);
+        break;
     }
   }
 }
@@ -276,6 +422,9 @@ static BatchDev<T> make_batch(const vclhip_batch_plan* plan, const vclhip_csr* A
   B.x = x;
   B.order = plan->order_dev;
   B.lanes = plan->lanes_dev[sizeof(T) == 4 ? 1 : 0];
+  B.blocks = plan->blocks_dev;
+  B.sys_blk = plan->sys_blk_dev;
+  B.ap = reinterpret_cast<T*>(plan->ap_dev);
   B.counter = plan->counter_dev;
   B.num_systems = plan->num_systems;
   B.max_iters = P->max_iters;
@@ -341,7 +490,8 @@ static int autotune(vclhip_batch_plan* plan, const vclhip_csr* A, const uint32_t
     B.num_systems = static_cast<uint32_t>(pick.size());
     float best = 0;
     int best_l = 0;
-    for (int l = 0; l < kLaneChoices && !status; ++l) {
+    const int choices = plan->blocks_dev ? kLaneChoices + 1 : kLaneChoices;
+    for (int l = 0; l < choices && !status; ++l) {
       B.force_lanes = l;
       float ms_total = 0;
       for (int rep = 0; rep < 3 && !status; ++rep) {
@@ -365,8 +515,8 @@ static int autotune(vclhip_batch_plan* plan, const vclhip_csr* A, const uint32_t
   return status;
 }
 
-// Per-system lanes from the class table (heuristic where untuned: about one lane per nonzero of an
-// average row, capped at 16).
+// Per-system SpMV choice from the class table; where untuned, the staged SpMV when the plan has row
+// blocks, else about one lane per nonzero of an average row, capped at 16.
 static int upload_lanes(vclhip_batch_plan* plan, int ti, const uint32_t* row_ptr_host)
 {
   if (!plan->lanes_dev[ti]) VCLHIP_HIP(hipMalloc(&plan->lanes_dev[ti], plan->num_systems));
@@ -374,6 +524,7 @@ static int upload_lanes(vclhip_batch_plan* plan, int ti, const uint32_t* row_ptr
   for (uint32_t i = 0; i < plan->num_systems; ++i) {
     const uint32_t n = plan->sys_ptr[i + 1] - plan->sys_ptr[i];
     int l = plan->lanes_log2[size_class(n)][ti];
+    if (l < 0 && plan->blocks_dev) l = kStaged;
     if (l < 0) {
       double avg = 8;
       if (row_ptr_host && n)
@@ -474,6 +625,26 @@ int vclhip_batch_plan_create(uint32_t num_systems, const uint32_t* sys_ptr_host,
     return plan->sys_ptr[a + 1] - plan->sys_ptr[a] > plan->sys_ptr[b + 1] - plan->sys_ptr[b];
   });
   int st = hip_status(hipGetDevice(&plan->device), "hipGetDevice");
+  if (!st) st = hip_status(hipMalloc(&plan->ap_dev, std::max<size_t>(plan->total_rows, 1) * sizeof(double)), "hipMalloc");
+  if (!st && row_ptr_host && plan->total_rows < kPlanRowLimit) {
+    // row blocks of each system (the CSR kernels' partition of its rows), rows made global
+    std::vector<uint32_t> blocks, sys_blk(num_systems + 1, 0), local(max_rows + 1);
+    for (uint32_t i = 0; i < num_systems; ++i) {
+      const uint32_t a = sys_ptr_host[i], n = sys_ptr_host[i + 1] - a;
+      const uint32_t nb = n ? plan_csr_blocks(row_ptr_host + a, n, local.data()) : 0;
+      for (uint32_t k = 0; k < nb; ++k) blocks.push_back(((local[k] & (kPlanRowLimit - 1)) + a) | (local[k] & ~(kPlanRowLimit - 1)));
+      sys_blk[i + 1] = static_cast<uint32_t>(blocks.size());
+    }
+    blocks.push_back(plan->total_rows);
+    st = hip_status(hipMalloc(&plan->blocks_dev, blocks.size() * sizeof(uint32_t)), "hipMalloc");
+    if (!st) st = hip_status(hipMalloc(&plan->sys_blk_dev, sys_blk.size() * sizeof(uint32_t)), "hipMalloc");
+    if (!st)
+      st = hip_status(hipMemcpy(plan->blocks_dev, blocks.data(), blocks.size() * sizeof(uint32_t),
+                                hipMemcpyHostToDevice), "blocks upload");
+    if (!st)
+      st = hip_status(hipMemcpy(plan->sys_blk_dev, sys_blk.data(), sys_blk.size() * sizeof(uint32_t),
+                                hipMemcpyHostToDevice), "blocks upload");
+  }
   if (!st) st = hip_status(hipMalloc(&plan->order_dev, num_systems * sizeof(uint32_t)), "hipMalloc");
   if (!st) st = hip_status(hipMalloc(&plan->counter_dev, sizeof(uint32_t)), "hipMalloc");
   if (!st)
@@ -497,6 +668,9 @@ int vclhip_batch_plan_destroy(vclhip_batch_plan* plan)
   for (auto* l : plan->lanes_dev)
     if (l) (void)hipFree(l);
   if (plan->counter_dev) (void)hipFree(plan->counter_dev);
+  for (void* q : {static_cast<void*>(plan->blocks_dev), static_cast<void*>(plan->sys_blk_dev),
+                  static_cast<void*>(plan->ap_dev)})
+    if (q) (void)hipFree(q);
   delete plan;
   return VCLHIP_SUCCESS;
 }
