@@ -543,18 +543,24 @@ int vclhip_gmres_solve_f32(const vclhip_matrix* A, const float* b_dev, float* x_
  * system has at most 4096 rows (VCLHIP_ERROR_TOO_LARGE otherwise).  A column outside a system's
  * block sets its converged flag to 2. */
 typedef struct vclhip_batch_plan vclhip_batch_plan;
-/* Launch order and lanes-per-row choice of a batch (host arrays sys_ptr, num_systems+1 entries,
- * and optionally row_ptr for the untuned heuristic).  Allocates device memory. */
+/* Launch order and SpMV choice of a batch (host arrays sys_ptr, num_systems+1 entries, and
+ * optionally row_ptr).  With row_ptr the plan also holds the row blocks of every system (the CSR
+ * kernels' partition, vclhip_csr_plan_blocks), which the staged SpMV reads, and that SpMV is the
+ * choice where untuned; the batch matrix passed to the solves must then have these row pointers.
+ * Without row_ptr every system takes L lanes per row (about one lane per nonzero of an average
+ * row).  Allocates device memory, among it an A p workspace of total rows doubles: one solve at a
+ * time per plan. */
 int vclhip_batch_plan_create(uint32_t num_systems, const uint32_t* sys_ptr_host, const uint32_t* row_ptr_host,
                              vclhip_batch_plan** plan);
 int vclhip_batch_plan_destroy(vclhip_batch_plan* plan);
-/* Autotuner: for every size class (floor(log2 rows)) times 1, 2, 4, 8 and 16 lanes per row on up to
- * 512 of the batch's own systems (12 iterations, 3 repetitions) and keeps the fastest, for the
- * value type given (single_precision = 0: double, 1: float).  A is the batch matrix (values of
- * that type). */
+/* Autotuner: for every size class (floor(log2 rows)) times 1, 2, 4, 8 and 16 lanes per row and,
+ * when the plan has row blocks, the staged SpMV on up to 512 of the batch's own systems
+ * (12 iterations, 3 repetitions) and keeps the fastest, for the value type given
+ * (single_precision = 0: double, 1: float).  A is the batch matrix (values of that type). */
 int vclhip_batch_plan_autotune(vclhip_batch_plan* plan, const vclhip_csr* A, const uint32_t* sys_ptr_dev,
                                int single_precision, void* stream);
-/* The tuned log2(lanes per row) per size class (16 entries, -1 = untuned). */
+/* The tuned choice per size class (16 entries): 0..4 = log2(lanes per row), 5 = staged over row
+ * blocks, -1 = untuned. */
 int vclhip_batch_plan_lanes(const vclhip_batch_plan* plan, int single_precision, int32_t* lanes_log2_by_class);
 /* Solves every system: per-system relative tolerance params->tol (and abs_tol) on the recursive
  * residual, params->max_iters; iters/errors/converged are device arrays of num_systems entries
