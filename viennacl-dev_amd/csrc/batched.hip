@@ -196,8 +196,9 @@ __device__ __forceinline__ void batch_spmv_staged(const BatchDev<T>& B, uint32_t
     }
     __syncthreads();
     if (tid < nrows) {
-      const uint32_t a = srp[tid];
-      const uint32_t z = srp[tid + 1];
+      // clamped to the block: row pointers that disagree with the plan cannot read past the stage
+      const uint32_t z = min(srp[tid + 1], cnt);
+      const uint32_t a = min(srp[tid], z);
       if (z - a <= kShortRow) {
         T sum = 0;
         for (uint32_t k = a; k < z; ++k) sum += stage[k];
@@ -213,15 +214,16 @@ __device__ __forceinline__ void batch_spmv_staged(const BatchDev<T>& B, uint32_t
       uint32_t turn = 0;
       for (uint32_t base = 0; base < nrows; base += kWave) {
         const uint32_t i = base + lane;
-        const bool is_long = i < nrows && srp[i + 1] - srp[i] > kShortRow;
+        const bool is_long = i < nrows && min(srp[i + 1], cnt) - min(srp[i], min(srp[i + 1], cnt)) > kShortRow;
         unsigned long long m = __ballot(is_long);
         while (m) {
           const uint32_t bit = static_cast<uint32_t>(__builtin_ctzll(m));
           m &= m - 1;
           if ((turn++ & (kWavesPerBlock - 1)) != wave) continue;
           const uint32_t row = base + bit;
+          const uint32_t z = min(srp[row + 1], cnt);
           T sum = 0;
-          for (uint32_t k = srp[row] + lane; k < srp[row + 1]; k += kWave) sum += stage[k];
+          for (uint32_t k = min(srp[row], z) + lane; k < z; k += kWave) sum += stage[k];
           sum = wave_sum(sum);
           if (lane == 0) {
             const uint32_t li = r0 - rs + row;
