@@ -333,11 +333,11 @@ __device__ void batch_solve_system(const BatchDev<T>& B, uint32_t s, T* __restri
   __syncthreads();  // p, stage, srp and red are reused by the next system
 }
 
-// Persistent kernel: each workgroup takes systems from the queue until it is empty.  Three
-// workgroups per CU (the LDS of three FP64 workgroups fits in 160 KB; the register budget is set to
-// match, which measured 17% faster in FP64 than two workgroups without spills).
+// Persistent kernel: each workgroup takes systems from the queue until it is empty.  The register
+// budget matches what LDS allows: three FP64 workgroups per CU (50 KB each; 17% faster than two),
+// four FP32 workgroups (16% faster than three; LDS would allow six, registers would spill).
 template <typename T>
-__global__ __launch_bounds__(kBlock, 3) void batched_cg_kernel(BatchDev<T> B)
+__global__ __launch_bounds__(kBlock, (sizeof(T) == 4 ? 4 : 3)) void batched_cg_kernel(BatchDev<T> B)
 {
   __shared__ T p[kBatchMaxRows];
   __shared__ T stage[kCsrNnzCap];
