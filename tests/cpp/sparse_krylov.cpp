@@ -21,7 +21,9 @@
 #include "viennacl/linalg/inner_prod.hpp"
 #include "viennacl/linalg/jacobi_precond.hpp"
 #include "viennacl/linalg/mixed_precision_cg.hpp"
+#include "viennacl/linalg/norm_1.hpp"
 #include "viennacl/linalg/norm_2.hpp"
+#include "viennacl/linalg/norm_inf.hpp"
 #include "viennacl/linalg/prod.hpp"
 #include "viennacl/sliced_ell_matrix.hpp"
 #include "viennacl/tools/matrix_generation.hpp"
@@ -529,6 +531,45 @@ void test_matrix_market()
   std::remove(file);
 }
 
+/* norm_1, norm_inf, index_norm_inf, plane_rotation and swap against host loops
+ * (tests/src/vector_float_double.cpp checks the same operations against uBLAS). */
+template <typename T>
+void test_vector_extras(double eps)
+{
+  const size_t n = 10007;
+  std::mt19937_64 rng(42);
+  std::uniform_real_distribution<double> u(-1.0, 1.0);
+  std::vector<T> hx(n), hy(n);
+  for (size_t i = 0; i < n; ++i) {
+    hx[i] = T(u(rng));
+    hy[i] = T(u(rng));
+  }
+  hx[777] = T(-3);  // the largest |x_i|, first of two
+  hx[9000] = T(3);
+  viennacl::vector<T> x(n), y(n);
+  viennacl::copy(hx, x);
+  viennacl::copy(hy, y);
+  double n1 = 0;
+  for (size_t i = 0; i < n; ++i) n1 += std::fabs(double(hx[i]));
+  CHECK(std::fabs(double(viennacl::linalg::norm_1(x)) - n1) <= eps * n1 * 10, "norm_1");
+  CHECK(viennacl::linalg::index_norm_inf(x) == 777, "index_norm_inf %lu", (unsigned long)viennacl::linalg::index_norm_inf(x));
+  CHECK(double(viennacl::linalg::norm_inf(x)) == 3.0, "norm_inf");
+  viennacl::vector<T> z(n);
+  z.clear();
+  CHECK(viennacl::linalg::norm_inf(z) == T(0) && viennacl::linalg::index_norm_inf(z) == 0, "norm_inf of zeros");
+  const T c = T(0.6), d = T(0.8);
+  viennacl::linalg::plane_rotation(x, y, c, d);
+  std::vector<T> rx(n), ry(n);
+  for (size_t i = 0; i < n; ++i) {
+    rx[i] = c * hx[i] + d * hy[i];
+    ry[i] = c * hy[i] - d * hx[i];
+  }
+  CHECK(rel_inf(to_host(x), rx) < eps && rel_inf(to_host(y), ry) < eps, "plane_rotation");
+  const std::vector<T> sx = to_host(x), sy = to_host(y);
+  viennacl::swap(x, y);
+  CHECK(to_host(x) == sy && to_host(y) == sx, "swap");
+}
+
 int main()
 {
   int ndev = 0;
@@ -544,6 +585,8 @@ int main()
   test_gmres<float>(1e-5, 1e-4);
   test_formats<double>(1e-13, 1e-10, 1e-9);
   test_formats<float>(1e-5, 1e-5, 1e-4);
+  test_vector_extras<double>(1e-13);
+  test_vector_extras<float>(1e-6);
   test_mixed_precision();
   test_matrix_market();
   if (failures) {

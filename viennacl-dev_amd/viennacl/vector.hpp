@@ -58,6 +58,12 @@ template <> struct abi<double>
                  double* r) { return vclhip_dot_f64(n, x, xs, xi, y, ys, yi, r, stream()); }
   static int norm2(uint32_t n, const double* x, uint32_t s, uint32_t i, double* r) { return vclhip_norm2_f64(n, x, s, i, r, stream()); }
   static int ediv(uint32_t n, const double* x, const double* y, double* z) { return vclhip_element_div_f64(n, x, y, z, stream()); }
+  static int asum(uint32_t n, const double* x, uint32_t s, uint32_t i, double* r) { return vclhip_asum_f64(n, x, s, i, r, stream()); }
+  static int iamax(uint32_t n, const double* x, uint32_t s, uint32_t i, uint32_t* r) { return vclhip_iamax_f64(n, x, s, i, r, stream()); }
+  static int swap(uint32_t n, double* x, uint32_t xs, uint32_t xi, double* y, uint32_t ys, uint32_t yi)
+  { return vclhip_swap_f64(n, x, xs, xi, y, ys, yi, stream()); }
+  static int rot(uint32_t n, double* x, uint32_t xs, uint32_t xi, double* y, uint32_t ys, uint32_t yi, double c, double d)
+  { return vclhip_rot_f64(n, x, xs, xi, y, ys, yi, c, d, stream()); }
 };
 template <> struct abi<float>
 {
@@ -66,6 +72,12 @@ template <> struct abi<float>
                  float* r) { return vclhip_dot_f32(n, x, xs, xi, y, ys, yi, r, stream()); }
   static int norm2(uint32_t n, const float* x, uint32_t s, uint32_t i, float* r) { return vclhip_norm2_f32(n, x, s, i, r, stream()); }
   static int ediv(uint32_t n, const float* x, const float* y, float* z) { return vclhip_element_div_f32(n, x, y, z, stream()); }
+  static int asum(uint32_t n, const float* x, uint32_t s, uint32_t i, float* r) { return vclhip_asum_f32(n, x, s, i, r, stream()); }
+  static int iamax(uint32_t n, const float* x, uint32_t s, uint32_t i, uint32_t* r) { return vclhip_iamax_f32(n, x, s, i, r, stream()); }
+  static int swap(uint32_t n, float* x, uint32_t xs, uint32_t xi, float* y, uint32_t ys, uint32_t yi)
+  { return vclhip_swap_f32(n, x, xs, xi, y, ys, yi, stream()); }
+  static int rot(uint32_t n, float* x, uint32_t xs, uint32_t xi, float* y, uint32_t ys, uint32_t yi, float c, float d)
+  { return vclhip_rot_f32(n, x, xs, xi, y, ys, yi, c, d, stream()); }
 };
 }  // namespace detail
 
@@ -583,7 +595,67 @@ auto norm_2(vector_expression<L, R, O> const& e) -> decltype(norm_2(vector_base<
   tmp = e;
   return norm_2(tmp);
 }
+
+/** norm_1(x) = sum |x_i| (linalg/norm_1.hpp:76-86; host_based/vector_operations.hpp:708-733). */
+template <typename T>
+T norm_1(vector_base<T> const& x)
+{
+  x.require_hip();
+  T r = 0;
+  backend::hip::check(viennacl::detail::abi<T>::asum(uint32_t(x.size()), x.device_ptr(), uint32_t(x.start()),
+                                                     uint32_t(x.stride()), &r),
+                      "norm_1");
+  return r;
+}
+
+/** index_norm_inf(x): index of the first largest |x_i| (linalg/vector_operations.hpp:952-967;
+ *  host_based/vector_operations.hpp:891-950, which returns the vector's start offset when no entry
+ *  is above zero). */
+template <typename T>
+vcl_size_t index_norm_inf(vector_base<T> const& x)
+{
+  x.require_hip();
+  uint32_t r = 0;
+  backend::hip::check(viennacl::detail::abi<T>::iamax(uint32_t(x.size()), x.device_ptr(), uint32_t(x.start()),
+                                                      uint32_t(x.stride()), &r),
+                      "index_norm_inf");
+  return r;
+}
+
+/** norm_inf(x) = max |x_i| (linalg/norm_inf.hpp:78-88); 0 for an empty or all-zero vector. */
+template <typename T>
+T norm_inf(vector_base<T> const& x)
+{
+  const vcl_size_t i = index_norm_inf(x);
+  if (i >= x.size()) return T(0);
+  return std::fabs(x.get_entry(i));
+}
+
+/** (x, y) <- (alpha x + beta y, alpha y - beta x) (plane_rotation, linalg/vector_operations.hpp:1259-1280;
+ *  host_based/vector_operations.hpp:1080-1110). */
+template <typename T>
+void plane_rotation(vector_base<T>& x, vector_base<T>& y, T alpha, T beta)
+{
+  x.require_hip();
+  y.check_size(x.size());
+  backend::hip::check(viennacl::detail::abi<T>::rot(uint32_t(x.size()), x.device_ptr(), uint32_t(x.start()),
+                                                    uint32_t(x.stride()), y.device_ptr(), uint32_t(y.start()),
+                                                    uint32_t(y.stride()), alpha, beta),
+                      "plane_rotation");
+}
 }  // namespace linalg
+
+/** swap(x, y): exchanges the entries of two vectors of one size (vector.hpp:1648-1660, vector_swap). */
+template <typename T>
+void swap(vector_base<T>& x, vector_base<T>& y)
+{
+  x.require_hip();
+  y.check_size(x.size());
+  backend::hip::check(viennacl::detail::abi<T>::swap(uint32_t(x.size()), x.device_ptr(), uint32_t(x.start()),
+                                                     uint32_t(x.stride()), y.device_ptr(), uint32_t(y.start()),
+                                                     uint32_t(y.stride())),
+                      "swap");
+}
 
 // ------------------------------------------------------------------ host <-> device copies
 /** copy(std::vector -> vector) (vector.hpp copy overloads); an empty target is sized. */
