@@ -467,8 +467,9 @@ def adjoint_bench(vh, grid, n_fd=64):
     return out
 
 
-def cpu_baseline(rp, col, val, x, n, nnz, seconds):
-    """The reference host backend (OpenMP) timed on this machine's cores over a bounded sample."""
+def cpu_baseline(rp, col, val, x, n, nnz, seconds, c1=None):
+    """The reference host backend (OpenMP) timed on this machine's cores over a bounded sample; with
+    c1 = (rp, col, val, b), also the reference's own CG solve of the 256x256 case."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as po
     if not po.ref_available():
@@ -479,9 +480,44 @@ def cpu_baseline(rp, col, val, x, n, nnz, seconds):
     reps = max(3, int(seconds / max(one, 1e-6)))
     t = R.ref_bench_csr_spmv(n, po.p(rp), po.p(col), po.p(val), po.p(x), po.p(y), reps)
     gbs = csr_bytes(n, nnz) * reps / t / 1e9
-    return {"value": round(gbs, 2), "unit": "GB/s", "cores": int(R.ref_max_threads()), "kind": "reference",
-            "sample": f"{reps} FP64 CSR SpMVs y=Ax of the same 128^3 Laplacian (viennacl::linalg::prod, "
-                      f"host_based OpenMP backend), {t:.1f} s"}, y
+    out = {"value": round(gbs, 2), "unit": "GB/s", "cores": int(R.ref_max_threads()), "kind": "reference",
+           "sample": f"{reps} FP64 CSR SpMVs y=Ax of the same 128^3 Laplacian (viennacl::linalg::prod, "
+                     f"host_based OpenMP backend), {t:.1f} s"}
+    if c1 is not None:
+        crp, ccol, cval, cb = (np.ascontiguousarray(a) for a in c1)
+        cx = np.zeros(crp.size - 1)
+        it, err = C.c_uint32(), C.c_double()
+        best = None
+        for _ in range(3):
+            ts = R.ref_bench_solve(crp.size - 1, po.p(crp), po.p(ccol), po.p(cval), po.p(cb), po.p(cx), 1e-8, 2000,
+                                   0, 0, C.byref(it), C.byref(err))
+            best = ts if best is None else min(best, ts)
+        out["c1_cg_fdm_256"] = {"seconds": round(best, 6), "iters": int(it.value), "error": float(err.value),
+                                "sample": "solve(A, b, cg_tag(1e-8, 2000)), generate_fdm_laplace(256, 256), "
+                                          "best of 3"}
+    return out, y
+
+
+def c1_solve_bench(vh):
+    """The reference's CPU-runnable case on the device: generate_fdm_laplace(256, 256) (n = 65,536,
+    nnz = 326,656), b ~ U[-1,1] (seed 0), cg_tag(1e-8, 2000).  5 MB per SpMV stays in the caches,
+    so the device solve is bound by launch latency, not bandwidth; hipGraph chunks take most of the
+    launches off the host."""
+    rp, col, val = vh.fdm_laplace_2d(256, 256)
+    n = rp.size - 1
+    bh = vh.random_uniform(0, n)
+    A = vh.CsrMatrix(rp, col, val)
+    b = vh.DeviceArray.from_numpy(bh)
+    rec = {"workload": f"generate_fdm_laplace(256, 256) (n={n}, nnz={int(rp[-1])}), CG to 1e-8", "n": n,
+           "nnz": int(rp[-1])}
+    rec["cg_fp64_csr"] = timed_solve(vh, "cg", A, b, n, reps=3, tol=1e-8, max_iters=2000)
+    rec["cg_fp64_csr_graph"] = timed_solve(vh, "cg", A, b, n, reps=3, tol=1e-8, max_iters=2000, use_graph=True)
+    rec["cg_fp64_csr_graph_chunk64"] = timed_solve(vh, "cg", A, b, n, reps=3, tol=1e-8, max_iters=2000,
+                                                   use_graph=True, chunk=64)
+    log("c1 cg", rec)
+    b.free()
+    A.free()
+    return rec, (rp, col, val, bh)
 
 
 def device_banner(vh):
@@ -606,6 +642,7 @@ def main():
         for k, v in sol.items():
             log(k, v)
         out["solvers_lap3d_128"] = sol
+        out["c1_cg_fdm_256"], c1_inputs = c1_solve_bench(vh)
         out["powerlaw_2M"] = powerlaw_bench(vh, stream, args.steps)
     if args.mixed_grid:
         out["mixed_vs_fp64"] = mixed_vs_fp64(vh, args.mixed_grid)
@@ -626,7 +663,7 @@ def main():
 
     if not args.no_cpu:
         log("CPU baseline (reference host backend) ...")
-        cb = cpu_baseline(rp, col, val, xh, n, nnz, args.cpu_seconds)
+        cb = cpu_baseline(rp, col, val, xh, n, nnz, args.cpu_seconds, None if args.no_solvers else c1_inputs)
         if cb is not None:
             base, y_cpu = cb
             out["cpu_baseline"] = base
