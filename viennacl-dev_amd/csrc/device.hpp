@@ -90,6 +90,56 @@ __device__ __forceinline__ A block_sum_partials(const A* part, uint32_t count)
   return block_sum(s);
 }
 
+// K block_sum_partials in one pass: lane t adds p[q][t], p[q][t+256], ... in the same order as
+// block_sum_partials, and the workgroup combines in the same tree, so the results equal K separate
+// calls; the K loads of a step are in flight together and the workgroup synchronizes once.
+template <int K, typename A>
+__device__ __forceinline__ void block_sum_partials_k(const A* const (&p)[K], const uint32_t (&count)[K], A (&out)[K])
+{
+  __shared__ A red[K][kWavesPerBlock];
+  uint32_t nmax = 0;
+#pragma unroll
+  for (int q = 0; q < K; ++q) nmax = count[q] > nmax ? count[q] : nmax;
+  A s[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) s[q] = 0;
+  for (uint32_t i = threadIdx.x; i < nmax; i += kBlock) {
+    A v[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) v[q] = i < count[q] ? p[q][i] : A(0);
+#pragma unroll
+    for (int q = 0; q < K; ++q)
+      if (i < count[q]) s[q] += v[q];
+  }
+  const int wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    s[q] = wave_sum(s[q]);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[q][wave] = s[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    out[q] = red[q][0];
+#pragma unroll
+    for (int w = 1; w < kWavesPerBlock; ++w) out[q] += red[q][w];
+  }
+  __syncthreads();
+}
+
+template <typename A>
+__device__ __forceinline__ void block_sum_partials3(const A* pa, uint32_t na, const A* pb, uint32_t nb,
+                                                    const A* pc, uint32_t nc, A& ra, A& rb, A& rc)
+{
+  const A* const p[3] = {pa, pb, pc};
+  const uint32_t count[3] = {na, nb, nc};
+  A out[3];
+  block_sum_partials_k<3, A>(p, count, out);
+  ra = out[0];
+  rb = out[1];
+  rc = out[2];
+}
+
 // [begin, end) of the lw-th of G chunks of n elements with both ends on multiples of W (the last
 // chunk also takes the tail), so that W-wide vector accesses of a W*sizeof(T)-aligned array stay
 // aligned.

@@ -33,6 +33,8 @@ enum : uint32_t {
 };
 
 constexpr uint32_t kSlots = 8;  // partial-sum slots of kMaxPartials doubles each
+// Compact CG: at most this many partials read by all update workgroups together per iteration.
+constexpr uint64_t kCompactPartialReads = 65536;
 
 struct KState {
   double alpha, beta, omega;
@@ -55,6 +57,7 @@ struct KState {
   float inner, initial;  // reference schedule: FP32 <r,r> and its value at the last restart
   float inner_tol;
   uint32_t updates;      // FP64 corrections done
+  uint32_t iter2[2];     // CG, compact mode: iteration count, read from [k & 1] and written to the other
 };
 
 __host__ __device__ __forceinline__ double* slot_ptr(double* part, uint32_t slot) { return part + slot * kMaxPartials; }
@@ -171,9 +174,8 @@ template <bool Jac>
 __global__ __launch_bounds__(kBlock) void cg_setup_kernel(KState* __restrict__ st, const double* __restrict__ part,
                                                           uint32_t g_vec, uint32_t g_prod)
 {
-  const double bb = block_sum_partials(slot_ptr(part, 0), g_vec);
-  const double ApAp = block_sum_partials(slot_ptr(part, 1), g_prod);
-  const double pAp = block_sum_partials(slot_ptr(part, 2), g_prod);
+  double bb, ApAp, pAp;
+  block_sum_partials3(slot_ptr(part, 0), g_vec, slot_ptr(part, 1), g_prod, slot_ptr(part, 2), g_prod, bb, ApAp, pAp);
   if (threadIdx.x != 0) return;
   double nrs;
   if (Jac) {
@@ -247,9 +249,8 @@ __global__ __launch_bounds__(kBlock) void cg_iter_kernel(KState* __restrict__ st
                                                          uint32_t g_vec, uint32_t g_prod)
 {
   if (st->flags & kFlagDone) return;
-  const double rr = block_sum_partials(slot_ptr(part, 0), g_vec);
-  const double ApAp = block_sum_partials(slot_ptr(part, 1), g_prod);
-  const double pAp = block_sum_partials(slot_ptr(part, 2), g_prod);
+  double rr, ApAp, pAp;
+  block_sum_partials3(slot_ptr(part, 0), g_vec, slot_ptr(part, 1), g_prod, slot_ptr(part, 2), g_prod, rr, ApAp, pAp);
   if (threadIdx.x != 0) return;
   const uint32_t iter = st->iter + 1;
   st->iter = iter;
@@ -267,6 +268,72 @@ __global__ __launch_bounds__(kBlock) void cg_iter_kernel(KState* __restrict__ st
   const double alpha = rr / pAp;
   st->alpha = alpha;
   st->beta = (alpha * alpha * ApAp - rr) / rr;
+}
+
+// Compact CG step for small systems, one kernel instead of two: the scalar step of iteration k
+// (cg.hpp:170-180; every workgroup reduces the few partials itself, in the same fixed order, so
+// all of them take the same decision and the same alpha, beta) followed by the vector update of
+// iteration k + 1.  The <r,z> partials of update k are in slot 0 or 3 by the parity of k and the
+// update writes the other one; likewise the iteration count, so that no workgroup reads what
+// another one writes in this kernel.  Workgroup 0 stores the state.
+template <typename T, bool Jac, int W>
+__global__ __launch_bounds__(kBlock) void cg_step_kernel(uint32_t n, T* __restrict__ x, T* __restrict__ p,
+                                                         T* __restrict__ r, const T* __restrict__ Ap,
+                                                         const T* __restrict__ d, KState* __restrict__ st,
+                                                         double* __restrict__ part, uint32_t par, uint32_t g_vec,
+                                                         uint32_t g_prod)
+{
+  if (st->flags & kFlagDone) return;
+  double rr, ApAp, pAp;
+  block_sum_partials3(slot_ptr(part, par ? 3u : 0u), g_vec, slot_ptr(part, 1), g_prod, slot_ptr(part, 2), g_prod,
+                      rr, ApAp, pAp);
+  const uint32_t iter = st->iter2[par] + 1;
+  const bool conv = fabs(rr / st->nrs) < st->tol * st->tol || fabs(rr) < st->abs_tol * st->abs_tol;
+  const bool done = conv || iter >= st->max_iters;
+  const double alpha = rr / pAp;
+  const double beta = (alpha * alpha * ApAp - rr) / rr;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->iter = iter;
+    st->iter2[par ^ 1u] = iter;
+    st->rr = rr;
+    record_history(st, iter, rr);
+    if (done) {
+      if (conv) st->converged = 1;
+      st->flags |= kFlagDone;
+    } else {
+      st->alpha = alpha;
+      st->beta = beta;
+    }
+  }
+  if (done) return;
+  const T ta = static_cast<T>(alpha);
+  const T tb = static_cast<T>(beta);
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  double acc[1] = {0};
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    Pack<V, T> vp = ldp<V>(p, i);
+    Pack<V, T> vr = ldp<V>(r, i);
+    Pack<V, T> vx = ldp<V>(x, i);
+    const Pack<V, T> va = ldp<V>(Ap, i);
+    Pack<V, T> vd;
+    if constexpr (Jac) vd = ldp<V>(d, i);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      vx.v[k] += ta * vp.v[k];
+      vr.v[k] -= ta * va.v[k];
+      T z = vr.v[k];
+      if constexpr (Jac) z = vr.v[k] / vd.v[k];
+      vp.v[k] = z + tb * vp.v[k];
+      acc[0] += static_cast<double>(vr.v[k] * z);
+    }
+    stp<V>(x, i, vx);
+    stp<V>(p, i, vp);
+    stp<V>(r, i, vr);
+  });
+  block_store_partials<1, double>(acc, slot_ptr(part, par ? 0u : 3u), kMaxPartials, lw);
 }
 
 // ------------------------------------------------------------------ BiCGStab kernels
@@ -307,8 +374,9 @@ __global__ __launch_bounds__(kBlock) void bicg_init_kernel(uint32_t n, const T* 
 __global__ __launch_bounds__(kBlock) void bicg_setup_kernel(KState* __restrict__ st, const double* __restrict__ part,
                                                             uint32_t g_vec)
 {
-  const double bb = block_sum_partials(slot_ptr(part, 0), g_vec);
-  const double vv = block_sum_partials(slot_ptr(part, 1), g_vec);
+  double sums[2];
+  block_sum_partials_k<2, double>({slot_ptr(part, 0), slot_ptr(part, 1)}, {g_vec, g_vec}, sums);
+  const double bb = sums[0], vv = sums[1];
   if (threadIdx.x != 0) return;
   const double norm_rhs = sqrt(bb);
   const double nv = sqrt(vv);
@@ -337,8 +405,9 @@ __global__ __launch_bounds__(kBlock) void bicg_alpha_kernel(KState* __restrict__
     return;
   }
   const bool from_part = st->rr_from_partials != 0;
-  const double rsum = block_sum_partials(slot_ptr(part, 0), from_part ? g_vec : 0u);
-  const double ap_r0 = block_sum_partials(slot_ptr(part, 2), g_prod);
+  double sums[2];
+  block_sum_partials_k<2, double>({slot_ptr(part, 0), slot_ptr(part, 2)}, {from_part ? g_vec : 0u, g_prod}, sums);
+  const double rsum = sums[0], ap_r0 = sums[1];
   if (threadIdx.x != 0) return;
   const double rr = from_part ? rsum : st->rr;
   st->rr = rr;
@@ -383,10 +452,10 @@ __global__ __launch_bounds__(kBlock) void bicg_scalars_kernel(KState* __restrict
                                                               uint32_t g_vec, uint32_t g_prod, int detect_breakdown)
 {
   if (st->flags & (kFlagDone | kFlagFinishing)) return;
-  const double s_dot_s = block_sum_partials(slot_ptr(part, 3), g_vec);
-  const double As_dot_As = block_sum_partials(slot_ptr(part, 4), g_prod);
-  const double As_dot_s = block_sum_partials(slot_ptr(part, 5), g_prod);
-  const double As_dot_r0 = block_sum_partials(slot_ptr(part, 6), g_prod);
+  double sums[4];
+  block_sum_partials_k<4, double>({slot_ptr(part, 3), slot_ptr(part, 4), slot_ptr(part, 5), slot_ptr(part, 6)},
+                                  {g_vec, g_prod, g_prod, g_prod}, sums);
+  const double s_dot_s = sums[0], As_dot_As = sums[1], As_dot_s = sums[2], As_dot_r0 = sums[3];
   if (threadIdx.x != 0) return;
   const double r_dot_r0 = st->rr;
   const double Ap_dot_r0 = st->ap_r0;
@@ -490,8 +559,9 @@ __global__ __launch_bounds__(kBlock) void bicgc_omega_kernel(KState* __restrict_
                                                              uint32_t g_prod)
 {
   if (st->flags & kFlagDone) return;
-  const double t1t1 = block_sum_partials(slot_ptr(part, 4), g_prod);
-  const double t1s = block_sum_partials(slot_ptr(part, 5), g_prod);
+  double sums[2];
+  block_sum_partials_k<2, double>({slot_ptr(part, 4), slot_ptr(part, 5)}, {g_prod, g_prod}, sums);
+  const double t1t1 = sums[0], t1s = sums[1];
   if (threadIdx.x != 0) return;
   const double nrm = sqrt(t1t1);
   st->omega = t1s / (nrm * nrm);
@@ -537,8 +607,9 @@ __global__ __launch_bounds__(kBlock) void bicgc_check_kernel(KState* __restrict_
                                                              uint32_t g_vec)
 {
   if (st->flags & kFlagDone) return;
-  const double rr2 = block_sum_partials(slot_ptr(part, 0), g_vec);
-  const double rr0 = block_sum_partials(slot_ptr(part, 1), g_vec);
+  double sums[2];
+  block_sum_partials_k<2, double>({slot_ptr(part, 0), slot_ptr(part, 1)}, {g_vec, g_vec}, sums);
+  const double rr2 = sums[0], rr0 = sums[1];
   if (threadIdx.x != 0) return;
   const double res = sqrt(rr2);
   const uint32_t iter = st->iter + 1;
@@ -923,24 +994,55 @@ static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_so
   VCLHIP_TRY(check_launch("cg_setup_kernel"));
 
   uint32_t* flags = &st->flags;
-  auto launch_iter = [&](uint32_t) -> int {
+  auto launch_update = [&]() {
     if (jac)
       VCLHIP_LAUNCH_W(v2, (cg_update_kernel<T, true, kVecW<T> >), (cg_update_kernel<T, true, 1>), gv, s, n, x, p, r,
                       Ap, d, st, part);
     else
       VCLHIP_LAUNCH_W(v2, (cg_update_kernel<T, false, kVecW<T> >), (cg_update_kernel<T, false, 1>), gv, s, n, x, p,
                       r, Ap, d, st, part);
+  };
+  auto launch_prod = [&]() {
     if (jac)
       op.launch(s, p, EpiCg<T, true>{Ap, p, d}, slot_ptr(part, 1), flags, kFlagDone);
     else
       op.launch(s, p, EpiCg<T, false>{Ap, p, d}, slot_ptr(part, 1), flags, kFlagDone);
+  };
+  auto launch_iter = [&](uint32_t) -> int {
+    launch_update();
+    launch_prod();
     hipLaunchKernelGGL(cg_iter_kernel, dim3(1), dim3(kBlock), 0, s, st, part, gv, op.grid);
     return check_launch("cg iteration");
   };
+  // Compact mode: two kernels per iteration (SpMV, then the scalar step fused with the next update)
+  // where every update workgroup can afford to reduce all partials itself.  Not with a monitor
+  // (the update of iteration k + 1 would run before the monitor sees iteration k) nor with graph
+  // chunks of odd length (the captured parities must alternate across replays).
+  const uint32_t chunk = P->chunk ? P->chunk : 16;
+  const bool compact = !P->monitor && static_cast<uint64_t>(gv) * (gv + 2ull * op.grid) <= kCompactPartialReads &&
+                       !(P->use_graph && (chunk & 1u));
+  auto launch_iter_compact = [&](uint32_t k) -> int {
+    launch_prod();
+    if (jac)
+      VCLHIP_LAUNCH_W(v2, (cg_step_kernel<T, true, kVecW<T> >), (cg_step_kernel<T, true, 1>), gv, s, n, x, p, r, Ap,
+                      d, st, part, k & 1u, gv, op.grid);
+    else
+      VCLHIP_LAUNCH_W(v2, (cg_step_kernel<T, false, kVecW<T> >), (cg_step_kernel<T, false, 1>), gv, s, n, x, p, r,
+                      Ap, d, st, part, k & 1u, gv, op.grid);
+    return check_launch("cg iteration");
+  };
   KState fin;
-  VCLHIP_TRY(run_iterations(
-      s, st, 0u, P->max_iters, P, x, launch_iter, [](uint32_t) { return false; }, [](uint32_t) { return 0; }, fin,
-      false));
+  if (compact) {
+    launch_update();  // the update of iteration 0; iteration k then closes with that of k + 1
+    VCLHIP_TRY(check_launch("cg update"));
+    VCLHIP_TRY(run_iterations(
+        s, st, 0u, P->max_iters, P, x, launch_iter_compact, [](uint32_t) { return false; },
+        [](uint32_t) { return 0; }, fin, false));
+  } else {
+    VCLHIP_TRY(run_iterations(
+        s, st, 0u, P->max_iters, P, x, launch_iter, [](uint32_t) { return false; }, [](uint32_t) { return 0; }, fin,
+        false));
+  }
   R->iters = fin.iter;
   R->error = fin.nrs != 0 ? std::sqrt(std::fabs(fin.rr) / fin.nrs) : 0.0;
   if (jac && fin.nrs != 0) R->error = std::sqrt(std::fabs(fin.rr / fin.nrs));
@@ -1133,9 +1235,13 @@ template <bool Jac>
 __global__ __launch_bounds__(kBlock) void mixed_setup_kernel(KState* __restrict__ st, const double* __restrict__ part,
                                                              uint32_t g_vec, uint32_t g_prod, int mode)
 {
-  const double rz = block_sum_partials(slot_ptr(part, kSlotInit), g_vec);
-  const double rr = Jac ? block_sum_partials(slot_ptr(part, kSlotInit + 1), g_vec) : rz;
-  const double bb = block_sum_partials(slot_ptr(part, kSlotInit + 2), g_vec);
+  double sums[3];
+  block_sum_partials_k<3, double>({slot_ptr(part, kSlotInit), slot_ptr(part, kSlotInit + 1),
+                                   slot_ptr(part, kSlotInit + 2)},
+                                  {g_vec, Jac ? g_vec : 0u, g_vec}, sums);
+  const double rz = sums[0];
+  const double rr = Jac ? sums[1] : rz;
+  const double bb = sums[2];
   double ApAp = 0, pAp = 0;
   if (mode == 1) {
     ApAp = block_sum_partials(slot_ptr(part, kSlotApAp), g_prod);
@@ -1301,10 +1407,13 @@ __global__ __launch_bounds__(kBlock) void mixed1_iter_kernel(KState* __restrict_
                                                              uint32_t g_vec, uint32_t g_prod)
 {
   if (st->flags & (kFlagDone | kFlagPause)) return;
-  const double rz = block_sum_partials(slot_ptr(part, kSlotRZ), g_vec);
-  const double rr = Jac ? block_sum_partials(slot_ptr(part, kSlotRR), g_vec) : rz;
-  const double ApAp = block_sum_partials(slot_ptr(part, kSlotApAp), g_prod);
-  const double pAp = block_sum_partials(slot_ptr(part, kSlotPAp), g_prod);
+  double sums[4];
+  block_sum_partials_k<4, double>({slot_ptr(part, kSlotRZ), slot_ptr(part, kSlotRR), slot_ptr(part, kSlotApAp),
+                                   slot_ptr(part, kSlotPAp)},
+                                  {g_vec, Jac ? g_vec : 0u, g_prod, g_prod}, sums);
+  const double rz = sums[0];
+  const double rr = Jac ? sums[1] : rz;
+  const double ApAp = sums[2], pAp = sums[3];
   if (threadIdx.x != 0) return;
   const uint32_t iter = st->iter + 1;
   st->iter = iter;
@@ -1435,10 +1544,11 @@ __global__ __launch_bounds__(kBlock) void mixed1_resume_kernel(KState* __restric
                                                                uint32_t g_vec, uint32_t g_prod)
 {
   if (st->flags & (kFlagDone | kFlagTailIdle)) return;
-  const double rz = block_sum_partials(slot_ptr(part, kSlotRZ), g_vec);
-  const double rr = block_sum_partials(slot_ptr(part, kSlotRR), g_vec);
-  const double ApAp = block_sum_partials(slot_ptr(part, kSlotApAp), g_prod);
-  const double pAp = block_sum_partials(slot_ptr(part, kSlotPAp), g_prod);
+  double sums[4];
+  block_sum_partials_k<4, double>({slot_ptr(part, kSlotRZ), slot_ptr(part, kSlotRR), slot_ptr(part, kSlotApAp),
+                                   slot_ptr(part, kSlotPAp)},
+                                  {g_vec, g_vec, g_prod, g_prod}, sums);
+  const double rz = sums[0], rr = sums[1], ApAp = sums[2], pAp = sums[3];
   if (threadIdx.x != 0) return;
   st->rr = rz;
   st->rr2 = rr;
