@@ -197,6 +197,33 @@ def test_launch_modes_are_deterministic(dev, vh):
     assert res.iters == bres.iters and np.array_equal(x, bx)
 
 
+@pytest.mark.parametrize("precond", [0, 1])
+def test_cg_compact_mode_matches_three_kernel_form(dev, vh, precond):
+    """Small systems run CG with the scalar step fused into the next update (two kernels per
+    iteration).  A monitor, or graph chunks of odd length, select the three-kernel form; every
+    form gives the same iterations, error, history and x, bit for bit, also when the iteration
+    limit ends the solve."""
+    rp, col, val = vh.fdm_laplace_2d(48, 40)
+    n = rp.size - 1
+    b = vh.random_uniform(5, n)
+    A = vh.CsrMatrix(rp, col, val)
+
+    @vh.MONITOR
+    def never(xptr, rel, user):
+        return 0
+
+    for mi in (1000, 37, 1):
+        hc, h3 = np.zeros(mi), np.zeros(mi)
+        xc, rc = solve(vh, "cg", A, b, precond=precond, tol=1e-10, max_iters=mi, history=hc)
+        x3, r3 = solve(vh, "cg", A, b, precond=precond, tol=1e-10, max_iters=mi, history=h3, monitor=never)
+        xg, rg = solve(vh, "cg", A, b, precond=precond, tol=1e-10, max_iters=mi, chunk=5, use_graph=True)
+        for x, r in ((x3, r3), (xg, rg)):
+            assert r.iters == rc.iters and r.error == rc.error and r.converged == rc.converged, mi
+            assert np.array_equal(x, xc), mi
+        assert np.array_equal(hc[: rc.iters], h3[: r3.iters]), mi
+        assert rc.converged == (mi == 1000) and (mi == 1000 or rc.iters == mi)
+
+
 def test_edge_cases(dev, vh):
     rp, col, val = vh.fdm_laplace_2d(16, 16)
     A = vh.CsrMatrix(rp, col, val)
