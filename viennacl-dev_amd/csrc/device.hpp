@@ -103,13 +103,25 @@ __device__ __forceinline__ void block_sum_partials_k(const A* const (&p)[K], con
   A s[K];
 #pragma unroll
   for (int q = 0; q < K; ++q) s[q] = 0;
-  for (uint32_t i = threadIdx.x; i < nmax; i += kBlock) {
-    A v[K];
+  // batches of B steps: the B*K loads of a batch are issued before any is added
+  constexpr uint32_t B = K <= 2 ? 8 : 4;
+  uint32_t i = threadIdx.x;
+  for (; i + (B - 1) * kBlock < nmax; i += B * kBlock) {
+    A v[B][K];
 #pragma unroll
-    for (int q = 0; q < K; ++q) v[q] = i < count[q] ? p[q][i] : A(0);
+    for (uint32_t b = 0; b < B; ++b)
+#pragma unroll
+      for (int q = 0; q < K; ++q) v[b][q] = i + b * kBlock < count[q] ? p[q][i + b * kBlock] : A(0);
+#pragma unroll
+    for (uint32_t b = 0; b < B; ++b)
+#pragma unroll
+      for (int q = 0; q < K; ++q)
+        if (i + b * kBlock < count[q]) s[q] += v[b][q];
+  }
+  for (; i < nmax; i += kBlock) {
 #pragma unroll
     for (int q = 0; q < K; ++q)
-      if (i < count[q]) s[q] += v[q];
+      if (i < count[q]) s[q] += p[q][i];
   }
   const int wave = threadIdx.x / kWave;
 #pragma unroll
