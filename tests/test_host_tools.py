@@ -29,6 +29,10 @@ def test_status_strings_and_errors_without_device(vh):
     with pytest.raises(vh.VclHipError) as e:
         vh.call("vclhip_csr_spmv_f64", None, None, 0, 1, 1.0, None, 0, 1, 0.0, None)
     assert e.value.status == -1
+    # row pointers that decrease would make the kernels read outside the arrays: the planner refuses
+    with pytest.raises(vh.VclHipError) as e:
+        vh.plan_blocks(np.array([0, 3, 2, 5], np.uint32))
+    assert e.value.status == -1 and b"row 1" in vh.lib().vclhip_last_error()
 
 
 def check_plan(rp, blocks):

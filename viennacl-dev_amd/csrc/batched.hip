@@ -628,6 +628,13 @@ int vclhip_batch_plan_create(uint32_t num_systems, const uint32_t* sys_ptr_host,
   });
   int st = hip_status(hipGetDevice(&plan->device), "hipGetDevice");
   if (!st) st = hip_status(hipMalloc(&plan->ap_dev, std::max<size_t>(plan->total_rows, 1) * sizeof(double)), "hipMalloc");
+  if (!st && row_ptr_host) {
+    for (uint32_t r = 0; r < plan->total_rows && !st; ++r)
+      if (row_ptr_host[r + 1] < row_ptr_host[r]) {
+        set_error("invalid argument: row_ptr must be nondecreasing (row %u)", r);
+        st = VCLHIP_ERROR_INVALID_VALUE;
+      }
+  }
   if (!st && row_ptr_host && plan->total_rows < kPlanRowLimit) {
     // row blocks of each system (the CSR kernels' partition of its rows), rows made global
     std::vector<uint32_t> blocks, sys_blk(num_systems + 1, 0), local(max_rows + 1);

@@ -281,6 +281,11 @@ int vclhip_csr_plan_blocks(const uint32_t* row_ptr, uint32_t n_rows, uint32_t* b
 {
   if (!row_ptr || !blocks || !num_blocks) return vclhip::invalid("null argument");
   if (n_rows >= vclhip::kPlanRowLimit) return vclhip::invalid("n_rows must be below 2^28");
+  for (uint32_t r = 0; r < n_rows; ++r)
+    if (row_ptr[r + 1] < row_ptr[r]) {
+      vclhip::set_error("invalid argument: row_ptr must be nondecreasing (row %u)", r);
+      return VCLHIP_ERROR_INVALID_VALUE;
+    }
   *num_blocks = vclhip::plan_csr_blocks(row_ptr, n_rows, blocks);
   return VCLHIP_SUCCESS;
 }
