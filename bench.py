@@ -319,6 +319,7 @@ def mixed_vs_fp64(vh, grid, reps=2):
     x64h = x64.to_numpy()
     xm = vh.DeviceArray.zeros(n, np.float64)
     for name, kw in (("mixed_reliable", dict(mixed_mode=1)),
+                     ("mixed_reliable_sell16_inner", dict(mixed_mode=1, inner_format=3)),
                      ("mixed_reliable_csr_inner", dict(mixed_mode=1, inner_format=1)),
                      ("mixed_reference_schedule", dict(mixed_mode=0))):
         rec = run(name, lambda x: vh.mixed_cg(A, b, x, tol=1e-10, max_iters=20000, **kw), xm)

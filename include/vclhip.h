@@ -423,10 +423,13 @@ typedef struct vclhip_solver_params {
   int (*monitor)(const void* x_dev, double rel_residual, void* user);
   void* monitor_user;
   int inner_format;                  /* mixed CG on a CSR matrix: storage of the FP32 inner copy,
-                                        VCLHIP_INNER_AUTO (SELL-64 when it stores at most 25% padding,
-                                        with 16-bit column offsets when every |col - row| <= 32767;
-                                        else CSR), VCLHIP_INNER_CSR, VCLHIP_INNER_SELL64 (32-bit
-                                        columns) or VCLHIP_INNER_SELL64_D16 */
+                                        VCLHIP_INNER_AUTO (DIA when the matrix has at most 32
+                                        diagonals storing at most 25% zeros; else SELL-64 when it
+                                        stores at most 25% padding, with 16-bit column offsets when
+                                        every |col - row| <= 32767; else CSR), VCLHIP_INNER_CSR,
+                                        VCLHIP_INNER_SELL64 (32-bit columns), VCLHIP_INNER_SELL64_D16
+                                        or VCLHIP_INNER_DIA.  DIA and SELL-64 sum each row in one
+                                        lane in column order and give the same results. */
   uint32_t krylov_dim;               /* GMRES: steps per restart cycle (gmres_tag default 20) */
 } vclhip_solver_params;
 
@@ -434,6 +437,7 @@ typedef struct vclhip_solver_params {
 #define VCLHIP_INNER_CSR 1
 #define VCLHIP_INNER_SELL64 2
 #define VCLHIP_INNER_SELL64_D16 3
+#define VCLHIP_INNER_DIA 4
 
 typedef struct vclhip_solver_result {
   uint32_t iters;     /* tag.iters() */

@@ -85,7 +85,7 @@ def test_values32_argument_and_graph_are_identical(dev, vh):
     v32 = vh.DeviceArray.from_numpy(val.astype(np.float32))
     x1, r1 = run_mixed(vh, rp, col, val, b, values32=v32, tol=1e-10, max_iters=2000, mixed_mode=1)
     assert r0.iters == r1.iters and r0.restarts == r1.restarts and np.array_equal(x0, x1)
-    # default (SELL-64 inner copy for this stencil), eager and graph-captured
+    # default (DIA inner copy for this stencil), eager and graph-captured
     x2, r2 = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=2000, mixed_mode=1)
     x3, r3 = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=2000, mixed_mode=1, use_graph=True, chunk=8)
     assert r2.iters == r3.iters and r2.restarts == r3.restarts and np.array_equal(x2, x3)
@@ -111,6 +111,56 @@ def test_inner_formats_vs_oracle(dev, vh, oracle, fmt):
     b2 = vh.random_uniform(1, 3000)
     x2, r2 = run_mixed(vh, rp2, col2, val2, b2, tol=1e-10, max_iters=3000, mixed_mode=1, inner_format=fmt)
     assert r2.converged and true_relres(rp2, col2, val2, x2, b2) < 1e-10
+
+
+@pytest.mark.parametrize("jacobi", [False, True])
+def test_dia_inner_copy_equals_sell(dev, vh, jacobi):
+    """The DIA inner copy sums each row in one lane in column order, skipping zeros, as the SELL-64
+    copy does: both schedules give the same iterations and the same x, bit for bit; the automatic
+    choice takes DIA for a 7-point operator."""
+    k = np.exp(vh.random_normal(6, 20 * 22 * 24))
+    rp, col, val = vh.diffusion(20, 22, 24, k)
+    b = vh.random_uniform(0, rp.size - 1)
+    pc = 1 if jacobi else 0
+    for mode in ((1,) if jacobi else (0, 1)):
+        xs, rs = run_mixed(vh, rp, col, val, b, precond=pc, tol=1e-10, max_iters=3000, mixed_mode=mode,
+                           inner_format=3)
+        xd, rd = run_mixed(vh, rp, col, val, b, precond=pc, tol=1e-10, max_iters=3000, mixed_mode=mode,
+                           inner_format=4)
+        xa, ra = run_mixed(vh, rp, col, val, b, precond=pc, tol=1e-10, max_iters=3000, mixed_mode=mode)
+        for x, r in ((xd, rd), (xa, ra)):
+            assert r.iters == rs.iters and r.restarts == rs.restarts and r.error == rs.error, mode
+            assert np.array_equal(x, xs), mode
+        assert rd.converged and true_relres(rp, col, val, xd, b) < 1e-10
+
+
+def test_dia_inner_copy_refuses_what_it_cannot_hold(dev, vh):
+    """More than 32 diagonals (a random graph, a band of 33) or more than 25% zeros: DIA, forced,
+    is refused; the automatic choice falls back to SELL-64 or CSR and solves."""
+    _, rp, col, val = vh.graph_laplacian_batch(1, 3000, 3000, 8.0, 0.1, 1.0, 0.01, 3)
+    b = vh.random_uniform(1, 3000)
+    with pytest.raises(vh.VclHipError) as e:
+        run_mixed(vh, rp, col, val, b, tol=1e-8, max_iters=3000, mixed_mode=1, inner_format=4)
+    assert e.value.status == -3  # VCLHIP_ERROR_NOT_SUPPORTED
+    x, res = run_mixed(vh, rp, col, val, b, tol=1e-8, max_iters=3000, mixed_mode=1)
+    assert res.converged and true_relres(rp, col, val, x, b) < 1e-8
+    # a symmetric band of half-width 16 (33 diagonals), diagonally dominant
+    n = 2000
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j in range(max(0, i - 16), min(n, i + 17)):
+            rows.append(i)
+            cols.append(j)
+            vals.append(40.0 if i == j else -1.0 / (1 + abs(i - j)))
+    rpb = np.zeros(n + 1, np.uint32)
+    np.add.at(rpb, np.array(rows) + 1, 1)
+    rpb = np.cumsum(rpb).astype(np.uint32)
+    colb, valb = np.array(cols, np.uint32), np.array(vals)
+    bb = vh.random_uniform(2, n)
+    with pytest.raises(vh.VclHipError):
+        run_mixed(vh, rpb, colb, valb, bb, tol=1e-8, max_iters=500, mixed_mode=1, inner_format=4)
+    x, res = run_mixed(vh, rpb, colb, valb, bb, tol=1e-8, max_iters=500, mixed_mode=1)
+    assert res.converged and true_relres(rpb, colb, valb, x, bb) < 1e-8
 
 
 def test_sell16_refuses_far_columns(dev, vh):

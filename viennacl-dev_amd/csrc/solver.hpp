@@ -13,7 +13,7 @@ namespace vclhip {
 
 // ------------------------------------------------------------------ matrix wrapper
 
-enum class OpFormat { Csr, Sell, Coo, Ell, Hyb };
+enum class OpFormat { Csr, Sell, Coo, Ell, Hyb, Dia };
 
 template <typename T>
 struct DevOp {
@@ -36,6 +36,7 @@ struct DevOp {
                        part, kMaxPartials, stop, mask);
   }
 
+  DiaDev<T> dia{};  // the mixed solver's inner DIA copy
   HybDev<T> hyb{};  // HYB with a plan for its CSR part
   bool hyb_plan = false;
 
@@ -70,6 +71,10 @@ struct DevOp {
       case OpFormat::Ell:
         hipLaunchKernelGGL((ell_kernel<T, true, Epi, false>), dim3(grid), dim3(kBlock), 0, s, ell, x, 0u, 1u, epi,
                            part, kMaxPartials, stop, mask);
+        break;
+      case OpFormat::Dia:
+        hipLaunchKernelGGL((dia_kernel<T, Epi>), dim3(grid), dim3(kBlock), 0, s, dia, x, epi, part, kMaxPartials, stop,
+                           mask);
         break;
       case OpFormat::Hyb:
         if (hyb_plan)

@@ -291,6 +291,53 @@ __global__ __launch_bounds__(kBlock) void sell_kernel(SellDev<T> A, const T* __r
   if constexpr (Epi::K > 0) block_store_partials<Epi::K, acc_t>(acc, part, pstride, lw);
 }
 
+// DIA: nd diagonals of n values each, diagonal-major (val[d * n + row] multiplies x[row + off[d]]),
+// offsets ascending.  The mixed-precision solver's inner copy of a stencil-like matrix.
+template <typename T>
+struct DiaDev {
+  const T* __restrict__ val;
+  const int32_t* __restrict__ off;
+  uint32_t n_rows;
+  uint32_t nd;
+};
+
+// One lane per row over the diagonals in offset order, i.e. the row's entries in column order, zero
+// values skipped as the SELL kernel skips them: the same sums as a row of the SELL copy.  There are
+// no column indices to load, and the x reads of a wavefront are 64 consecutive values per diagonal.
+template <typename T, class Epi>
+__global__ __launch_bounds__(kBlock) void dia_kernel(DiaDev<T> A, const T* __restrict__ x, Epi epi,
+                                                     typename Epi::acc_t* __restrict__ part, uint32_t pstride,
+                                                     const uint32_t* __restrict__ stop, uint32_t stop_mask)
+{
+  using acc_t = typename Epi::acc_t;
+  constexpr int KA = Epi::K > 0 ? Epi::K : 1;
+  if (stop && (*stop & stop_mask)) return;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lw = xcd_logical_block();
+  const uint32_t tiles = (A.n_rows + kBlock - 1) / kBlock;
+  uint32_t t0, t1;
+  chunk_range(tiles, lw, gridDim.x, t0, t1);
+  acc_t acc[KA];
+#pragma unroll
+  for (int q = 0; q < KA; ++q) acc[q] = 0;
+  for (uint32_t t = t0; t < t1; ++t) {
+    const uint32_t row = t * kBlock + tid;
+    if (row < A.n_rows) {
+      T sum = 0;
+      // x is read wherever the column exists, whatever the value, so that no load waits for another
+#pragma unroll 8
+      for (uint32_t d = 0; d < A.nd; ++d) {
+        const uint32_t c = row + static_cast<uint32_t>(A.off[d]);  // out of range (also below 0): no entry
+        const T v = A.val[static_cast<size_t>(d) * A.n_rows + row];
+        const T xv = c < A.n_rows ? x[c] : T(0);
+        sum += (v > 0 || v < 0) ? xv * v : T(0);
+      }
+      epi(row, sum, acc);
+    }
+  }
+  if constexpr (Epi::K > 0) block_store_partials<Epi::K, acc_t>(acc, part, pstride, lw);
+}
+
 // ELL (one slice of height ld, column-major) and HYB (ELL part plus a CSR part per row).
 template <typename T>
 struct EllDev {
