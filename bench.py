@@ -317,6 +317,10 @@ def mixed_vs_fp64(vh, grid, reps=2):
     x64 = vh.DeviceArray.zeros(n, np.float64)
     f64 = run("cg_fp64", lambda x: vh.cg(A, b, x, tol=1e-10, max_iters=20000), x64)
     x64h = x64.to_numpy()
+    xg = vh.DeviceArray.zeros(n, np.float64)
+    g64 = run("cg_fp64_csr_as_given", lambda x: vh.cg(A, b, x, tol=1e-10, max_iters=20000, operator_format=1), xg)
+    g64["same_x_as_cg_fp64"] = bool(np.array_equal(xg.to_numpy(), x64h))
+    xg.free()
     xm = vh.DeviceArray.zeros(n, np.float64)
     for name, kw in (("mixed_reliable", dict(mixed_mode=1)),
                      ("mixed_reliable_sell16_inner", dict(mixed_mode=1, inner_format=3)),
@@ -629,11 +633,16 @@ def main():
     if not args.no_solvers:
         b = vh.DeviceArray.from_numpy(vh.random_uniform(0, n))
         sol = {}
-        sol["cg_fp64_csr"] = timed_solve(vh, "cg", A, b, n, tol=1e-10, max_iters=5000)
+        # a CSR stencil is solved on a DIA copy by default (the same results); "_as_given" keeps CSR
+        sol["cg_fp64_csr"] = timed_solve(vh, "cg", A, b, n, reps=3, tol=1e-10, max_iters=5000)
+        sol["cg_fp64_csr_as_given"] = timed_solve(vh, "cg", A, b, n, reps=3, tol=1e-10, max_iters=5000,
+                                                  operator_format=1)
         sol["cg_fp64_csr_graph"] = timed_solve(vh, "cg", A, b, n, tol=1e-10, max_iters=5000, use_graph=True)
         sol["cg_fp64_sell64"] = timed_solve(vh, "cg", S, b, n, tol=1e-10, max_iters=5000)
         sol["jacobi_cg_fp64_csr"] = timed_solve(vh, "cg", A, b, n, precond=1, tol=1e-10, max_iters=5000)
         sol["bicgstab_fp64_csr"] = timed_solve(vh, "bicgstab", A, b, n, tol=1e-10, max_iters=5000)
+        sol["bicgstab_fp64_csr_as_given"] = timed_solve(vh, "bicgstab", A, b, n, tol=1e-10, max_iters=5000,
+                                                        operator_format=1)
         sol["jacobi_bicgstab_fp64_csr"] = timed_solve(vh, "bicgstab", A, b, n, precond=1, tol=1e-10,
                                                       max_iters=5000)
         sol["jacobi_cg_fp64_sell64"] = timed_solve(vh, "cg", S, b, n, precond=1, tol=1e-10, max_iters=5000)

@@ -431,7 +431,15 @@ typedef struct vclhip_solver_params {
                                         or VCLHIP_INNER_DIA.  DIA and SELL-64 sum each row in one
                                         lane in column order and give the same results. */
   uint32_t krylov_dim;               /* GMRES: steps per restart cycle (gmres_tag default 20) */
+  int operator_format;               /* CG, BiCGStab, GMRES on a CSR matrix: VCLHIP_OPERATOR_AUTO
+                                        (default) solves on a DIA copy in workspace memory when the
+                                        matrix has at most 32 diagonals storing at most 25% zeros
+                                        and the copy takes at most a quarter of the free device
+                                        memory -- the same results, fewer bytes per product;
+                                        VCLHIP_OPERATOR_AS_GIVEN always uses the CSR matrix */
 } vclhip_solver_params;
+#define VCLHIP_OPERATOR_AUTO 0
+#define VCLHIP_OPERATOR_AS_GIVEN 1
 
 #define VCLHIP_INNER_AUTO 0
 #define VCLHIP_INNER_CSR 1

@@ -1,0 +1,173 @@
+// dia.hpp -- DIA copies of CSR matrices with few diagonals, built on the device at solve time.
+//
+// A CSR matrix whose entries lie on at most kDiaMax diagonals (stencil operators) is stored as one
+// value per row and diagonal, offsets ascending; dia_kernel (spmv_kernels.hpp) then needs no column
+// indices and issues every load of a row at once.  With at most kDiaMax diagonals no row has more
+// than kDiaMax <= kShortRow entries, so the CSR kernel sums every row in one lane in column order,
+// and so does dia_kernel: a solve on the DIA copy gives the same results as on the CSR matrix.
+// Used for the mixed-precision solver's FP32 inner copy and, unless the caller asks for the
+// matrix as given, for the operator of the CG, BiCGStab and GMRES solvers.
+#pragma once
+
+#include <vector>
+
+#include "internal.hpp"
+#include "solver.hpp"
+
+namespace vclhip {
+
+constexpr uint32_t kDiaMax = 32;    // diagonals of a DIA copy at most
+constexpr uint32_t kDiaHash = 128;  // LDS hash slots per workgroup for the offsets of its rows
+static_assert(kDiaMax <= kShortRow, "a DIA row must be one the CSR kernel sums in one lane");
+
+// Marks every offset col - row of the matrix in a bitmap (bit col - row + n).  A workgroup first
+// collects the distinct offsets of its 256 rows in an LDS hash table, so that the global bitmap
+// sees one atomic OR per distinct offset and workgroup; a workgroup whose rows hold more distinct
+// offsets than the table, or a column outside the square matrix, raises *overflow (the matrix is
+// then not a DIA candidate).
+template <typename I>
+__global__ __launch_bounds__(kBlock) void dia_offsets_kernel(const I* __restrict__ rp, const I* __restrict__ col,
+                                                             uint32_t n, unsigned long long* __restrict__ bits,
+                                                             uint32_t* __restrict__ overflow)
+{
+  __shared__ uint32_t keys[kDiaHash];
+  __shared__ uint32_t full;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < kDiaHash; i += kBlock) keys[i] = 0;
+  if (tid == 0) full = 0;
+  __syncthreads();
+  const uint32_t row = blockIdx.x * kBlock + tid;
+  if (row < n) {
+    const uint32_t e1 = rp[row + 1];
+    for (uint32_t e = rp[row]; e < e1; ++e) {
+      const uint32_t c = col[e];
+      if (c >= n) {  // not a square-matrix column: no DIA copy
+        full = 1;
+        continue;
+      }
+      const uint32_t key = c + n - row;  // in [1, 2n - 1]; 0 marks an empty slot
+      uint32_t h = (key * 2654435761u) >> 25;
+      uint32_t probe = 0;
+      for (; probe < kDiaHash; ++probe) {
+        const uint32_t prev = atomicCAS(&keys[h], 0u, key);
+        if (prev == 0u || prev == key) break;
+        h = (h + 1) & (kDiaHash - 1);
+      }
+      if (probe == kDiaHash) full = 1;
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = tid; i < kDiaHash; i += kBlock) {
+    const uint32_t k = keys[i];
+    if (k) atomicOr(&bits[k >> 6], 1ull << (k & 63));
+  }
+  if (tid == 0 && full) *overflow = 1u;
+}
+
+// val[d * n + row] = A(row, row + off[d]) converted to T (val zeroed before); off ascending,
+// nd <= kDiaMax, and every offset of the matrix is in off.
+template <typename S, typename T>
+__global__ __launch_bounds__(kBlock) void dia_fill_kernel(const uint32_t* __restrict__ rp,
+                                                          const uint32_t* __restrict__ col, const S* __restrict__ a,
+                                                          uint32_t n, const int32_t* __restrict__ off, uint32_t nd,
+                                                          T* __restrict__ val)
+{
+  __shared__ int32_t so[kDiaMax];
+  if (threadIdx.x < nd) so[threadIdx.x] = off[threadIdx.x];
+  __syncthreads();
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= n) return;
+  const uint32_t e1 = rp[row + 1];
+  for (uint32_t e = rp[row]; e < e1; ++e) {
+    const int32_t o = static_cast<int32_t>(col[e] - row);
+    uint32_t lo = 0, hi = nd;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (so[mid] <= o)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    val[static_cast<size_t>(lo) * n + row] = static_cast<T>(a[e]);
+  }
+}
+
+// Builds the DIA copy (values converted from S to T) of a square CSR matrix into workspace memory
+// when it has at most kDiaMax distinct diagonals, they store at most `max_fill` times nnz values,
+// and (max_bytes > 0) the values take at most max_bytes; *built tells whether it was made.
+template <typename S, typename T>
+static int build_dia(const vclhip_csr* A, Workspace& ws, hipStream_t s, double max_fill, size_t max_bytes,
+                     DiaDev<T>* out, bool* built)
+{
+  *built = false;
+  const uint32_t n = A->n_rows;
+  if (n == 0 || A->n_cols != n || n >= (1u << 30) || !A->row_ptr || !A->col_idx || !A->values)
+    return VCLHIP_SUCCESS;
+  const size_t words = (2ull * n + 64) / 64;
+  unsigned long long* bits;
+  uint32_t* overflow;
+  VCLHIP_TRY(ws.alloc(&bits, words));
+  VCLHIP_TRY(ws.alloc(&overflow, 1));
+  VCLHIP_HIP(hipMemsetAsync(bits, 0, words * sizeof(unsigned long long), s));
+  VCLHIP_HIP(hipMemsetAsync(overflow, 0, sizeof(uint32_t), s));
+  const uint32_t g = (n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(dia_offsets_kernel<uint32_t>, dim3(g), dim3(kBlock), 0, s, A->row_ptr, A->col_idx, n, bits,
+                     overflow);
+  VCLHIP_TRY(check_launch("dia_offsets_kernel"));
+  std::vector<unsigned long long> hb(words);
+  uint32_t hover = 0;
+  VCLHIP_HIP(hipMemcpyAsync(hb.data(), bits, words * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  VCLHIP_HIP(hipMemcpyAsync(&hover, overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  VCLHIP_HIP(hipStreamSynchronize(s));
+  if (hover) return VCLHIP_SUCCESS;
+  std::vector<int32_t> offs;
+  for (size_t w = 0; w < words; ++w)
+    for (unsigned long long b = hb[w]; b; b &= b - 1) {
+      if (offs.size() == kDiaMax) return VCLHIP_SUCCESS;
+      const uint64_t k = w * 64 + static_cast<uint64_t>(__builtin_ctzll(b));
+      offs.push_back(static_cast<int32_t>(static_cast<int64_t>(k) - n));
+    }
+  const uint32_t nd = static_cast<uint32_t>(offs.size());
+  if (nd == 0 || static_cast<double>(nd) * n > max_fill * static_cast<double>(A->nnz)) return VCLHIP_SUCCESS;
+  const size_t vbytes = static_cast<size_t>(nd) * n * sizeof(T);
+  if (max_bytes && vbytes > max_bytes) return VCLHIP_SUCCESS;
+  int32_t* doff;
+  T* val;
+  VCLHIP_TRY(ws.alloc(&doff, nd));
+  VCLHIP_TRY(ws.alloc(&val, static_cast<size_t>(nd) * n));
+  VCLHIP_HIP(hipMemcpyAsync(doff, offs.data(), nd * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  VCLHIP_HIP(hipMemsetAsync(val, 0, vbytes, s));
+  hipLaunchKernelGGL((dia_fill_kernel<S, T>), dim3(g), dim3(kBlock), 0, s, A->row_ptr, A->col_idx,
+                     static_cast<const S*>(A->values), n, doff, nd, val);
+  VCLHIP_TRY(check_launch("dia_fill_kernel"));
+  VCLHIP_HIP(hipStreamSynchronize(s));  // offs lives on the host stack
+  out->val = val;
+  out->off = doff;
+  out->n_rows = n;
+  out->nd = nd;
+  *built = true;
+  return VCLHIP_SUCCESS;
+}
+
+// The solvers' operator: a CSR matrix with few diagonals is replaced by a DIA copy in workspace
+// memory (same results, fewer bytes per product) unless params->operator_format asks for the
+// matrix as given, or the copy would take more than a quarter of the free device memory.
+template <typename T>
+static int dia_operator(DevOp<T>& op, Workspace& ws, hipStream_t s, const vclhip_solver_params* P)
+{
+  if (!P || P->operator_format != VCLHIP_OPERATOR_AUTO || op.fmt != OpFormat::Csr || !op.csr_desc)
+    return VCLHIP_SUCCESS;
+  size_t free_b = 0, total_b = 0;
+  VCLHIP_HIP(hipMemGetInfo(&free_b, &total_b));
+  DiaDev<T> dia{};
+  bool built = false;
+  VCLHIP_TRY((build_dia<T, T>(op.csr_desc, ws, s, 1.25, free_b / 4 ? free_b / 4 : 1, &dia, &built)));
+  if (!built) return VCLHIP_SUCCESS;
+  op.fmt = OpFormat::Dia;
+  op.sell = false;
+  op.dia = dia;
+  op.grid = chunk_grid((op.n + kBlock - 1) / kBlock, sell_gmax());
+  return VCLHIP_SUCCESS;
+}
+
+}  // namespace vclhip

@@ -30,6 +30,7 @@
 
 #include "internal.hpp"
 #include "solver.hpp"
+#include "dia.hpp"
 #include "spmv_kernels.hpp"
 
 namespace vclhip {
@@ -545,6 +546,7 @@ int gmres_solve(const M* Ain, const T* b, T* x, const vclhip_solver_params* P, v
   const uint32_t max_cycles = max_restarts + 1;
 
   Workspace ws(s);
+  VCLHIP_TRY(dia_operator(op, ws, s, P));
   T *V, *r;
   double *R, *sigma, *xi, *coef, *upd, *c, *part, *hist = nullptr;
   GState* st;

@@ -16,6 +16,7 @@
 #include "internal.hpp"
 #include "solver.hpp"
 #include "spmv_kernels.hpp"
+#include "dia.hpp"
 
 namespace vclhip {
 
@@ -970,6 +971,7 @@ static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_so
   const uint32_t n = op.n;
   const bool jac = precond == VCLHIP_PRECOND_JACOBI;
   Workspace ws(s);
+  VCLHIP_TRY(dia_operator(op, ws, s, P));
   T *r, *p, *Ap, *d = nullptr;
   double *part, *hist = nullptr;
   KState* st;
@@ -1084,6 +1086,7 @@ static int bicgstab_solve(const M* Ain, const T* b, T* x, int precond, const vcl
   const uint32_t n = op.n;
   const bool jac = precond == VCLHIP_PRECOND_JACOBI;
   Workspace ws(s);
+  VCLHIP_TRY(dia_operator(op, ws, s, P));
   T *r, *p, *r0, *Ap, *sv, *As, *d = nullptr;
   double *part, *hist = nullptr;
   KState* st;
@@ -1731,133 +1734,6 @@ static int build_inner_sell(const vclhip_csr* A, Workspace& ws, hipStream_t s, d
   return VCLHIP_SUCCESS;
 }
 
-// ------------------------------------------------------------------ DIA inner copy
-
-constexpr uint32_t kDiaMax = 32;    // diagonals of a DIA inner copy at most
-constexpr uint32_t kDiaHash = 128;  // LDS hash slots per workgroup for the offsets of its rows
-
-// Marks every offset col - row of the matrix in a bitmap (bit col - row + n).  A workgroup first
-// collects the distinct offsets of its 256 rows in an LDS hash table, so that the global bitmap
-// sees one atomic OR per distinct offset and workgroup; a workgroup whose rows hold more distinct
-// offsets than the table raises *overflow (the matrix is then not a DIA candidate).
-__global__ __launch_bounds__(kBlock) void dia_offsets_kernel(const uint32_t* __restrict__ rp,
-                                                             const uint32_t* __restrict__ col, uint32_t n,
-                                                             unsigned long long* __restrict__ bits,
-                                                             uint32_t* __restrict__ overflow)
-{
-  __shared__ uint32_t keys[kDiaHash];
-  __shared__ uint32_t full;
-  const uint32_t tid = threadIdx.x;
-  for (uint32_t i = tid; i < kDiaHash; i += kBlock) keys[i] = 0;
-  if (tid == 0) full = 0;
-  __syncthreads();
-  const uint32_t row = blockIdx.x * kBlock + tid;
-  if (row < n) {
-    const uint32_t e1 = rp[row + 1];
-    for (uint32_t e = rp[row]; e < e1; ++e) {
-      const uint32_t c = col[e];
-      if (c >= n) {  // not a square-matrix column: no DIA copy
-        full = 1;
-        continue;
-      }
-      const uint32_t key = c + n - row;  // in [1, 2n - 1]; 0 marks an empty slot
-      uint32_t h = (key * 2654435761u) >> 25;
-      uint32_t probe = 0;
-      for (; probe < kDiaHash; ++probe) {
-        const uint32_t prev = atomicCAS(&keys[h], 0u, key);
-        if (prev == 0u || prev == key) break;
-        h = (h + 1) & (kDiaHash - 1);
-      }
-      if (probe == kDiaHash) full = 1;
-    }
-  }
-  __syncthreads();
-  for (uint32_t i = tid; i < kDiaHash; i += kBlock) {
-    const uint32_t k = keys[i];
-    if (k) atomicOr(&bits[k >> 6], 1ull << (k & 63));
-  }
-  if (tid == 0 && full) *overflow = 1u;
-}
-
-// val[d * n + row] = A(row, row + off[d]) in FP32 (val zeroed before); off ascending, nd <= kDiaMax.
-__global__ __launch_bounds__(kBlock) void dia_fill_kernel(const uint32_t* __restrict__ rp,
-                                                          const uint32_t* __restrict__ col,
-                                                          const double* __restrict__ a, uint32_t n,
-                                                          const int32_t* __restrict__ off, uint32_t nd,
-                                                          float* __restrict__ val)
-{
-  __shared__ int32_t so[kDiaMax];
-  if (threadIdx.x < nd) so[threadIdx.x] = off[threadIdx.x];
-  __syncthreads();
-  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
-  if (row >= n) return;
-  const uint32_t e1 = rp[row + 1];
-  for (uint32_t e = rp[row]; e < e1; ++e) {
-    const int32_t o = static_cast<int32_t>(col[e] - row);
-    uint32_t lo = 0, hi = nd;  // the offset is in the list: every offset of the matrix was collected
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) / 2;
-      if (so[mid] <= o)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    val[static_cast<size_t>(lo) * n + row] = static_cast<float>(a[e]);
-  }
-}
-
-// Builds the DIA FP32 copy of a square CSR matrix into workspace memory when it has at most
-// kDiaMax distinct diagonals and they store at most `max_fill` times nnz values; *built tells
-// whether it was made.
-static int build_inner_dia(const vclhip_csr* A, Workspace& ws, hipStream_t s, double max_fill, DiaDev<float>* out,
-                           bool* built)
-{
-  *built = false;
-  const uint32_t n = A->n_rows;
-  if (n == 0 || A->n_cols != n || n >= (1u << 30)) return VCLHIP_SUCCESS;
-  const size_t words = (2ull * n + 64) / 64;
-  unsigned long long* bits;
-  uint32_t* overflow;
-  VCLHIP_TRY(ws.alloc(&bits, words));
-  VCLHIP_TRY(ws.alloc(&overflow, 1));
-  VCLHIP_HIP(hipMemsetAsync(bits, 0, words * sizeof(unsigned long long), s));
-  VCLHIP_HIP(hipMemsetAsync(overflow, 0, sizeof(uint32_t), s));
-  const uint32_t g = (n + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(dia_offsets_kernel, dim3(g), dim3(kBlock), 0, s, A->row_ptr, A->col_idx, n, bits, overflow);
-  VCLHIP_TRY(check_launch("dia_offsets_kernel"));
-  std::vector<unsigned long long> hb(words);
-  uint32_t hover = 0;
-  VCLHIP_HIP(hipMemcpyAsync(hb.data(), bits, words * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-  VCLHIP_HIP(hipMemcpyAsync(&hover, overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  VCLHIP_HIP(hipStreamSynchronize(s));
-  if (hover) return VCLHIP_SUCCESS;
-  std::vector<int32_t> offs;
-  for (size_t w = 0; w < words; ++w)
-    for (unsigned long long b = hb[w]; b; b &= b - 1) {
-      if (offs.size() == kDiaMax) return VCLHIP_SUCCESS;
-      const uint64_t k = w * 64 + static_cast<uint64_t>(__builtin_ctzll(b));
-      offs.push_back(static_cast<int32_t>(static_cast<int64_t>(k) - n));
-    }
-  const uint32_t nd = static_cast<uint32_t>(offs.size());
-  if (nd == 0 || static_cast<double>(nd) * n > max_fill * static_cast<double>(A->nnz)) return VCLHIP_SUCCESS;
-  int32_t* doff;
-  float* val;
-  VCLHIP_TRY(ws.alloc(&doff, nd));
-  VCLHIP_TRY(ws.alloc(&val, static_cast<size_t>(nd) * n));
-  VCLHIP_HIP(hipMemcpyAsync(doff, offs.data(), nd * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  VCLHIP_HIP(hipMemsetAsync(val, 0, static_cast<size_t>(nd) * n * sizeof(float), s));
-  hipLaunchKernelGGL(dia_fill_kernel, dim3(g), dim3(kBlock), 0, s, A->row_ptr, A->col_idx,
-                     static_cast<const double*>(A->values), n, doff, nd, val);
-  VCLHIP_TRY(check_launch("dia_fill_kernel"));
-  VCLHIP_HIP(hipStreamSynchronize(s));  // offs lives on the host stack
-  out->val = val;
-  out->off = doff;
-  out->n_rows = n;
-  out->nd = nd;
-  *built = true;
-  return VCLHIP_SUCCESS;
-}
-
 inline const double* values_of(const vclhip_csr* A) { return static_cast<const double*>(A->values); }
 inline const double* values_of(const vclhip_sell* A) { return static_cast<const double*>(A->elements); }
 // Number of stored values: nnz for CSR; block_start[last] + columns_per_block[last] * C for SELL
@@ -1922,7 +1798,7 @@ static int mixed_cg_solve(const M* A, const float* values32, const double* b, do
   if (P->inner_format < VCLHIP_INNER_AUTO || P->inner_format > VCLHIP_INNER_DIA) return invalid("inner_format");
   if constexpr (std::is_same<M, vclhip_csr>::value) {
     if (!values32 && (P->inner_format == VCLHIP_INNER_AUTO || P->inner_format == VCLHIP_INNER_DIA)) {
-      VCLHIP_TRY(build_inner_dia(A, ws, s, 1.25, &inner_dia, &inner_is_dia));
+      VCLHIP_TRY((build_dia<double, float>(A, ws, s, 1.25, 0, &inner_dia, &inner_is_dia)));
       if (P->inner_format == VCLHIP_INNER_DIA && !inner_is_dia) {
         set_error("inner_format DIA: more than %u diagonals, or their padding exceeds 25%% of nnz", kDiaMax);
         return VCLHIP_ERROR_NOT_SUPPORTED;

@@ -333,7 +333,7 @@ def matrix_diag(A, d, stream=None):
 
 def solver_params(tol=1e-8, max_iters=300, abs_tol=0.0, max_iters_before_restart=200, inner_tol=1e-2,
                   chunk=16, use_graph=False, mixed_mode=0, variant=0, history=None, monitor=None, inner_format=0,
-                  krylov_dim=20):
+                  krylov_dim=20, operator_format=0):
     p = SolverParams()
     lib().vclhip_solver_params_default(C.byref(p))
     p.variant = variant
@@ -347,6 +347,7 @@ def solver_params(tol=1e-8, max_iters=300, abs_tol=0.0, max_iters_before_restart
     p.mixed_mode = mixed_mode
     p.inner_format = inner_format
     p.krylov_dim = krylov_dim
+    p.operator_format = operator_format
     if history is not None:
         p.history_host = history.ctypes.data_as(C.POINTER(C.c_double))
         p.history_len = history.size

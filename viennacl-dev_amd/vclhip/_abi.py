@@ -74,7 +74,7 @@ class SolverParams(C.Structure):
                 ("max_iters_before_restart", u32), ("inner_tol", f32), ("chunk", u32),
                 ("use_graph", i32), ("mixed_mode", i32), ("variant", i32), ("history_host", P(f64)),
                 ("history_len", u32), ("monitor", MONITOR), ("monitor_user", vp), ("inner_format", i32),
-                ("krylov_dim", u32)]
+                ("krylov_dim", u32), ("operator_format", i32)]
 
 
 class SolverResult(C.Structure):
