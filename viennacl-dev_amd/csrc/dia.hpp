@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kBlock) void dia_offsets_kernel(const I* __restrict
                                                              uint32_t* __restrict__ overflow)
 {
   __shared__ uint32_t keys[kDiaHash];
-  __shared__ uint32_t full;
+  __shared__ volatile uint32_t full;
   const uint32_t tid = threadIdx.x;
   for (uint32_t i = tid; i < kDiaHash; i += kBlock) keys[i] = 0;
   if (tid == 0) full = 0;
@@ -39,21 +39,22 @@ __global__ __launch_bounds__(kBlock) void dia_offsets_kernel(const I* __restrict
   const uint32_t row = blockIdx.x * kBlock + tid;
   if (row < n) {
     const uint32_t e1 = rp[row + 1];
-    for (uint32_t e = rp[row]; e < e1; ++e) {
+    for (uint32_t e = rp[row]; e < e1 && !full; ++e) {  // once the table is full the answer is known
       const uint32_t c = col[e];
       if (c >= n) {  // not a square-matrix column: no DIA copy
         full = 1;
-        continue;
+        break;
       }
       const uint32_t key = c + n - row;  // in [1, 2n - 1]; 0 marks an empty slot
       uint32_t h = (key * 2654435761u) >> 25;
       uint32_t probe = 0;
-      for (; probe < kDiaHash; ++probe) {
+      // kDiaMax + 1 occupied slots in a row mean more than kDiaMax distinct offsets: not a candidate
+      for (; probe <= kDiaMax; ++probe) {
         const uint32_t prev = atomicCAS(&keys[h], 0u, key);
         if (prev == 0u || prev == key) break;
         h = (h + 1) & (kDiaHash - 1);
       }
-      if (probe == kDiaHash) full = 1;
+      if (probe > kDiaMax) full = 1;
     }
   }
   __syncthreads();
@@ -103,6 +104,7 @@ static int build_dia(const vclhip_csr* A, Workspace& ws, hipStream_t s, double m
   const uint32_t n = A->n_rows;
   if (n == 0 || A->n_cols != n || n >= (1u << 30) || !A->row_ptr || !A->col_idx || !A->values)
     return VCLHIP_SUCCESS;
+  if (static_cast<uint64_t>(A->nnz) > static_cast<uint64_t>(kDiaMax) * n) return VCLHIP_SUCCESS;
   const size_t words = (2ull * n + 64) / 64;
   unsigned long long* bits;
   uint32_t* overflow;
