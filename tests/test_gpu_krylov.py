@@ -222,6 +222,37 @@ def test_dia_operator_equals_csr(dev, vh, kind):
     assert ra.iters == rg.iters and ra.error == rg.error and ra.converged and np.array_equal(xa, xg)
 
 
+def test_dia_operator_skips_unordered_rows(dev, vh):
+    """A row whose columns are not strictly increasing (unsorted, or an entry split in two) keeps the
+    CSR operator: the default solve equals the as-given one bit for bit and solves the summed matrix."""
+    rp, col, val = vh.laplace_3d(14, 13, 12)
+    n = rp.size - 1
+    b = vh.random_uniform(4, n)
+    # row 100: reverse its entries; row 200: split its diagonal into two entries of half the value
+    col, val = col.copy(), val.copy()
+    a, e = int(rp[100]), int(rp[101])
+    col[a:e], val[a:e] = col[a:e][::-1].copy(), val[a:e][::-1].copy()
+    a, e = int(rp[200]), int(rp[201])
+    d = a + int(np.nonzero(col[a:e] == 200)[0][0])
+    col = np.insert(col, d, 200)
+    val = np.insert(val, d, val[d] / 2)
+    val[d + 1] /= 2
+    rp = rp.astype(np.int64)
+    rp[201:] += 1
+    rp = rp.astype(np.uint32)
+    A = vh.CsrMatrix(rp, col, val)
+    out = []
+    for of in (0, 1):
+        db, dx = vh.DeviceArray.from_numpy(b), vh.DeviceArray.zeros(n, np.float64)
+        res = vh.cg(A, db, dx, tol=1e-10, max_iters=2000, operator_format=of)
+        out.append((dx.to_numpy(), res))
+    (xa, ra), (xg, rg) = out
+    assert ra.converged and ra.iters == rg.iters and np.array_equal(xa, xg)
+    import scipy.sparse as sp
+    M = sp.csr_matrix((val, col.astype(np.int64), rp.astype(np.int64)), shape=(n, n))
+    assert np.linalg.norm(b - M @ xa) / np.linalg.norm(b) < 1e-9
+
+
 @pytest.mark.parametrize("precond", [0, 1])
 def test_cg_compact_mode_matches_three_kernel_form(dev, vh, precond):
     """Small systems run CG with the scalar step fused into the next update (two kernels per

@@ -66,20 +66,23 @@ __global__ __launch_bounds__(kBlock) void dia_offsets_kernel(const I* __restrict
 }
 
 // val[d * n + row] = A(row, row + off[d]) converted to T (val zeroed before); off ascending,
-// nd <= kDiaMax, and every offset of the matrix is in off.
+// nd <= kDiaMax, and every offset of the matrix is in off.  A row whose columns do not strictly
+// increase (unsorted, or an entry given twice) raises *unordered: its sum in column order would not
+// be the CSR kernel's, and a repeated entry would be lost, so the copy is then not used.
 template <typename S, typename T>
 __global__ __launch_bounds__(kBlock) void dia_fill_kernel(const uint32_t* __restrict__ rp,
                                                           const uint32_t* __restrict__ col, const S* __restrict__ a,
                                                           uint32_t n, const int32_t* __restrict__ off, uint32_t nd,
-                                                          T* __restrict__ val)
+                                                          T* __restrict__ val, uint32_t* __restrict__ unordered)
 {
   __shared__ int32_t so[kDiaMax];
   if (threadIdx.x < nd) so[threadIdx.x] = off[threadIdx.x];
   __syncthreads();
   const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
   if (row >= n) return;
-  const uint32_t e1 = rp[row + 1];
-  for (uint32_t e = rp[row]; e < e1; ++e) {
+  const uint32_t e0 = rp[row], e1 = rp[row + 1];
+  for (uint32_t e = e0; e < e1; ++e) {
+    if (e > e0 && col[e] <= col[e - 1]) *unordered = 1u;
     const int32_t o = static_cast<int32_t>(col[e] - row);
     uint32_t lo = 0, hi = nd;
     while (hi - lo > 1) {
@@ -139,10 +142,13 @@ static int build_dia(const vclhip_csr* A, Workspace& ws, hipStream_t s, double m
   VCLHIP_TRY(ws.alloc(&val, static_cast<size_t>(nd) * n));
   VCLHIP_HIP(hipMemcpyAsync(doff, offs.data(), nd * sizeof(int32_t), hipMemcpyHostToDevice, s));
   VCLHIP_HIP(hipMemsetAsync(val, 0, vbytes, s));
+  VCLHIP_HIP(hipMemsetAsync(overflow, 0, sizeof(uint32_t), s));  // reused as the unordered-row flag
   hipLaunchKernelGGL((dia_fill_kernel<S, T>), dim3(g), dim3(kBlock), 0, s, A->row_ptr, A->col_idx,
-                     static_cast<const S*>(A->values), n, doff, nd, val);
+                     static_cast<const S*>(A->values), n, doff, nd, val, overflow);
   VCLHIP_TRY(check_launch("dia_fill_kernel"));
+  VCLHIP_HIP(hipMemcpyAsync(&hover, overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   VCLHIP_HIP(hipStreamSynchronize(s));  // offs lives on the host stack
+  if (hover) return VCLHIP_SUCCESS;
   out->val = val;
   out->off = doff;
   out->n_rows = n;
