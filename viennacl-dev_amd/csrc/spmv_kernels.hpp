@@ -70,8 +70,10 @@ __device__ __forceinline__ T load_x(const T* __restrict__ x, uint32_t c, uint32_
   return x[static_cast<size_t>(c) * xi + xs];
 }
 
-// Sum of the ELL entries of one row in column order; zero entries are skipped without reading x
-// (host_based/sparse_matrix_operations.hpp:1516-1524, :1897-1904).
+// Sum of the ELL entries of one row in column order; zero entries are left out of the sum
+// (host_based/sparse_matrix_operations.hpp:1516-1524, :1897-1904).  The value and the column are
+// loaded together and x then once, at the column or, for a zero value (whose column may be any
+// padding), at x's first element: two dependent round trips instead of three.
 template <typename T, bool Contig>
 __device__ __forceinline__ T ell_row_sum(const uint32_t* __restrict__ coords, const T* __restrict__ val,
                                          uint32_t ld, uint32_t width, uint32_t row, const T* __restrict__ x,
@@ -82,7 +84,10 @@ __device__ __forceinline__ T ell_row_sum(const uint32_t* __restrict__ coords, co
   for (uint32_t k = 0; k < width; ++k) {
     const size_t off = static_cast<size_t>(k) * ld + row;
     const T v = val[off];
-    if (v > 0 || v < 0) sum += load_x<T, Contig>(x, coords[off], xs, xi) * v;
+    const uint32_t c = coords[off];
+    const bool nz = v > 0 || v < 0;
+    const T xv = load_x<T, Contig>(x, nz ? c : 0u, xs, xi);
+    if (nz) sum += xv * v;
   }
   return sum;
 }
