@@ -330,6 +330,7 @@ def mixed_vs_fp64(vh, grid, reps=2):
         xh = xm.to_numpy()
         rec["rel_error_vs_fp64_solution"] = float(np.linalg.norm(xh - x64h) / np.linalg.norm(x64h))
         rec["time_ratio_vs_fp64"] = round(rec["seconds"] / f64["seconds"], 4)
+        rec["time_ratio_vs_fp64_csr_as_given"] = round(rec["seconds"] / g64["seconds"], 4)
     S = vh.SellMatrix(rp, col, val, 64)
     s64 = run("cg_fp64_sell64", lambda x: vh.cg(S, b, x, tol=1e-10, max_iters=20000), xm)
     rec = run("mixed_reliable_sell64", lambda x: vh.mixed_cg(S, b, x, tol=1e-10, max_iters=20000, mixed_mode=1), xm)
