@@ -182,6 +182,11 @@ __device__ __forceinline__ void batch_spmv_staged(const BatchDev<T>& B, uint32_t
         v[k] = B.val[e];
       }
 #pragma unroll
+      for (int k = 0; k < U; ++k) {  // every entry's loads in flight before the first conditional store
+        keep_here(c[k]);
+        keep_here(v[k]);
+      }
+#pragma unroll
       for (int k = 0; k < U; ++k) {
         const uint32_t i = tid + k * kBlock;
         if (i < cnt) {

@@ -18,6 +18,17 @@ constexpr int kBlock = 256;           // lanes per workgroup of every kernel her
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr uint32_t kMaxPartials = 4096; // upper bound on workgroups that write partials
 
+// Makes a register value opaque to the compiler at this point: the loads that produce it are
+// issued and waited for here, and cannot be sunk into a later conditional use.  The staged SpMV
+// loops use it so that every entry's loads are in flight together; without it hipcc moved the
+// first entry's column, value and x loads into its conditional LDS store, behind full waits for
+// all the other loads (two more memory round trips per row block).
+template <typename T>
+__device__ __forceinline__ void keep_here(T& v)
+{
+  asm volatile("" : "+v"(v));
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
 {

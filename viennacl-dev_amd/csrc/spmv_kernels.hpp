@@ -183,13 +183,18 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(std::conditional_t<Hyb, Hyb
         c[k] = ld_stream<NT>(A.col + static_cast<size_t>(e) * CS);
         v[k] = ld_stream<NT>(A.val + e);
       }
-      T xv[U];
+      T pr[U];
 #pragma unroll
-      for (int k = 0; k < U; ++k) xv[k] = load_x<T, Contig>(x, c[k], xs, xi);
+      for (int k = 0; k < U; ++k) pr[k] = load_x<T, Contig>(x, c[k], xs, xi);
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        pr[k] = v[k] * pr[k];
+        keep_here(pr[k]);
+      }
 #pragma unroll
       for (int k = 0; k < U; ++k) {
         const uint32_t i = tid + k * kBlock;
-        if (i < cnt) stage[i] = v[k] * xv[k];
+        if (i < cnt) stage[i] = pr[k];
       }
     }
     if (tid == 0) srp[0] = 0;
