@@ -100,6 +100,7 @@ template <typename T, bool Jac, bool Full>
 struct EpiBicg {
   using acc_t = double;
   static constexpr int K = Full ? 3 : 1;
+  static constexpr bool kDiaGrouped = false;  // BiCGStab on a 128^3 DIA operator: 38 ms counted, 41-43 ms grouped
   T* __restrict__ out;
   const T* __restrict__ p;
   const T* __restrict__ r0;

@@ -241,6 +241,8 @@ __global__ __launch_bounds__(kBlock) void csr_kernel(std::conditional_t<Hyb, Hyb
   if constexpr (Epi::K > 0) block_store_partials<Epi::K, acc_t>(acc, part, pstride, lw);
 }
 
+constexpr uint32_t kSellGroup = 8;  // columns whose loads a SELL row issues together
+
 // SELL-C SpMV, one lane per row; with C = 64 a wavefront is exactly one slice, so all lanes of a
 // wavefront run the same number of columns and every load is one contiguous 64-lane access.
 // Delta: the columns are stored as 16-bit offsets from the row (the mixed solver's inner copy),
@@ -274,25 +276,27 @@ __global__ __launch_bounds__(kBlock) void sell_kernel(SellDev<T> A, const T* __r
       const size_t base = static_cast<size_t>(A.start[s]) + lane;
       const T* __restrict__ ev = A.val + base;
       T sum = 0;
-      if constexpr (Delta) {
-        const int16_t* __restrict__ dv = A.dcol + base;
-#pragma unroll 4
-        for (uint32_t k = 0; k < ncols; ++k) {
-          const size_t off = static_cast<size_t>(k) * A.C;
-          const T v = ld_stream<NT>(ev + off);
-          const uint32_t c = static_cast<uint32_t>(static_cast<int32_t>(row) + ld_stream<NT>(dv + off));
-          const T xv = load_x<T, Contig>(x, c, xs, xi);
-          sum += (v > 0 || v < 0) ? xv * v : T(0);
+      // Groups of kSellGroup columns: every load of a group is issued before the first sum, the
+      // column index clamped so that a short last group re-reads its last column instead of
+      // branching (a counted loop left a 7-column slice mostly to the unroll remainder, one column
+      // per memory round trip).  Zero values add nothing (host_based/sparse_matrix_operations.hpp:1831-1833).
+      for (uint32_t k0 = 0; k0 < ncols; k0 += kSellGroup) {
+        T v[kSellGroup], xv[kSellGroup];
+        uint32_t c[kSellGroup];
+#pragma unroll
+        for (uint32_t j = 0; j < kSellGroup; ++j) {
+          const size_t off = static_cast<size_t>(min(k0 + j, ncols - 1)) * A.C;
+          v[j] = ld_stream<NT>(ev + off);
+          if constexpr (Delta)
+            c[j] = static_cast<uint32_t>(static_cast<int32_t>(row) + ld_stream<NT>(A.dcol + base + off));
+          else
+            c[j] = ld_stream<NT>(A.col + base + off);
         }
-      } else {
-        const uint32_t* __restrict__ cv = A.col + base;
-#pragma unroll 4
-        for (uint32_t k = 0; k < ncols; ++k) {
-          const size_t off = static_cast<size_t>(k) * A.C;
-          const T v = ld_stream<NT>(ev + off);
-          const T xv = load_x<T, Contig>(x, ld_stream<NT>(cv + off), xs, xi);
-          sum += (v > 0 || v < 0) ? xv * v : T(0);  // host_based/sparse_matrix_operations.hpp:1831-1833
-        }
+#pragma unroll
+        for (uint32_t j = 0; j < kSellGroup; ++j) xv[j] = load_x<T, Contig>(x, c[j], xs, xi);
+#pragma unroll
+        for (uint32_t j = 0; j < kSellGroup; ++j)
+          if (k0 + j < ncols) sum += (v[j] > 0 || v[j] < 0) ? xv[j] * v[j] : T(0);
       }
       epi(row, sum, acc);
     }
@@ -300,6 +304,15 @@ __global__ __launch_bounds__(kBlock) void sell_kernel(SellDev<T> A, const T* __r
 
   if constexpr (Epi::K > 0) block_store_partials<Epi::K, acc_t>(acc, part, pstride, lw);
 }
+
+constexpr uint32_t kDiaGroup = 8;  // diagonals whose loads a DIA row issues together
+
+// Whether dia_kernel issues a group's loads together (the default) or runs the counted loop: an
+// epilogue opts out with `static constexpr bool kDiaGrouped = false`.
+template <class E, class = void>
+struct dia_grouped : std::true_type {};
+template <class E>
+struct dia_grouped<E, std::void_t<decltype(E::kDiaGrouped)> > : std::bool_constant<E::kDiaGrouped> {};
 
 // DIA: nd diagonals of n values each, diagonal-major (val[d * n + row] multiplies x[row + off[d]]),
 // offsets ascending.  The mixed-precision solver's inner copy of a stencil-like matrix.
@@ -334,13 +347,37 @@ __global__ __launch_bounds__(kBlock) void dia_kernel(DiaDev<T> A, const T* __res
     const uint32_t row = t * kBlock + tid;
     if (row < A.n_rows) {
       T sum = 0;
-      // x is read wherever the column exists, whatever the value, so that no load waits for another
+      if constexpr (dia_grouped<Epi>::value) {
+        // Groups of kDiaGroup diagonals: every load of a group is issued before the first sum (the
+        // test for the diagonals past A.nd is uniform, a scalar branch), and x is read wherever the
+        // column exists, whatever the value.  A counted loop over A.nd runs a 7-diagonal stencil
+        // entirely in the compiler's unroll remainder, one diagonal per memory round trip.
+        for (uint32_t d0 = 0; d0 < A.nd; d0 += kDiaGroup) {
+          T v[kDiaGroup], xv[kDiaGroup];
+#pragma unroll
+          for (uint32_t j = 0; j < kDiaGroup; ++j) {
+            v[j] = T(0);
+            xv[j] = T(0);
+            if (d0 + j < A.nd) {
+              const uint32_t d = d0 + j;
+              const uint32_t c = row + static_cast<uint32_t>(A.off[d]);  // out of range (also below 0): no entry
+              v[j] = A.val[static_cast<size_t>(d) * A.n_rows + row];
+              if (c < A.n_rows) xv[j] = x[c];
+            }
+          }
+#pragma unroll
+          for (uint32_t j = 0; j < kDiaGroup; ++j)
+            if (d0 + j < A.nd) sum += (v[j] > 0 || v[j] < 0) ? xv[j] * v[j] : T(0);
+        }
+      } else {
+        // the counted loop: fewer loads in flight, measured faster under the BiCGStab epilogues
 #pragma unroll 8
-      for (uint32_t d = 0; d < A.nd; ++d) {
-        const uint32_t c = row + static_cast<uint32_t>(A.off[d]);  // out of range (also below 0): no entry
-        const T v = A.val[static_cast<size_t>(d) * A.n_rows + row];
-        const T xv = c < A.n_rows ? x[c] : T(0);
-        sum += (v > 0 || v < 0) ? xv * v : T(0);
+        for (uint32_t d = 0; d < A.nd; ++d) {
+          const uint32_t c = row + static_cast<uint32_t>(A.off[d]);
+          const T v = A.val[static_cast<size_t>(d) * A.n_rows + row];
+          const T xv = c < A.n_rows ? x[c] : T(0);
+          sum += (v > 0 || v < 0) ? xv * v : T(0);
+        }
       }
       epi(row, sum, acc);
     }
