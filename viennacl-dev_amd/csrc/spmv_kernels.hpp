@@ -74,20 +74,29 @@ __device__ __forceinline__ T load_x(const T* __restrict__ x, uint32_t c, uint32_
 // (host_based/sparse_matrix_operations.hpp:1516-1524, :1897-1904).  The value and the column are
 // loaded together and x then once, at the column or, for a zero value (whose column may be any
 // padding), at x's first element: two dependent round trips instead of three.
+constexpr uint32_t kEllGroup = 8;  // ELL entries whose loads a row issues together
+
 template <typename T, bool Contig>
 __device__ __forceinline__ T ell_row_sum(const uint32_t* __restrict__ coords, const T* __restrict__ val,
                                          uint32_t ld, uint32_t width, uint32_t row, const T* __restrict__ x,
                                          uint32_t xs, uint32_t xi)
 {
+  // groups of kEllGroup entries, all loads of a group before its sums (clamped indices, no branch)
   T sum = 0;
-#pragma unroll 4
-  for (uint32_t k = 0; k < width; ++k) {
-    const size_t off = static_cast<size_t>(k) * ld + row;
-    const T v = val[off];
-    const uint32_t c = coords[off];
-    const bool nz = v > 0 || v < 0;
-    const T xv = load_x<T, Contig>(x, nz ? c : 0u, xs, xi);
-    if (nz) sum += xv * v;
+  for (uint32_t k0 = 0; k0 < width; k0 += kEllGroup) {
+    T v[kEllGroup], xv[kEllGroup];
+    uint32_t c[kEllGroup];
+#pragma unroll
+    for (uint32_t j = 0; j < kEllGroup; ++j) {
+      const size_t off = static_cast<size_t>(min(k0 + j, width - 1)) * ld + row;
+      v[j] = val[off];
+      c[j] = coords[off];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kEllGroup; ++j) xv[j] = load_x<T, Contig>(x, (v[j] > 0 || v[j] < 0) ? c[j] : 0u, xs, xi);
+#pragma unroll
+    for (uint32_t j = 0; j < kEllGroup; ++j)
+      if (k0 + j < width && (v[j] > 0 || v[j] < 0)) sum += xv[j] * v[j];
   }
   return sum;
 }
