@@ -21,7 +21,12 @@ import subprocess
 import sys
 import time
 
-import numpy as np
+# The CPU baseline pins its OpenMP threads as the survey's measurement rules say (close binding,
+# one thread per core); libgomp reads these when the reference build is first loaded.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+import numpy as np  # noqa: E402
 
 ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "viennacl-dev_amd"))
@@ -102,8 +107,14 @@ def time_launches(vh, launch, stream, steps, warmup, evict=None):
     return cold, warm, wall
 
 
-def stream_reference(vh, stream, steps):
-    """z = a x + b y over 64M doubles (1.5 GB moved per launch): the streaming rate of this box."""
+def stream_reference(vh, stream, steps, evict=None, spmv_bytes=None, spmv_blocks=None):
+    """The streaming rates of this device, kernel-timed like the SpMV:
+    * axpby: z = a x + b y over 64M doubles (1.5 GB moved per launch);
+    * read_1gib: a read-only kernel (vclhip_stream_read_f64) over 1 GiB, 16-byte loads, 8 in flight
+      per lane, one contiguous chunk per workgroup: the read ceiling;
+    * read_spmv_shaped: the same kernel over as many bytes as the CSR SpMV reads, one contiguous
+      chunk per row block of the SpMV (about 24 KB each), cold (after the 1 GiB eviction read) and
+      warm: what a perfect stream of the SpMV's bytes in its launch shape would take."""
     x = vh.DeviceArray.zeros(STREAM_N, np.float64)
     y = vh.DeviceArray.zeros(STREAM_N, np.float64)
     z = vh.DeviceArray.zeros(STREAM_N, np.float64)
@@ -112,11 +123,31 @@ def stream_reference(vh, stream, steps):
         vh.call("vclhip_axpby_f64", STREAM_N, 1.5, x.ptr, -0.5, y.ptr, z.ptr, stream.handle)
 
     _, warm, _ = time_launches(vh, launch, stream, steps, 2)
-    gbs = 24 * STREAM_N / (warm * 1e-3) / 1e9
-    for a in (x, y, z):
-        a.free()
-    return {"kernel": "axpby_kernel<double>", "bytes_per_launch": 24 * STREAM_N, "ms": round(warm, 6),
-            "gbs": round(gbs, 2)}
+    out = {"axpby": {"kernel": "axpby_kernel<double>", "bytes_per_launch": 24 * STREAM_N, "ms": round(warm, 6),
+                     "gbs": round(24 * STREAM_N / (warm * 1e-3) / 1e9, 2)}}
+    y.free()
+    z.free()
+    # read-only: 1 GiB (x and y's storage, 128M doubles; larger than the Infinity Cache)
+    big = vh.DeviceArray.zeros(2 * STREAM_N, np.float64)
+    sums = vh.DeviceArray.zeros(1 << 16, np.float64)
+    grid = 8 * 256 * 8  # eight workgroups per CU
+    _, warm, _ = time_launches(vh, lambda: vh.call("vclhip_stream_read_f64", C.c_uint64(2 * STREAM_N), big.ptr,
+                                                   sums.ptr, grid, stream.handle), stream, steps, 2)
+    out["read_1gib"] = {"kernel": "stream_read_kernel", "bytes_per_launch": 16 * STREAM_N, "workgroups": grid,
+                        "ms": round(warm, 6), "gbs": round(16 * STREAM_N / (warm * 1e-3) / 1e9, 2)}
+    if spmv_bytes and spmv_blocks and evict is not None:
+        m = spmv_bytes // 8
+        g = (spmv_blocks + 7) // 8 * 8
+        cold, warm, _ = time_launches(vh, lambda: vh.call("vclhip_stream_read_f64", C.c_uint64(m), big.ptr, sums.ptr,
+                                                          g, stream.handle), stream, steps, 2, evict)
+        out["read_spmv_shaped"] = {"kernel": "stream_read_kernel", "bytes_per_launch": 8 * m, "workgroups": g,
+                                   "cold_ms": round(float(cold.mean()), 6), "warm_ms": round(warm, 6),
+                                   "cold_gbs": round(8 * m / (cold.mean() * 1e-3) / 1e9, 2),
+                                   "warm_gbs": round(8 * m / (warm * 1e-3) / 1e9, 2)}
+    big.free()
+    sums.free()
+    x.free()
+    return out
 
 
 def true_relres(vh, A, b, x, n):
@@ -473,6 +504,43 @@ def adjoint_bench(vh, grid, n_fd=64):
     return out
 
 
+def host_solve(po, R, rp, col, val, b, tol, max_iters, kind, reps=1):
+    """The reference's own solve() on its host backend (ref_bench_solve: kind 0 CG, 2 mixed-precision
+    CG), best of `reps`, timed around solve() only."""
+    rp, col, val, b = (np.ascontiguousarray(a) for a in (rp, col, val, b))
+    x = np.zeros(rp.size - 1)
+    it, err = C.c_uint32(), C.c_double()
+    best = None
+    for _ in range(reps):
+        t = R.ref_bench_solve(rp.size - 1, po.p(rp), po.p(col), po.p(val), po.p(b), po.p(x), tol, max_iters, kind, 0,
+                              C.byref(it), C.byref(err))
+        best = t if best is None else min(best, t)
+    r = b - po.csr_spmv(rp, col, val, x)
+    return {"seconds": round(best, 4), "iters": int(it.value), "error": float(err.value),
+            "ms_per_iter": round(best * 1e3 / max(it.value, 1), 4),
+            "true_relres": float(np.linalg.norm(r) / np.linalg.norm(b))}
+
+
+def cpu_solver_baseline(vh, po, R, mixed_grid):
+    """The CG metric on the host: the reference's own CG (cg.hpp:128-187) on the 128^3 Laplacian
+    (C2) and the FP64 and mixed-precision CG (mixed_precision_cg.hpp:95-187) on the 160^3 diffusion
+    operator (C3), same inputs as the device runs, to 1e-10."""
+    out = {}
+    rp, col, val = vh.laplace_3d(128)
+    out["c2_cg_lap3d_128"] = host_solve(po, R, rp, col, val, vh.random_uniform(0, rp.size - 1), 1e-10, 5000, 0)
+    log("CPU baseline c2", out["c2_cg_lap3d_128"])
+    if mixed_grid:
+        g = mixed_grid
+        k = np.exp(vh.random_normal(1, g ** 3))
+        rp, col, val = vh.diffusion(g, g, g, k)
+        b = vh.random_uniform(0, rp.size - 1)
+        out[f"c3_cg_fp64_diffusion_{g}"] = host_solve(po, R, rp, col, val, b, 1e-10, 20000, 0)
+        log("CPU baseline c3 fp64", out[f"c3_cg_fp64_diffusion_{g}"])
+        out[f"c3_mixed_cg_diffusion_{g}"] = host_solve(po, R, rp, col, val, b, 1e-10, 20000, 2)
+        log("CPU baseline c3 mixed", out[f"c3_mixed_cg_diffusion_{g}"])
+    return out
+
+
 def cpu_baseline(rp, col, val, x, n, nnz, seconds, c1=None):
     """The reference host backend (OpenMP) timed on this machine's cores over a bounded sample; with
     c1 = (rp, col, val, b), also the reference's own CG solve of the 256x256 case."""
@@ -488,7 +556,9 @@ def cpu_baseline(rp, col, val, x, n, nnz, seconds, c1=None):
     gbs = csr_bytes(n, nnz) * reps / t / 1e9
     out = {"value": round(gbs, 2), "unit": "GB/s", "cores": int(R.ref_max_threads()), "kind": "reference",
            "sample": f"{reps} FP64 CSR SpMVs y=Ax of the same 128^3 Laplacian (viennacl::linalg::prod, "
-                     f"host_based OpenMP backend), {t:.1f} s"}
+                     f"host_based OpenMP backend), {t:.1f} s",
+           "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+           "host_cpus": os.cpu_count()}
     if c1 is not None:
         crp, ccol, cval, cb = (np.ascontiguousarray(a) for a in c1)
         cx = np.zeros(crp.size - 1)
@@ -606,8 +676,8 @@ def main():
     sell_gbs = bytes_sell / (s_cold.mean() * 1e-3) / 1e9
     log(f"SELL-64 SpMV cold {s_cold.mean() * 1e3:.1f} us ({sell_gbs:.0f} GB/s), warm {s_warm * 1e3:.1f} us")
     y_gpu = y.to_numpy()
+    ref = stream_reference(vh, stream, 10, evict, bytes_csr, A.num_blocks)
     evict.free()
-    ref = stream_reference(vh, stream, 10)
     log("streaming reference", ref)
 
     traffic = None
@@ -677,6 +747,9 @@ def main():
         cb = cpu_baseline(rp, col, val, xh, n, nnz, args.cpu_seconds, None if args.no_solvers else c1_inputs)
         if cb is not None:
             base, y_cpu = cb
+            if not args.no_solvers:
+                import pyoracle as po
+                base.update(cpu_solver_baseline(vh, po, po.ref(), args.mixed_grid))
             out["cpu_baseline"] = base
             out["cpu_vs_gpu_relinf"] = float(np.max(np.abs(y_cpu - y_gpu)) / np.max(np.abs(y_cpu)))
             log("CPU baseline", base)

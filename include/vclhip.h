@@ -78,6 +78,11 @@ int vclhip_event_elapsed_ms(void* start, void* stop, float* ms);
  * stop_event at the kernel's own start and end (hipExtLaunchKernelGGL), so that one kernel can be
  * timed without the latency of separate event packets.  The request applies to one launch. */
 int vclhip_time_next_launch(void* start_event, void* stop_event);
+/* Read-only streaming reference (no counterpart in the reference; measurement only): num_sums
+ * workgroups (a multiple of 8, at most 2^20) each read a contiguous chunk of x[0..n) with 16-byte
+ * loads and store its sum in sums[0..num_sums).  x must be 16-byte aligned.  Its kernel time over
+ * 8 n bytes is the rate at which the device reads HBM; vclhip_time_next_launch applies to it. */
+int vclhip_stream_read_f64(uint64_t n, const double* x_dev, double* sums_dev, uint32_t num_sums, void* stream);
 
 /* ------------------------------------------------------------ sparse formats ---- */
 /* CSR matrix on the device (compressed_matrix<T>: handle1 = row_ptr, handle2 = col, handle = values,
@@ -433,10 +438,17 @@ typedef struct vclhip_solver_params {
   uint32_t krylov_dim;               /* GMRES: steps per restart cycle (gmres_tag default 20) */
   int operator_format;               /* CG, BiCGStab, GMRES on a CSR matrix: VCLHIP_OPERATOR_AUTO
                                         (default) solves on a DIA copy in workspace memory when the
-                                        matrix has at most 32 diagonals storing at most 25% zeros
-                                        and the copy takes at most a quarter of the free device
-                                        memory -- the same results, fewer bytes per product;
-                                        VCLHIP_OPERATOR_AS_GIVEN always uses the CSR matrix */
+                                        matrix has at most 32 diagonals storing at most 25% zeros,
+                                        every stored value is nonzero and not NaN, every row's columns
+                                        strictly increase, and the copy takes at most a quarter of the
+                                        free device memory -- the same results bit for bit (also for
+                                        non-finite x), fewer bytes per product;
+                                        VCLHIP_OPERATOR_AS_GIVEN always uses the CSR matrix; other
+                                        values are rejected */
+  const struct vclhip_dia* dia;      /* optional, with VCLHIP_OPERATOR_AUTO: the detection and DIA
+                                        copy made once by vclhip_dia_create_* for this CSR matrix,
+                                        used instead of detecting and copying in every solve (the
+                                        matrix's arrays must be the ones it was made from) */
 } vclhip_solver_params;
 #define VCLHIP_OPERATOR_AUTO 0
 #define VCLHIP_OPERATOR_AS_GIVEN 1
@@ -452,10 +464,29 @@ typedef struct vclhip_solver_result {
   double error;       /* tag.error() */
   uint32_t restarts;  /* mixed CG outer corrections / BiCGStab restarts */
   int converged;
+  int operator_used;  /* the storage the products ran on: VCLHIP_USED_AS_GIVEN or VCLHIP_USED_DIA;
+                         mixed CG: its FP32 inner copy, VCLHIP_INNER_CSR (the matrix's own format),
+                         VCLHIP_INNER_SELL64, VCLHIP_INNER_SELL64_D16 or VCLHIP_INNER_DIA */
 } vclhip_solver_result;
+#define VCLHIP_USED_AS_GIVEN 0
+#define VCLHIP_USED_DIA 1
 
 /* Fill params with the defaults of cg_tag() / bicgstab_tag() / mixed_precision_cg_tag(). */
 void vclhip_solver_params_default(vclhip_solver_params* params);
+
+/* A DIA copy of a CSR matrix kept across solves (no reference counterpart; the solvers otherwise
+ * detect the diagonals and copy the matrix in every solve).  vclhip_dia_create_* runs that
+ * detection once: *dia always receives an object, holding the copy in device memory when A is a
+ * candidate (at most 32 diagonals storing at most 25% zeros, nonzero non-NaN values, strictly
+ * increasing columns, at most a quarter of the free device memory) and only the negative answer
+ * otherwise.  Pass it as params->dia to solves of this matrix; it stays valid while A's arrays and
+ * values are unchanged, and the caller rebuilds it after changing them.  Synchronizes the stream. */
+typedef struct vclhip_dia vclhip_dia;
+int vclhip_dia_create_f64(const vclhip_csr* A, vclhip_dia** dia, void* stream);
+int vclhip_dia_create_f32(const vclhip_csr* A, vclhip_dia** dia, void* stream);
+/* 1 when the object holds a DIA copy (the solves then run on it), else 0; nd receives its diagonal count. */
+int vclhip_dia_info(const vclhip_dia* dia, int* built, uint32_t* nd);
+int vclhip_dia_destroy(vclhip_dia* dia);
 
 #define VCLHIP_PRECOND_NONE 0
 #define VCLHIP_PRECOND_JACOBI 1   /* jacobi_precond<compressed_matrix> (jacobi_precond.hpp:103-130) */

@@ -276,7 +276,8 @@ double ref_bench_csr_spmv(uint32_t n, const uint32_t* rp, const uint32_t* col, c
   return t1 - t0;
 }
 
-// Solve time of cg (jacobi = 0/1) or bicgstab (kind = 1); iterations and error returned.
+// Solve time of cg (kind = 0, jacobi = 0/1), bicgstab (kind = 1) or mixed_precision_cg (kind = 2); iterations
+// and error returned.
 double ref_bench_solve(uint32_t n, const uint32_t* rp, const uint32_t* col, const double* val, const double* b,
                        double* x, double tol, uint32_t max_iters, int kind, int jacobi, uint32_t* iters, double* error)
 {
@@ -297,6 +298,15 @@ double ref_bench_solve(uint32_t n, const uint32_t* rp, const uint32_t* col, cons
       vx = viennacl::linalg::solve(A, vb, tag);
       t1 = omp_get_wtime();
     }
+    *iters = tag.iters();
+    *error = tag.error();
+  } else if (kind == 2) {
+    // mixed_precision_cg.hpp:95-187 with its default inner tolerance: the FP32 copy of A is made
+    // inside solve(), as the reference does, so it is part of the timed solve
+    viennacl::linalg::mixed_precision_cg_tag tag(tol, max_iters);
+    t0 = omp_get_wtime();
+    vx = viennacl::linalg::solve(A, vb, tag);
+    t1 = omp_get_wtime();
     *iters = tag.iters();
     *error = tag.error();
   } else {

@@ -76,13 +76,85 @@ def gradients(nx=48, ny=40, n_fd=16):
     return ok, {"host_rel": host_rel, "db_rel": db_rel, "fd_worst_rel": worst, "fd_bad": fd_bad}
 
 
+def gradients_full(nx=512, n_fd=64):
+    """Config C5 at its size: the 512 x 512 variable-coefficient diffusion operator (n = 262,144),
+    theta = exp(N(0, 0.5)) per cell, L = 0.5 ||x - x_t||^2 with x_t ~ U[-1, 1].  dL/dtheta from the
+    device (forward CG, adjoint CG, fused dA kernel, torch chain rule) against (1) the FP64 host
+    gradient: oracle CG for x and lambda (16 threads), oracle dA, the same chain rule on the CPU, and
+    (2) central differences with h = 1e-6 on 64 random entries.  Solves run to 1e-13.  Bounds: the
+    two gradients of (1) are two solutions of the same systems to the same tolerance, so they agree
+    to about the condition number times 1e-13 (relative 2-norm 1e-7 asserted); the differences of (2)
+    carry the rounding of L (about 1e-16 |L| / h) and the solves' error, so each entry is compared
+    on a floor of |g|_inf: |fd_i - g_i| <= 1e-4 max(|g_i|, 1e-2 |g|_inf)."""
+    ny = nx
+    n = nx * ny
+    theta = np.exp(0.5 * vh.random_normal(1, n))
+    b = vh.random_uniform(0, n)
+    xt = vh.random_uniform(2, n)
+    rp, col, pos, masks = ts.diffusion_pattern_2d(nx, ny)
+    pat = ts.CsrPattern(rp, col)
+    dev = torch.device("cuda")
+    th = torch.tensor(theta.reshape(ny, nx), dtype=torch.float64, device=dev, requires_grad=True)
+    bt = torch.tensor(b, dtype=torch.float64, device=dev, requires_grad=True)
+    xtt = torch.tensor(xt, dtype=torch.float64, device=dev)
+    tol = 1e-13
+
+    def loss_of(theta_t):
+        vals = ts.diffusion_values_2d(theta_t, nx, ny, pos, masks)
+        x = ts.sparse_solve(vals, bt, pat, tol=tol, max_iters=50000)
+        return 0.5 * torch.sum((x - xtt) ** 2)
+
+    L = loss_of(th)
+    L.backward()
+    g = th.grad.detach().cpu().numpy().ravel()
+    gb = bt.grad.detach().cpu().numpy()
+
+    oracle.set_threads(16)
+    vals_h = ts.diffusion_values_2d(torch.tensor(theta.reshape(ny, nx)), nx, ny, pos, masks).numpy()
+    Ah = oracle.CsrHost(rp, col, vals_h)
+    xh, _, _, _ = oracle.cg_pipelined(Ah, b, tol, 50000)
+    lamh, _, _, _ = oracle.cg_pipelined(Ah, xh - xt, tol, 50000)
+    oracle.set_threads(1)
+    dAh = oracle.adjoint_grad(rp, col, lamh, xh)
+    th_c = torch.tensor(theta.reshape(ny, nx), requires_grad=True)
+    ts.diffusion_values_2d(th_c, nx, ny, pos, masks).backward(torch.tensor(dAh))
+    gh = th_c.grad.numpy().ravel()
+    host_rel = float(np.linalg.norm(g - gh) / np.linalg.norm(gh))
+    db_rel = float(np.linalg.norm(gb - lamh) / np.linalg.norm(lamh))
+
+    rng = np.random.default_rng(11)
+    idx = rng.choice(n, n_fd, replace=False)
+    h = 1e-6
+    g_inf = float(np.abs(g).max())
+    fd_bad, worst = [], 0.0
+    for i in idx:
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        with torch.no_grad():
+            Lp = loss_of(torch.tensor(tp.reshape(ny, nx), device=dev)).item()
+            Lm = loss_of(torch.tensor(tm.reshape(ny, nx), device=dev)).item()
+        fd = (Lp - Lm) / (2 * h)
+        scale = max(abs(g[i]), 1e-2 * g_inf)
+        err = abs(fd - g[i]) / scale
+        worst = max(worst, err)
+        if err > 1e-4:
+            fd_bad.append([int(i), fd, float(g[i])])
+    ok = host_rel < 1e-7 and db_rel < 1e-7 and not fd_bad
+    return ok, {"n": n, "host_rel": host_rel, "db_rel": db_rel, "fd_entries": int(n_fd), "fd_worst_scaled": worst,
+                "g_inf": g_inf, "fd_bad": fd_bad}
+
+
 def input_checks():
     rp, col, pos, masks = ts.diffusion_pattern_2d(8, 6)
     pat = ts.CsrPattern(rp, col)
     vals = torch.ones(int(rp[-1]), dtype=torch.float64, device="cuda")
     raised = []
+    nonsym = ts.diffusion_values_2d(torch.ones(6, 8, dtype=torch.float64, device="cuda"), 8, 6, pos, masks)
+    nonsym[int(pos["xp"][3])] -= 0.25  # one entry above the diagonal without its mirror
     for v, b in ((vals.float(), torch.ones(48, dtype=torch.float64, device="cuda")),
-                 (vals, torch.ones(47, dtype=torch.float64, device="cuda"))):
+                 (vals, torch.ones(47, dtype=torch.float64, device="cuda")),
+                 (nonsym, torch.ones(48, dtype=torch.float64, device="cuda"))):
         try:
             ts.sparse_solve(v, b, pat)
             raised.append(False)
@@ -92,7 +164,7 @@ def input_checks():
 
 
 if __name__ == "__main__":
-    check = {"gradients": gradients, "input_checks": input_checks}[sys.argv[1]]
+    check = {"gradients": gradients, "gradients_full": gradients_full, "input_checks": input_checks}[sys.argv[1]]
     ok, info = check()
     print(json.dumps({"ok": ok, **info}), flush=True)
     sys.exit(0 if ok else 1)

@@ -1,4 +1,4 @@
-"""The benchmark configurations at their full sizes (C1-C4 of the survey), through the C ABI.
+"""The benchmark configurations at their full sizes (C1-C5 of the survey), through the C ABI.
 
 C1 is checked against the reference itself (oracle/_ref, the reference's host backend built from
 its sources; the restated oracle where that build is absent).  At the larger sizes the checks are
@@ -68,6 +68,67 @@ def test_c2_cg_lap3d_128(dev, vh, oracle, fmt):
     assert device_relres(vh, Acsr, b, dx.to_numpy()) < 1e-10
 
 
+_C2_ORACLE = {}
+
+
+def c2_oracle(vh, oracle, solver):
+    """The oracle's solve of the 128^3 Laplacian (8 threads, its OpenMP reduction order) for one of
+    the solvers of config C2, computed once per session: (iterations, error, x)."""
+    if solver not in _C2_ORACLE:
+        rp, col, val = vh.laplace_3d(128)
+        b = vh.random_uniform(0, rp.size - 1)
+        Ah = oracle.CsrHost(rp, col, val)
+        d = np.full(rp.size - 1, 6.0)  # the Laplacian's diagonal: the Jacobi preconditioner
+        oracle.set_threads(8)
+        try:
+            if solver == "jacobi_cg":  # cg.hpp:257-322 with jacobi_precond
+                x, it, err, _ = oracle.cg_classic(Ah, b, 1e-10, 5000, diag=d)
+            elif solver == "bicgstab":  # bicgstab.hpp:97-215
+                x, it, err, _ = oracle.bicgstab_pipelined(Ah, b, 1e-10, 5000)
+            else:  # bicgstab.hpp:398-489 with jacobi_precond
+                x, it, err, _ = oracle.bicgstab_classic(Ah, b, 1e-10, 5000, diag=d)
+        finally:
+            oracle.set_threads(1)
+        _C2_ORACLE[solver] = (it, err, x)
+    return _C2_ORACLE[solver]
+
+
+@pytest.mark.parametrize("fmt", ["csr", "sell"])
+@pytest.mark.parametrize("solver", ["jacobi_cg", "bicgstab", "jacobi_bicgstab"])
+def test_c2_solvers_lap3d_128(dev, vh, oracle, solver, fmt):
+    """The other solvers of config C2 on the 128^3 Laplacian (n = 2,097,152), tol 1e-10, in CSR
+    (solved on its DIA copy, the default) and SELL-64: the iteration count within 2% of the
+    oracle's (8 threads), the reported error below the tolerance, x close to the oracle's, and the
+    FP64 true residual.  The criteria differ by solver: Jacobi-PCG stops on the M^-1-norm of the
+    residual (cg.hpp:306-309), which with the constant diagonal 6 of this operator equals the
+    2-norm ratio, so ||b - A x|| / ||b|| < tol; Jacobi-BiCGStab stops on ||M^-1 r|| / ||b|| =
+    ||r|| / (6 ||b||) (bicgstab.hpp:460-461), so its 2-norm ratio is below 6 tol; plain BiCGStab
+    stops on its residual estimate (bicgstab.hpp:195), so below tol up to the estimate's drift."""
+    rp, col, val = vh.laplace_3d(128)
+    n = rp.size - 1
+    b = vh.random_uniform(0, n)
+    A = vh.CsrMatrix(rp, col, val) if fmt == "csr" else vh.SellMatrix(rp, col, val, 64)
+    db = vh.DeviceArray.from_numpy(b)
+    dx = vh.DeviceArray.zeros(n, np.float64)
+    jac = 1 if solver.startswith("jacobi") else 0
+    if solver == "jacobi_cg":
+        res = vh.cg(A, db, dx, precond=jac, tol=1e-10, max_iters=5000)
+    else:
+        res = vh.bicgstab(A, db, dx, precond=jac, tol=1e-10, max_iters=5000)
+    it_o, err_o, x_o = c2_oracle(vh, oracle, solver)
+    print(f"{solver} {fmt}: {res.iters} iterations (oracle {it_o}), error {res.error:.3e} (oracle {err_o:.3e})")
+    assert res.converged and res.error < 1e-10
+    assert iters_close(res.iters, it_o), (res.iters, it_o)
+    if fmt == "csr":
+        assert res.operator_used == vh.USED_DIA
+    x = dx.to_numpy()
+    bound = {"jacobi_cg": 1.0, "bicgstab": 1.05, "jacobi_bicgstab": 6.0}[solver] * 1e-10
+    rel = device_relres(vh, vh.CsrMatrix(rp, col, val), b, x)
+    print(f"  true residual {rel:.3e} (bound {bound:.2e})")
+    assert rel < bound
+    assert np.linalg.norm(x - x_o) / np.linalg.norm(x_o) < 1e-6
+
+
 def test_c2_powerlaw_bicgstab_full_size(dev, vh, oracle):
     """Power-law nonsymmetric matrix, n = 2,097,152 (row lengths P(k) ~ k^-2 on 1..512), Jacobi
     BiCGStab to 1e-10: FP64 true residual at the tolerance (the preconditioned criterion is
@@ -111,22 +172,35 @@ def test_c3_mixed_vs_fp64_160(dev, vh):
     assert np.linalg.norm(a - c) / np.linalg.norm(c) < 1e-6
 
 
-def test_c4_batched_4096(dev, vh):
-    """4096 graph-Laplacian systems, n_i ~ U{64..4096}, tol 1e-8: every system converged with its
-    FP64 true residual at the tolerance."""
+@pytest.mark.parametrize("single", [False, True], ids=["fp64", "fp32"])
+def test_c4_batched_4096(dev, vh, single):
+    """4096 graph-Laplacian systems, n_i ~ U{64..4096}, in FP64 at tol 1e-8 and in FP32 at tol 1e-5
+    (an FP32 solve cannot reach 1e-8: its recursive residual drifts from the true one by about the
+    FP32 resolution times the condition number).  Every system converges on its recursive residual,
+    and its FP64 true residual ||b_i - A_i x_i|| / ||b_i|| (x_i and b_i as the solve holds them,
+    A_i in FP64) is checked: FP64 at the tolerance, FP32 within TRUE_FP32 of it."""
     sys_ptr, rp, col, val = vh.graph_laplacian_batch(4096, 64, 4096, 8.0, 0.1, 1.0, 0.01, 1)
     n = rp.size - 1
-    b = vh.random_uniform(0, n)
-    A = vh.CsrMatrix(rp, col, val)
+    dtype = np.float32 if single else np.float64
+    tol = 1e-5 if single else 1e-8
+    b = vh.random_uniform(0, n).astype(dtype)
+    A = vh.CsrMatrix(rp, col, val, dtype=dtype)
     plan = vh.BatchPlan(sys_ptr, rp)
-    plan.autotune(A)
+    plan.autotune(A, single=single)
     db = vh.DeviceArray.from_numpy(b)
-    dx = vh.DeviceArray.zeros(n, np.float64)
-    it, err, conv = plan.solve(A, db, dx, tol=1e-8, max_iters=5000)
-    assert (conv == 1).all() and (err < 1e-8).all()
+    dx = vh.DeviceArray.zeros(n, dtype)
+    it, err, conv = plan.solve(A, db, dx, tol=tol, max_iters=5000)
+    assert (conv == 1).all() and (err < tol).all()
+    A64 = vh.CsrMatrix(rp, col, val)
     y = vh.DeviceArray.zeros(n, np.float64)
-    A.spmv(dx, y)
-    r = b - y.to_numpy()
+    A64.spmv(vh.DeviceArray.from_numpy(dx.to_numpy().astype(np.float64)), y)
+    b64 = b.astype(np.float64)
+    r = b64 - y.to_numpy()
     starts = sys_ptr[:-1].astype(np.int64)
-    rel = np.sqrt(np.add.reduceat(r * r, starts) / np.add.reduceat(b * b, starts))
-    assert rel.max() < 1.05e-8, rel.max()
+    rel = np.sqrt(np.add.reduceat(r * r, starts) / np.add.reduceat(b64 * b64, starts))
+    print(f"{'fp32' if single else 'fp64'}: true residual max {rel.max():.3e} median {np.median(rel):.3e}")
+    assert rel.max() < (TRUE_FP32 if single else 1.05e-8), rel.max()
+
+
+# FP64 true residual that the FP32 batched solve at tol 1e-5 stays below on every system
+TRUE_FP32 = 5e-5

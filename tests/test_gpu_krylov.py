@@ -224,6 +224,138 @@ def test_dia_operator_equals_csr(dev, vh, kind, dtype):
     assert ra.iters == rg.iters and ra.error == rg.error and ra.converged and np.array_equal(xa, xg)
 
 
+def stencil_27(nx, ny, nz):
+    """27-point stencil on an nx*ny*nz grid (row = x + nx (y + ny z)): -1 to every neighbour in the
+    box, 26.5 on the diagonal (symmetric, strictly diagonally dominant: SPD).  27 diagonals, so the
+    DIA kernel runs three full groups of eight and a partial fourth."""
+    rows, cols = [], []
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    r = (x + nx * (y + ny * z)).ravel()
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                ok = ((x + dx >= 0) & (x + dx < nx) & (y + dy >= 0) & (y + dy < ny) & (z + dz >= 0) & (z + dz < nz)).ravel()
+                rows.append(r[ok])
+                cols.append(r[ok] + dx + nx * (dy + ny * dz))
+    return _csr_from_entries(nx * ny * nz, np.concatenate(rows), np.concatenate(cols), 26.5)
+
+
+def band(n, offsets):
+    """Band matrix with the given diagonals: -1/|o| on diagonal o != 0 (symmetric when the offsets
+    are), the diagonal = the row's absolute off-diagonal sum + 0.5."""
+    rows, cols, vals = [], [], []
+    for o in offsets:
+        if o == 0:
+            continue
+        i = np.arange(max(0, -o), min(n, n - o))
+        rows.append(i)
+        cols.append(i + o)
+        vals.append(np.full(i.size, -1.0 / abs(o)))
+    return _csr_from_entries(n, np.concatenate(rows), np.concatenate(cols), None, np.concatenate(vals))
+
+
+def _csr_from_entries(n, rows, cols, diag, vals=None):
+    import scipy.sparse as sp
+    if vals is None:
+        vals = np.where(rows == cols, diag, -1.0)
+    M = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    if diag is None:
+        M = M + sp.diags(np.asarray(abs(M).sum(axis=1)).ravel() + 0.5)
+    M = sp.csr_matrix(M)
+    M.sort_indices()
+    return M.indptr.astype(np.uint32), M.indices.astype(np.uint32), M.data.astype(np.float64)
+
+
+DIA_CASES = {
+    "lap27": (lambda: stencil_27(14, 13, 12), 27, True),
+    "band9": (lambda: band(3000, range(-4, 5)), 9, True),
+    "band17": (lambda: band(3000, range(-8, 9)), 17, True),
+    "band31": (lambda: band(3000, range(-15, 16)), 31, True),
+    "band32": (lambda: band(3000, range(-15, 17)), 32, False),   # the kDiaMax boundary (nonsymmetric)
+    "band33": (lambda: band(3000, range(-16, 17)), 33, True),    # one diagonal too many: keeps CSR
+}
+
+
+@pytest.mark.parametrize("case", list(DIA_CASES))
+@pytest.mark.parametrize("kind", ["cg", "jacobi_cg", "bicgstab", "gmres"])
+def test_dia_operator_diagonal_counts(dev, vh, case, kind):
+    """Operators with 9 to 33 diagonals: up to 32 the default solve runs on the DIA copy (reported
+    in result.operator_used), bit for bit equal to the solve on the CSR matrix as given, across the
+    DIA kernel's full and partial groups of eight diagonals and at the 32-diagonal limit of the
+    offset detection; 33 diagonals keep the CSR operator."""
+    make, nd, symmetric = DIA_CASES[case]
+    if kind.endswith("cg") and not symmetric:
+        pytest.skip("CG needs a symmetric matrix")
+    rp, col, val = make()
+    n = rp.size - 1
+    assert np.unique(col.astype(np.int64) - np.repeat(np.arange(n), np.diff(rp.astype(np.int64)))).size == nd
+    b = vh.random_uniform(5, n)
+    A = vh.CsrMatrix(rp, col, val)
+    pc = 1 if kind.startswith("jacobi") else 0
+    base = kind.replace("jacobi_", "")
+    out = []
+    for of in (0, 1):
+        db, dx = vh.DeviceArray.from_numpy(b), vh.DeviceArray.zeros(n, np.float64)
+        if base == "gmres":
+            res = vh.gmres(A, db, dx, tol=1e-10, max_iters=3000, krylov_dim=30, operator_format=of)
+        else:
+            fn = vh.cg if base == "cg" else vh.bicgstab
+            res = fn(A, db, dx, precond=pc, tol=1e-10, max_iters=3000, operator_format=of)
+        out.append((dx.to_numpy(), res))
+    (xa, ra), (xg, rg) = out
+    assert ra.operator_used == (vh.USED_DIA if nd <= 32 else vh.USED_AS_GIVEN)
+    assert rg.operator_used == vh.USED_AS_GIVEN
+    assert ra.converged and ra.iters == rg.iters and ra.error == rg.error and np.array_equal(xa, xg)
+
+
+@pytest.mark.parametrize("bad", ["zero", "negative_zero", "nan"])
+def test_dia_operator_refuses_values_it_would_skip(dev, vh, bad):
+    """The DIA kernel skips zero (either sign) and NaN values, where the CSR kernel multiplies every
+    stored entry (0 * Inf and NaN reach its sums): a matrix storing such a value keeps the CSR
+    operator, so the default solve is the as-given one."""
+    rp, col, val = vh.laplace_3d(10, 9, 8)
+    n = rp.size - 1
+    val = val.copy()
+    k = int(rp[57]) + 1  # an off-diagonal entry of row 57
+    val[k] = {"zero": 0.0, "negative_zero": -0.0, "nan": np.nan}[bad]
+    A = vh.CsrMatrix(rp, col, val)
+    b = vh.random_uniform(6, n)
+    db, dx = vh.DeviceArray.from_numpy(b), vh.DeviceArray.zeros(n, np.float64)
+    res = vh.cg(A, db, dx, tol=1e-10, max_iters=50)
+    assert res.operator_used == vh.USED_AS_GIVEN
+    d = vh.DiaOperator(A)
+    assert not d.built and d.nd == 0
+    d.close()
+
+
+def test_dia_object_reused_across_solves(dev, vh):
+    """vclhip_dia_create_f64 detects and copies once; solves given the object (params.dia) run on it
+    with the results of a solve that builds its own copy; an object made for other arrays is
+    rejected."""
+    rp, col, val = stencil_27(12, 11, 10)
+    n = rp.size - 1
+    A = vh.CsrMatrix(rp, col, val)
+    d = vh.DiaOperator(A)
+    assert d.built and d.nd == 27
+    outs = []
+    for seed in (7, 8):
+        b = vh.random_uniform(seed, n)
+        for dia in (None, d):
+            db, dx = vh.DeviceArray.from_numpy(b), vh.DeviceArray.zeros(n, np.float64)
+            res = vh.cg(A, db, dx, tol=1e-10, max_iters=2000, dia=dia)
+            assert res.operator_used == vh.USED_DIA
+            outs.append((dx.to_numpy(), res.iters))
+    assert np.array_equal(outs[0][0], outs[1][0]) and outs[0][1] == outs[1][1]
+    assert np.array_equal(outs[2][0], outs[3][0]) and outs[2][1] == outs[3][1]
+    B = vh.CsrMatrix(rp, col, val)  # same values, other arrays
+    db, dx = vh.DeviceArray.from_numpy(vh.random_uniform(7, n)), vh.DeviceArray.zeros(n, np.float64)
+    with pytest.raises(vh.VclHipError):
+        vh.cg(B, db, dx, tol=1e-10, max_iters=10, dia=d)
+    with pytest.raises(vh.VclHipError):
+        vh.cg(A, db, dx, tol=1e-10, max_iters=10, operator_format=2)
+    d.close()
+
+
 def test_dia_operator_skips_unordered_rows(dev, vh):
     """A row whose columns are not strictly increasing (unsorted, or an entry split in two) keeps the
     CSR operator: the default solve equals the as-given one bit for bit and solves the summed matrix."""

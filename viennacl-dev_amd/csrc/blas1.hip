@@ -63,6 +63,34 @@ __global__ __launch_bounds__(kBlock) void dot_partial_vec_kernel(uint32_t n, con
   block_store_partials<1, A>(acc, part, 0, lw);
 }
 
+// Read-only streaming reference: workgroup lw sums the lw-th contiguous chunk of x with 16-byte
+// loads, kStreamU of them per lane in flight (no address depends on a load), and stores one partial
+// sum.  It measures the rate at which this device reads HBM; nothing on the solver path uses it.
+constexpr int kStreamU = 8;
+
+__global__ __launch_bounds__(kBlock) void stream_read_kernel(uint64_t n, const double* __restrict__ x,
+                                                             double* __restrict__ sums)
+{
+  const uint32_t lw = xcd_logical_block();
+  const uint64_t G = gridDim.x;
+  const uint64_t packs = n / 2;
+  const uint64_t b = packs * lw / G, e = packs * (lw + 1) / G;
+  const double2* __restrict__ p = reinterpret_cast<const double2*>(x);
+  double s = 0;
+  uint64_t i = b + threadIdx.x;
+  for (; i + static_cast<uint64_t>(kStreamU - 1) * kBlock < e; i += static_cast<uint64_t>(kStreamU) * kBlock) {
+    double2 v[kStreamU];
+#pragma unroll
+    for (int u = 0; u < kStreamU; ++u) v[u] = p[i + static_cast<uint64_t>(u) * kBlock];
+#pragma unroll
+    for (int u = 0; u < kStreamU; ++u) s += v[u].x + v[u].y;
+  }
+  for (; i < e; i += kBlock) s += p[i].x + p[i].y;
+  if (lw == G - 1 && (n & 1) && threadIdx.x == 0) s += x[n - 1];
+  s = block_sum(s);
+  if (threadIdx.x == 0) sums[lw] = s;
+}
+
 template <typename T, int W>
 __global__ __launch_bounds__(kBlock) void axpby_kernel(uint32_t n, T a, const T* x, T b, const T* y, T* z)
 {
@@ -399,6 +427,15 @@ int vclhip_norm2_f32(uint32_t n, const float* x, uint32_t xs, uint32_t xi, float
   VCLHIP_TRY((dot_to_host<float, float>(n, x, xs, xi, x, xs, xi, result, stream)));
   *result = std::sqrt(*result);
   return VCLHIP_SUCCESS;
+}
+
+int vclhip_stream_read_f64(uint64_t n, const double* x, double* sums, uint32_t num_sums, void* stream)
+{
+  if (!x || !sums) return invalid("null pointer");
+  if (num_sums < 8 || num_sums > (1u << 20) || (num_sums & 7u)) return invalid("num_sums: a multiple of 8 in [8, 2^20]");
+  if (!aligned16({x})) return invalid("x must be 16-byte aligned");
+  launch_timed(stream_read_kernel, dim3(num_sums), dim3(kBlock), static_cast<hipStream_t>(stream), n, x, sums);
+  return check_launch("stream_read_kernel");
 }
 
 int vclhip_axpby_f64(uint32_t n, double a, const double* x, double b, const double* y, double* z, void* stream)

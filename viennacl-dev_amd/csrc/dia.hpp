@@ -67,13 +67,15 @@ __global__ __launch_bounds__(kBlock) void dia_offsets_kernel(const I* __restrict
 
 // val[d * n + row] = A(row, row + off[d]) converted to T (val zeroed before); off ascending,
 // nd <= kDiaMax, and every offset of the matrix is in off.  A row whose columns do not strictly
-// increase (unsorted, or an entry given twice) raises *unordered: its sum in column order would not
-// be the CSR kernel's, and a repeated entry would be lost, so the copy is then not used.
+// increase (unsorted, or an entry given twice) raises *refuse: its sum in column order would not be
+// the CSR kernel's, and a repeated entry would be lost.  With `exact`, so does a stored value that
+// the DIA kernel would skip (a zero of either sign, or NaN): the CSR kernel multiplies every stored
+// entry, so 0 * Inf or a NaN value would reach its sum and not the DIA one.
 template <typename S, typename T>
 __global__ __launch_bounds__(kBlock) void dia_fill_kernel(const uint32_t* __restrict__ rp,
                                                           const uint32_t* __restrict__ col, const S* __restrict__ a,
                                                           uint32_t n, const int32_t* __restrict__ off, uint32_t nd,
-                                                          T* __restrict__ val, uint32_t* __restrict__ unordered)
+                                                          int exact, T* __restrict__ val, uint32_t* __restrict__ refuse)
 {
   __shared__ int32_t so[kDiaMax];
   if (threadIdx.x < nd) so[threadIdx.x] = off[threadIdx.x];
@@ -82,7 +84,9 @@ __global__ __launch_bounds__(kBlock) void dia_fill_kernel(const uint32_t* __rest
   if (row >= n) return;
   const uint32_t e0 = rp[row], e1 = rp[row + 1];
   for (uint32_t e = e0; e < e1; ++e) {
-    if (e > e0 && col[e] <= col[e - 1]) *unordered = 1u;
+    if (e > e0 && col[e] <= col[e - 1]) *refuse = 1u;
+    const S v = a[e];
+    if (exact && !(v > S(0) || v < S(0))) *refuse = 1u;
     const int32_t o = static_cast<int32_t>(col[e] - row);
     uint32_t lo = 0, hi = nd;
     while (hi - lo > 1) {
@@ -92,16 +96,26 @@ __global__ __launch_bounds__(kBlock) void dia_fill_kernel(const uint32_t* __rest
       else
         hi = mid;
     }
-    val[static_cast<size_t>(lo) * n + row] = static_cast<T>(a[e]);
+    val[static_cast<size_t>(lo) * n + row] = static_cast<T>(v);
   }
 }
 
-// Builds the DIA copy (values converted from S to T) of a square CSR matrix into workspace memory
-// when it has at most kDiaMax distinct diagonals, they store at most `max_fill` times nnz values,
-// and (max_bytes > 0) the values take at most max_bytes; *built tells whether it was made.
-template <typename S, typename T>
-static int build_dia(const vclhip_csr* A, Workspace& ws, hipStream_t s, double max_fill, size_t max_bytes,
-                     DiaDev<T>* out, bool* built)
+// Device memory of at most `bytes` taken for an optional copy: a failure leaves *p null and clears
+// the HIP error, so that the caller keeps the matrix as given.
+struct DiaArrays {
+  int32_t* off = nullptr;
+  void* val = nullptr;
+};
+
+// Builds the DIA copy (values converted from S to T) of a square CSR matrix when it has at most
+// kDiaMax distinct diagonals, they store at most `max_fill` times nnz values and the values take at
+// most max_bytes; *built tells whether it was made.  Temporaries come from `ws`; the copy's arrays
+// from `alloc_out(bytes_off, bytes_val, &arrays)`, which returns false when it cannot provide them
+// (then no copy is made).  Every failure of this optional step other than a launch error leaves
+// *built false.
+template <typename S, typename T, class OutAlloc>
+static int build_dia(const vclhip_csr* A, Workspace& ws, hipStream_t s, double max_fill, size_t max_bytes, bool exact,
+                     OutAlloc&& alloc_out, DiaDev<T>* out, bool* built)
 {
   *built = false;
   const uint32_t n = A->n_rows;
@@ -111,11 +125,10 @@ static int build_dia(const vclhip_csr* A, Workspace& ws, hipStream_t s, double m
   const size_t words = (2ull * n + 64) / 64;
   unsigned long long* bits;
   uint32_t* overflow;
-  VCLHIP_TRY(ws.alloc(&bits, words));
-  VCLHIP_TRY(ws.alloc(&overflow, 1));
-  VCLHIP_HIP(hipMemsetAsync(bits, 0, words * sizeof(unsigned long long), s));
-  VCLHIP_HIP(hipMemsetAsync(overflow, 0, sizeof(uint32_t), s));
+  if (!ws.try_alloc(&bits, words) || !ws.try_alloc(&overflow, 1)) return VCLHIP_SUCCESS;
   const uint32_t g = (n + kBlock - 1) / kBlock;
+  VCLHIP_TRY(vclhip_fill_zero(bits, words * sizeof(unsigned long long), s));
+  VCLHIP_TRY(vclhip_fill_zero(overflow, sizeof(uint32_t), s));
   hipLaunchKernelGGL(dia_offsets_kernel<uint32_t>, dim3(g), dim3(kBlock), 0, s, A->row_ptr, A->col_idx, n, bits,
                      overflow);
   VCLHIP_TRY(check_launch("dia_offsets_kernel"));
@@ -135,46 +148,104 @@ static int build_dia(const vclhip_csr* A, Workspace& ws, hipStream_t s, double m
   const uint32_t nd = static_cast<uint32_t>(offs.size());
   if (nd == 0 || static_cast<double>(nd) * n > max_fill * static_cast<double>(A->nnz)) return VCLHIP_SUCCESS;
   const size_t vbytes = static_cast<size_t>(nd) * n * sizeof(T);
-  if (max_bytes && vbytes > max_bytes) return VCLHIP_SUCCESS;
-  int32_t* doff;
-  T* val;
-  VCLHIP_TRY(ws.alloc(&doff, nd));
-  VCLHIP_TRY(ws.alloc(&val, static_cast<size_t>(nd) * n));
-  VCLHIP_HIP(hipMemcpyAsync(doff, offs.data(), nd * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  VCLHIP_HIP(hipMemsetAsync(val, 0, vbytes, s));
-  VCLHIP_HIP(hipMemsetAsync(overflow, 0, sizeof(uint32_t), s));  // reused as the unordered-row flag
+  if (vbytes > max_bytes) return VCLHIP_SUCCESS;
+  DiaArrays arr;
+  if (!alloc_out(nd * sizeof(int32_t), vbytes, &arr)) return VCLHIP_SUCCESS;
+  T* val = static_cast<T*>(arr.val);
+  VCLHIP_HIP(hipMemcpyAsync(arr.off, offs.data(), nd * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  VCLHIP_TRY(vclhip_fill_zero(val, vbytes, s));
+  VCLHIP_TRY(vclhip_fill_zero(overflow, sizeof(uint32_t), s));  // reused as the refusal flag
   hipLaunchKernelGGL((dia_fill_kernel<S, T>), dim3(g), dim3(kBlock), 0, s, A->row_ptr, A->col_idx,
-                     static_cast<const S*>(A->values), n, doff, nd, val, overflow);
+                     static_cast<const S*>(A->values), n, arr.off, nd, exact ? 1 : 0, val, overflow);
   VCLHIP_TRY(check_launch("dia_fill_kernel"));
   VCLHIP_HIP(hipMemcpyAsync(&hover, overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   VCLHIP_HIP(hipStreamSynchronize(s));  // offs lives on the host stack
   if (hover) return VCLHIP_SUCCESS;
   out->val = val;
-  out->off = doff;
+  out->off = arr.off;
   out->n_rows = n;
   out->nd = nd;
   *built = true;
   return VCLHIP_SUCCESS;
 }
 
-// The solvers' operator: a CSR matrix with few diagonals is replaced by a DIA copy in workspace
-// memory (same results, fewer bytes per product) unless params->operator_format asks for the
-// matrix as given, or the copy would take more than a quarter of the free device memory.
-template <typename T>
-static int dia_operator(DevOp<T>& op, Workspace& ws, hipStream_t s, const vclhip_solver_params* P)
+// The copy's arrays as workspace of one solve, released with it.
+struct WorkspaceDiaAlloc {
+  Workspace& ws;
+  bool operator()(size_t off_bytes, size_t val_bytes, DiaArrays* a) const
+  {
+    char* v = nullptr;
+    if (!ws.try_alloc(&a->off, off_bytes / sizeof(int32_t)) || !ws.try_alloc(&v, val_bytes)) return false;
+    a->val = v;
+    return true;
+  }
+};
+
+// A quarter of the free device memory: the most an optional copy may take.
+inline size_t dia_memory_cap()
 {
-  if (!P || P->operator_format != VCLHIP_OPERATOR_AUTO || op.fmt != OpFormat::Csr || !op.csr_desc)
-    return VCLHIP_SUCCESS;
   size_t free_b = 0, total_b = 0;
-  VCLHIP_HIP(hipMemGetInfo(&free_b, &total_b));
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return free_b / 4;
+}
+
+}  // namespace vclhip
+
+// A DIA copy kept across solves (vclhip_dia_create_*): the outcome of the detection for one CSR
+// matrix and, when it is a candidate, the copy in device memory owned by this object.
+struct vclhip_dia {
+  int single;  // value type: 0 double, 1 float
+  vclhip_csr matrix;  // the descriptor it was built from (arrays compared at use)
+  bool built;
+  vclhip::DiaDev<double> d64;
+  vclhip::DiaDev<float> d32;
+  void* val;
+  int32_t* off;
+};
+
+namespace vclhip {
+
+// The solvers' operator: a CSR matrix with few diagonals is replaced by a DIA copy (same results,
+// fewer bytes per product) unless params->operator_format asks for the matrix as given, or the copy
+// would take more than a quarter of the free device memory.  params->dia, when given, is the
+// detection and copy made once by vclhip_dia_create_* for this matrix; otherwise the copy is built
+// in the solve's workspace.  *used_dia tells whether the solve runs on a DIA copy.
+template <typename T>
+static int dia_operator(DevOp<T>& op, Workspace& ws, hipStream_t s, const vclhip_solver_params* P, int* used_dia)
+{
+  *used_dia = 0;
+  if (!P) return VCLHIP_SUCCESS;
+  if (P->operator_format != VCLHIP_OPERATOR_AUTO && P->operator_format != VCLHIP_OPERATOR_AS_GIVEN)
+    return invalid("operator_format must be VCLHIP_OPERATOR_AUTO or VCLHIP_OPERATOR_AS_GIVEN");
+  if (P->operator_format != VCLHIP_OPERATOR_AUTO || op.fmt != OpFormat::Csr || !op.csr_desc) return VCLHIP_SUCCESS;
   DiaDev<T> dia{};
   bool built = false;
-  VCLHIP_TRY((build_dia<T, T>(op.csr_desc, ws, s, 1.25, free_b / 4 ? free_b / 4 : 1, &dia, &built)));
+  if (P->dia) {
+    const vclhip_dia* c = P->dia;
+    const vclhip_csr* A = op.csr_desc;
+    if (c->single != (sizeof(T) == 4 ? 1 : 0)) return invalid("params->dia was built for the other value type");
+    if (c->matrix.n_rows != A->n_rows || c->matrix.nnz != A->nnz || c->matrix.row_ptr != A->row_ptr ||
+        c->matrix.col_idx != A->col_idx || c->matrix.values != A->values)
+      return invalid("params->dia was built for another matrix");
+    if (!c->built) return VCLHIP_SUCCESS;
+    if constexpr (sizeof(T) == 4)
+      dia = c->d32;
+    else
+      dia = c->d64;
+    built = true;
+  } else {
+    VCLHIP_TRY((build_dia<T, T>(op.csr_desc, ws, s, 1.25, dia_memory_cap(), true, WorkspaceDiaAlloc{ws}, &dia,
+                                &built)));
+  }
   if (!built) return VCLHIP_SUCCESS;
   op.fmt = OpFormat::Dia;
   op.sell = false;
   op.dia = dia;
   op.grid = chunk_grid((op.n + kBlock - 1) / kBlock, sell_gmax());
+  *used_dia = 1;
   return VCLHIP_SUCCESS;
 }
 

@@ -546,7 +546,8 @@ int gmres_solve(const M* Ain, const T* b, T* x, const vclhip_solver_params* P, v
   const uint32_t max_cycles = max_restarts + 1;
 
   Workspace ws(s);
-  VCLHIP_TRY(dia_operator(op, ws, s, P));
+  int used_dia = 0;
+  VCLHIP_TRY(dia_operator(op, ws, s, P, &used_dia));
   T *V, *r;
   double *R, *sigma, *xi, *coef, *upd, *c, *part, *hist = nullptr;
   GState* st;
@@ -683,6 +684,7 @@ int gmres_solve(const M* Ain, const T* b, T* x, const vclhip_solver_params* P, v
   Res->iters = fin.iters;
   Res->error = fin.error;
   Res->restarts = fin.cycles;
+  Res->operator_used = used_dia ? VCLHIP_USED_DIA : VCLHIP_USED_AS_GIVEN;
   Res->converged = static_cast<int>(fin.converged);
   if (hist && fin.iters) {
     VCLHIP_HIP(hipMemcpyAsync(P->history_host, hist, std::min(hist_len, fin.iters) * sizeof(double),

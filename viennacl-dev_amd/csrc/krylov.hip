@@ -972,7 +972,8 @@ static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_so
   const uint32_t n = op.n;
   const bool jac = precond == VCLHIP_PRECOND_JACOBI;
   Workspace ws(s);
-  VCLHIP_TRY(dia_operator(op, ws, s, P));
+  int used_dia = 0;
+  VCLHIP_TRY(dia_operator(op, ws, s, P, &used_dia));
   T *r, *p, *Ap, *d = nullptr;
   double *part, *hist = nullptr;
   KState* st;
@@ -1068,6 +1069,7 @@ static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_so
   if (jac && fin.nrs != 0) R->error = std::sqrt(std::fabs(fin.rr / fin.nrs));
   R->restarts = 0;
   R->converged = static_cast<int>(fin.converged);
+  R->operator_used = used_dia ? VCLHIP_USED_DIA : VCLHIP_USED_AS_GIVEN;
   return copy_history(P, hist, fin.iter, s);
 }
 
@@ -1087,7 +1089,8 @@ static int bicgstab_solve(const M* Ain, const T* b, T* x, int precond, const vcl
   const uint32_t n = op.n;
   const bool jac = precond == VCLHIP_PRECOND_JACOBI;
   Workspace ws(s);
-  VCLHIP_TRY(dia_operator(op, ws, s, P));
+  int used_dia = 0;
+  VCLHIP_TRY(dia_operator(op, ws, s, P, &used_dia));
   T *r, *p, *r0, *Ap, *sv, *As, *d = nullptr;
   double *part, *hist = nullptr;
   KState* st;
@@ -1192,6 +1195,7 @@ static int bicgstab_solve(const M* Ain, const T* b, T* x, int precond, const vcl
   R->error = fin.nrs != 0 ? fin.res / fin.nrs : 0.0;
   R->restarts = restarts;
   R->converged = static_cast<int>(fin.converged);
+  R->operator_used = used_dia ? VCLHIP_USED_DIA : VCLHIP_USED_AS_GIVEN;
   return copy_history(P, hist, fin.iter, s);
 }
 
@@ -1799,7 +1803,9 @@ static int mixed_cg_solve(const M* A, const float* values32, const double* b, do
   if (P->inner_format < VCLHIP_INNER_AUTO || P->inner_format > VCLHIP_INNER_DIA) return invalid("inner_format");
   if constexpr (std::is_same<M, vclhip_csr>::value) {
     if (!values32 && (P->inner_format == VCLHIP_INNER_AUTO || P->inner_format == VCLHIP_INNER_DIA)) {
-      VCLHIP_TRY((build_dia<double, float>(A, ws, s, 1.25, 0, &inner_dia, &inner_is_dia)));
+      // the inner copy is summed like the SELL copy (zeros skipped), so stored zeros do not refuse it
+      VCLHIP_TRY((build_dia<double, float>(A, ws, s, 1.25, dia_memory_cap(), false, WorkspaceDiaAlloc{ws},
+                                           &inner_dia, &inner_is_dia)));
       if (P->inner_format == VCLHIP_INNER_DIA && !inner_is_dia) {
         set_error("inner_format DIA: more than %u diagonals, or their padding exceeds 25%% of nnz", kDiaMax);
         return VCLHIP_ERROR_NOT_SUPPORTED;
@@ -1943,6 +1949,9 @@ static int mixed_cg_solve(const M* A, const float* values32, const double* b, do
   R->error = fin.nrs > 0 ? std::sqrt(fin.res / fin.nrs) : 0.0;
   R->restarts = fin.updates;
   R->converged = static_cast<int>(fin.converged);
+  R->operator_used = inner_is_dia    ? VCLHIP_INNER_DIA
+                     : inner_is_sell ? (inner_dcol ? VCLHIP_INNER_SELL64_D16 : VCLHIP_INNER_SELL64)
+                                     : VCLHIP_INNER_CSR;
   return copy_history(P, hist, fin.iter, s);
 }
 
@@ -2072,7 +2081,98 @@ static int pipelined_prod(const M* A, const double* p, double* Ap, const double*
 
 using namespace vclhip;
 
+namespace vclhip {
+
+// The copy's arrays in device memory owned by a vclhip_dia (freed by vclhip_dia_destroy).
+struct OwnedDiaAlloc {
+  vclhip_dia* d;
+  bool operator()(size_t off_bytes, size_t val_bytes, DiaArrays* a) const
+  {
+    void* o = nullptr;
+    void* v = nullptr;
+    if (hipMalloc(&o, off_bytes) != hipSuccess || hipMalloc(&v, val_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      if (o) (void)hipFree(o);
+      return false;
+    }
+    d->off = static_cast<int32_t*>(o);
+    d->val = v;
+    a->off = d->off;
+    a->val = v;
+    return true;
+  }
+};
+
+template <typename T>
+static int dia_create(const vclhip_csr* A, vclhip_dia** out, void* stream)
+{
+  if (!out) return invalid("dia is NULL");
+  *out = nullptr;
+  if (!A || !A->row_ptr || !A->col_idx || !A->values) return invalid("CSR matrix arrays");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  vclhip_dia* d = new vclhip_dia();
+  d->single = sizeof(T) == 4 ? 1 : 0;
+  d->matrix = *A;
+  d->built = false;
+  d->val = nullptr;
+  d->off = nullptr;
+  int status = VCLHIP_SUCCESS;
+  {
+    Workspace ws(s);
+    DiaDev<T> dia{};
+    bool built = false;
+    status = build_dia<T, T>(A, ws, s, 1.25, dia_memory_cap(), true, OwnedDiaAlloc{d}, &dia, &built);
+    if (!status) {
+      d->built = built;
+      if constexpr (sizeof(T) == 4)
+        d->d32 = dia;
+      else
+        d->d64 = dia;
+    }
+  }
+  if (!status) status = hip_status(hipStreamSynchronize(s), "dia create");  // the workspace is released
+  if (status) {
+    (void)vclhip_dia_destroy(d);
+    return status;
+  }
+  *out = d;
+  return VCLHIP_SUCCESS;
+}
+
+}  // namespace vclhip
+
 extern "C" {
+
+int vclhip_dia_create_f64(const vclhip_csr* A, vclhip_dia** dia, void* stream)
+{
+  return vclhip::dia_create<double>(A, dia, stream);
+}
+
+int vclhip_dia_create_f32(const vclhip_csr* A, vclhip_dia** dia, void* stream)
+{
+  return vclhip::dia_create<float>(A, dia, stream);
+}
+
+int vclhip_dia_info(const vclhip_dia* dia, int* built, uint32_t* nd)
+{
+  if (!dia || !built || !nd) return vclhip::invalid("null argument");
+  *built = dia->built ? 1 : 0;
+  *nd = !dia->built ? 0u : (dia->single ? dia->d32.nd : dia->d64.nd);
+  return VCLHIP_SUCCESS;
+}
+
+int vclhip_dia_destroy(vclhip_dia* dia)
+{
+  if (!dia) return VCLHIP_SUCCESS;
+  int status = VCLHIP_SUCCESS;
+  if (dia->val) status = vclhip::hip_status(hipFree(dia->val), "hipFree");
+  if (dia->off) {
+    const int st2 = vclhip::hip_status(hipFree(dia->off), "hipFree");
+    if (!status) status = st2;
+  }
+  delete dia;
+  return status;
+}
 
 void vclhip_solver_params_default(vclhip_solver_params* P)
 {

@@ -271,6 +271,20 @@ class Workspace {
     *p = static_cast<U*>(v);
     return VCLHIP_SUCCESS;
   }
+  // For optional buffers: false (and the HIP error cleared) when the allocation fails.
+  template <typename U>
+  bool try_alloc(U** p, size_t count)
+  {
+    void* v = nullptr;
+    if (hipMallocAsync(&v, count * sizeof(U) + 256, s_) != hipSuccess || !v) {
+      (void)hipGetLastError();
+      *p = nullptr;
+      return false;
+    }
+    ptrs_.push_back(v);
+    *p = static_cast<U*>(v);
+    return true;
+  }
 
  private:
   hipStream_t s_;

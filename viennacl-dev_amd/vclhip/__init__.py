@@ -10,10 +10,11 @@ import numpy as np
 
 from . import _abi
 from ._abi import (Csr, Sell, Coo, Ell, Hyb, Matrix, SolverParams, SolverResult, MONITOR, VclHipError, call, lib,
-                   PRECOND_NONE, PRECOND_JACOBI, LIB_PATH, declared_functions,
+                   PRECOND_NONE, PRECOND_JACOBI, LIB_PATH, declared_functions, OPERATOR_AUTO, OPERATOR_AS_GIVEN,
+                   USED_AS_GIVEN, USED_DIA, INNER_AUTO, INNER_CSR, INNER_SELL64, INNER_SELL64_D16, INNER_DIA,
                    FORMAT_CSR, FORMAT_SELL, FORMAT_COO, FORMAT_ELL, FORMAT_HYB)
 
-__all__ = ["DeviceArray", "Stream", "CsrMatrix", "SellMatrix", "CooMatrix", "EllMatrix", "HybMatrix", "BatchPlan",
+__all__ = ["DeviceArray", "Stream", "CsrMatrix", "DiaOperator", "SellMatrix", "CooMatrix", "EllMatrix", "HybMatrix", "BatchPlan",
            "lib", "call", "VclHipError",
            "device_count", "PRECOND_NONE", "PRECOND_JACOBI", "SolverParams", "SolverResult"]
 
@@ -331,9 +332,36 @@ def matrix_diag(A, d, stream=None):
     call(fn, C.byref(m), d.ptr, stream)
 
 
+class DiaOperator:
+    """The solvers' DIA detection and copy of a CSR matrix made once (vclhip_dia_create_*), for
+    repeated solves of the same matrix: pass it as `dia=` to cg / bicgstab / gmres.  `built` says
+    whether the matrix qualified (then the solves run on the copy), `nd` its diagonal count.  The
+    matrix's arrays and values must not change while it is used."""
+
+    def __init__(self, A, stream=None):
+        self.handle = C.c_void_p()
+        fn = "vclhip_dia_create_f64" if A.dtype == np.float64 else "vclhip_dia_create_f32"
+        desc = A.desc if hasattr(A, "desc") else A
+        call(fn, C.byref(desc), C.byref(self.handle), stream)
+        built, nd = C.c_int(), C.c_uint32()
+        call("vclhip_dia_info", self.handle, C.byref(built), C.byref(nd))
+        self.built, self.nd = bool(built.value), int(nd.value)
+
+    def close(self):
+        if self.handle:
+            call("vclhip_dia_destroy", self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def solver_params(tol=1e-8, max_iters=300, abs_tol=0.0, max_iters_before_restart=200, inner_tol=1e-2,
                   chunk=16, use_graph=False, mixed_mode=0, variant=0, history=None, monitor=None, inner_format=0,
-                  krylov_dim=20, operator_format=0):
+                  krylov_dim=20, operator_format=0, dia=None):
     p = SolverParams()
     lib().vclhip_solver_params_default(C.byref(p))
     p.variant = variant
@@ -348,6 +376,8 @@ def solver_params(tol=1e-8, max_iters=300, abs_tol=0.0, max_iters_before_restart
     p.inner_format = inner_format
     p.krylov_dim = krylov_dim
     p.operator_format = operator_format
+    if dia is not None:
+        p.dia = dia.handle
     if history is not None:
         p.history_host = history.ctypes.data_as(C.POINTER(C.c_double))
         p.history_len = history.size

@@ -31,6 +31,12 @@ ERROR_TOO_LARGE = -5
 PRECOND_NONE = 0
 PRECOND_JACOBI = 1
 
+OPERATOR_AUTO = 0
+OPERATOR_AS_GIVEN = 1
+USED_AS_GIVEN = 0
+USED_DIA = 1
+INNER_AUTO, INNER_CSR, INNER_SELL64, INNER_SELL64_D16, INNER_DIA = 0, 1, 2, 3, 4
+
 
 class Csr(C.Structure):
     _fields_ = [("n_rows", u32), ("n_cols", u32), ("nnz", u32), ("num_blocks", u32),
@@ -74,11 +80,11 @@ class SolverParams(C.Structure):
                 ("max_iters_before_restart", u32), ("inner_tol", f32), ("chunk", u32),
                 ("use_graph", i32), ("mixed_mode", i32), ("variant", i32), ("history_host", P(f64)),
                 ("history_len", u32), ("monitor", MONITOR), ("monitor_user", vp), ("inner_format", i32),
-                ("krylov_dim", u32), ("operator_format", i32)]
+                ("krylov_dim", u32), ("operator_format", i32), ("dia", vp)]
 
 
 class SolverResult(C.Structure):
-    _fields_ = [("iters", u32), ("error", f64), ("restarts", u32), ("converged", i32)]
+    _fields_ = [("iters", u32), ("error", f64), ("restarts", u32), ("converged", i32), ("operator_used", i32)]
 
 
 # name -> (restype, argtypes); every function of include/vclhip.h
@@ -96,6 +102,7 @@ _SIGS = {
     "vclhip_memcpy_d2h": (i32, [vp, vp, sz, vp]),
     "vclhip_memcpy_d2d": (i32, [vp, vp, sz, vp]),
     "vclhip_fill_zero": (i32, [vp, sz, vp]),
+    "vclhip_stream_read_f64": (i32, [u64, vp, vp, u32, vp]),
     "vclhip_stream_create": (i32, [P(vp)]),
     "vclhip_stream_destroy": (i32, [vp]),
     "vclhip_stream_synchronize": (i32, [vp]),
@@ -177,6 +184,10 @@ _SIGS = {
     "vclhip_gmres_solve_f64": (i32, [P(Matrix), vp, vp, P(SolverParams), P(SolverResult), vp]),
     "vclhip_gmres_solve_f32": (i32, [P(Matrix), vp, vp, P(SolverParams), P(SolverResult), vp]),
     "vclhip_solver_params_default": (None, [P(SolverParams)]),
+    "vclhip_dia_create_f64": (i32, [P(Csr), P(vp), vp]),
+    "vclhip_dia_create_f32": (i32, [P(Csr), P(vp), vp]),
+    "vclhip_dia_info": (i32, [vp, P(i32), P(u32)]),
+    "vclhip_dia_destroy": (i32, [vp]),
     "vclhip_cg_solve_csr_f64": (i32, [P(Csr), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_cg_solve_sell_f64": (i32, [P(Sell), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_bicgstab_solve_csr_f64": (i32, [P(Csr), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),

@@ -1,11 +1,14 @@
 """Differentiable sparse solve: x = A(values)^{-1} b as a torch.autograd.Function.
 
 The forward pass runs the device CG of libvclhip (vclhip_cg_solve_csr_f64) on torch's current
-stream; the backward pass solves A^T lambda = dL/dx with the same CG (A symmetric, so A^T = A),
-returns dL/db = lambda, and forms dL/dA_ij = -lambda_i x_j on A's pattern with one kernel
-(vclhip_csr_adjoint_grad_f64).  Gradients with respect to whatever produced the values (a
-coefficient field, say) then follow from torch's autograd.  torch supplies device memory and
-streams only; every solve and gradient runs in libvclhip.
+stream; the backward pass solves A^T lambda = dL/dx with the same CG, returns dL/db = lambda, and
+forms dL/dA_ij = -lambda_i x_j on A's pattern with one kernel (vclhip_csr_adjoint_grad_f64).  CG
+needs A symmetric positive definite, so A^T = A: the pattern's symmetry is checked once when the
+CsrPattern is made and the values' symmetry in every forward pass (a nonsymmetric A raises
+ValueError instead of returning a wrong gradient).  Both solves share one DIA detection and copy of
+A (vclhip_dia_create_f64), made in the forward pass.  Gradients with respect to whatever produced
+the values (a coefficient field, say) then follow from torch's autograd.  torch supplies device
+memory and streams only; every solve and gradient runs in libvclhip.
 """
 import ctypes as C
 
@@ -31,6 +34,15 @@ class CsrPattern:
         self.row_ptr = torch.from_numpy(rp.view(np.int32).copy()).to(device)
         self.col = torch.from_numpy((cl if cl.size else np.zeros(1, np.uint32)).view(np.int32).copy()).to(device)
         self.blocks = torch.from_numpy(blocks.view(np.int32).copy()).to(device)
+        # transpose[k] = the entry (col(k), row(k)); None when some entry has no mirror
+        rows = np.repeat(np.arange(self.n, dtype=np.int64), np.diff(rp.astype(np.int64)))
+        key = rows * self.n + cl.astype(np.int64)
+        key_t = cl.astype(np.int64) * self.n + rows
+        order = np.argsort(key, kind="stable")
+        at = np.searchsorted(key, key_t, sorter=order) if key.size else np.zeros(0, np.int64)
+        at = np.minimum(at, max(key.size - 1, 0))
+        found = key[order[at]] == key_t if key.size else np.ones(0, bool)
+        self.transpose = torch.from_numpy(order[at]).to(device) if bool(np.all(found)) else None
 
     def descriptor(self, values):
         return Csr(self.n, self.n, self.nnz, self.num_blocks, self.row_ptr.data_ptr(), self.col.data_ptr(),
@@ -50,14 +62,50 @@ def _check(values, vec, pattern):
         raise ValueError("sizes do not match the pattern")
 
 
-def cg_solve(pattern, values, b, tol=1e-10, max_iters=10000, precond=PRECOND_NONE):
-    """x = A^{-1} b by the device CG; returns (x, SolverResult)."""
+def check_symmetric(pattern, values):
+    """Raises ValueError unless A = (pattern, values) equals its transpose."""
+    if pattern.transpose is None:
+        raise ValueError("the CG solve needs a symmetric matrix: the pattern is not symmetric")
+    if not torch.equal(values, values[pattern.transpose]):
+        raise ValueError("the CG solve needs a symmetric matrix: values[i, j] != values[j, i]")
+
+
+class DiaCache:
+    """The DIA detection and copy of A = (pattern, values) made once (vclhip_dia_create_f64) and
+    shared by the solves with these values."""
+
+    def __init__(self, pattern, values):
+        self.handle = C.c_void_p()
+        if not values.is_cuda or values.dtype != torch.float64 or values.numel() != pattern.nnz:
+            raise ValueError("the DIA copy needs the float64 device values of the pattern")
+        if not values.is_contiguous():
+            raise ValueError("values must be contiguous")
+        self.desc = pattern.descriptor(values)
+        call("vclhip_dia_create_f64", C.byref(self.desc), C.byref(self.handle), _stream())
+
+    def close(self):
+        if self.handle:
+            call("vclhip_dia_destroy", self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cg_solve(pattern, values, b, tol=1e-10, max_iters=10000, precond=PRECOND_NONE, dia=None):
+    """x = A^{-1} b by the device CG; returns (x, SolverResult).  dia: an optional DiaCache of these
+    values (otherwise the solve detects and copies the diagonals itself)."""
     _check(values, b, pattern)
     values = values.contiguous()
     b = b.contiguous()
     x = torch.empty_like(b)
     desc = pattern.descriptor(values)
     params = solver_params(tol=tol, max_iters=max_iters)
+    if dia is not None:
+        params.dia = dia.handle
     res = SolverResult()
     call("vclhip_cg_solve_csr_f64", C.byref(desc), C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), precond,
          C.byref(params), C.byref(res), _stream())
@@ -78,28 +126,38 @@ class SparseSolve(torch.autograd.Function):
     """x = A^{-1} b for a symmetric positive definite A given by (pattern, values)."""
 
     @staticmethod
-    def forward(ctx, values, b, pattern, tol, max_iters, precond):
-        x, res = cg_solve(pattern, values, b, tol, max_iters, precond)
+    def forward(ctx, values, b, pattern, tol, max_iters, precond, check):
+        _check(values, b, pattern)  # types and sizes before any device work on these arrays
+        values = values.contiguous()
+        if check:
+            check_symmetric(pattern, values)
+        dia = DiaCache(pattern, values)
+        x, res = cg_solve(pattern, values, b, tol, max_iters, precond, dia)
         if not res.converged:
+            dia.close()
             raise RuntimeError(f"forward CG did not reach {tol} in {max_iters} iterations (error {res.error:.3e})")
         ctx.save_for_backward(values, x)
         ctx.pattern, ctx.tol, ctx.max_iters, ctx.precond = pattern, tol, max_iters, precond
         ctx.forward_iters = res.iters
+        ctx.dia = dia
         return x
 
     @staticmethod
     def backward(ctx, gx):
         values, x = ctx.saved_tensors
-        lam, res = cg_solve(ctx.pattern, values, gx.contiguous(), ctx.tol, ctx.max_iters, ctx.precond)
+        lam, res = cg_solve(ctx.pattern, values, gx.contiguous(), ctx.tol, ctx.max_iters, ctx.precond, ctx.dia)
+        ctx.dia.close()
         if not res.converged:
             raise RuntimeError(f"adjoint CG did not reach {ctx.tol} in {ctx.max_iters} iterations")
         dA = adjoint_grad(ctx.pattern, values, lam, x) if ctx.needs_input_grad[0] else None
-        return dA, lam, None, None, None, None
+        return dA, lam, None, None, None, None, None
 
 
-def sparse_solve(values, b, pattern, tol=1e-10, max_iters=10000, jacobi=False):
-    """Differentiable x = A(values)^{-1} b (A symmetric positive definite)."""
-    return SparseSolve.apply(values, b, pattern, tol, max_iters, PRECOND_JACOBI if jacobi else PRECOND_NONE)
+def sparse_solve(values, b, pattern, tol=1e-10, max_iters=10000, jacobi=False, check_symmetric=True):
+    """Differentiable x = A(values)^{-1} b (A symmetric positive definite; check_symmetric=False skips
+    the per-call value check for callers that build A symmetric by construction)."""
+    return SparseSolve.apply(values, b, pattern, tol, max_iters, PRECOND_JACOBI if jacobi else PRECOND_NONE,
+                             bool(check_symmetric))
 
 
 # ------------------------------------------------------------------ 2-D diffusion operator in torch

@@ -6,6 +6,7 @@
 // structure is set, like the reference's generate_row_block_information).
 
 #include <map>
+#include <memory>
 #include <vector>
 
 #include "viennacl/backend/memory.hpp"
@@ -22,11 +23,11 @@ public:
   typedef backend::mem_handle handle_type;
   typedef vcl_size_t size_type;
 
-  compressed_matrix() : rows_(0), cols_(0), nonzeros_(0), row_block_num_(0) {}
+  compressed_matrix() : rows_(0), cols_(0), nonzeros_(0), row_block_num_(0), wrapped_(false) {}
 
   /** Allocates (zeroed) storage for a rows x cols matrix with nonzeros entries (compressed_matrix.hpp:642-676). */
   compressed_matrix(size_type rows, size_type cols, size_type nonzeros = 0, viennacl::context ctx = viennacl::context())
-    : rows_(rows), cols_(cols), nonzeros_(nonzeros), row_block_num_(0)
+    : rows_(rows), cols_(cols), nonzeros_(nonzeros), row_block_num_(0), wrapped_(false)
   {
     std::vector<uint32_t> rp(rows + 1, 0);
     backend::memory_create(row_buffer_, sizeof(uint32_t) * (rows + 1), ctx, rp.data());
@@ -37,15 +38,16 @@ public:
     generate_row_block_information();
   }
 
-  explicit compressed_matrix(viennacl::context) : rows_(0), cols_(0), nonzeros_(0), row_block_num_(0) {}
+  explicit compressed_matrix(viennacl::context) : rows_(0), cols_(0), nonzeros_(0), row_block_num_(0), wrapped_(false) {}
 
   /** Wraps caller-owned device CSR arrays (compressed_matrix.hpp:740-781); the caller keeps ownership
    *  and the arrays must outlive the matrix.  The row-block plan is built from a host copy of row_ptr. */
   compressed_matrix(const uint32_t* row_ptr_dev, const uint32_t* col_dev, const NumericT* values_dev,
                     memory_types mem_type, size_type rows, size_type cols, size_type nonzeros)
-    : rows_(rows), cols_(cols), nonzeros_(nonzeros), row_block_num_(0)
+    : rows_(rows), cols_(cols), nonzeros_(nonzeros), row_block_num_(0), wrapped_(false)
   {
     if (mem_type != HIP_MEMORY) throw memory_exception("only HIP_MEMORY arrays can be wrapped");
+    wrapped_ = true;
     row_buffer_.wrap_hip(const_cast<uint32_t*>(row_ptr_dev), sizeof(uint32_t) * (rows + 1));
     col_buffer_.wrap_hip(const_cast<uint32_t*>(col_dev), sizeof(uint32_t) * nonzeros);
     elements_.wrap_hip(const_cast<NumericT*>(values_dev), sizeof(NumericT) * nonzeros);
@@ -53,7 +55,7 @@ public:
   }
 
   compressed_matrix(compressed_matrix const& other)
-    : rows_(0), cols_(0), nonzeros_(0), row_block_num_(0)
+    : rows_(0), cols_(0), nonzeros_(0), row_block_num_(0), wrapped_(false)
   {
     *this = other;
   }
@@ -61,6 +63,8 @@ public:
   compressed_matrix& operator=(compressed_matrix const& other)
   {
     if (this == &other) return *this;
+    dia_.reset();
+    wrapped_ = false;
     rows_ = other.rows_;
     cols_ = other.cols_;
     nonzeros_ = other.nonzeros_;
@@ -81,6 +85,8 @@ public:
   void set(const void* row_jumper, const void* col_buffer, const NumericT* elements, size_type rows, size_type cols,
            size_type nonzeros)
   {
+    dia_.reset();
+    wrapped_ = false;
     viennacl::context ctx(HIP_MEMORY);
     backend::memory_create(row_buffer_, sizeof(uint32_t) * (rows + 1), ctx, row_jumper);
     backend::memory_create(col_buffer_, sizeof(uint32_t) * nonzeros, ctx, col_buffer);
@@ -107,24 +113,67 @@ public:
     backend::memory_create(row_blocks_, sizeof(uint32_t) * (nb + 1), viennacl::context(HIP_MEMORY), blocks.data());
   }
 
+  /** Room for new_nonzeros entries (compressed_matrix.hpp:946-973): the column and value arrays grow,
+   *  keeping their entries when `preserve`; nnz() then reports the new capacity, as in the reference. */
+  void reserve(size_type new_nonzeros, bool preserve = true)
+  {
+    dia_.reset();
+    if (new_nonzeros <= nonzeros_) return;
+    viennacl::context ctx(HIP_MEMORY);
+    handle_type col_new, val_new;
+    backend::memory_create(col_new, sizeof(uint32_t) * new_nonzeros, ctx);
+    backend::memory_create(val_new, sizeof(NumericT) * new_nonzeros, ctx);
+    if (preserve && nonzeros_ > 0) {
+      backend::memory_copy(col_buffer_, col_new, 0, 0, sizeof(uint32_t) * nonzeros_);
+      backend::memory_copy(elements_, val_new, 0, 0, sizeof(NumericT) * nonzeros_);
+    }
+    col_buffer_.swap(col_new);
+    elements_.swap(val_new);
+    nonzeros_ = new_nonzeros;
+  }
+
+  /** New dimensions (compressed_matrix.hpp:982-1040).  With `preserve` the entries inside the new
+   *  dimensions are kept and those in removed rows or columns are dropped (growing a matrix that had
+   *  no rows stores an explicit zero at (0, 0), as the reference does); without it the matrix is
+   *  left with no entries. */
   void resize(size_type new_rows, size_type new_cols, bool preserve = true)
   {
-    if (preserve && nonzeros_ > 0 && (new_rows < rows_ || new_cols < cols_))
-      throw memory_exception("shrinking resize with preserve is not provided by the HIP backend");
-    if (!preserve || nonzeros_ == 0) {
+    if (new_rows == 0 || new_cols == 0) throw std::invalid_argument("ViennaCL (HIP): cannot resize a sparse matrix to zero size");
+    if (new_rows == rows_ && new_cols == cols_) return;
+    if (!preserve) {
       *this = compressed_matrix(new_rows, new_cols, 0);
       return;
     }
-    std::vector<uint32_t> rp(rows_ + 1);
-    backend::memory_read(row_buffer_, 0, sizeof(uint32_t) * (rows_ + 1), rp.data());
-    rp.resize(new_rows + 1, rp[rows_]);
-    backend::memory_create(row_buffer_, sizeof(uint32_t) * (new_rows + 1), viennacl::context(HIP_MEMORY), rp.data());
-    rows_ = new_rows;
-    cols_ = new_cols;
-    generate_row_block_information(rp.data());
+    std::vector<std::map<uint32_t, NumericT> > stl(rows_);
+    if (rows_ > 0) {
+      copy_to_stl(stl);
+    } else {
+      stl.resize(new_rows);
+      stl[0][0] = NumericT(0);
+    }
+    stl.resize(new_rows);
+    if (new_cols < cols_)
+      for (vcl_size_t i = 0; i < stl.size(); ++i) stl[i].erase(stl[i].lower_bound(uint32_t(new_cols)), stl[i].end());
+    set_from_stl(stl, new_cols);
   }
 
+  /** Removes every entry, keeping the dimensions (compressed_matrix.hpp:1043-1055). */
   void clear() { *this = compressed_matrix(rows_, cols_, 0); }
+
+  /** Entry (i, j) as a host-side proxy (compressed_matrix.hpp:1057-1076); an entry outside the
+   *  sparsity pattern is inserted first (with the value 0), which rebuilds the matrix. */
+  entry_proxy<NumericT> operator()(size_type i, size_type j)
+  {
+    dia_.reset();
+    if (i >= rows_ || j >= cols_) throw std::out_of_range("ViennaCL (HIP): compressed_matrix index out of range");
+    size_type index = element_index(i, j);
+    if (index < nonzeros_) return entry_proxy<NumericT>(index, elements_);
+    std::vector<std::map<uint32_t, NumericT> > stl(rows_);
+    copy_to_stl(stl);
+    stl[i][uint32_t(j)] = NumericT(0);
+    set_from_stl(stl, cols_);
+    return entry_proxy<NumericT>(element_index(i, j), elements_);
+  }
 
   size_type size1() const { return rows_; }
   size_type size2() const { return cols_; }
@@ -135,10 +184,29 @@ public:
   handle_type const& handle2() const { return col_buffer_; }
   handle_type const& handle3() const { return row_blocks_; }
   handle_type const& handle() const { return elements_; }
-  handle_type& handle1() { return row_buffer_; }
-  handle_type& handle2() { return col_buffer_; }
-  handle_type& handle3() { return row_blocks_; }
-  handle_type& handle() { return elements_; }
+  // Writable access may change the matrix: it drops the solvers' DIA copy.
+  handle_type& handle1() { dia_.reset(); return row_buffer_; }
+  handle_type& handle2() { dia_.reset(); return col_buffer_; }
+  handle_type& handle3() { dia_.reset(); return row_blocks_; }
+  handle_type& handle() { dia_.reset(); return elements_; }
+
+  /** The solvers' DIA detection and copy of this matrix (vclhip_dia_create_*), made on the first
+   *  solve and kept until the matrix can have changed: set(), resize(), reserve(), operator()(i, j)
+   *  and writable handle access drop it.  A matrix wrapping caller-owned arrays keeps none (the
+   *  caller may change them at any time), and neither does an empty one. */
+  vclhip_dia const* dia_cache() const
+  {
+    if (wrapped_ || rows_ == 0 || rows_ != cols_ || nonzeros_ == 0) return nullptr;
+    if (!dia_) {
+      vclhip_csr d = descriptor();
+      vclhip_dia* p = nullptr;
+      const int st = std::is_same<NumericT, float>::value ? vclhip_dia_create_f32(&d, &p, backend::hip::current_stream())
+                                                          : vclhip_dia_create_f64(&d, &p, backend::hip::current_stream());
+      backend::hip::check(st, "DIA operator");
+      dia_.reset(p, [](vclhip_dia* q) { (void)vclhip_dia_destroy(q); });
+    }
+    return dia_.get();
+  }
 
   /** The descriptor passed to the C ABI. */
   vclhip_csr descriptor() const
@@ -161,10 +229,54 @@ public:
   }
 
 private:
+  /** Position of entry (i, j) in the column and value arrays, or nonzeros_ when it is not stored. */
+  size_type element_index(size_type i, size_type j) const
+  {
+    uint32_t rr[2];
+    backend::memory_read(row_buffer_, sizeof(uint32_t) * i, sizeof(rr), rr);
+    if (rr[1] <= rr[0]) return nonzeros_;
+    std::vector<uint32_t> cols(rr[1] - rr[0]);
+    backend::memory_read(col_buffer_, sizeof(uint32_t) * rr[0], sizeof(uint32_t) * cols.size(), cols.data());
+    for (size_type k = 0; k < cols.size(); ++k)
+      if (cols[k] == j) return rr[0] + k;
+    return nonzeros_;
+  }
+
+  void copy_to_stl(std::vector<std::map<uint32_t, NumericT> >& stl) const
+  {
+    std::vector<uint32_t> rp(rows_ + 1, 0);
+    if (row_buffer_.raw_size()) backend::memory_read(row_buffer_, 0, sizeof(uint32_t) * rp.size(), rp.data());
+    const uint32_t stored = rp[rows_];
+    std::vector<uint32_t> col(stored);
+    std::vector<NumericT> val(stored);
+    if (stored) {
+      backend::memory_read(col_buffer_, 0, sizeof(uint32_t) * stored, col.data());
+      backend::memory_read(elements_, 0, sizeof(NumericT) * stored, val.data());
+    }
+    for (size_type i = 0; i < rows_ && i < stl.size(); ++i)
+      for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) stl[i][col[k]] = val[k];
+  }
+
+  void set_from_stl(std::vector<std::map<uint32_t, NumericT> > const& stl, size_type cols)
+  {
+    std::vector<uint32_t> rp(stl.size() + 1, 0), col;
+    std::vector<NumericT> val;
+    for (size_type i = 0; i < stl.size(); ++i) {
+      for (typename std::map<uint32_t, NumericT>::const_iterator it = stl[i].begin(); it != stl[i].end(); ++it) {
+        col.push_back(it->first);
+        val.push_back(it->second);
+      }
+      rp[i + 1] = uint32_t(col.size());
+    }
+    set(rp.data(), col.data(), val.data(), stl.size(), cols, val.size());
+  }
+
   size_type rows_;
   size_type cols_;
   size_type nonzeros_;
   size_type row_block_num_;
+  bool wrapped_;
+  mutable std::shared_ptr<vclhip_dia> dia_;
   handle_type row_buffer_;
   handle_type row_blocks_;
   handle_type col_buffer_;

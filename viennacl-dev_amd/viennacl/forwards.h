@@ -73,6 +73,10 @@ inline void check(int status, const char* where)
 
 template <typename NumericT> class vector_base;
 template <typename NumericT, unsigned int AlignmentV = 1> class vector;
+template <class NumericT, unsigned int AlignmentV = 1> class const_vector_iterator;
+template <class NumericT, unsigned int AlignmentV = 1> class vector_iterator;
+template <typename VectorType> class vector_range;
+template <typename VectorType> class vector_slice;
 template <typename NumericT, unsigned int AlignmentV = 1> class compressed_matrix;
 template <typename ScalarT, typename IndexT = unsigned int> class sliced_ell_matrix;
 template <typename NumericT, unsigned int AlignmentV = 128> class coordinate_matrix;

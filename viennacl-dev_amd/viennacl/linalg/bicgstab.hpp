@@ -53,6 +53,7 @@ vector<T> bicgstab_device(MatrixT const& A, vector_base<T> const& rhs, bicgstab_
   vector<T> b = contiguous_copy(rhs);
   vector<T> x(rhs.size());
   vclhip_solver_params P = default_params();
+  P.dia = dia_of(A);
   P.tol = tag.tolerance();
   P.abs_tol = tag.abs_tolerance();
   P.max_iters = static_cast<uint32_t>(tag.max_iterations());
@@ -136,6 +137,16 @@ solve_impl(MatrixT const& A, vector_base<T> const& rhs, bicgstab_tag const& tag,
     return bicgstab_device(A, rhs, tag, VCLHIP_PRECOND_NONE, monitor, monitor_data);
   if (is_jacobi<PreconditionerT>::value)
     return bicgstab_device(A, rhs, tag, VCLHIP_PRECOND_JACOBI, monitor, monitor_data);
+  vector<T> b(rhs);
+  return bicgstab_generic(A, b, tag, precond, monitor, monitor_data);
+}
+/** A user operator (apply(x, y) and size1(); examples/tutorial/matrix-free.cpp:58-82): the classic
+ *  recurrences on the HIP vector operations, each product through apply(). */
+template <typename MatrixT, typename T, typename PreconditionerT>
+typename std::enable_if<!is_hip_sparse<MatrixT>::value, vector<T> >::type
+solve_impl(MatrixT const& A, vector_base<T> const& rhs, bicgstab_tag const& tag, PreconditionerT const& precond,
+           bool (*monitor)(vector<T> const&, T, void*) = nullptr, void* monitor_data = nullptr)
+{
   vector<T> b(rhs);
   return bicgstab_generic(A, b, tag, precond, monitor, monitor_data);
 }

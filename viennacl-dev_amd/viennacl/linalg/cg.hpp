@@ -51,6 +51,7 @@ vector<T> cg_device(MatrixT const& A, vector_base<T> const& rhs, cg_tag const& t
   vector<T> b = contiguous_copy(rhs);
   vector<T> x(rhs.size());
   vclhip_solver_params P = default_params();
+  P.dia = dia_of(A);
   P.tol = tag.tolerance();
   P.abs_tol = tag.abs_tolerance();
   P.max_iters = tag.max_iterations();
@@ -115,6 +116,16 @@ solve_impl(MatrixT const& A, vector_base<T> const& rhs, cg_tag const& tag, Preco
     return cg_device(A, rhs, tag, VCLHIP_PRECOND_NONE, monitor, monitor_data);
   if (is_jacobi<PreconditionerT>::value)
     return cg_device(A, rhs, tag, VCLHIP_PRECOND_JACOBI, monitor, monitor_data);
+  vector<T> b(rhs);
+  return cg_generic(A, b, tag, precond, monitor, monitor_data);
+}
+/** A user operator (apply(x, y) and size1(); examples/tutorial/matrix-free.cpp:58-82): the classic
+ *  recurrences on the HIP vector operations, each product through apply(). */
+template <typename MatrixT, typename T, typename PreconditionerT>
+typename std::enable_if<!is_hip_sparse<MatrixT>::value, vector<T> >::type
+solve_impl(MatrixT const& A, vector_base<T> const& rhs, cg_tag const& tag, PreconditionerT const& precond,
+           bool (*monitor)(vector<T> const&, T, void*) = nullptr, void* monitor_data = nullptr)
+{
   vector<T> b(rhs);
   return cg_generic(A, b, tag, precond, monitor, monitor_data);
 }

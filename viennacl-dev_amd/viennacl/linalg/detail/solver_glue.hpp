@@ -42,6 +42,13 @@ inline vclhip_solver_params default_params()
   return p;
 }
 
+/** The DIA copy a CSR matrix keeps for its solves (compressed_matrix::dia_cache); none for the
+ *  other formats. */
+template <typename MatrixT>
+vclhip_dia const* dia_of(MatrixT const&) { return nullptr; }
+template <typename T, unsigned A>
+vclhip_dia const* dia_of(compressed_matrix<T, A> const& m) { return m.dia_cache(); }
+
 // ---- dispatch on value type and format
 inline int cg_solve(vclhip_csr const& A, const double* b, double* x, int pc, vclhip_solver_params const& P,
                     vclhip_solver_result& R)

@@ -90,6 +90,7 @@ vector<T> gmres_device(MatrixT const& A, vector_base<T> const& rhs, gmres_tag co
   vector<T> b = contiguous_copy(rhs);
   vector<T> x(rhs.size());
   vclhip_solver_params P = default_params();
+  P.dia = dia_of(A);
   P.tol = tag.tolerance();
   P.abs_tol = tag.abs_tolerance();
   P.max_iters = tag.max_iterations();
@@ -229,6 +230,16 @@ solve_impl(MatrixT const& A, vector_base<T> const& rhs, gmres_tag const& tag, Pr
            bool (*monitor)(vector<T> const&, T, void*) = nullptr, void* monitor_data = nullptr)
 {
   if (std::is_same<PreconditionerT, no_precond>::value) return gmres_device(A, rhs, tag, monitor, monitor_data);
+  vector<T> b(rhs);
+  return gmres_generic(A, b, tag, precond, monitor, monitor_data);
+}
+/** A user operator (apply(x, y) and size1(); examples/tutorial/matrix-free.cpp:58-82): the classic
+ *  recurrences on the HIP vector operations, each product through apply(). */
+template <typename MatrixT, typename T, typename PreconditionerT>
+typename std::enable_if<!is_hip_sparse<MatrixT>::value, vector<T> >::type
+solve_impl(MatrixT const& A, vector_base<T> const& rhs, gmres_tag const& tag, PreconditionerT const& precond,
+           bool (*monitor)(vector<T> const&, T, void*) = nullptr, void* monitor_data = nullptr)
+{
   vector<T> b(rhs);
   return gmres_generic(A, b, tag, precond, monitor, monitor_data);
 }

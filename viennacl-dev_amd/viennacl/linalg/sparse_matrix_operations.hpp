@@ -5,6 +5,11 @@
 // (reference linalg/sparse_matrix_operations.hpp:48-121 and the op_executors of
 // compressed_matrix.hpp:1231-1318 / sliced_ell_matrix.hpp:286-375).  HIP_MEMORY only.
 
+#include <map>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
 #include "viennacl/forwards.h"
 #include "viennacl/vector.hpp"
 
@@ -125,7 +130,56 @@ row_info(MatrixT const& mat, vector_base<T>& vec, row_info_types info)
 }  // namespace detail
 }  // namespace linalg
 
-/** y = prod(A, x); aliasing of y and x goes through a temporary (compressed_matrix.hpp:1231-1246). */
+namespace linalg
+{
+namespace detail
+{
+/** Whether MatrixT is one of the sparse formats (it hands a device descriptor to the C ABI) or a
+ *  user-provided operator, which only has apply(x, y) and size1() (examples/tutorial/matrix-free.cpp:58-82). */
+template <typename MatrixT, typename = void>
+struct has_descriptor : std::false_type {};
+template <typename MatrixT>
+struct has_descriptor<MatrixT, decltype(void(std::declval<MatrixT const&>().descriptor()))> : std::true_type {};
+
+/** y = A x for a sparse matrix (y, x distinct). */
+template <typename MatrixT, typename T>
+typename std::enable_if<has_descriptor<MatrixT>::value>::type
+apply_operator(MatrixT const& A, vector_base<T> const& x, vector_base<T>& y)
+{
+  linalg::prod_impl(A, x, T(1), y, T(0));
+}
+
+/** y = A x for a user operator: its apply() (the generic op_executor of vector.hpp:3315-3320). */
+template <typename MatrixT, typename T>
+typename std::enable_if<!has_descriptor<MatrixT>::value>::type
+apply_operator(MatrixT const& A, vector_base<T> const& x, vector_base<T>& y)
+{
+  A.apply(x, y);
+}
+
+/** y += sign * A x for a sparse matrix: one fused product (beta = 1). */
+template <typename MatrixT, typename T>
+typename std::enable_if<has_descriptor<MatrixT>::value>::type
+accumulate_operator(MatrixT const& A, vector_base<T> const& x, T sign, vector_base<T>& y)
+{
+  linalg::prod_impl(A, x, sign, y, T(1));
+}
+
+/** y += sign * A x for a user operator, through a temporary. */
+template <typename MatrixT, typename T>
+typename std::enable_if<!has_descriptor<MatrixT>::value>::type
+accumulate_operator(MatrixT const& A, vector_base<T> const& x, T sign, vector_base<T>& y)
+{
+  vector_base<T> tmp(A.size1());
+  A.apply(x, tmp);
+  y.require_hip();
+  backend::hip::check(viennacl::detail::avbv(y, y, T(1), tmp, sign), "y += A x");
+}
+}  // namespace detail
+}  // namespace linalg
+
+/** y = prod(A, x) for a sparse matrix or a user operator; aliasing of y and x goes through a
+ *  temporary (compressed_matrix.hpp:1231-1246). */
 template <typename T, typename MatrixT>
 void vcl_eval(vector_base<T>& y, vector_expression<MatrixT, vector_base<T>, op_prod> const& e)
 {
@@ -134,14 +188,15 @@ void vcl_eval(vector_base<T>& y, vector_expression<MatrixT, vector_base<T>, op_p
   if (y.size() == 0) y.init(A.size1(), viennacl::context(HIP_MEMORY));
   if (y.handle() == x.handle()) {
     vector_base<T> tmp(A.size1());
-    linalg::prod_impl(A, x, T(1), tmp, T(0));
+    linalg::detail::apply_operator(A, x, tmp);
     y.assign_linear(tmp, T(1));
     return;
   }
-  linalg::prod_impl(A, x, T(1), y, T(0));
+  linalg::detail::apply_operator(A, x, y);
 }
 
-/** y += sign * prod(A, x) (compressed_matrix.hpp:1248-1318). */
+/** y += sign * prod(A, x) (compressed_matrix.hpp:1248-1318); a user operator's product goes
+ *  through a temporary. */
 template <typename T, typename MatrixT>
 void vcl_inplace(vector_base<T>& y, vector_expression<MatrixT, vector_base<T>, op_prod> const& e, T sign)
 {
@@ -149,12 +204,12 @@ void vcl_inplace(vector_base<T>& y, vector_expression<MatrixT, vector_base<T>, o
   vector_base<T> const& x = e.rhs();
   if (y.handle() == x.handle()) {
     vector_base<T> tmp(A.size1());
-    linalg::prod_impl(A, x, T(1), tmp, T(0));
+    linalg::detail::apply_operator(A, x, tmp);
     y.require_hip();
     backend::hip::check(detail::avbv(y, y, T(1), tmp, sign), "y += A x");
     return;
   }
-  linalg::prod_impl(A, x, sign, y, T(1));
+  linalg::detail::accumulate_operator(A, x, sign, y);
 }
 
 }  // namespace viennacl

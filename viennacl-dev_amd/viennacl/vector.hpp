@@ -5,7 +5,10 @@
 // copies, and the expressions the Krylov path uses (x = a*y + b*z, x += a*y, x = prod(A, y),
 // element_div, inner_prod, norm_2).  Every operation on HIP_MEMORY runs through libvclhip.
 
+#include <algorithm>
 #include <cmath>
+#include <iterator>
+#include <memory>
 #include <type_traits>
 #include <vector>
 
@@ -47,6 +50,8 @@ template <typename T> struct is_vector_like : std::false_type {};
 template <typename T> struct is_vector_like<vector_base<T> > : std::true_type {};
 template <typename T, unsigned int A> struct is_vector_like<vector<T, A> > : std::true_type {};
 template <typename L, typename R, typename O> struct is_vector_like<vector_expression<L, R, O> > : std::true_type {};
+template <typename V> struct is_vector_like<vector_range<V> > : std::true_type {};
+template <typename V> struct is_vector_like<vector_slice<V> > : std::true_type {};
 
 inline void* stream() { return backend::hip::current_stream(); }
 
@@ -81,6 +86,112 @@ template <> struct abi<float>
 };
 }  // namespace detail
 
+// ------------------------------------------------------------------ entry access and iterators
+
+/** Read/write access to one entry of a device buffer through host round trips (reference
+ *  entry_proxy, scalar.hpp:41-196): `offset` counts elements from the start of the buffer. */
+template <typename NumericT>
+class entry_proxy
+{
+public:
+  entry_proxy(vcl_size_t offset, backend::mem_handle& h) : offset_(offset), h_(&h) {}
+  operator NumericT() const
+  {
+    NumericT v;
+    backend::memory_read(*h_, sizeof(NumericT) * offset_, sizeof(NumericT), &v);
+    return v;
+  }
+  entry_proxy& operator=(NumericT v)
+  {
+    backend::memory_write(*h_, sizeof(NumericT) * offset_, sizeof(NumericT), &v);
+    return *this;
+  }
+  entry_proxy& operator=(entry_proxy const& o) { return *this = NumericT(o); }
+  entry_proxy& operator+=(NumericT v) { return *this = NumericT(*this) + v; }
+  entry_proxy& operator-=(NumericT v) { return *this = NumericT(*this) - v; }
+  entry_proxy& operator*=(NumericT v) { return *this = NumericT(*this) * v; }
+  entry_proxy& operator/=(NumericT v) { return *this = NumericT(*this) / v; }
+  vcl_size_t index() const { return offset_; }
+  backend::mem_handle const& handle() const { return *h_; }
+
+private:
+  vcl_size_t offset_;
+  backend::mem_handle* h_;
+};
+
+/** Read-only iterator over the entries of a vector or vector proxy (reference
+ *  const_vector_iterator, vector.hpp:108-201): entry i sits at start + i * stride of the buffer;
+ *  dereferencing reads it back to the host. */
+template <class NumericT, unsigned int AlignmentV>
+class const_vector_iterator
+{
+  typedef const_vector_iterator<NumericT, AlignmentV> self_type;
+
+public:
+  typedef NumericT value_type;
+  typedef vcl_size_t size_type;
+  typedef vcl_ptrdiff_t difference_type;
+  typedef backend::mem_handle handle_type;
+
+  const_vector_iterator(vector_base<NumericT> const& vec, size_type index, size_type start = 0, size_type stride = 1)
+    : elements_(&vec.handle()), index_(index), start_(start), stride_(stride) {}
+  const_vector_iterator(handle_type const& elements, size_type index, size_type start = 0, size_type stride = 1)
+    : elements_(&elements), index_(index), start_(start), stride_(stride) {}
+
+  NumericT operator*() const
+  {
+    NumericT v;
+    backend::memory_read(*elements_, sizeof(NumericT) * offset(), sizeof(NumericT), &v);
+    return v;
+  }
+  self_type& operator++() { ++index_; return *this; }
+  self_type operator++(int) { self_type t = *this; ++index_; return t; }
+  bool operator==(self_type const& o) const { return index_ == o.index_; }
+  bool operator!=(self_type const& o) const { return index_ != o.index_; }
+  difference_type operator-(self_type const& o) const { return difference_type(index_) - difference_type(o.index_); }
+  self_type operator+(difference_type d) const { return self_type(*elements_, size_type(difference_type(index_) + d), start_, stride_); }
+
+  size_type index() const { return index_; }
+  size_type offset() const { return start_ + index_ * stride_; }
+  size_type stride() const { return stride_; }
+  handle_type const& handle() const { return *elements_; }
+
+protected:
+  handle_type const* elements_;
+  size_type index_;
+  size_type start_;
+  size_type stride_;
+};
+
+/** Writable iterator (reference vector_iterator, vector.hpp:205-249): dereferencing gives an
+ *  entry_proxy. */
+template <class NumericT, unsigned int AlignmentV>
+class vector_iterator : public const_vector_iterator<NumericT, AlignmentV>
+{
+  typedef const_vector_iterator<NumericT, AlignmentV> base_type;
+  typedef vector_iterator<NumericT, AlignmentV> self_type;
+
+public:
+  typedef typename base_type::handle_type handle_type;
+  typedef typename base_type::size_type size_type;
+  typedef typename base_type::difference_type difference_type;
+
+  vector_iterator(handle_type& elements, size_type index, size_type start = 0, size_type stride = 1)
+    : base_type(elements, index, start, stride) {}
+  vector_iterator(vector_base<NumericT>& vec, size_type index, size_type start = 0, size_type stride = 1)
+    : base_type(vec.handle(), index, start, stride) {}
+
+  entry_proxy<NumericT> operator*() const { return entry_proxy<NumericT>(this->offset(), handle()); }
+  self_type& operator++() { ++this->index_; return *this; }
+  self_type operator++(int) { self_type t = *this; ++this->index_; return t; }
+  difference_type operator-(self_type const& o) const { return difference_type(this->index_) - difference_type(o.index_); }
+  self_type operator+(difference_type d) const
+  {
+    return self_type(handle(), size_type(difference_type(this->index_) + d), this->start_, this->stride_);
+  }
+  handle_type& handle() const { return const_cast<handle_type&>(*this->elements_); }
+};
+
 // ------------------------------------------------------------------ vector_base
 
 template <typename NumericT>
@@ -95,34 +206,8 @@ public:
   typedef vcl_size_t size_type;
   typedef vcl_ptrdiff_t difference_type;
 
-  /** Read/write access to one entry through host round trips (reference entry_proxy). */
-  class entry_proxy
-  {
-  public:
-    entry_proxy(vector_base& v, size_type i) : v_(v), i_(i) {}
-    operator NumericT() const { return v_.get_entry(i_); }
-    entry_proxy& operator=(NumericT val) { v_.set_entry(i_, val); return *this; }
-  private:
-    vector_base& v_;
-    size_type i_;
-  };
-
-  /** Position in a vector for fast_copy / copy ranges (reference vector_iterator). */
-  class const_iterator
-  {
-  public:
-    const_iterator(vector_base const* v, size_type i) : v_(v), i_(i) {}
-    size_type index() const { return i_; }
-    vector_base const& vec() const { return *v_; }
-    difference_type operator-(const_iterator const& o) const { return difference_type(i_) - difference_type(o.i_); }
-    const_iterator operator+(difference_type d) const { return const_iterator(v_, i_ + d); }
-    bool operator==(const_iterator const& o) const { return v_ == o.v_ && i_ == o.i_; }
-    bool operator!=(const_iterator const& o) const { return !(*this == o); }
-  private:
-    vector_base const* v_;
-    size_type i_;
-  };
-  typedef const_iterator iterator;
+  typedef const_vector_iterator<NumericT, 1> const_iterator;
+  typedef vector_iterator<NumericT, 1> iterator;
 
   vector_base() : size_(0), start_(0), stride_(1), internal_size_(0) {}
 
@@ -190,11 +275,13 @@ public:
   backend::mem_handle const& handle() const { return elements_; }
   NumericT* device_ptr() const { return elements_.template hip_ptr<NumericT>(); }
 
-  const_iterator begin() const { return const_iterator(this, 0); }
-  const_iterator end() const { return const_iterator(this, size_); }
+  const_iterator begin() const { return const_iterator(elements_, 0, start_, stride_); }
+  const_iterator end() const { return const_iterator(elements_, size_, start_, stride_); }
+  iterator begin() { return iterator(elements_, 0, start_, stride_); }
+  iterator end() { return iterator(elements_, size_, start_, stride_); }
 
-  entry_proxy operator[](size_type i) { return entry_proxy(*this, i); }
-  entry_proxy operator()(size_type i) { return entry_proxy(*this, i); }
+  entry_proxy<NumericT> operator[](size_type i) { return entry_proxy<NumericT>(start_ + i * stride_, elements_); }
+  entry_proxy<NumericT> operator()(size_type i) { return entry_proxy<NumericT>(start_ + i * stride_, elements_); }
   NumericT operator[](size_type i) const { return get_entry(i); }
   NumericT operator()(size_type i) const { return get_entry(i); }
 
@@ -414,10 +501,11 @@ inline int avbv<double>(vector_base<double>& y, vector_base<double> const& a, do
 template <>
 inline int avbv<float>(vector_base<float>& y, vector_base<float> const& a, float ca, vector_base<float> const& b, float cb)
 {
-  // contiguous float vectors only (the Krylov path's FP32 vectors are never strided)
-  if (!y.contiguous() || !a.contiguous() || !b.contiguous())
-    throw memory_exception("strided float vector expressions are not provided by the HIP backend");
-  return vclhip_axpby_f32(uint32_t(y.size()), ca, a.device_ptr(), cb, b.device_ptr(), y.device_ptr(), stream());
+  if (y.contiguous() && a.contiguous() && b.contiguous())
+    return vclhip_axpby_f32(uint32_t(y.size()), ca, a.device_ptr(), cb, b.device_ptr(), y.device_ptr(), stream());
+  return vclhip_avbv_f32(uint32_t(y.size()), y.device_ptr(), uint32_t(y.start()), uint32_t(y.stride()), a.device_ptr(),
+                         uint32_t(a.start()), uint32_t(a.stride()), ca, b.device_ptr(), uint32_t(b.start()),
+                         uint32_t(b.stride()), cb, stream());
 }
 
 /** y = sum of the terms; y may appear among them. */
@@ -658,68 +746,127 @@ void swap(vector_base<T>& x, vector_base<T>& y)
 }
 
 // ------------------------------------------------------------------ host <-> device copies
-/** copy(std::vector -> vector) (vector.hpp copy overloads); an empty target is sized. */
-template <typename T, typename CpuT>
-void copy(std::vector<CpuT> const& cpu, vector_base<T>& gpu)
+/** copy(host vector -> vector) (vector.hpp:1509-1527): any host container with begin()/end(); an empty
+ *  target is sized, a view is written in place. */
+template <typename HostVectorT, typename T>
+typename std::enable_if<!detail::is_vector_like<HostVectorT>::value>::type
+copy(HostVectorT const& cpu, vector_base<T>& gpu)
 {
-  if (gpu.size() == 0) gpu.init(cpu.size(), viennacl::context(HIP_MEMORY));
-  gpu.check_size(cpu.size());
-  std::vector<T> tmp(cpu.begin(), cpu.end());
-  if (gpu.contiguous()) {
-    backend::memory_write(gpu.handle(), 0, sizeof(T) * tmp.size(), tmp.data());
-  } else {
-    for (size_t i = 0; i < tmp.size(); ++i) gpu.set_entry(i, tmp[i]);
-  }
+  const vcl_size_t n = vcl_size_t(std::distance(cpu.begin(), cpu.end()));
+  if (gpu.size() == 0) gpu.init(n, viennacl::context(HIP_MEMORY));
+  gpu.check_size(n);
+  copy(cpu.begin(), cpu.end(), gpu.begin());
 }
 
-/** copy(vector -> std::vector); the target is resized. */
+/** copy(vector -> host vector) (vector.hpp:1358-1366): the host container must hold size() entries;
+ *  a std::vector is resized. */
 template <typename T, typename CpuT>
 void copy(vector_base<T> const& gpu, std::vector<CpuT>& cpu)
 {
-  std::vector<T> tmp(gpu.size());
-  if (gpu.contiguous()) {
-    backend::memory_read(gpu.handle(), 0, sizeof(T) * tmp.size(), tmp.data());
-  } else {
-    for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = gpu.get_entry(i);
-  }
-  cpu.assign(tmp.begin(), tmp.end());
+  cpu.resize(gpu.size());
+  copy(gpu.begin(), gpu.end(), cpu.begin());
 }
 
-/** fast_copy(gpu_begin, gpu_end, cpu_begin) (vector.hpp:1234-1262): contiguous device range -> host. */
-template <typename T>
-void fast_copy(typename vector_base<T>::const_iterator const& gpu_begin,
-               typename vector_base<T>::const_iterator const& gpu_end, T* cpu_begin)
+/** fast_copy(gpu_begin, gpu_end, cpu_begin) (vector.hpp:1233-1258): device range -> host memory that is
+ *  one linear piece (&*cpu_begin onwards); a strided range is read with its gaps and unpacked. */
+template <typename NumericT, unsigned int AlignmentV, typename CpuIt>
+void fast_copy(const_vector_iterator<NumericT, AlignmentV> const& gpu_begin,
+               const_vector_iterator<NumericT, AlignmentV> const& gpu_end, CpuIt cpu_begin)
 {
-  vector_base<T> const& v = gpu_begin.vec();
+  if (gpu_end - gpu_begin <= 0) return;
   const vcl_size_t count = vcl_size_t(gpu_end - gpu_begin);
-  if (count == 0) return;
-  if (v.stride() != 1) throw memory_exception("fast_copy needs a unit-stride vector");
-  backend::memory_read(v.handle(), sizeof(T) * (v.start() + gpu_begin.index()), sizeof(T) * count, cpu_begin);
+  NumericT* out = &(*cpu_begin);
+  if (gpu_begin.stride() == 1) {
+    backend::memory_read(gpu_begin.handle(), sizeof(NumericT) * gpu_begin.offset(), sizeof(NumericT) * count, out);
+    return;
+  }
+  std::vector<NumericT> span((count - 1) * gpu_begin.stride() + 1);
+  backend::memory_read(gpu_begin.handle(), sizeof(NumericT) * gpu_begin.offset(), sizeof(NumericT) * span.size(), span.data());
+  for (vcl_size_t i = 0; i < count; ++i) out[i] = span[i * gpu_begin.stride()];
 }
 
-template <typename T>
-void fast_copy(vector_base<T> const& gpu, std::vector<T>& cpu)
+/** fast_copy(cpu_begin, cpu_end, gpu_begin) (vector.hpp:1399-1426): linear host memory -> device range; a
+ *  strided range is read, patched and written back so that the gaps keep their values. */
+template <typename CpuIt, typename NumericT, unsigned int AlignmentV>
+void fast_copy(CpuIt const& cpu_begin, CpuIt const& cpu_end, vector_iterator<NumericT, AlignmentV> gpu_begin)
+{
+  if (cpu_end - cpu_begin <= 0) return;
+  const vcl_size_t count = vcl_size_t(cpu_end - cpu_begin);
+  const NumericT* in = &(*cpu_begin);
+  if (gpu_begin.stride() == 1) {
+    backend::memory_write(gpu_begin.handle(), sizeof(NumericT) * gpu_begin.offset(), sizeof(NumericT) * count, in);
+    return;
+  }
+  std::vector<NumericT> span((count - 1) * gpu_begin.stride() + 1);
+  backend::memory_read(gpu_begin.handle(), sizeof(NumericT) * gpu_begin.offset(), sizeof(NumericT) * span.size(), span.data());
+  for (vcl_size_t i = 0; i < count; ++i) span[i * gpu_begin.stride()] = in[i];
+  backend::memory_write(gpu_begin.handle(), sizeof(NumericT) * gpu_begin.offset(), sizeof(NumericT) * span.size(), span.data());
+}
+
+template <typename NumericT>
+void fast_copy(vector_base<NumericT> const& gpu, std::vector<NumericT>& cpu)
 {
   cpu.resize(gpu.size());
-  if (gpu.size()) fast_copy<T>(gpu.begin(), gpu.end(), cpu.data());
+  if (gpu.size()) fast_copy(gpu.begin(), gpu.end(), cpu.begin());
 }
 
-/** fast_copy(cpu_begin, cpu_end, gpu_begin): host range -> contiguous device range. */
-template <typename T>
-void fast_copy(const T* cpu_begin, const T* cpu_end, typename vector_base<T>::const_iterator gpu_begin)
-{
-  vector_base<T>& v = const_cast<vector_base<T>&>(gpu_begin.vec());
-  if (v.stride() != 1) throw memory_exception("fast_copy needs a unit-stride vector");
-  backend::memory_write(v.handle(), sizeof(T) * (v.start() + gpu_begin.index()), sizeof(T) * vcl_size_t(cpu_end - cpu_begin),
-                        cpu_begin);
-}
-
-template <typename T>
-void fast_copy(std::vector<T> const& cpu, vector_base<T>& gpu)
+template <typename NumericT>
+void fast_copy(std::vector<NumericT> const& cpu, vector_base<NumericT>& gpu)
 {
   if (gpu.size() == 0) gpu.init(cpu.size(), viennacl::context(HIP_MEMORY));
   gpu.check_size(cpu.size());
-  if (cpu.size()) fast_copy<T>(cpu.data(), cpu.data() + cpu.size(), gpu.begin());
+  if (cpu.size()) fast_copy(cpu.begin(), cpu.end(), gpu.begin());
+}
+
+/** copy(gpu_begin, gpu_end, cpu_begin) (vector.hpp:1321-1356): device range -> any host output iterator,
+ *  converting to its value type. */
+template <typename NumericT, unsigned int AlignmentV, typename CpuIt>
+void copy(const_vector_iterator<NumericT, AlignmentV> const& gpu_begin,
+          const_vector_iterator<NumericT, AlignmentV> const& gpu_end, CpuIt cpu_begin)
+{
+  if (gpu_end - gpu_begin <= 0) return;
+  std::vector<NumericT> tmp(static_cast<vcl_size_t>(gpu_end - gpu_begin));
+  fast_copy(gpu_begin, gpu_end, tmp.begin());
+  std::copy(tmp.begin(), tmp.end(), cpu_begin);
+}
+
+template <typename NumericT, unsigned int AlignmentV, typename CpuIt>
+void copy(vector_iterator<NumericT, AlignmentV> const& gpu_begin, vector_iterator<NumericT, AlignmentV> const& gpu_end,
+          CpuIt cpu_begin)
+{
+  copy(static_cast<const_vector_iterator<NumericT, AlignmentV> const&>(gpu_begin),
+       static_cast<const_vector_iterator<NumericT, AlignmentV> const&>(gpu_end), cpu_begin);
+}
+
+/** copy(cpu_begin, cpu_end, gpu_begin) (vector.hpp:1487-1500): any host input range -> device range. */
+template <typename CpuIt, typename NumericT, unsigned int AlignmentV>
+void copy(CpuIt const& cpu_begin, CpuIt const& cpu_end, vector_iterator<NumericT, AlignmentV> gpu_begin)
+{
+  const vcl_ptrdiff_t count = static_cast<vcl_ptrdiff_t>(std::distance(cpu_begin, cpu_end));
+  if (count <= 0) return;
+  std::vector<NumericT> tmp(static_cast<vcl_size_t>(count));
+  std::copy(cpu_begin, cpu_end, tmp.begin());
+  fast_copy(tmp.begin(), tmp.end(), gpu_begin);
+}
+
+/** Device range -> device range (vector.hpp:1565-1600): an entry-wise copy on the device. */
+template <typename NumericT, unsigned int AlignmentV>
+void copy(const_vector_iterator<NumericT, AlignmentV> const& src_begin,
+          const_vector_iterator<NumericT, AlignmentV> const& src_end, vector_iterator<NumericT, AlignmentV> dst_begin)
+{
+  if (src_end - src_begin <= 0) return;
+  const vcl_size_t count = vcl_size_t(src_end - src_begin);
+  vector_base<NumericT> src(const_cast<backend::mem_handle&>(src_begin.handle()), count, src_begin.offset(), src_begin.stride());
+  vector_base<NumericT> dst(dst_begin.handle(), count, dst_begin.offset(), dst_begin.stride());
+  dst.assign_linear(src, NumericT(1));
+}
+
+template <typename NumericT, unsigned int AlignmentV>
+void copy(vector_iterator<NumericT, AlignmentV> const& src_begin, vector_iterator<NumericT, AlignmentV> const& src_end,
+          vector_iterator<NumericT, AlignmentV> dst_begin)
+{
+  copy(static_cast<const_vector_iterator<NumericT, AlignmentV> const&>(src_begin),
+       static_cast<const_vector_iterator<NumericT, AlignmentV> const&>(src_end), dst_begin);
 }
 
 /** Precision conversion y = x between float and double vectors (vector.hpp:410-427 -> convert). */
@@ -746,6 +893,9 @@ namespace traits
 template <typename T> vcl_size_t size(vector_base<T> const& v) { return v.size(); }
 template <typename T> viennacl::context context(vector_base<T> const& v) { return context_of(v.handle()); }
 template <typename T> void clear(vector_base<T>& v) { v.clear(); }
+/** The memory domain of an object's buffer (traits/handle.hpp:155-165): HIP_MEMORY for everything
+ *  that computes in this build. */
+template <typename T> memory_types active_handle_id(T const& obj) { return obj.handle().get_active_handle_id(); }
 }  // namespace traits
 
 }  // namespace viennacl
