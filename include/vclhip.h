@@ -202,7 +202,7 @@ int vclhip_ell_from_csr_f64(uint32_t n_rows, uint32_t align, const uint32_t* row
 /* CSR -> HYB on the host as hyb_matrix's copy() splits it (hyb_matrix.hpp:136-240): the ELL width is
  * the smallest row length that at least csr_threshold * n_rows rows do not exceed, rounded up to
  * `align`; longer rows continue in the CSR part (one zero entry when it is empty).  First call with
- * NULL outputs for the sizes. */
+ * NULL outputs for the sizes; when *internal_ellnnz is 0 the ELL arrays may stay NULL. */
 int vclhip_hyb_from_csr_f64(uint32_t n_rows, double csr_threshold, uint32_t align, const uint32_t* row_ptr,
                             const uint32_t* col, const double* val, uint32_t* internal_ellnnz, uint32_t* csr_nnz,
                             uint32_t* ell_coords, double* ell_elements, uint32_t* csr_rows, uint32_t* csr_cols,

@@ -110,6 +110,8 @@ _REF_SIGS = {
     "ref_bench_csr_spmv": (f64, [u32, vp, vp, vp, vp, vp, u32]),
     "ref_bench_solve": (f64, [u32, vp, vp, vp, vp, vp, f64, u32, i32, i32, P(u32), P(f64)]),
     "ref_max_threads": (i32, []),
+    "ref_mm_read": (C.c_long, [C.c_char_p, C.c_long, P(u32), P(u32)]),
+    "ref_mm_csr": (None, [vp, vp, vp]),
 }
 
 _orc = None
@@ -522,3 +524,16 @@ def ref_mixed_cg(rp, col, val, b, tol, max_iters, inner_tol=1e-2):
     ref().ref_mixed_cg_csr(rp.size - 1, p(rp), p(col), p(val), p(b), p(x), tol, max_iters, inner_tol,
                            C.byref(it), C.byref(err))
     return x, it.value, err.value
+
+
+def ref_read_matrix_market(path, index_base=1):
+    """The reference reader (io/matrix_market.hpp:102-326, into std::vector<std::map>): returns
+    (result, row_ptr, col, val); result is the reader's own return value (0 on a parse error, 1 when
+    the file cannot be opened, else the number of lines read)."""
+    nr, nz = u32(), u32()
+    r = ref().ref_mm_read(str(path).encode(), index_base, C.byref(nr), C.byref(nz))
+    rp = np.zeros(nr.value + 1, np.uint32)
+    col = np.zeros(nz.value, np.uint32)
+    val = np.zeros(nz.value, np.float64)
+    ref().ref_mm_csr(p(rp), p(col), p(val))
+    return int(r), rp, col, val

@@ -28,6 +28,7 @@
 #include "viennacl/linalg/jacobi_precond.hpp"
 #include "viennacl/linalg/iterative_operations.hpp"
 #include "viennacl/tools/matrix_generation.hpp"
+#include "viennacl/io/matrix_market.hpp"
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -35,6 +36,8 @@
 namespace {
 
 typedef std::vector<std::map<unsigned int, double> > stl_matrix;
+
+stl_matrix mm_last;  // the last matrix read by ref_mm_read
 
 void fill_csr(viennacl::compressed_matrix<double>& A, uint32_t n, const uint32_t* rp, const uint32_t* col,
               const double* val)
@@ -585,6 +588,35 @@ void ref_mixed_cg_csr(uint32_t n, const uint32_t* rp, const uint32_t* col, const
   from_vcl(vx, x);
   *iters = tag.iters();
   *error = tag.error();
+}
+
+// io/matrix_market.hpp:102-326 through its std::vector<std::map> overload (:354-361), the type the
+// reference's own tests read files into.  Returns the reader's own result (0 on a parse error, 1 when
+// the file cannot be opened, else the number of lines read); the matrix is kept until the next call
+// and copied out by ref_mm_csr.
+long ref_mm_read(const char* path, long index_base, uint32_t* n_rows, uint32_t* nnz)
+{
+  mm_last.clear();
+  const long r = viennacl::io::read_matrix_market_file(mm_last, path, index_base);
+  std::size_t k = 0;
+  for (std::size_t i = 0; i < mm_last.size(); ++i) k += mm_last[i].size();
+  *n_rows = static_cast<uint32_t>(mm_last.size());
+  *nnz = static_cast<uint32_t>(k);
+  return r;
+}
+
+void ref_mm_csr(uint32_t* rp, uint32_t* col, double* val)
+{
+  uint32_t k = 0;
+  rp[0] = 0;
+  for (std::size_t i = 0; i < mm_last.size(); ++i) {
+    for (std::map<unsigned int, double>::const_iterator it = mm_last[i].begin(); it != mm_last[i].end(); ++it) {
+      col[k] = it->first;
+      val[k] = it->second;
+      ++k;
+    }
+    rp[i + 1] = k;
+  }
 }
 
 }  // extern "C"

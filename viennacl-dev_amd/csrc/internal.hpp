@@ -1,21 +1,16 @@
-// internal.hpp -- shared host-side helpers of libvclhip (error reporting, argument checks).
+// internal.hpp -- shared host-side helpers of libvclhip that touch the HIP runtime (status
+// conversion, launch checks, the measurement hook); error reporting is in errors.hpp.
 #pragma once
 
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
-#include <cstdarg>
 #include <cstdint>
-#include <cstdio>
 #include <string>
 
-#include "vclhip.h"
+#include "errors.hpp"
 
 namespace vclhip {
-
-// Thread-local text of the last failure; returned by vclhip_last_error().
-void set_error(const char* fmt, ...);
-const char* last_error();
 
 // Converts a HIP error into the ABI's status (positive hipError_t) and records its text.
 inline int hip_status(hipError_t e, const char* where)
@@ -25,24 +20,11 @@ inline int hip_status(hipError_t e, const char* where)
   return static_cast<int>(e);
 }
 
-inline int invalid(const char* what)
-{
-  set_error("invalid argument: %s", what);
-  return VCLHIP_ERROR_INVALID_VALUE;
-}
-
 // Checks the launch that was just enqueued (VIENNACL_CUDA_LAST_ERROR_CHECK, cuda/common.hpp:165-175).
 inline int check_launch(const char* kernel)
 {
   return hip_status(hipGetLastError(), kernel);
 }
-
-#define VCLHIP_TRY(expr)              \
-  do {                                \
-    int vclhip_status_ = (expr);      \
-    if (vclhip_status_ != 0)          \
-      return vclhip_status_;          \
-  } while (0)
 
 #define VCLHIP_HIP(expr) VCLHIP_TRY(::vclhip::hip_status((expr), #expr))
 

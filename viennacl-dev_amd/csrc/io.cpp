@@ -3,10 +3,14 @@
 // Reading follows the reference reader's rules: an optional "%%MatrixMarket matrix coordinate"
 // header with a real, pattern or complex field (pattern entries are 1; of a complex entry the real part is kept)
 // and a general or symmetric structure; '%' comment lines and blank lines are skipped, leading
-// blanks are ignored; the size line gives rows, columns and the entry count; reading stops after
-// that many entries; indices are shifted by index_base and checked against the size; a symmetric
-// file also sets (col, row); an entry given twice keeps the last value (the reference assigns
-// mat(row, col) = value).  Writing produces the reference writer's output: a general coordinate
+// blanks are ignored (blanks only: a line led by a tab is a data line, as there); the size line gives
+// rows, columns and the entry count; reading stops after that many entries, and a file that ends
+// before them is an error; indices are shifted by index_base and checked against the size; a
+// symmetric file also sets (col, row); an entry given twice keeps the last value (the reference
+// assigns mat(row, col) = value).  Two cases differ from the reference on purpose: a size line that
+// declares 0 entries gives an empty matrix (the reference reads on and reports a parse error at the
+// end of the file), and a file that ends early without a final newline is an error too (the
+// reference then returns what it read).  Writing produces the reference writer's output: a general coordinate
 // header, the size line, then one "row col value" line per entry in row order with the value in
 // the default iostream format (6 significant digits).
 #include <algorithm>
@@ -18,7 +22,7 @@
 #include <string>
 #include <vector>
 
-#include "internal.hpp"
+#include "errors.hpp"
 
 struct vclhip_mm_matrix {
   uint32_t n_rows = 0, n_cols = 0;
@@ -111,9 +115,11 @@ int vclhip_mm_read(const char* path, int64_t index_base, vclhip_mm_matrix** out,
     }
     if (!got) break;
     ++linenum;
+    // only blanks are trimmed (io/matrix_market.hpp:44-75): a line that starts with a tab or holds
+    // only a tab or a carriage return is a data line, and not one that parses
     const char* p = line.c_str();
-    while (*p == ' ' || *p == '\t') ++p;
-    if (*p == '\n' || *p == '\r' || *p == 0) continue;
+    while (*p == ' ') ++p;
+    if (*p == '\n' || *p == 0) continue;
     if (*p == '%') {
       if (p[1] != '%') continue;
       const char* q = p + 2;
@@ -160,6 +166,7 @@ int vclhip_mm_read(const char* path, int64_t index_base, vclhip_mm_matrix** out,
   std::fclose(f);
   if (status) return status;
   if (!size_seen) return fail(linenum, "missing size line");
+  if (valid < count) return fail(linenum, "the file ends before the entries its size line declares");
   // rows in order, columns ascending; of repeated (row, col) the last assignment stays
   std::sort(entries.begin(), entries.end(), [](const Entry& a, const Entry& b) {
     if (a.row != b.row) return a.row < b.row;

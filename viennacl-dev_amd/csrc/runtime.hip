@@ -1,5 +1,5 @@
 // runtime.hip -- the HIP memory domain of libvclhip: device selection, allocation, copies,
-// streams, events, and the error text of the last failure.
+// streams and events.
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -8,18 +8,6 @@
 #include "internal.hpp"
 
 namespace vclhip {
-
-static thread_local char g_last_error[512] = "";
-
-void set_error(const char* fmt, ...)
-{
-  va_list ap;
-  va_start(ap, fmt);
-  std::vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
-  va_end(ap);
-}
-
-const char* last_error() { return g_last_error; }
 
 static thread_local LaunchTiming g_timing;
 
@@ -65,8 +53,6 @@ __global__ __launch_bounds__(kBlock) void fill_bytes_kernel(uint8_t* __restrict_
 extern "C" {
 
 const char* vclhip_version(void) { return "vclhip 0.1 (gfx950)"; }
-
-const char* vclhip_last_error(void) { return vclhip::last_error(); }
 
 const char* vclhip_status_string(int status)
 {

@@ -10,7 +10,7 @@
 #include <unordered_set>
 #include <vector>
 
-#include "internal.hpp"
+#include "errors.hpp"
 #include "plan.hpp"
 
 namespace {
@@ -422,10 +422,13 @@ int vclhip_hyb_from_csr_f64(uint32_t n_rows, double csr_threshold, uint32_t alig
   }
   *internal_ellnnz = static_cast<uint32_t>(iw);
   *csr_nnz = static_cast<uint32_t>(rest ? rest : 1);  // an empty CSR part keeps one zero entry
-  if (!ell_coords || !ell_elements || !csr_rows || !csr_cols || !csr_elements) return VCLHIP_SUCCESS;
+  // sizes only when the outputs are not given (the ELL arrays may be null when the ELL part is empty)
+  if (!csr_rows || !csr_cols || !csr_elements || (iw && (!ell_coords || !ell_elements))) return VCLHIP_SUCCESS;
   if (row_ptr[n_rows] && (!col || !val)) return vclhip::invalid("hyb arguments");
-  std::fill(ell_coords, ell_coords + ld * iw, 0u);
-  std::fill(ell_elements, ell_elements + ld * iw, 0.0);
+  if (iw) {
+    std::fill(ell_coords, ell_coords + ld * iw, 0u);
+    std::fill(ell_elements, ell_elements + ld * iw, 0.0);
+  }
   uint32_t c = 0;
   for (uint32_t r = 0; r < n_rows; ++r) {
     csr_rows[r] = c;
