@@ -393,7 +393,11 @@ def batched_cg_bench(vh, num_systems, reps=3):
                        f"CSR, per-system tol 1e-8, b ~ U[-1,1]", "rows": n, "nnz": nnz}
     plan = vh.BatchPlan(sys_ptr, rp)
     stream = vh.Stream()
-    for name, dtype in (("fp64", np.float64), ("fp32", np.float32)):
+    # FP32 at the configuration's 1e-8 stops on its recursive residual, which drifts from the true
+    # one by about the FP32 resolution times the condition number; the FP32 run at 1e-5 is one whose
+    # tolerance the true residual can meet.  "converged" counts systems whose FP64 true residual is
+    # at most the tolerance; "converged_recursive" the device's own flags.
+    for name, dtype, tol in (("fp64", np.float64, 1e-8), ("fp32", np.float32, 1e-8), ("fp32_tol1e-5", np.float32, 1e-5)):
         A = vh.CsrMatrix(rp, col, val, dtype=dtype)
         t0 = time.perf_counter()
         plan.autotune(A, single=(dtype == np.float32), stream=stream.handle)
@@ -407,7 +411,7 @@ def batched_cg_bench(vh, num_systems, reps=3):
         best = None
         for r in range(reps + 1):
             ev.record(0, stream.handle)
-            plan.solve_into(A, db, dx, it, er, cv, stream.handle, tol=1e-8, max_iters=20000, use_graph=True)
+            plan.solve_into(A, db, dx, it, er, cv, stream.handle, tol=tol, max_iters=20000, use_graph=True)
             ev.record(1, stream.handle)
             stream.synchronize()
             ms = ev.ms(0, 1)
@@ -430,9 +434,10 @@ def batched_cg_bench(vh, num_systems, reps=3):
         rows_i = np.diff(sys_ptr).astype(np.float64)
         nnz_i = np.diff(rp[sys_ptr].astype(np.int64)).astype(np.float64)
         moved = float(np.sum(iters * ((vb + 4) * nnz_i + 4 * (rows_i + 1))))
-        out[name] = {"ms": round(best, 4), "systems_per_s": round(num_systems / (best * 1e-3), 1),
+        out[name] = {"tol": tol, "ms": round(best, 4), "systems_per_s": round(num_systems / (best * 1e-3), 1),
                      "iters_total": int(iters.sum()), "iters_max": int(iters.max()), "iters_mean": float(iters.mean()),
-                     "converged": int((conv == 1).sum()), "max_true_relres": float(rel.max()),
+                     "converged": int((rel <= tol).sum()), "converged_recursive": int((conv == 1).sum()),
+                     "max_true_relres": float(rel.max()),
                      "median_true_relres": float(np.median(rel)), "matrix_gbs": round(moved / (best * 1e-3) / 1e9, 1),
                      "autotune_s": round(tune_s, 3), "lanes_log2_by_size_class": plan.lanes(dtype == np.float32).tolist()}
         log("batched", name, out[name])
@@ -494,12 +499,17 @@ def adjoint_bench(vh, grid, n_fd=64):
             tm[i] -= h
             fd = (loss(torch.tensor(tp.reshape(ny, nx), device=dev), 1e-13).item()
                   - loss(torch.tensor(tm.reshape(ny, nx), device=dev), 1e-13).item()) / (2 * h)
-            errs.append(abs(fd - gref[i]) / max(abs(gref[i]), 1e-12))
+            errs.append(abs(fd - gref[i]))
+    # errors on a floor of 1e-2 |g|_inf: a finite difference of L (about 2.5e7 here) at h = 1e-6 carries
+    # an absolute error of about 1e-2 from the rounding of L and of the perturbed matrix, whatever g_i is
+    g_inf = float(np.abs(gref).max())
+    scaled = [e / max(abs(gref[i]), 1e-2 * g_inf) for e, i in zip(errs, idx)]
     out = {"workload": f"2-D variable-coefficient diffusion {grid}^2 (n={n}, nnz={int(rp[-1])}), FP64 CG forward "
                        f"and adjoint to 1e-10, dL/dA on the CSR pattern chained to dL/dtheta",
            "forward_backward_s": round(best, 6), "fd_entries": n_fd, "fd_h": h,
-           "fd_max_rel_err": float(max(errs)), "fd_median_rel_err": float(np.median(errs)),
-           "grad_norm": float(np.linalg.norm(g))}
+           "fd_max_abs_err": float(max(errs)), "fd_max_err_on_floor": float(max(scaled)),
+           "fd_median_err_on_floor": float(np.median(scaled)), "fd_floor": "max(|g_i|, 1e-2 |g|_inf)",
+           "grad_inf": g_inf, "grad_norm": float(np.linalg.norm(g))}
     log("adjoint", out)
     return out
 
@@ -596,6 +606,62 @@ def c1_solve_bench(vh):
     return rec, (rp, col, val, bh)
 
 
+def profile_workloads(vh, names, steps, warmup):
+    """Short runs of single kernels for rocprofv3 counter passes (one workload per dispatch group,
+    each printed as it starts)."""
+    out = {}
+    stream = vh.Stream()
+    for name in names:
+        log("profile workload", name)
+        if name == "spmv":
+            rp, col, val = vh.laplace_3d(128)
+            n = rp.size - 1
+            A, S = vh.CsrMatrix(rp, col, val), vh.SellMatrix(rp, col, val, 64)
+            x, y = vh.DeviceArray.from_numpy(vh.random_uniform(1, n)), vh.DeviceArray.zeros(n, np.float64)
+            _, warm, _ = time_launches(vh, lambda: A.spmv(x, y, stream=stream.handle), stream, steps, warmup)
+            _, swarm, _ = time_launches(vh, lambda: S.spmv(x, y, stream=stream.handle), stream, steps, warmup)
+            out[name] = {"csr_ms": warm, "sell_ms": swarm, "stream": stream_reference(vh, stream, steps)}
+            for a in (x, y, A, S):
+                a.free()
+        elif name == "powerlaw":
+            n = 2097152
+            rp, col, val = vh.powerlaw(n, 512, 2.0, 1.1, 1)
+            A = vh.CsrMatrix(rp, col, val)
+            x, y = vh.DeviceArray.from_numpy(vh.random_uniform(1, n)), vh.DeviceArray.zeros(n, np.float64)
+            _, warm, _ = time_launches(vh, lambda: A.spmv(x, y, stream=stream.handle), stream, steps, warmup)
+            out[name] = {"csr_ms": warm, "nnz": int(rp[-1]), "algorithmic_bytes": csr_bytes(n, int(rp[-1]))}
+            for a in (x, y, A):
+                a.free()
+        elif name == "dia_cg":
+            g = 160
+            rp, col, val = vh.diffusion(g, g, g, np.exp(vh.random_normal(1, g ** 3)))
+            A = vh.CsrMatrix(rp, col, val)
+            b, x = vh.DeviceArray.from_numpy(vh.random_uniform(0, g ** 3)), vh.DeviceArray.zeros(g ** 3, np.float64)
+            res = vh.cg(A, b, x, tol=1e-30, max_iters=50, stream=stream.handle)
+            stream.synchronize()
+            out[name] = {"iters": int(res.iters), "operator_used": int(res.operator_used)}
+            for a in (b, x, A):
+                a.free()
+        elif name == "batched":
+            sys_ptr, rp, col, val = vh.graph_laplacian_batch(4096, 64, 4096, 8.0, 0.1, 1.0, 0.01, 1)
+            n = rp.size - 1
+            plan = vh.BatchPlan(sys_ptr, rp)
+            A = vh.CsrMatrix(rp, col, val)
+            plan.autotune(A, single=False, stream=stream.handle)
+            db, dx = vh.DeviceArray.from_numpy(vh.random_uniform(0, n)), vh.DeviceArray.zeros(n, np.float64)
+            it, cv = vh.DeviceArray.zeros(4096, np.uint32), vh.DeviceArray.zeros(4096, np.uint32)
+            er = vh.DeviceArray.zeros(4096, np.float64)
+            plan.solve_into(A, db, dx, it, er, cv, stream.handle, tol=1e-8, max_iters=20000, use_graph=False)
+            stream.synchronize()
+            out[name] = {"iters_total": int(it.to_numpy().sum())}
+            for a in (db, dx, it, cv, er, A):
+                a.free()
+            plan.close()
+        else:
+            raise SystemExit(f"unknown profile workload {name}")
+    return out
+
+
 def device_banner(vh):
     name = C.create_string_buffer(64)
     mem, cu, l2, llc = C.c_size_t(), C.c_int(), C.c_int(), C.c_int()
@@ -623,6 +689,11 @@ def main():
                          "before libvclhip so that the process holds one HIP runtime)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only warm CSR/SELL SpMV and the streaming kernel (for counter collection)")
+    ap.add_argument("--profile", default=None,
+                    help="comma-separated short workloads for counter collection, nothing else: spmv (CSR and "
+                         "SELL-64 SpMV of the 128^3 Laplacian and the streaming kernels), powerlaw (CSR SpMV of the "
+                         "power-law matrix), dia_cg (50 FP64 CG iterations on the 160^3 diffusion operator's DIA "
+                         "copy), batched (one FP64 batched CG solve of the 4096 systems)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the sharded (row-partitioned) path is not built; run with --gpus 1.",
@@ -640,6 +711,9 @@ def main():
     assert vh.device_count() > 0, "no HIP device"
     vh.call("vclhip_set_device", 0)
     device_banner(vh)
+    if args.profile:
+        print(json.dumps(profile_workloads(vh, args.profile.split(","), args.steps, args.warmup)), flush=True)
+        return
 
     N = args.grid
     t = time.perf_counter()

@@ -441,8 +441,12 @@ typedef struct vclhip_solver_params {
                                         matrix has at most 32 diagonals storing at most 25% zeros,
                                         every stored value is nonzero and not NaN, every row's columns
                                         strictly increase, and the copy takes at most a quarter of the
-                                        free device memory -- the same results bit for bit (also for
-                                        non-finite x), fewer bytes per product;
+                                        free device memory -- the same products bit for bit (also
+                                        for non-finite x) with fewer bytes each; the inner products
+                                        fused into them are summed per 256-row tile, which is the CSR
+                                        plan's grouping when no row has more than 8 entries (then the
+                                        whole solve is the same bit for bit; with longer rows it
+                                        agrees to rounding);
                                         VCLHIP_OPERATOR_AS_GIVEN always uses the CSR matrix; other
                                         values are rejected */
   const struct vclhip_dia* dia;      /* optional, with VCLHIP_OPERATOR_AUTO: the detection and DIA

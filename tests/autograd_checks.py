@@ -85,15 +85,20 @@ def gradients_full(nx=512, n_fd=64):
         they agree to about the condition number times 1e-13: relative 2-norm 1e-7 asserted;
     (2) central differences with h = 1e-6 on 64 random entries, (L(theta + h e_i) - L(theta - h e_i))
         / 2h, formed two ways from the device solves:
-        * plainly, from the two losses.  L is about 2.5e7 here, so the FP64 rounding of each loss
-          alone moves the quotient by about 2.5e7 * 1.1e-16 / 2e-6 = 1.4e-3, and the two solves'
-          errors (relative residual 1e-13, condition number about 1e5) add (x - x_t) . dx / h, a few
-          1e-2: |fd_i - g_i| <= 1e-3 max(|g_i|, 1e-2 |g|_inf) asserted;
         * in solution space: with x_- the solve at theta - h e_i, d = x_+ - x_- solves
           A(theta + h e_i) d = -(A(theta + h e_i) - A(theta - h e_i)) x_-, so the difference of the
-          losses is 0.5 d . (2 x_- + d - 2 x_t) exactly; d is solved on the device with its own
-          right-hand side, so its error is relative to d, not to x, and no large number is
-          subtracted: |fd_i - g_i| <= 1e-6 max(|g_i|, 1e-2 |g|_inf) asserted."""
+          losses is 0.5 d . (2 x_- + d - 2 x_t) exactly, with no large number subtracted.  What is
+          left is the rounding of the perturbed matrix itself: each value that depends on theta_i
+          (the stencil of cell i and its neighbours' diagonals) carries a relative error of at most
+          8 eps (the harmonic mean and the four-term diagonal), and dL/da_k = dA_k, so the quotient
+          moves by at most noise_i = 8 eps sum_k |dA_k| |a_k| / h over those values -- about 1e-1
+          where |dA| reaches 7e6 (dA_k = -lambda_r x_c is large while dL/dtheta_i is a difference of
+          such terms).  |fd_i - g_i| <= noise_i asserted (first order; the errors seen are 1/30 to
+          1/100 of it);
+        * plainly, from the two losses.  L is about 2.5e7 here, so its rounding adds about
+          2.5e7 * 1.1e-16 / 2e-6 = 1.4e-3, and the two solves' errors (relative residual 1e-13,
+          condition number about 1e5) add (x - x_t) . dx / h, a few 1e-2:
+          |fd_i - g_i| <= 1e-3 max(|g_i|, 1e-2 |g|_inf) asserted."""
     ny = nx
     n = nx * ny
     theta = np.exp(0.5 * vh.random_normal(1, n))
@@ -136,6 +141,18 @@ def gradients_full(nx=512, n_fd=64):
     idx = rng.choice(n, n_fd, replace=False)
     h = 1e-6
     g_inf = float(np.abs(g).max())
+    eps = np.finfo(np.float64).eps
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp.astype(np.int64)))
+    cols = col.astype(np.int64)
+
+    def rounding_noise(i):
+        # the values that depend on theta_i: entries (r, c) with r and c in cell i's stencil
+        y0, x0 = divmod(int(i), nx)
+        cells = [y0 * nx + x0] + [(y0 + dy) * nx + x0 + dx for dy, dx in ((-1, 0), (1, 0), (0, -1), (0, 1))
+                                  if 0 <= y0 + dy < ny and 0 <= x0 + dx < nx]
+        m = np.isin(rows, cells) & np.isin(cols, cells)
+        return 8 * eps * float(np.sum(np.abs(dAh[m]) * np.abs(vals_h[m]))) / h
+
     fd_bad, worst, worst_sol = [], 0.0, 0.0
     with torch.no_grad():
         for i in idx:
@@ -156,13 +173,13 @@ def gradients_full(nx=512, n_fd=64):
             d = ts.sparse_solve(vp, torch.tensor(rhs, device=dev), pat, tol=tol, max_iters=50000).cpu().numpy()
             fd_sol = float(0.5 * np.dot(d, 2 * xm_h + d - 2 * xt)) / (2 * h)
             scale = max(abs(g[i]), 1e-2 * g_inf)
-            err, err_sol = abs(fd - g[i]) / scale, abs(fd_sol - g[i]) / scale
+            err, err_sol = abs(fd - g[i]) / scale, abs(fd_sol - g[i]) / rounding_noise(i)
             worst, worst_sol = max(worst, err), max(worst_sol, err_sol)
-            if err > 1e-3 or err_sol > 1e-6:
-                fd_bad.append([int(i), fd, fd_sol, float(g[i])])
+            if err > 1e-3 or err_sol > 1.0:
+                fd_bad.append([int(i), fd, fd_sol, float(g[i]), rounding_noise(i)])
     ok = host_rel < 1e-7 and db_rel < 1e-7 and not fd_bad
     return ok, {"n": n, "host_rel": host_rel, "db_rel": db_rel, "fd_entries": int(n_fd), "fd_worst_scaled": worst,
-                "fd_solution_space_worst_scaled": worst_sol, "g_inf": g_inf, "fd_bad": fd_bad}
+                "fd_solution_space_worst_over_noise": worst_sol, "g_inf": g_inf, "fd_bad": fd_bad}
 
 
 def input_checks():

@@ -6,8 +6,8 @@ inputs and the reference's outputs, so the tests pin the oracle and the HIP path
 reference being present.  The reference's own SpMV test matrix (examples/testdata/mat65k.mtx) is
 not available, so these seeded problems stand in for it.
 
-Run from the repository root:  python tests/golden/make_golden.py  (add `gmres` or `formats` to write
-only those fixtures)
+Run from the repository root:  python tests/golden/make_golden.py  (add `gmres`, `formats` or
+`c2_envelope` to write only those fixtures)
 """
 import pathlib
 import sys
@@ -283,12 +283,45 @@ def formats_main():
         save(f"solve_formats_{name}", **out)
 
 
+def c2_envelope_main():
+    """BiCGStab iteration counts on the 128^3 Laplacian (b = U[-1,1] seed 0, tol 1e-10), plain and
+    Jacobi: the reference's own solve() at 1, 2, 4, 8 and 16 OpenMP threads, and the oracle's
+    restatement over its k-block reduction orders.  Only the summation order of the inner products
+    differs between these runs; the spread of the counts is the reference's own."""
+    rp, col, val = vh.laplace_3d(128)
+    n = rp.size - 1
+    b = vh.random_uniform(0, n)
+    d = np.full(n, 6.0)
+    out = {"threads": np.array([1, 2, 4, 8, 16], np.int32), "oracle_blocks": np.array(po.ENVELOPE_BLOCKS, np.int32)}
+    ref_p, ref_j = [], []
+    for t in out["threads"]:
+        po.set_threads(int(t))
+        ref_p.append(po.ref_bicgstab(rp, col, val, b, 1e-10, 5000)[1])
+        ref_j.append(po.ref_bicgstab(rp, col, val, b, 1e-10, 5000, jacobi=True)[1])
+        print("threads", t, ref_p[-1], ref_j[-1], flush=True)
+    po.set_threads(8)
+    A = po.CsrHost(rp, col, val)
+    orc_p, orc_j = [], []
+    for k in po.ENVELOPE_BLOCKS:
+        with po.reduction_blocks(k):
+            orc_p.append(po.bicgstab_pipelined(A, b, 1e-10, 5000)[1])
+            orc_j.append(po.bicgstab_classic(A, b, 1e-10, 5000, diag=d)[1])
+        print("blocks", k, orc_p[-1], orc_j[-1], flush=True)
+    po.set_threads(1)
+    save("c2_bicgstab_lap3d_128_iters", ref_bicgstab=np.array(ref_p, np.int32),
+         ref_jacobi_bicgstab=np.array(ref_j, np.int32), oracle_bicgstab=np.array(orc_p, np.int32),
+         oracle_jacobi_bicgstab=np.array(orc_j, np.int32), **out)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["gmres"]:
         gmres_main()
     elif sys.argv[1:] == ["formats"]:
         formats_main()
+    elif sys.argv[1:] == ["c2_envelope"]:
+        c2_envelope_main()
     else:
         main()
         gmres_main()
         formats_main()
+        c2_envelope_main()

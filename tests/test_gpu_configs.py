@@ -6,10 +6,14 @@ the ones that do not depend on size: the FP64 true residual ||b - A x|| / ||b||,
 against the oracle's (same recurrences), the mixed solve's distance to the FP64 solution, and
 every system of the batch.
 """
+import pathlib
+
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
 
 
 def iters_close(got, want):
@@ -93,13 +97,27 @@ def c2_oracle(vh, oracle, solver):
     return _C2_ORACLE[solver]
 
 
+def c2_bicgstab_envelope(solver):
+    """The range of BiCGStab iteration counts on the 128^3 Laplacian over the reference's thread
+    counts (1-16) and the oracle's reduction orders."""
+    g = np.load(GOLDEN / "c2_bicgstab_lap3d_128_iters.npz")
+    counts = np.concatenate([g["ref_" + solver], g["oracle_" + solver]])
+    return int(counts.min()), int(counts.max())
+
+
 @pytest.mark.parametrize("fmt", ["csr", "sell"])
 @pytest.mark.parametrize("solver", ["jacobi_cg", "bicgstab", "jacobi_bicgstab"])
 def test_c2_solvers_lap3d_128(dev, vh, oracle, solver, fmt):
     """The other solvers of config C2 on the 128^3 Laplacian (n = 2,097,152), tol 1e-10, in CSR
     (solved on its DIA copy, the default) and SELL-64: the iteration count within 2% of the
-    oracle's (8 threads), the reported error below the tolerance, x close to the oracle's, and the
-    FP64 true residual.  The criteria differ by solver: Jacobi-PCG stops on the M^-1-norm of the
+    oracle's (8 threads) for Jacobi-PCG, the reported error below the tolerance, x close to the
+    oracle's, and the FP64 true residual.  BiCGStab's count at this size is set by the rounding of
+    its inner products: the reference's own solve() took 365-394 iterations (306-345 with Jacobi)
+    over 1-16 OpenMP threads, and not the same count twice at 4 threads (its reductions combine
+    the threads' sums in arrival order); the oracle takes 348-398 (313-348) over its reduction
+    orders.  These counts are the fixture c2_bicgstab_lap3d_128_iters (tests/golden/make_golden.py,
+    from the reference build), and the BiCGStab counts are held to within 2% of their range.
+    The criteria differ by solver: Jacobi-PCG stops on the M^-1-norm of the
     residual (cg.hpp:306-309), which with the constant diagonal 6 of this operator equals the
     2-norm ratio, so ||b - A x|| / ||b|| < tol; Jacobi-BiCGStab stops on ||M^-1 r|| / ||b|| =
     ||r|| / (6 ||b||) (bicgstab.hpp:460-461), so its 2-norm ratio is below 6 tol; plain BiCGStab
@@ -118,7 +136,11 @@ def test_c2_solvers_lap3d_128(dev, vh, oracle, solver, fmt):
     it_o, err_o, x_o = c2_oracle(vh, oracle, solver)
     print(f"{solver} {fmt}: {res.iters} iterations (oracle {it_o}), error {res.error:.3e} (oracle {err_o:.3e})")
     assert res.converged and res.error < 1e-10
-    assert iters_close(res.iters, it_o), (res.iters, it_o)
+    if solver == "jacobi_cg":
+        assert iters_close(res.iters, it_o), (res.iters, it_o)
+    else:
+        lo, hi = c2_bicgstab_envelope(solver)
+        assert int(np.floor(0.98 * lo)) <= res.iters <= int(np.ceil(1.02 * hi)), (res.iters, lo, hi)
     if fmt == "csr":
         assert res.operator_used == vh.USED_DIA
     x = dx.to_numpy()

@@ -267,6 +267,7 @@ def _csr_from_entries(n, rows, cols, diag, vals=None):
 
 
 DIA_CASES = {
+    "band5": (lambda: band(3000, range(-2, 3)), 5, True),
     "lap27": (lambda: stencil_27(14, 13, 12), 27, True),
     "band9": (lambda: band(3000, range(-4, 5)), 9, True),
     "band17": (lambda: band(3000, range(-8, 9)), 17, True),
@@ -279,10 +280,14 @@ DIA_CASES = {
 @pytest.mark.parametrize("case", list(DIA_CASES))
 @pytest.mark.parametrize("kind", ["cg", "jacobi_cg", "bicgstab", "gmres"])
 def test_dia_operator_diagonal_counts(dev, vh, case, kind):
-    """Operators with 9 to 33 diagonals: up to 32 the default solve runs on the DIA copy (reported
-    in result.operator_used), bit for bit equal to the solve on the CSR matrix as given, across the
-    DIA kernel's full and partial groups of eight diagonals and at the 32-diagonal limit of the
-    offset detection; 33 diagonals keep the CSR operator."""
+    """Operators with 5 to 33 diagonals: up to 32 the default solve runs on the DIA copy (reported
+    in result.operator_used), across the DIA kernel's full and partial groups of eight diagonals and
+    at the 32-diagonal limit of the offset detection; 33 diagonals keep the CSR operator.  The DIA
+    products equal the CSR kernel's bit for bit; the inner products fused into them are summed per
+    256-row tile on the DIA copy and per row block of the CSR plan on the matrix as given.  With at
+    most 8 entries per row every block is a 256-row tile, the partitions coincide and the two solves
+    are equal bit for bit; with more, the blocks are shorter, the sums are grouped differently and
+    the solves agree to rounding: iterations within 2% and x within 1e-9."""
     make, nd, symmetric = DIA_CASES[case]
     if kind.endswith("cg") and not symmetric:
         pytest.skip("CG needs a symmetric matrix")
@@ -305,7 +310,13 @@ def test_dia_operator_diagonal_counts(dev, vh, case, kind):
     (xa, ra), (xg, rg) = out
     assert ra.operator_used == (vh.USED_DIA if nd <= 32 else vh.USED_AS_GIVEN)
     assert rg.operator_used == vh.USED_AS_GIVEN
-    assert ra.converged and ra.iters == rg.iters and ra.error == rg.error and np.array_equal(xa, xg)
+    assert ra.converged and rg.converged
+    if int(np.diff(rp.astype(np.int64)).max()) <= 8:
+        assert A.num_blocks == (n + 255) // 256  # the CSR plan's blocks are the 256-row tiles
+        assert ra.iters == rg.iters and ra.error == rg.error and np.array_equal(xa, xg)
+    else:
+        assert iters_close(ra.iters, rg.iters), (ra.iters, rg.iters)
+        assert np.linalg.norm(xa - xg) / np.linalg.norm(xg) < 1e-9
 
 
 @pytest.mark.parametrize("bad", ["zero", "negative_zero", "nan"])

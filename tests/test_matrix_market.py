@@ -218,7 +218,9 @@ def test_zero_entry_file_is_an_empty_matrix(vh, oracle, tmp_path):
 @pytest.mark.gpu
 def test_file_read_into_device_and_host_gives_the_same_product(vh, dev, oracle, tmp_path):
     """One file, two domains: vclhip_mm_read -> compressed_matrix on the device -> SpMV, and the
-    reference reader -> the oracle's host CSR product; bit for bit (same arrays, same per-row order)."""
+    reference reader -> the oracle's host CSR product: the same arrays, and the same products bit for
+    bit on rows of at most 32 entries (one lane sums them in the host's order); longer rows are
+    summed by a wavefront and agree to 1e-13 relative (inf-norm), the SpMV parity bar."""
     rng = np.random.default_rng(5)
     n = 20000
     lens = np.minimum((rng.pareto(1.0, n) + 1).astype(np.int64), 512)
@@ -242,4 +244,7 @@ def test_file_read_into_device_and_host_gives_the_same_product(vh, dev, oracle, 
     dA.spmv(dx, dy)
     y = dy.to_numpy()
     y_host = oracle.csr_spmv(hrp, hcol, hval, x)
-    assert np.array_equal(y, y_host)
+    short = np.diff(rp.astype(np.int64)) <= 32
+    assert short.sum() > n // 2 and (~short).any()
+    assert np.array_equal(y[short], y_host[short])
+    assert np.max(np.abs(y - y_host)) <= 1e-13 * np.max(np.abs(y_host))

@@ -4,7 +4,11 @@
 // value per row and diagonal, offsets ascending; dia_kernel (spmv_kernels.hpp) then needs no column
 // indices and issues every load of a row at once.  With at most kDiaMax diagonals no row has more
 // than kDiaMax <= kShortRow entries, so the CSR kernel sums every row in one lane in column order,
-// and so does dia_kernel: a solve on the DIA copy gives the same results as on the CSR matrix.
+// and so does dia_kernel: the products are the same bit for bit.  The inner products fused into
+// the product are summed per 256-row tile here and per row block of the CSR plan there; the two
+// groupings coincide when no row has more than 8 entries (every block is then a 256-row tile, as
+// for the 7-point stencil) and a solve is the same bit for bit; with longer rows it agrees to
+// rounding.
 // Used for the mixed-precision solver's FP32 inner copy and, unless the caller asks for the
 // matrix as given, for the operator of the CG, BiCGStab and GMRES solvers.
 #pragma once
@@ -208,8 +212,8 @@ struct vclhip_dia {
 
 namespace vclhip {
 
-// The solvers' operator: a CSR matrix with few diagonals is replaced by a DIA copy (same results,
-// fewer bytes per product) unless params->operator_format asks for the matrix as given, or the copy
+// The solvers' operator: a CSR matrix with few diagonals is replaced by a DIA copy (same products,
+// fewer bytes each) unless params->operator_format asks for the matrix as given, or the copy
 // would take more than a quarter of the free device memory.  params->dia, when given, is the
 // detection and copy made once by vclhip_dia_create_* for this matrix; otherwise the copy is built
 // in the solve's workspace.  *used_dia tells whether the solve runs on a DIA copy.

@@ -306,8 +306,9 @@ void copy(std::vector<std::map<IndexT, CpuT> > const& cpu, compressed_matrix<Num
     }
     rp[i + 1] = uint32_t(col.size());
   }
-  vcl_size_t cols = gpu.size2() > max_col ? gpu.size2() : max_col;
-  if (cols < cpu.size()) cols = cpu.size();
+  // columns: one past the largest column index (the reference's max_col + 1), or the matrix's own
+  // column count when that is larger
+  const vcl_size_t cols = gpu.size2() > max_col ? gpu.size2() : max_col;
   gpu.set(rp.data(), col.data(), val.data(), cpu.size(), cols, val.size());
 }
 
