@@ -642,6 +642,31 @@ def profile_workloads(vh, names, steps, warmup):
             out[name] = {"iters": int(res.iters), "operator_used": int(res.operator_used)}
             for a in (b, x, A):
                 a.free()
+        elif name in ("c1_cg", "c2_cg", "c2_bicgstab", "c3_cg_fp64", "c3_mixed"):
+            # one solve of the configuration as the bench line times it
+            if name == "c1_cg":
+                rp, col, val = vh.fdm_laplace_2d(256, 256)
+                kw = dict(tol=1e-8, max_iters=2000)
+            elif name.startswith("c2"):
+                rp, col, val = vh.laplace_3d(128)
+                kw = dict(tol=1e-10, max_iters=5000)
+            else:
+                g = 160
+                rp, col, val = vh.diffusion(g, g, g, np.exp(vh.random_normal(1, g ** 3)))
+                kw = dict(tol=1e-10, max_iters=20000)
+            n = rp.size - 1
+            A = vh.CsrMatrix(rp, col, val)
+            b, x = vh.DeviceArray.from_numpy(vh.random_uniform(0, n)), vh.DeviceArray.zeros(n, np.float64)
+            if name == "c2_bicgstab":
+                res = vh.bicgstab(A, b, x, stream=stream.handle, **kw)
+            elif name == "c3_mixed":
+                res = vh.mixed_cg(A, b, x, stream=stream.handle, mixed_mode=1, **kw)
+            else:
+                res = vh.cg(A, b, x, stream=stream.handle, **kw)
+            stream.synchronize()
+            out[name] = {"n": n, "iters": int(res.iters), "error": float(res.error), "converged": bool(res.converged)}
+            for a in (b, x, A):
+                a.free()
         elif name == "batched":
             sys_ptr, rp, col, val = vh.graph_laplacian_batch(4096, 64, 4096, 8.0, 0.1, 1.0, 0.01, 1)
             n = rp.size - 1
@@ -690,10 +715,11 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="only warm CSR/SELL SpMV and the streaming kernel (for counter collection)")
     ap.add_argument("--profile", default=None,
-                    help="comma-separated short workloads for counter collection, nothing else: spmv (CSR and "
+                    help="comma-separated short workloads for rocprofv3 runs, nothing else: spmv (CSR and "
                          "SELL-64 SpMV of the 128^3 Laplacian and the streaming kernels), powerlaw (CSR SpMV of the "
                          "power-law matrix), dia_cg (50 FP64 CG iterations on the 160^3 diffusion operator's DIA "
-                         "copy), batched (one FP64 batched CG solve of the 4096 systems)")
+                         "copy), batched (one FP64 batched CG solve of the 4096 systems), and one solve of a "
+                         "configuration: c1_cg, c2_cg, c2_bicgstab, c3_cg_fp64, c3_mixed")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the sharded (row-partitioned) path is not built; run with --gpus 1.",
