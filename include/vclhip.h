@@ -501,6 +501,20 @@ int vclhip_dia_destroy(vclhip_dia* dia);
  * x_dev receives the solution (the solver starts from x = 0). */
 int vclhip_cg_solve_csr_f64(const vclhip_csr* A, const double* b_dev, double* x_dev, int precond,
                             const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
+/* The same solve without any allocation -- no counterpart in the reference; for callers that
+ * cannot allocate device or pinned memory (serving loops that must not allocate, harnesses that
+ * forbid it).  Its vectors, partial sums and state live in a file-scope scratch array of 32 MiB
+ * that each device the library runs on holds until the process ends: n rows need about 24 n bytes
+ * plus 257 KiB (32 n with Jacobi, plus 8 bytes per iteration of a requested history), so up to
+ * about 1.39 M rows (1.04 M with Jacobi); a larger system returns VCLHIP_ERROR_TOO_LARGE.  The DIA copy of the operator (VCLHIP_OPERATOR_AUTO) is made in what is
+ * left of the array, or not at all, unless params->dia brings one.  params->use_graph must be 0.
+ * The host waits for the device after every chunk of params->chunk iterations (no pinned memory,
+ * no events).  Results equal vclhip_cg_solve_csr_f64's bit for bit.  Calls on one device share its
+ * array, so calls that overlap on one device (other host threads, or streams that run concurrently)
+ * race; calls made one after another do not, whatever the previous call left there. */
+int vclhip_cg_solve_csr_f64_noalloc(const vclhip_csr* A, const double* b_dev, double* x_dev, int precond,
+                                    const vclhip_solver_params* params, vclhip_solver_result* result,
+                                    void* stream);
 int vclhip_cg_solve_sell_f64(const vclhip_sell* A, const double* b_dev, double* x_dev, int precond,
                              const vclhip_solver_params* params, vclhip_solver_result* result, void* stream);
 /* solve(A, b, bicgstab_tag [, jacobi_precond]) (bicgstab.hpp:495-533).  Pipelined BiCGStab of

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "internal.hpp"
@@ -824,22 +825,27 @@ struct NoTail {
 // iteration i (0-based); needs_hook(i) says whether iteration i needs extra work before it (a
 // restart), which keeps it out of a captured graph; hook(i) enqueues that work.  tail() enqueues
 // work that closes every chunk (the mixed-precision defect correction, whose kernels return at
-// once unless an iteration of the chunk asked for it).
+// once unless an iteration of the chunk asked for it).  With no_alloc nothing is allocated: no
+// pinned buffer, no events and no graph; the state of each chunk is copied to the host stack and
+// waited for before the next chunk is enqueued.
 template <class LaunchIter, class NeedsHook, class Hook, class Tail = NoTail>
 static int run_iterations(hipStream_t s, KState* st_dev, uint32_t first_iter, uint32_t max_iters,
                           const vclhip_solver_params* P, const void* x_dev, LaunchIter&& launch_iter,
-                          NeedsHook&& needs_hook, Hook&& hook, KState& final_state, bool bicg, Tail tail = Tail())
+                          NeedsHook&& needs_hook, Hook&& hook, KState& final_state, bool bicg, Tail tail = Tail(),
+                          bool no_alloc = false)
 {
   PinnedState pin;
-  VCLHIP_HIP(hipHostMalloc(reinterpret_cast<void**>(&pin.h), 2 * sizeof(KState), hipHostMallocDefault));
   EventPair ev;
-  VCLHIP_HIP(hipEventCreateWithFlags(&ev.e[0], hipEventDisableTiming));
-  VCLHIP_HIP(hipEventCreateWithFlags(&ev.e[1], hipEventDisableTiming));
+  if (!no_alloc) {
+    VCLHIP_HIP(hipHostMalloc(reinterpret_cast<void**>(&pin.h), 2 * sizeof(KState), hipHostMallocDefault));
+    VCLHIP_HIP(hipEventCreateWithFlags(&ev.e[0], hipEventDisableTiming));
+    VCLHIP_HIP(hipEventCreateWithFlags(&ev.e[1], hipEventDisableTiming));
+  }
 
   const bool monitor = P && P->monitor;
   uint32_t chunk = (P && P->chunk) ? P->chunk : 16;
   if (monitor) chunk = 1;
-  const bool want_graph = P && P->use_graph && !monitor && chunk > 1;
+  const bool want_graph = P && P->use_graph && !monitor && chunk > 1 && !no_alloc;
 
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -884,13 +890,19 @@ static int run_iterations(hipStream_t s, KState* st_dev, uint32_t first_iter, ui
       if (status) break;
     }
     launched += cnt;
-    status = hip_status(hipMemcpyAsync(&pin.h[slot], st_dev, sizeof(KState), hipMemcpyDeviceToHost, s), "state D2H");
-    if (!status) status = hip_status(hipEventRecord(ev.e[slot], s), "hipEventRecord");
+    KState polled;  // no_alloc: the state after this chunk
+    if (no_alloc) {
+      status = hip_status(hipMemcpyAsync(&polled, st_dev, sizeof(KState), hipMemcpyDeviceToHost, s), "state D2H");
+      if (!status) status = hip_status(hipStreamSynchronize(s), "solver sync");
+    } else {
+      status = hip_status(hipMemcpyAsync(&pin.h[slot], st_dev, sizeof(KState), hipMemcpyDeviceToHost, s), "state D2H");
+      if (!status) status = hip_status(hipEventRecord(ev.e[slot], s), "hipEventRecord");
+    }
     if (status) break;
     if (monitor) {
-      status = hip_status(hipEventSynchronize(ev.e[slot]), "hipEventSynchronize");
+      if (!no_alloc) status = hip_status(hipEventSynchronize(ev.e[slot]), "hipEventSynchronize");
       if (status) break;
-      const KState& h = pin.h[slot];
+      const KState& h = no_alloc ? polled : pin.h[slot];
       if (h.iter > 0 && !h.breakdown) {
         const double rel = bicg ? std::fabs(h.res / h.nrs) : std::sqrt(std::fabs(h.rr / h.nrs));
         if (P->monitor(x_dev, rel, P->monitor_user)) {
@@ -901,6 +913,8 @@ static int run_iterations(hipStream_t s, KState* st_dev, uint32_t first_iter, ui
         }
       }
       if (h.flags & end_mask) finished = true;
+    } else if (no_alloc) {
+      if (polled.flags & end_mask) finished = true;
     } else if (pending >= 0) {
       status = hip_status(hipEventSynchronize(ev.e[pending]), "hipEventSynchronize");
       if (status) break;
@@ -958,22 +972,34 @@ static int copy_history(const vclhip_solver_params* P, const double* history_dev
 
 // ------------------------------------------------------------------ CG driver
 
+// With an arena (arena_bytes of device memory) the solve allocates nothing: its vectors, partial
+// sums and state are pieces of the arena, an optional DIA copy takes what is left or is not made,
+// and the host waits for each chunk's state instead of using pinned memory, events or graphs.
 template <typename T, class M>
 static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_solver_params* P,
-                    vclhip_solver_result* R, void* stream)
+                    vclhip_solver_result* R, void* stream, void* arena = nullptr, size_t arena_bytes = 0)
 {
   if (!P || !R || !b || !x) return invalid("null argument");
   if (precond != VCLHIP_PRECOND_NONE && precond != VCLHIP_PRECOND_JACOBI) return invalid("precond");
   DevOp<T> op;
   VCLHIP_TRY(make_op(Ain, op));
-  SolveStream ss(stream, P->use_graph != 0);
-  VCLHIP_TRY(ss.status());
-  hipStream_t s = ss.get();
   const uint32_t n = op.n;
   const bool jac = precond == VCLHIP_PRECOND_JACOBI;
-  Workspace ws(s);
-  int used_dia = 0;
-  VCLHIP_TRY(dia_operator(op, ws, s, P, &used_dia));
+  const uint32_t hist_len = (P->history_host && P->history_len && P->max_iters) ? std::min(P->history_len, P->max_iters) : 0;
+  if (arena) {
+    const size_t need = 3 * Workspace::arena_bytes(sizeof(T), n) + Workspace::arena_bytes(sizeof(double), kSlots * kMaxPartials) +
+                        Workspace::arena_bytes(sizeof(KState), 1) + (hist_len ? Workspace::arena_bytes(sizeof(double), hist_len) : 0) +
+                        (jac ? Workspace::arena_bytes(sizeof(T), n) + Workspace::arena_bytes(sizeof(uint32_t), 1) : 0);
+    if (need > arena_bytes) {
+      set_error("CG without allocations: n = %u needs %zu bytes of scratch memory, more than its %zu", n, need,
+                arena_bytes);
+      return VCLHIP_ERROR_TOO_LARGE;
+    }
+  }
+  SolveStream ss(stream, P->use_graph != 0 && !arena);
+  VCLHIP_TRY(ss.status());
+  hipStream_t s = ss.get();
+  Workspace ws = arena ? Workspace(s, arena, arena_bytes) : Workspace(s);
   T *r, *p, *Ap, *d = nullptr;
   double *part, *hist = nullptr;
   KState* st;
@@ -982,13 +1008,15 @@ static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_so
   VCLHIP_TRY(ws.alloc(&Ap, n));
   VCLHIP_TRY(ws.alloc(&part, kSlots * kMaxPartials));
   VCLHIP_TRY(ws.alloc(&st, 1));
-  if (P->history_host && P->history_len && P->max_iters) VCLHIP_TRY(ws.alloc(&hist, std::min(P->history_len, P->max_iters)));
+  if (hist_len) VCLHIP_TRY(ws.alloc(&hist, hist_len));
   if (jac) {
     uint32_t* zc;
     VCLHIP_TRY(ws.alloc(&d, n));
     VCLHIP_TRY(ws.alloc(&zc, 1));
     VCLHIP_TRY(jacobi_diag(op, d, zc, s));
   }
+  int used_dia = 0;
+  VCLHIP_TRY(dia_operator(op, ws, s, P, &used_dia));
   KState h;
   init_state(h, P, hist);
   VCLHIP_TRY(upload_state(st, h, s));
@@ -1053,16 +1081,17 @@ static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_so
     return check_launch("cg iteration");
   };
   KState fin;
+  const bool no_alloc = arena != nullptr;
   if (compact) {
     launch_update();  // the update of iteration 0; iteration k then closes with that of k + 1
     VCLHIP_TRY(check_launch("cg update"));
     VCLHIP_TRY(run_iterations(
         s, st, 0u, P->max_iters, P, x, launch_iter_compact, [](uint32_t) { return false; },
-        [](uint32_t) { return 0; }, fin, false));
+        [](uint32_t) { return 0; }, fin, false, NoTail(), no_alloc));
   } else {
     VCLHIP_TRY(run_iterations(
         s, st, 0u, P->max_iters, P, x, launch_iter, [](uint32_t) { return false; }, [](uint32_t) { return 0; }, fin,
-        false));
+        false, NoTail(), no_alloc));
   }
   R->iters = fin.iter;
   R->error = fin.nrs != 0 ? std::sqrt(std::fabs(fin.rr) / fin.nrs) : 0.0;
@@ -2141,6 +2170,35 @@ static int dia_create(const vclhip_csr* A, vclhip_dia** out, void* stream)
 
 }  // namespace vclhip
 
+// ------------------------------------------------------------------ CG without allocations
+
+// Scratch memory of vclhip_cg_solve_csr_f64_noalloc: a file-scope device array that every device the
+// library runs on holds until the process ends.  Host code never passes `scratch` itself (its host
+// name is a stand-in, not a device address): scratch_address() takes each device's own address once,
+// with that device current, and the solver receives that address.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[kScratchDevices] = {};
+
+static int scratch_address(void** out)
+{
+  int dev = 0;
+  VCLHIP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) return invalid("device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void* ptr = nullptr;
+    VCLHIP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return hip_status(hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch_addr[dev] = ptr;
+  }
+  *out = g_scratch_addr[dev];
+  return VCLHIP_SUCCESS;
+}
+
 extern "C" {
 
 int vclhip_dia_create_f64(const vclhip_csr* A, vclhip_dia** dia, void* stream)
@@ -2199,6 +2257,15 @@ int vclhip_cg_solve_sell_f64(const vclhip_sell* A, const double* b, double* x, i
                              const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
 {
   return cg_solve(A, b, x, precond, P, R, stream);
+}
+
+int vclhip_cg_solve_csr_f64_noalloc(const vclhip_csr* A, const double* b, double* x, int precond,
+                                    const vclhip_solver_params* P, vclhip_solver_result* R, void* stream)
+{
+  if (P && P->use_graph) return invalid("use_graph must be 0: instantiating a graph allocates");
+  void* arena = nullptr;
+  VCLHIP_TRY(scratch_address(&arena));
+  return cg_solve(A, b, x, precond, P, R, stream, arena, kScratchBytes);
 }
 
 int vclhip_bicgstab_solve_csr_f64(const vclhip_csr* A, const double* b, double* x, int precond,

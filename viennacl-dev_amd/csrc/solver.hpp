@@ -254,10 +254,12 @@ class SolveStream {
 
 // ------------------------------------------------------------------ device workspace
 
-// Stream-ordered allocations released when the object goes out of scope.
+// Stream-ordered allocations released when the object goes out of scope; or, made over an arena
+// (a caller's device buffer), 256-byte aligned pieces of that buffer and no allocation at all.
 class Workspace {
  public:
   explicit Workspace(hipStream_t s) : s_(s) {}
+  Workspace(hipStream_t s, void* arena, size_t bytes) : s_(s), arena_(static_cast<char*>(arena)), cap_(bytes) {}
   ~Workspace()
   {
     for (void* p : ptrs_) (void)hipFreeAsync(p, s_);
@@ -265,6 +267,13 @@ class Workspace {
   template <typename U>
   int alloc(U** p, size_t count)
   {
+    if (arena_) {
+      if (!take(p, count)) {
+        set_error("scratch memory: %zu more bytes needed, %zu of %zu left", count * sizeof(U), cap_ - used_, cap_);
+        return VCLHIP_ERROR_TOO_LARGE;
+      }
+      return VCLHIP_SUCCESS;
+    }
     void* v = nullptr;
     VCLHIP_HIP(hipMallocAsync(&v, count * sizeof(U) + 256, s_));
     ptrs_.push_back(v);
@@ -275,6 +284,7 @@ class Workspace {
   template <typename U>
   bool try_alloc(U** p, size_t count)
   {
+    if (arena_) return take(p, count);
     void* v = nullptr;
     if (hipMallocAsync(&v, count * sizeof(U) + 256, s_) != hipSuccess || !v) {
       (void)hipGetLastError();
@@ -285,10 +295,30 @@ class Workspace {
     *p = static_cast<U*>(v);
     return true;
   }
+  // Bytes still free in the arena (no limit without one).
+  size_t arena_left() const { return arena_ ? cap_ - used_ : ~size_t(0); }
+  // The arena bytes that alloc(count elements of `elem` bytes) takes: like a device allocation here,
+  // 256 bytes beyond the request, rounded up to keep the next piece 256-byte aligned.
+  static size_t arena_bytes(size_t elem, size_t count) { return (elem * count + 256 + 255) & ~size_t(255); }
 
  private:
+  template <typename U>
+  bool take(U** p, size_t count)
+  {
+    const size_t bytes = arena_bytes(sizeof(U), count);
+    if (bytes > cap_ - used_) {
+      *p = nullptr;
+      return false;
+    }
+    *p = reinterpret_cast<U*>(arena_ + used_);
+    used_ += bytes;
+    return true;
+  }
+
   hipStream_t s_;
   std::vector<void*> ptrs_;
+  char* arena_ = nullptr;  // 256-byte aligned
+  size_t cap_ = 0, used_ = 0;
 };
 
 }  // namespace vclhip

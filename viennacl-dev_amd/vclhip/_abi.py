@@ -189,6 +189,7 @@ _SIGS = {
     "vclhip_dia_info": (i32, [vp, P(i32), P(u32)]),
     "vclhip_dia_destroy": (i32, [vp]),
     "vclhip_cg_solve_csr_f64": (i32, [P(Csr), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
+    "vclhip_cg_solve_csr_f64_noalloc": (i32, [P(Csr), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_cg_solve_sell_f64": (i32, [P(Sell), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_bicgstab_solve_csr_f64": (i32, [P(Csr), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
     "vclhip_bicgstab_solve_sell_f64": (i32, [P(Sell), vp, vp, i32, P(SolverParams), P(SolverResult), vp]),
