@@ -15,6 +15,7 @@
 #include "viennacl/backend/memory.hpp"
 #include "viennacl/context.hpp"
 #include "viennacl/forwards.h"
+#include "viennacl/linalg/host_based/common.hpp"
 
 namespace viennacl
 {
