@@ -418,7 +418,10 @@ typedef struct vclhip_solver_params {
   int use_graph;                     /* capture one chunk as a hipGraph and replay it */
   int mixed_mode;                    /* mixed CG: 0 = restart as the reference, 1 = reliable update (keeps p) */
   int variant;                       /* BiCGStab with Jacobi: 0 = the reference's recurrences (bicgstab.hpp:447-480),
-                                        1 = fully pipelined (fewer passes; rounding differs) */
+                                        1 = fully pipelined (fewer passes; rounding differs).
+                                        CG: 0 = automatic (a small system on a DIA operator runs one
+                                        kernel per iteration), 1 = at least two kernels per
+                                        iteration; the results are the same bit for bit */
   double* history_host;              /* optional: per-iteration convergence quantity (may be NULL) */
   uint32_t history_len;
   /* optional monitor, called once per iteration with the device solution and the relative residual
@@ -471,6 +474,8 @@ typedef struct vclhip_solver_result {
   int operator_used;  /* the storage the products ran on: VCLHIP_USED_AS_GIVEN or VCLHIP_USED_DIA;
                          mixed CG: its FP32 inner copy, VCLHIP_INNER_CSR (the matrix's own format),
                          VCLHIP_INNER_SELL64, VCLHIP_INNER_SELL64_D16 or VCLHIP_INNER_DIA */
+  uint32_t kernels_per_iter;  /* CG: kernels launched per iteration after the first (1 in fused mode,
+                                 2 in compact mode, else 3); 0 from the other solvers */
 } vclhip_solver_result;
 #define VCLHIP_USED_AS_GIVEN 0
 #define VCLHIP_USED_DIA 1

@@ -494,3 +494,70 @@ def test_jacobi_bicgstab_powerlaw_vs_oracle(dev, vh, oracle):
                                             diag=oracle.row_info(rp, col, val, 3))
     assert iters_close(res.iters, it), (res.iters, it)
     assert res.converged and true_relres(rp, col, val, x, b) < 1e-8
+
+
+FUSED_CASES = {
+    "fdm_256": (lambda vh: vh.fdm_laplace_2d(256, 256), 0, dict(tol=1e-8, max_iters=2000)),
+    "diffusion_jacobi": (lambda vh: vh.diffusion(90, 70, 1, np.exp(vh.random_normal(3, 90 * 70))), 1,
+                         dict(tol=1e-10, max_iters=3000)),
+    "lap27": (lambda vh: stencil_27(14, 13, 12), 0, dict(tol=1e-10, max_iters=3000)),
+    "graph_chunks": (lambda vh: vh.fdm_laplace_2d(128, 100), 0, dict(tol=1e-10, max_iters=3000, use_graph=True)),
+}
+
+
+@pytest.mark.parametrize("case", list(FUSED_CASES))
+def test_cg_one_kernel_per_iteration_equals_two(dev, vh, oracle, case):
+    """Small systems on a DIA operator run one kernel per CG iteration (cg_fused_kernel: the scalar
+    step of iteration k, the update of k + 1 and the product of k + 1 in one launch, reading r, p and
+    A p from one copy and writing the other).  Against the two-kernel compact mode (variant 1): the
+    same iteration count, error, per-iteration history and x bit for bit, with and without Jacobi, on
+    a 27-diagonal stencil (full and partial groups of eight diagonals) and with hipGraph chunks; and
+    the oracle's iteration count (same recurrences) within 2%."""
+    make, pc, kw = FUSED_CASES[case]
+    rp, col, val = make(vh)
+    n = rp.size - 1
+    b = vh.random_uniform(7, n)
+    A = vh.CsrMatrix(rp, col, val)
+    out = []
+    for variant in (0, 1):
+        db, dx = vh.DeviceArray.from_numpy(b), vh.DeviceArray.zeros(n, np.float64)
+        hist = np.zeros(kw["max_iters"])
+        res = vh.cg(A, db, dx, precond=pc, variant=variant, history=hist, **kw)
+        out.append((dx.to_numpy(), res, hist))
+    (xf, rf, hf), (xc, rc, hc) = out
+    print(f"{case}: {rf.iters} iterations, kernels per iteration {rf.kernels_per_iter} / {rc.kernels_per_iter}")
+    assert rf.operator_used == vh.USED_DIA and rf.kernels_per_iter == 1 and rc.kernels_per_iter == 2
+    assert rf.converged and rf.iters == rc.iters and rf.error == rc.error
+    assert np.array_equal(xf, xc) and np.array_equal(hf, hc)
+    Ah = oracle.CsrHost(rp, col, val)
+    rows = np.repeat(np.arange(n), np.diff(rp.astype(np.int64)))
+    diag = np.zeros(n)
+    on = col.astype(np.int64) == rows
+    diag[rows[on]] = val[on]
+    if pc:
+        _, it_o, _, _ = oracle.cg_classic(Ah, b, kw["tol"], kw["max_iters"], diag=diag)
+    else:
+        _, it_o, _, _ = oracle.cg_pipelined(Ah, b, kw["tol"], kw["max_iters"])
+    assert iters_close(rf.iters, it_o), (rf.iters, it_o)
+    # Jacobi stops on the D^-1-norm of r, which bounds the 2-norm ratio by sqrt(max d / min d) tol
+    bound = 1.05 * kw["tol"] * (np.sqrt(diag.max() / diag.min()) if pc else 1.0)
+    r = b - oracle.csr_spmv(rp, col, val, xf)
+    assert np.linalg.norm(r) / np.linalg.norm(b) < bound
+
+
+def test_cg_fused_mode_only_when_grid_fits_one_round(dev, vh):
+    """One kernel per iteration pays only while its grid is resident at once (two workgroups per CU
+    with the fused kernel's registers): a 392 x 392 Laplacian (608 row blocks of 256 rows, more
+    than twice the 256 CUs) keeps the two-kernel compact mode, with the same results as variant 1."""
+    rp, col, val = vh.fdm_laplace_2d(392, 392)
+    n = rp.size - 1
+    b = vh.random_uniform(9, n)
+    A = vh.CsrMatrix(rp, col, val)
+    out = []
+    for variant in (0, 1):
+        db, dx = vh.DeviceArray.from_numpy(b), vh.DeviceArray.zeros(n, np.float64)
+        res = vh.cg(A, db, dx, tol=1e-8, max_iters=3000, variant=variant)
+        out.append((dx.to_numpy(), res))
+    (x0, r0), (x1, r1) = out
+    assert r0.operator_used == vh.USED_DIA and r0.kernels_per_iter == 2 and r1.kernels_per_iter == 2
+    assert r0.converged and r0.iters == r1.iters and np.array_equal(x0, x1)

@@ -101,13 +101,34 @@ __device__ __forceinline__ A block_sum_partials(const A* part, uint32_t count)
   return block_sum(s);
 }
 
+// The workgroup sum of K per-lane values: a wave64 xor butterfly per wave, then the four waves in
+// order (the combination step of block_sum_partials_k).
+template <int K, typename A>
+__device__ __forceinline__ void block_combine_k(A (&s)[K], A (&out)[K])
+{
+  __shared__ A red[K][kWavesPerBlock];
+  const int wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    s[q] = wave_sum(s[q]);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[q][wave] = s[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    out[q] = red[q][0];
+#pragma unroll
+    for (int w = 1; w < kWavesPerBlock; ++w) out[q] += red[q][w];
+  }
+  __syncthreads();
+}
+
 // K block_sum_partials in one pass: lane t adds p[q][t], p[q][t+256], ... in the same order as
 // block_sum_partials, and the workgroup combines in the same tree, so the results equal K separate
 // calls; the K loads of a step are in flight together and the workgroup synchronizes once.
 template <int K, typename A>
 __device__ __forceinline__ void block_sum_partials_k(const A* const (&p)[K], const uint32_t (&count)[K], A (&out)[K])
 {
-  __shared__ A red[K][kWavesPerBlock];
   uint32_t nmax = 0;
 #pragma unroll
   for (int q = 0; q < K; ++q) nmax = count[q] > nmax ? count[q] : nmax;
@@ -134,20 +155,7 @@ __device__ __forceinline__ void block_sum_partials_k(const A* const (&p)[K], con
     for (int q = 0; q < K; ++q)
       if (i < count[q]) s[q] += p[q][i];
   }
-  const int wave = threadIdx.x / kWave;
-#pragma unroll
-  for (int q = 0; q < K; ++q) {
-    s[q] = wave_sum(s[q]);
-    if ((threadIdx.x & (kWave - 1)) == 0) red[q][wave] = s[q];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < K; ++q) {
-    out[q] = red[q][0];
-#pragma unroll
-    for (int w = 1; w < kWavesPerBlock; ++w) out[q] += red[q][w];
-  }
-  __syncthreads();
+  block_combine_k<K, A>(s, out);
 }
 
 template <typename A>

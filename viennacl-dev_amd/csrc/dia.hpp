@@ -246,6 +246,7 @@ static int dia_operator(DevOp<T>& op, Workspace& ws, hipStream_t s, const vclhip
   }
   if (!built) return VCLHIP_SUCCESS;
   op.fmt = OpFormat::Dia;
+  op.dia_nt = dia_stream_loads((static_cast<size_t>(dia.nd) + 4) * dia.n_rows * sizeof(T));
   op.sell = false;
   op.dia = dia;
   op.grid = chunk_grid((op.n + kBlock - 1) / kBlock, sell_gmax());

@@ -686,6 +686,7 @@ int gmres_solve(const M* Ain, const T* b, T* x, const vclhip_solver_params* P, v
   Res->restarts = fin.cycles;
   Res->operator_used = used_dia ? VCLHIP_USED_DIA : VCLHIP_USED_AS_GIVEN;
   Res->converged = static_cast<int>(fin.converged);
+  Res->kernels_per_iter = 0;
   if (hist && fin.iters) {
     VCLHIP_HIP(hipMemcpyAsync(P->history_host, hist, std::min(hist_len, fin.iters) * sizeof(double),
                               hipMemcpyDeviceToHost, s));

@@ -341,6 +341,241 @@ __global__ __launch_bounds__(kBlock) void cg_step_kernel(uint32_t n, T* __restri
   block_store_partials<1, double>(acc, slot_ptr(part, par ? 0u : 3u), kMaxPartials, lw);
 }
 
+// One kernel per CG iteration for small systems on a DIA operator: the scalar step of iteration k
+// (as cg_step_kernel), the vector update of iteration k + 1 on this workgroup's vector chunk, and the
+// product A p_{k+1} with its inner products on this workgroup's product chunk.  The product needs
+// p_{k+1} of rows that other workgroups update in this same kernel, so it forms them itself from the
+// previous iteration's r, A p and p with the update's own expressions (the same values bit for bit);
+// r, p and A p are kept twice, read from the `in` copies and written to the `out` ones.  Workgroup
+// lw < g_vec takes the update kernel's chunk lw and lw < g_prod the DIA product kernel's chunk lw,
+// so every partial sum is the one of the two-kernel compact mode and the solve is the same bit for
+// bit.  Partial slots by the parity of k: <r,z> of update k in 0 / 3 (update k + 1 writes the
+// other), <Ap,Ap> and <p,Ap> of product k in 1, 2 / 4, 5 (product k + 1 writes the other pair).
+//
+// The loads that do not depend on this iteration's scalars -- the partial sums, the first
+// kFusedPacks packs of the lane's share of the update and the first tile of the product (the
+// values of up to kDiaGroup diagonals and the r, A p, p entries they gather) -- are all issued
+// before the partial sums are reduced, so that they are in flight together: on a system held in the
+// caches the iteration is then about one memory round trip long instead of four.
+constexpr int kFusedPacks = 2;
+constexpr uint32_t kFusedPartials = 4;
+
+template <typename T, bool Jac, int W>
+__global__ __launch_bounds__(kBlock) void cg_fused_kernel(uint32_t n, T* __restrict__ x, const T* __restrict__ r_in,
+                                                          T* __restrict__ r_out, const T* __restrict__ p_in,
+                                                          T* __restrict__ p_out, const T* __restrict__ Ap_in,
+                                                          T* __restrict__ Ap_out, const T* __restrict__ d, DiaDev<T> A,
+                                                          KState* __restrict__ st, double* __restrict__ part,
+                                                          uint32_t par, uint32_t g_vec, uint32_t g_prod)
+{
+  // the state is read with everything else and tested once the loads are in flight
+  const uint32_t flags = st->flags;
+  const uint32_t iter_prev = st->iter2[par];
+  const uint32_t max_iters = st->max_iters;
+  const double nrs = st->nrs, tol = st->tol, abs_tol = st->abs_tol;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lw = xcd_logical_block();
+
+  // ---- loads issued ahead of the scalar step: first the partial sums it reduces (up to
+  // kFusedPartials per slot and lane; lane t takes entries t, t + 256, ... as block_sum_partials_k)
+  const double* const ps[3] = {slot_ptr(part, par ? 3u : 0u), slot_ptr(part, par ? 4u : 1u),
+                               slot_ptr(part, par ? 5u : 2u)};
+  const uint32_t pc[3] = {g_vec, g_prod, g_prod};
+  double pv[kFusedPartials][3];
+#pragma unroll
+  for (uint32_t b = 0; b < kFusedPartials; ++b)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      // unconditional (clamped) loads, so that no sum is placed next to its load and waits for it
+      const uint32_t i = tid + b * kBlock;
+      pv[b][q] = ps[q][i < pc[q] ? i : pc[q] - 1];
+    }
+  const bool has_vec = lw < g_vec;
+  uint32_t c0 = 0, c1 = 0;
+  if (has_vec) chunk_range_w(n, W, lw, g_vec, c0, c1);
+  const uint32_t packed_end = c0 + ((c1 - c0) / W) * W;
+  Pack<W, T> fp[kFusedPacks], fr[kFusedPacks], fx[kFusedPacks], fa[kFusedPacks], fd[kFusedPacks];
+#pragma unroll
+  for (int k = 0; k < kFusedPacks; ++k) {
+    const uint32_t i = c0 + W * tid + static_cast<uint32_t>(k) * W * kBlock;
+    if (has_vec && i < packed_end) {
+      fp[k] = ldp<W>(p_in, i);
+      fr[k] = ldp<W>(r_in, i);
+      fx[k] = ldp<W>(x, i);
+      fa[k] = ldp<W>(Ap_in, i);
+      if constexpr (Jac) fd[k] = ldp<W>(d, i);
+    }
+  }
+  const bool has_prod = lw < g_prod;
+  const uint32_t tiles = (n + kBlock - 1) / kBlock;
+  uint32_t t0 = 0, t1 = 0;
+  if (has_prod) chunk_range(tiles, lw, g_prod, t0, t1);
+  const uint32_t row0 = t0 * kBlock + tid;
+  const bool first_row = has_prod && t0 < t1 && row0 < n;
+  T gv[kDiaGroup], gr[kDiaGroup], ga[kDiaGroup], gp[kDiaGroup], gd[kDiaGroup];
+  T own_r = 0, own_a = 0, own_p = 0, own_d = 1;
+  if (first_row) {
+#pragma unroll
+    for (uint32_t j = 0; j < kDiaGroup; ++j) {
+      gv[j] = T(0);
+      gr[j] = ga[j] = gp[j] = T(0);
+      gd[j] = T(1);
+      if (j < A.nd) {
+        const uint32_t c = row0 + static_cast<uint32_t>(A.off[j]);  // out of range (also below 0): no entry
+        gv[j] = A.val[static_cast<size_t>(j) * A.n_rows + row0];
+        if (c < A.n_rows) {
+          gr[j] = r_in[c];
+          ga[j] = Ap_in[c];
+          gp[j] = p_in[c];
+          if constexpr (Jac) gd[j] = d[c];
+        }
+      }
+    }
+    own_r = r_in[row0];
+    own_a = Ap_in[row0];
+    own_p = p_in[row0];
+    if constexpr (Jac) own_d = d[row0];
+  }
+
+  if (flags & kFlagDone) return;  // the same value for every lane: all return or none
+
+  // ---- the scalar step of iteration k: the partial sums added in block_sum_partials_k's order
+  double sums[3] = {0, 0, 0}, red3[3];
+#pragma unroll
+  for (uint32_t b = 0; b < kFusedPartials; ++b)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if (tid + b * kBlock < pc[q]) sums[q] += pv[b][q];
+  const uint32_t pmax = g_vec > g_prod ? g_vec : g_prod;
+  for (uint32_t i = tid + kFusedPartials * kBlock; i < pmax; i += kBlock)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if (i < pc[q]) sums[q] += ps[q][i];
+  block_combine_k<3, double>(sums, red3);
+  const double rr = red3[0], ApAp = red3[1], pAp = red3[2];
+  const uint32_t iter = iter_prev + 1;
+  const bool conv = fabs(rr / nrs) < tol * tol || fabs(rr) < abs_tol * abs_tol;
+  const bool done = conv || iter >= max_iters;
+  const double alpha = rr / pAp;
+  const double beta = (alpha * alpha * ApAp - rr) / rr;
+  if (blockIdx.x == 0 && tid == 0) {
+    st->iter = iter;
+    st->iter2[par ^ 1u] = iter;
+    st->rr = rr;
+    record_history(st, iter, rr);
+    if (done) {
+      if (conv) st->converged = 1;
+      st->flags |= kFlagDone;
+    } else {
+      st->alpha = alpha;
+      st->beta = beta;
+    }
+  }
+  if (done) return;
+  const T ta = static_cast<T>(alpha);
+  const T tb = static_cast<T>(beta);
+
+  // ---- update k + 1 (cg_step_kernel's update, chunk lw of g_vec, in for_packs' order)
+  if (has_vec) {
+    double acc[1] = {0};
+    auto update = [&](uint32_t i, auto& vp, auto& vr, auto& vx, const auto& va, const auto& vd) {
+      constexpr int V = sizeof(vp.v) / sizeof(T);
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        vx.v[k] += ta * vp.v[k];
+        vr.v[k] -= ta * va.v[k];
+        T z = vr.v[k];
+        if constexpr (Jac) z = vr.v[k] / vd.v[k];
+        vp.v[k] = z + tb * vp.v[k];
+        acc[0] += static_cast<double>(vr.v[k] * z);
+      }
+      stp<V>(x, i, vx);
+      stp<V>(p_out, i, vp);
+      stp<V>(r_out, i, vr);
+    };
+#pragma unroll
+    for (int k = 0; k < kFusedPacks; ++k) {
+      const uint32_t i = c0 + W * tid + static_cast<uint32_t>(k) * W * kBlock;
+      if (i < packed_end) update(i, fp[k], fr[k], fx[k], fa[k], fd[k]);
+    }
+    for (uint32_t i = c0 + W * tid + kFusedPacks * W * kBlock; i < packed_end; i += W * kBlock) {
+      Pack<W, T> vp = ldp<W>(p_in, i), vr = ldp<W>(r_in, i), vx = ldp<W>(x, i);
+      const Pack<W, T> va = ldp<W>(Ap_in, i);
+      Pack<W, T> vd;
+      if constexpr (Jac) vd = ldp<W>(d, i);
+      update(i, vp, vr, vx, va, vd);
+    }
+    if (W > 1) {
+      for (uint32_t i = packed_end + tid; i < c1; i += kBlock) {
+        Pack<1, T> vp = ldp<1>(p_in, i), vr = ldp<1>(r_in, i), vx = ldp<1>(x, i);
+        const Pack<1, T> va = ldp<1>(Ap_in, i);
+        Pack<1, T> vd;
+        if constexpr (Jac) vd = ldp<1>(d, i);
+        update(i, vp, vr, vx, va, vd);
+      }
+    }
+    block_store_partials<1, double>(acc, slot_ptr(part, par ? 0u : 3u), kMaxPartials, lw);
+  }
+
+  // ---- A p_{k+1} with EpiCg (dia_kernel's sum, chunk lw of g_prod)
+  if (has_prod) {
+    auto next = [&](T rj, T aj, T pj, T dj) -> T {
+      rj -= ta * aj;
+      T z = rj;
+      if constexpr (Jac) z = rj / dj;
+      return z + tb * pj;
+    };
+    auto p_next = [&](uint32_t j) -> T { return next(r_in[j], Ap_in[j], p_in[j], Jac ? d[j] : T(1)); };
+    double acc[2] = {0, 0};
+    for (uint32_t t = t0; t < t1; ++t) {
+      const uint32_t row = t * kBlock + tid;
+      if (row < n) {
+        T sum = 0;
+        uint32_t d_from = 0;
+        T p_row;
+        if (t == t0) {  // the first group of diagonals from the registers loaded ahead
+#pragma unroll
+          for (uint32_t j = 0; j < kDiaGroup; ++j) {
+            if (j < A.nd) {
+              const uint32_t c = row + static_cast<uint32_t>(A.off[j]);
+              const T xv = c < A.n_rows ? next(gr[j], ga[j], gp[j], gd[j]) : T(0);
+              sum += (gv[j] > 0 || gv[j] < 0) ? xv * gv[j] : T(0);
+            }
+          }
+          d_from = kDiaGroup;
+          p_row = next(own_r, own_a, own_p, own_d);
+        } else {
+          p_row = p_next(row);
+        }
+        for (uint32_t d0 = d_from; d0 < A.nd; d0 += kDiaGroup) {
+          T v[kDiaGroup], xv[kDiaGroup];
+#pragma unroll
+          for (uint32_t j = 0; j < kDiaGroup; ++j) {
+            v[j] = T(0);
+            xv[j] = T(0);
+            if (d0 + j < A.nd) {
+              const uint32_t dd = d0 + j;
+              const uint32_t c = row + static_cast<uint32_t>(A.off[dd]);
+              v[j] = A.val[static_cast<size_t>(dd) * A.n_rows + row];
+              if (c < A.n_rows) xv[j] = p_next(c);
+            }
+          }
+#pragma unroll
+          for (uint32_t j = 0; j < kDiaGroup; ++j)
+            if (d0 + j < A.nd) sum += (v[j] > 0 || v[j] < 0) ? xv[j] * v[j] : T(0);
+        }
+        Ap_out[row] = sum;
+        if (Jac)
+          acc[0] += static_cast<double>(sum * (sum / d[row]));
+        else
+          acc[0] += static_cast<double>(sum * sum);
+        acc[1] += static_cast<double>(p_row * sum);
+      }
+    }
+    block_store_partials<2, double>(acc, slot_ptr(part, par ? 1u : 4u), kMaxPartials, lw);
+  }
+}
+
 // ------------------------------------------------------------------ BiCGStab kernels
 
 // x = 0; r = p = r0* = M^-1 b; partials <b,b>, <r,r>  (bicgstab.hpp:105-131, :408-445)
@@ -1080,14 +1315,48 @@ static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_so
                       Ap, d, st, part, k & 1u, gv, op.grid);
     return check_launch("cg iteration");
   };
+  // Fused mode: one kernel per iteration (cg_fused_kernel) where compact mode runs on a DIA operator
+  // whose grid is resident in one round (the kernel's registers leave room for two workgroups per
+  // CU; a second round costs more than the launch it saves), unless params->variant is 1.  Its
+  // second copies of r, p and A p are optional memory: without them the solve stays in compact
+  // mode (the same results either way).
+  T *r2 = nullptr, *p2 = nullptr, *Ap2 = nullptr;
+  const uint32_t g_fused = std::max(gv, op.grid);
+  bool fused = compact && op.fmt == OpFormat::Dia && P->variant != 1 &&
+               g_fused <= 2u * static_cast<uint32_t>(device_cu_count());
+  if (fused) fused = ws.try_alloc(&r2, n) && ws.try_alloc(&p2, n) && ws.try_alloc(&Ap2, n);
+  auto launch_iter_fused = [&](uint32_t k) -> int {
+    const uint32_t par = k & 1u;
+    T* const r_in = par ? r2 : r;
+    T* const r_out = par ? r : r2;
+    T* const p_in = par ? p2 : p;
+    T* const p_out = par ? p : p2;
+    T* const Ap_in = par ? Ap2 : Ap;
+    T* const Ap_out = par ? Ap : Ap2;
+    if (jac)
+      VCLHIP_LAUNCH_W(v2, (cg_fused_kernel<T, true, kVecW<T> >), (cg_fused_kernel<T, true, 1>), g_fused, s, n, x,
+                      r_in, r_out, p_in, p_out, Ap_in, Ap_out, d, op.dia, st, part, par, gv, op.grid);
+    else
+      VCLHIP_LAUNCH_W(v2, (cg_fused_kernel<T, false, kVecW<T> >), (cg_fused_kernel<T, false, 1>), g_fused, s, n, x,
+                      r_in, r_out, p_in, p_out, Ap_in, Ap_out, d, op.dia, st, part, par, gv, op.grid);
+    return check_launch("cg iteration");
+  };
   KState fin;
   const bool no_alloc = arena != nullptr;
   if (compact) {
     launch_update();  // the update of iteration 0; iteration k then closes with that of k + 1
     VCLHIP_TRY(check_launch("cg update"));
-    VCLHIP_TRY(run_iterations(
-        s, st, 0u, P->max_iters, P, x, launch_iter_compact, [](uint32_t) { return false; },
-        [](uint32_t) { return 0; }, fin, false, NoTail(), no_alloc));
+    if (fused) {
+      launch_prod();  // the product of iteration 0; iteration k's kernel then forms that of k + 1
+      VCLHIP_TRY(check_launch("cg prod"));
+      VCLHIP_TRY(run_iterations(
+          s, st, 0u, P->max_iters, P, x, launch_iter_fused, [](uint32_t) { return false; },
+          [](uint32_t) { return 0; }, fin, false, NoTail(), no_alloc));
+    } else {
+      VCLHIP_TRY(run_iterations(
+          s, st, 0u, P->max_iters, P, x, launch_iter_compact, [](uint32_t) { return false; },
+          [](uint32_t) { return 0; }, fin, false, NoTail(), no_alloc));
+    }
   } else {
     VCLHIP_TRY(run_iterations(
         s, st, 0u, P->max_iters, P, x, launch_iter, [](uint32_t) { return false; }, [](uint32_t) { return 0; }, fin,
@@ -1099,6 +1368,7 @@ static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_so
   R->restarts = 0;
   R->converged = static_cast<int>(fin.converged);
   R->operator_used = used_dia ? VCLHIP_USED_DIA : VCLHIP_USED_AS_GIVEN;
+  R->kernels_per_iter = compact ? (fused ? 1u : 2u) : 3u;
   return copy_history(P, hist, fin.iter, s);
 }
 
@@ -1225,6 +1495,7 @@ static int bicgstab_solve(const M* Ain, const T* b, T* x, int precond, const vcl
   R->restarts = restarts;
   R->converged = static_cast<int>(fin.converged);
   R->operator_used = used_dia ? VCLHIP_USED_DIA : VCLHIP_USED_AS_GIVEN;
+  R->kernels_per_iter = 0;
   return copy_history(P, hist, fin.iter, s);
 }
 
@@ -1856,6 +2127,7 @@ static int mixed_cg_solve(const M* A, const float* values32, const double* b, do
     op32.fmt = OpFormat::Dia;
     op32.sell = false;
     op32.dia = inner_dia;
+    op32.dia_nt = dia_stream_loads((static_cast<size_t>(inner_dia.nd) + 4) * n * sizeof(float));
     op32.n = n;
     op32.grid = chunk_grid((n + kBlock - 1) / kBlock, sell_gmax());
   } else if (inner_is_sell) {
@@ -1978,6 +2250,7 @@ static int mixed_cg_solve(const M* A, const float* values32, const double* b, do
   R->error = fin.nrs > 0 ? std::sqrt(fin.res / fin.nrs) : 0.0;
   R->restarts = fin.updates;
   R->converged = static_cast<int>(fin.converged);
+  R->kernels_per_iter = 0;
   R->operator_used = inner_is_dia    ? VCLHIP_INNER_DIA
                      : inner_is_sell ? (inner_dcol ? VCLHIP_INNER_SELL64_D16 : VCLHIP_INNER_SELL64)
                                      : VCLHIP_INNER_CSR;

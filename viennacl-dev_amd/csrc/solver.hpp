@@ -36,7 +36,8 @@ struct DevOp {
                        part, kMaxPartials, stop, mask);
   }
 
-  DiaDev<T> dia{};  // the mixed solver's inner DIA copy
+  DiaDev<T> dia{};  // a DIA copy (the solvers' operator for stencils, the mixed solver's inner copy)
+  bool dia_nt = false;  // its values read nontemporally (dia_stream_loads)
   HybDev<T> hyb{};  // HYB with a plan for its CSR part
   bool hyb_plan = false;
 
@@ -73,8 +74,12 @@ struct DevOp {
                            part, kMaxPartials, stop, mask);
         break;
       case OpFormat::Dia:
-        hipLaunchKernelGGL((dia_kernel<T, Epi>), dim3(grid), dim3(kBlock), 0, s, dia, x, epi, part, kMaxPartials, stop,
-                           mask);
+        if (dia_nt)
+          hipLaunchKernelGGL((dia_kernel<T, Epi, true>), dim3(grid), dim3(kBlock), 0, s, dia, x, epi, part, kMaxPartials,
+                             stop, mask);
+        else
+          hipLaunchKernelGGL((dia_kernel<T, Epi>), dim3(grid), dim3(kBlock), 0, s, dia, x, epi, part, kMaxPartials, stop,
+                             mask);
         break;
       case OpFormat::Hyb:
         if (hyb_plan)

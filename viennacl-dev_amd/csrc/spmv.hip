@@ -23,6 +23,13 @@ bool stream_loads(size_t bytes)
   return bytes > (size_t(256) << 20);
 }
 
+bool dia_stream_loads(size_t working_set)
+{
+  static const uint32_t mode = env_u32("VCLHIP_DIA_NT", 2, 0, 2);
+  if (mode < 2) return mode == 1;
+  return working_set > (size_t(256) << 20);
+}
+
 uint32_t csr_gmax()
 {
   static const uint32_t g = env_u32("VCLHIP_CSR_GMAX", 2048, 8, kMaxPartials);

@@ -336,7 +336,7 @@ struct DiaDev {
 // One lane per row over the diagonals in offset order, i.e. the row's entries in column order, zero
 // values skipped as the SELL kernel skips them: the same sums as a row of the SELL copy.  There are
 // no column indices to load, and the x reads of a wavefront are 64 consecutive values per diagonal.
-template <typename T, class Epi>
+template <typename T, class Epi, bool NT = false>
 __global__ __launch_bounds__(kBlock) void dia_kernel(DiaDev<T> A, const T* __restrict__ x, Epi epi,
                                                      typename Epi::acc_t* __restrict__ part, uint32_t pstride,
                                                      const uint32_t* __restrict__ stop, uint32_t stop_mask)
@@ -370,7 +370,7 @@ __global__ __launch_bounds__(kBlock) void dia_kernel(DiaDev<T> A, const T* __res
             if (d0 + j < A.nd) {
               const uint32_t d = d0 + j;
               const uint32_t c = row + static_cast<uint32_t>(A.off[d]);  // out of range (also below 0): no entry
-              v[j] = A.val[static_cast<size_t>(d) * A.n_rows + row];
+              v[j] = ld_stream<NT>(A.val + static_cast<size_t>(d) * A.n_rows + row);
               if (c < A.n_rows) xv[j] = x[c];
             }
           }
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void dia_kernel(DiaDev<T> A, const T* __res
 #pragma unroll 8
         for (uint32_t d = 0; d < A.nd; ++d) {
           const uint32_t c = row + static_cast<uint32_t>(A.off[d]);
-          const T v = A.val[static_cast<size_t>(d) * A.n_rows + row];
+          const T v = ld_stream<NT>(A.val + static_cast<size_t>(d) * A.n_rows + row);
           const T xv = c < A.n_rows ? x[c] : T(0);
           sum += (v > 0 || v < 0) ? xv * v : T(0);
         }
@@ -462,6 +462,10 @@ uint32_t csr_gmax();
 // Nontemporal matrix loads for a matrix of `bytes`: VCLHIP_STREAM_LOADS = 0 never, 1 always; by
 // default when the matrix is larger than the 256 MiB Infinity Cache.
 bool stream_loads(size_t bytes);
+// Nontemporal loads of a DIA copy's values in a solver whose working set (the values and four
+// vectors) takes `working_set` bytes: VCLHIP_DIA_NT = 0 never, 1 always; by default when it is
+// larger than the 256 MiB Infinity Cache (the values then do not push the vectors out of L2).
+bool dia_stream_loads(size_t working_set);
 uint32_t sell_gmax();
 uint32_t vec_gmax();
 uint32_t spmv_gmax();

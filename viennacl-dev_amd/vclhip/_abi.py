@@ -84,7 +84,8 @@ class SolverParams(C.Structure):
 
 
 class SolverResult(C.Structure):
-    _fields_ = [("iters", u32), ("error", f64), ("restarts", u32), ("converged", i32), ("operator_used", i32)]
+    _fields_ = [("iters", u32), ("error", f64), ("restarts", u32), ("converged", i32), ("operator_used", i32),
+                ("kernels_per_iter", u32)]
 
 
 # name -> (restype, argtypes); every function of include/vclhip.h
