@@ -181,6 +181,9 @@ def timed_solve(vh, kind, A, b, n, precond=0, reps=2, **kw):
            "true_relres": float(rel), "converged": bool(res.converged)}
     if kind == "gmres":
         out["cycles"] = int(res.restarts)
+    else:
+        out["kernels_per_iter"] = int(res.kernels_per_iter)
+        out["us_per_iter"] = round(best * 1e6 / max(int(res.iters), 1), 2)
     return out
 
 
@@ -584,6 +587,47 @@ def cpu_baseline(rp, col, val, x, n, nnz, seconds, c1=None):
     return out, y
 
 
+def latency_reference(vh):
+    """What a kernel boundary and a device-wide synchronisation cost on this device, the two ways of
+    ordering one solver iteration after the previous one: back-to-back launches of an empty kernel
+    (256 workgroups), and steps of vclhip_grid_sync_probe (co-resident workgroups that publish their
+    stores to all XCDs and synchronise through flags, as a single-launch solver would per
+    iteration), each timed by events around the whole sequence."""
+    stream = vh.Stream()
+    ev = Events(vh, 2)
+    out = {}
+    launches = 2000
+    for _ in range(2):  # the second pass is the measurement
+        ev.record(0, stream.handle)
+        for _ in range(launches):
+            vh.call("vclhip_empty_launch", 256, stream.handle)
+        ev.record(1, stream.handle)
+        stream.synchronize()
+    out["empty_kernel_us_per_launch"] = round(ev.ms(0, 1) * 1e3 / launches, 3)
+    flags = vh.DeviceArray.zeros(1024, np.uint32)
+    steps = 2000
+    sync = {}
+    for g in (64, 256, 512):
+        best = None
+        for _ in range(2):
+            ev.record(0, stream.handle)
+            vh.call("vclhip_grid_sync_probe", g, steps, flags.ptr, stream.handle)
+            ev.record(1, stream.handle)
+            stream.synchronize()
+            gave_up = int(flags.to_numpy()[g + 1])
+            if gave_up:
+                raise RuntimeError(f"grid_sync_probe: a workgroup gave up waiting (workgroups {g})")
+            t = ev.ms(0, 1)
+            best = t if best is None else min(best, t)
+        sync[str(g)] = round(best * 1e3 / steps, 3)
+    out["grid_sync_us_per_step_by_workgroups"] = sync
+    flags.free()
+    ev.close()
+    stream.close()
+    log("latency reference", out)
+    return out
+
+
 def c1_solve_bench(vh):
     """The reference's CPU-runnable case on the device: generate_fdm_laplace(256, 256) (n = 65,536,
     nnz = 326,656), b ~ U[-1,1] (seed 0), cg_tag(1e-8, 2000).  5 MB per SpMV stays in the caches,
@@ -600,6 +644,8 @@ def c1_solve_bench(vh):
     rec["cg_fp64_csr_graph"] = timed_solve(vh, "cg", A, b, n, reps=3, tol=1e-8, max_iters=2000, use_graph=True)
     rec["cg_fp64_csr_graph_chunk64"] = timed_solve(vh, "cg", A, b, n, reps=3, tol=1e-8, max_iters=2000,
                                                    use_graph=True, chunk=64)
+    rec["cg_fp64_csr_two_kernels"] = timed_solve(vh, "cg", A, b, n, reps=3, tol=1e-8, max_iters=2000, variant=1)
+    rec["latency_reference"] = latency_reference(vh)
     log("c1 cg", rec)
     b.free()
     A.free()

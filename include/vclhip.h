@@ -83,6 +83,17 @@ int vclhip_time_next_launch(void* start_event, void* stop_event);
  * loads and store its sum in sums[0..num_sums).  x must be 16-byte aligned.  Its kernel time over
  * 8 n bytes is the rate at which the device reads HBM; vclhip_time_next_launch applies to it. */
 int vclhip_stream_read_f64(uint64_t n, const double* x_dev, double* sums_dev, uint32_t num_sums, void* stream);
+/* Latency references (no counterpart in the reference; measurement only).  vclhip_empty_launch
+ * launches a kernel of `workgroups` workgroups that does nothing: back to back, its launches time
+ * the floor of a kernel boundary.  vclhip_grid_sync_probe launches `workgroups` workgroups (at most
+ * what the device holds at once, and at most 4096; VCLHIP_ERROR_INVALID_VALUE otherwise) that
+ * synchronise `steps` times device-wide (each publishes its stores to all XCDs and an arrival flag;
+ * workgroup 0 collects the flags and releases the others): the price that a single-launch solver
+ * would pay per iteration in place of a kernel boundary.  flags_dev holds workgroups + 2 words
+ * (zeroed by the call); every wait is bounded, and flags_dev[workgroups + 1] is nonzero afterwards
+ * if a workgroup gave up waiting. */
+int vclhip_empty_launch(uint32_t workgroups, void* stream);
+int vclhip_grid_sync_probe(uint32_t workgroups, uint32_t steps, uint32_t* flags_dev, void* stream);
 
 /* ------------------------------------------------------------ sparse formats ---- */
 /* CSR matrix on the device (compressed_matrix<T>: handle1 = row_ptr, handle2 = col, handle = values,
@@ -419,6 +430,9 @@ typedef struct vclhip_solver_params {
   int mixed_mode;                    /* mixed CG: 0 = restart as the reference, 1 = reliable update (keeps p) */
   int variant;                       /* BiCGStab with Jacobi: 0 = the reference's recurrences (bicgstab.hpp:447-480),
                                         1 = fully pipelined (fewer passes; rounding differs).
+                                        BiCGStab without a preconditioner: 0 = automatic (a small
+                                        system runs four kernels per iteration), 1 = six; the
+                                        results are the same bit for bit.
                                         CG: 0 = automatic (a small system on a DIA operator runs one
                                         kernel per iteration), 1 = at least two kernels per
                                         iteration; the results are the same bit for bit */
@@ -475,7 +489,8 @@ typedef struct vclhip_solver_result {
                          mixed CG: its FP32 inner copy, VCLHIP_INNER_CSR (the matrix's own format),
                          VCLHIP_INNER_SELL64, VCLHIP_INNER_SELL64_D16 or VCLHIP_INNER_DIA */
   uint32_t kernels_per_iter;  /* CG: kernels launched per iteration after the first (1 in fused mode,
-                                 2 in compact mode, else 3); 0 from the other solvers */
+                                 2 in compact mode, else 3); BiCGStab: 4 in compact mode, 6 pipelined,
+                                 8 with the reference's Jacobi recurrences; 0 from the other solvers */
 } vclhip_solver_result;
 #define VCLHIP_USED_AS_GIVEN 0
 #define VCLHIP_USED_DIA 1

@@ -149,3 +149,19 @@ def test_scal_copy_swap_rot(dev, vh):
     got = v1.to_numpy()[1::2]
     want = (xf[:1000] * 2) * np.float32(1.5) + dyf.to_numpy()[0:3000:3] * np.float32(-0.5)
     assert np.allclose(got, want, rtol=1e-6, atol=1e-6)
+
+
+def test_latency_probes(dev, vh):
+    """The measurement kernels behind the bench's latency reference: empty launches run, the
+    device-wide synchronisation completes every step without a workgroup giving up (every arrival
+    flag ends at `steps`, the go flag too), and a grid larger than the device holds at once, or
+    beyond 4096 workgroups, is refused before anything is launched."""
+    flags = vh.DeviceArray.zeros(4200, np.uint32)
+    assert vh.call("vclhip_empty_launch", 256, None) == 0
+    for g, steps in ((8, 50), (256, 200)):
+        assert vh.call("vclhip_grid_sync_probe", g, steps, flags.ptr, None) == 0
+        f = flags.to_numpy()
+        assert np.all(f[:g] == steps) and f[g] == steps and f[g + 1] == 0
+    with pytest.raises(vh.VclHipError) as e:
+        vh.call("vclhip_grid_sync_probe", 4097, 1, flags.ptr, None)
+    assert e.value.status == vh.ERROR_INVALID_VALUE

@@ -561,3 +561,99 @@ def test_cg_fused_mode_only_when_grid_fits_one_round(dev, vh):
     (x0, r0), (x1, r1) = out
     assert r0.operator_used == vh.USED_DIA and r0.kernels_per_iter == 2 and r1.kernels_per_iter == 2
     assert r0.converged and r0.iters == r1.iters and np.array_equal(x0, x1)
+
+
+BICG_COMPACT_CASES = {
+    "fdm_256": (lambda vh: vh.fdm_laplace_2d(256, 256), np.float64, "csr", dict(tol=1e-8, max_iters=2000)),
+    "convdiff_nonsym": (lambda vh: vh.convection_diffusion(40, 36, 30), np.float64, "csr",
+                        dict(tol=1e-10, max_iters=2000)),
+    "sell_fp32": (lambda vh: vh.laplace_3d(30, 28, 26), np.float32, "sell", dict(tol=1e-5, max_iters=2000)),
+    "max_iters": (lambda vh: vh.powerlaw(20000, seed=3), np.float64, "csr", dict(tol=1e-12, max_iters=60)),
+    "graph_chunks": (lambda vh: vh.convection_diffusion(30, 30, 30), np.float64, "csr",
+                     dict(tol=1e-10, max_iters=2000, use_graph=True, chunk=8)),
+}
+
+
+@pytest.mark.parametrize("case", list(BICG_COMPACT_CASES))
+def test_bicgstab_four_kernels_per_iteration_equals_six(dev, vh, oracle, case):
+    """Small unpreconditioned BiCGStab solves run four kernels per iteration (the alpha step at the
+    head of the s update, the scalar step at the head of the vector update, each workgroup reducing
+    the partials itself).  Against the six-kernel form (variant 1): the same iteration count, error,
+    residual history and x bit for bit, on a symmetric and a nonsymmetric operator, in FP32 on
+    SELL-64, when max_iters ends the solve (the final full update) and with hipGraph chunks; and the
+    oracle's pipelined BiCGStab iteration count within its envelope."""
+    make, dt, fmt, kw = BICG_COMPACT_CASES[case]
+    rp, col, val = make(vh)
+    n = rp.size - 1
+    b = vh.random_uniform(5, n).astype(dt)
+    if fmt == "csr":
+        A = vh.CsrMatrix(rp, col, val, dtype=dt)
+    else:
+        A = vh.SellMatrix(rp, col, val, 64, dtype=dt)
+    out = []
+    for variant in (0, 1):
+        db, dx = vh.DeviceArray.from_numpy(b), vh.DeviceArray.zeros(n, dt)
+        hist = np.zeros(kw["max_iters"])
+        res = vh.bicgstab(A, db, dx, variant=variant, history=hist, **kw)
+        out.append((dx.to_numpy(), res, hist))
+    (xc, rc, hc), (x6, r6, h6) = out
+    print(f"{case}: {rc.iters} iterations, kernels per iteration {rc.kernels_per_iter} / {r6.kernels_per_iter}")
+    assert rc.kernels_per_iter == 4 and r6.kernels_per_iter == 6
+    assert rc.iters == r6.iters and rc.error == r6.error and rc.converged == r6.converged
+    assert np.array_equal(xc, x6) and np.array_equal(hc, h6)
+    if case == "max_iters":
+        assert rc.iters == kw["max_iters"] and not rc.converged
+        return
+    assert rc.converged
+    if dt == np.float64:
+        H = oracle.CsrHost(rp, col, val)
+        oenv = oracle.iteration_envelope(lambda: oracle.bicgstab_pipelined(H, b, kw["tol"], kw["max_iters"]))
+        assert iters_in_envelope(rc.iters, oenv), (rc.iters, oenv)
+
+
+def test_bicgstab_compact_mode_only_for_small_systems(dev, vh):
+    """At 70^3 rows (343,000) every vector-update workgroup would read more than 2^20 partial sums per
+    iteration in all, so the solve keeps six kernels; a monitor keeps compact mode (it sees each
+    iteration after its vector update) and stops the solve where it asks to."""
+    rp, col, val = vh.laplace_3d(70)
+    n = rp.size - 1
+    A = vh.CsrMatrix(rp, col, val)
+    db = vh.DeviceArray.from_numpy(vh.random_uniform(2, n))
+    dx = vh.DeviceArray.zeros(n, np.float64)
+    res = vh.bicgstab(A, db, dx, tol=1e-6, max_iters=1000)
+    assert res.kernels_per_iter == 6 and res.converged
+    rp, col, val = vh.fdm_laplace_2d(128, 128)
+    n = rp.size - 1
+    A = vh.CsrMatrix(rp, col, val)
+    db = vh.DeviceArray.from_numpy(vh.random_uniform(3, n))
+    seen = []
+
+    @vh.MONITOR
+    def mon(x_dev, rel, user):
+        seen.append(rel)
+        return 1 if len(seen) == 25 else 0
+
+    dx = vh.DeviceArray.zeros(n, np.float64)
+    res = vh.bicgstab(A, db, dx, tol=1e-14, max_iters=1000, monitor=mon)
+    assert res.kernels_per_iter == 4 and res.iters == 25 and len(seen) == 25
+
+
+def test_solvers_refuse_vectors_of_another_precision_or_size(dev, vh):
+    """The C ABI takes plain device pointers, so the binding refuses, before any launch, a right-hand
+    side or solution whose precision differs from the matrix's or which is shorter than the matrix."""
+    rp, col, val = vh.fdm_laplace_2d(16, 16)
+    n = rp.size - 1
+    A64 = vh.CsrMatrix(rp, col, val)
+    S32 = vh.SellMatrix(rp, col, val, 64, dtype=np.float32)
+    b32, x32 = vh.DeviceArray.zeros(n, np.float32), vh.DeviceArray.zeros(n, np.float32)
+    b64, x64 = vh.DeviceArray.zeros(n, np.float64), vh.DeviceArray.zeros(n, np.float64)
+    short = vh.DeviceArray.zeros(n - 1, np.float64)
+    for fn in (vh.cg, vh.bicgstab, vh.gmres):
+        with pytest.raises(ValueError):
+            fn(A64, b32, x32)
+        with pytest.raises(ValueError):
+            fn(S32, b64, x64)
+        with pytest.raises(ValueError):
+            fn(A64, b64, short)
+    with pytest.raises(ValueError):
+        vh.mixed_cg(A64, b32, x32)

@@ -91,6 +91,61 @@ __global__ __launch_bounds__(kBlock) void stream_read_kernel(uint64_t n, const d
   if (threadIdx.x == 0) sums[lw] = s;
 }
 
+// Latency references (measurement only; nothing on the solver path uses them).  empty_kernel has
+// no memory work: its launch-to-launch time is the floor of a kernel boundary.  grid_sync_kernel
+// runs `steps` device-wide synchronisations of co-resident workgroups, the price that a persistent
+// (single-launch) solver would pay per iteration instead of a kernel boundary.  Per step, every
+// workgroup waits for its own stores, makes them visible to the other XCDs (agent-scope release:
+// its XCD's L2 written back) and publishes its arrival flag; workgroup 0 polls all flags, then
+// publishes a go flag that the others poll, and every workgroup acquires.  Every wait is bounded
+// (spin_limit polls); a workgroup that gives up raises the abort flag, which ends all other waits,
+// so the kernel always exits.  flags: G arrival flags, then the go flag, then the abort flag.
+__global__ __launch_bounds__(kBlock) void empty_kernel() {}
+
+__device__ __forceinline__ bool probe_spin(const uint32_t* word, uint32_t target, uint32_t* abort_flag,
+                                           uint32_t spin_limit)
+{
+  uint32_t spins = 0;
+  while (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u || ++spins > spin_limit) {
+      __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(kBlock) void grid_sync_kernel(uint32_t steps, uint32_t* flags, uint32_t spin_limit)
+{
+  __shared__ uint32_t ok;
+  const uint32_t G = gridDim.x;
+  uint32_t* const go = flags + G;
+  uint32_t* const abort_flag = flags + G + 1;
+  for (uint32_t k = 1; k <= steps; ++k) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_store(&flags[blockIdx.x], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 64) {
+      bool good = true;
+      for (uint32_t i = threadIdx.x; i < G && good; i += 64) good = probe_spin(&flags[i], k, abort_flag, spin_limit);
+      good = __all(good);
+      if (threadIdx.x == 0) {
+        if (good) __hip_atomic_store(go, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = good;
+      }
+    } else if (blockIdx.x != 0 && threadIdx.x == 0) {
+      ok = probe_spin(go, k, abort_flag, spin_limit);
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (!ok) return;
+  }
+}
+
 template <typename T, int W>
 __global__ __launch_bounds__(kBlock) void axpby_kernel(uint32_t n, T a, const T* x, T b, const T* y, T* z)
 {
@@ -436,6 +491,29 @@ int vclhip_stream_read_f64(uint64_t n, const double* x, double* sums, uint32_t n
   if (!aligned16({x})) return invalid("x must be 16-byte aligned");
   launch_timed(stream_read_kernel, dim3(num_sums), dim3(kBlock), static_cast<hipStream_t>(stream), n, x, sums);
   return check_launch("stream_read_kernel");
+}
+
+int vclhip_empty_launch(uint32_t workgroups, void* stream)
+{
+  if (workgroups == 0 || workgroups > (1u << 20)) return invalid("workgroups: 1 to 2^20");
+  hipLaunchKernelGGL(empty_kernel, dim3(workgroups), dim3(kBlock), 0, static_cast<hipStream_t>(stream));
+  return check_launch("empty_kernel");
+}
+
+int vclhip_grid_sync_probe(uint32_t workgroups, uint32_t steps, uint32_t* flags, void* stream)
+{
+  if (!flags) return invalid("null pointer");
+  if (workgroups == 0 || workgroups > 4096) return invalid("workgroups: 1 to 4096");
+  int per_cu = 0;
+  VCLHIP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, grid_sync_kernel, kBlock, 0));
+  if (static_cast<uint64_t>(workgroups) > static_cast<uint64_t>(per_cu) * device_cu_count())
+    return invalid("workgroups: more than the device holds at once");
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint64_t bytes = sizeof(uint32_t) * (static_cast<uint64_t>(workgroups) + 2);
+  VCLHIP_TRY(vclhip_fill_zero(flags, bytes, stream));
+  // about 10^7 polls of at least 64 cycles each: seconds, far beyond any step of a resident grid
+  hipLaunchKernelGGL(grid_sync_kernel, dim3(workgroups), dim3(kBlock), 0, s, steps, flags, 1u << 24);
+  return check_launch("grid_sync_kernel");
 }
 
 int vclhip_axpby_f64(uint32_t n, double a, const double* x, double b, const double* y, double* z, void* stream)

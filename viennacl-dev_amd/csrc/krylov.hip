@@ -781,6 +781,138 @@ __global__ __launch_bounds__(kBlock) void bicg_vec_update_kernel(uint32_t n, T* 
   block_store_partials<1, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
 }
 
+// ---- compact BiCGStab for small systems (unpreconditioned): four kernels per iteration instead of
+// six.  The alpha step runs at the head of the s update and the scalar step (alpha, beta, omega,
+// residual estimate, test) at the head of the vector update: every workgroup reduces the few partial
+// sums itself with block_sum_partials_k (the order of the scalar kernels, so every workgroup takes
+// the same decision with the same scalars, and the solve is the six-kernel one bit for bit);
+// workgroup 0 stores the state.  No workgroup reads a state field that another one writes in the
+// same kernel, except the flags at the start, where either value leads to the same action; the
+// iteration count is read from iter2[k & 1] and written to the other one.
+
+// bicg_alpha_kernel + bicg_update_s_kernel
+template <typename T, int W>
+__global__ __launch_bounds__(kBlock) void bicg_update_s_compact_kernel(uint32_t n, T* __restrict__ s,
+                                                                       const T* __restrict__ r,
+                                                                       const T* __restrict__ Ap,
+                                                                       KState* __restrict__ st,
+                                                                       double* __restrict__ part, uint32_t g_vec,
+                                                                       uint32_t g_prod)
+{
+  const uint32_t flags = st->flags;
+  if (flags & (kFlagDone | kFlagFinishing)) {
+    // a pending finish becomes done (workgroup 0; the others return on either value)
+    if (!(flags & kFlagDone) && blockIdx.x == 0 && threadIdx.x == 0) st->flags = flags | kFlagDone;
+    return;
+  }
+  const bool from_part = st->rr_from_partials != 0;
+  double sums[2];
+  block_sum_partials_k<2, double>({slot_ptr(part, 0), slot_ptr(part, 2)}, {from_part ? g_vec : 0u, g_prod}, sums);
+  const double rr = from_part ? sums[0] : st->rr;
+  const double ap_r0 = sums[1];
+  const double alpha_d = rr / ap_r0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rr = rr;
+    st->ap_r0 = ap_r0;
+    st->alpha = alpha_d;
+  }
+  const T alpha = static_cast<T>(alpha_d);
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  double acc[1] = {0};
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    const Pack<V, T> vr = ldp<V>(r, i);
+    const Pack<V, T> va = ldp<V>(Ap, i);
+    Pack<V, T> vs;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      vs.v[k] = vr.v[k] - alpha * va.v[k];
+      acc[0] += static_cast<double>(vs.v[k] * vs.v[k]);
+    }
+    stp<V>(s, i, vs);
+  });
+  block_store_partials<1, double>(acc, slot_ptr(part, 3), kMaxPartials, lw);
+}
+
+// bicg_scalars_kernel + bicg_vec_update_kernel.  A finish flag seen at the start can only be the one
+// workgroup 0 sets in this kernel (the s update turns an earlier one into done), so only done stops.
+template <typename T, int W>
+__global__ __launch_bounds__(kBlock) void bicg_vec_update_compact_kernel(
+    uint32_t n, T* __restrict__ x, T* __restrict__ p, const T* __restrict__ s, T* __restrict__ r,
+    const T* __restrict__ As, const T* __restrict__ Ap, const T* __restrict__ r0, KState* __restrict__ st,
+    double* __restrict__ part, uint32_t par, uint32_t g_vec, uint32_t g_prod)
+{
+  if (st->flags & kFlagDone) return;
+  double sums[4];
+  block_sum_partials_k<4, double>({slot_ptr(part, 3), slot_ptr(part, 4), slot_ptr(part, 5), slot_ptr(part, 6)},
+                                  {g_vec, g_prod, g_prod, g_prod}, sums);
+  const double s_dot_s = sums[0], As_dot_As = sums[1], As_dot_s = sums[2], As_dot_r0 = sums[3];
+  const double r_dot_r0 = st->rr;
+  const double Ap_dot_r0 = st->ap_r0;
+  const double alpha_d = r_dot_r0 / Ap_dot_r0;
+  const double beta_d = -As_dot_r0 / Ap_dot_r0;
+  const double omega_d = As_dot_s / As_dot_As;
+  const double res = sqrt(s_dot_s - 2.0 * omega_d * As_dot_s + omega_d * omega_d * As_dot_As);
+  const uint32_t iter = st->iter2[par] + 1;
+  const bool conv = fabs(res / st->nrs) < st->tol || res < st->abs_tol;
+  const uint32_t mode = conv ? 1u : (iter >= st->max_iters ? 2u : 0u);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->alpha = alpha_d;
+    st->beta = beta_d;
+    st->omega = omega_d;
+    st->res = res;
+    st->iter = iter;
+    st->iter2[par ^ 1u] = iter;
+    record_history(st, iter, res);
+    st->rr_from_partials = 1;
+    st->final_mode = mode;
+    if (conv) st->converged = 1;
+    if (mode != 0) st->flags |= kFlagFinishing;
+  }
+  const T alpha = static_cast<T>(alpha_d);
+  const T beta = static_cast<T>(beta_d);
+  const T omega = static_cast<T>(omega_d);
+  const uint32_t lw = xcd_logical_block();
+  uint32_t c0, c1;
+  chunk_range_w(n, W, lw, gridDim.x, c0, c1);
+  if (mode == 1) {
+    for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+      constexpr int V = decltype(w)::value;
+      Pack<V, T> vx = ldp<V>(x, i);
+      const Pack<V, T> vp = ldp<V>(p, i);
+      const Pack<V, T> vs = ldp<V>(s, i);
+#pragma unroll
+      for (int k = 0; k < V; ++k) vx.v[k] += alpha * vp.v[k] + omega * vs.v[k];
+      stp<V>(x, i, vx);
+    });
+    return;
+  }
+  double acc[1] = {0};
+  for_packs<W>(c0, c1, [&](uint32_t i, auto w) {
+    constexpr int V = decltype(w)::value;
+    Pack<V, T> vx = ldp<V>(x, i);
+    Pack<V, T> vp = ldp<V>(p, i);
+    const Pack<V, T> vs = ldp<V>(s, i);
+    const Pack<V, T> vas = ldp<V>(As, i);
+    const Pack<V, T> vap = ldp<V>(Ap, i);
+    const Pack<V, T> vr0 = ldp<V>(r0, i);
+    Pack<V, T> vr;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      vx.v[k] += alpha * vp.v[k] + omega * vs.v[k];
+      vr.v[k] = vs.v[k] - omega * vas.v[k];
+      vp.v[k] = vr.v[k] + beta * (vp.v[k] - omega * vap.v[k]);
+      acc[0] += static_cast<double>(vr.v[k] * vr0.v[k]);
+    }
+    stp<V>(x, i, vx);
+    stp<V>(r, i, vr);
+    stp<V>(p, i, vp);
+  });
+  block_store_partials<1, double>(acc, slot_ptr(part, 0), kMaxPartials, lw);
+}
+
 // ---- left-preconditioned BiCGStab with the reference's recurrences (bicgstab.hpp:447-480):
 // explicit <r,r0*> and ||r|| after the x/r update, beta = <r,r0*>_new / <r,r0*> * alpha / omega.
 
@@ -1442,7 +1574,24 @@ static int bicgstab_solve(const M* Ain, const T* b, T* x, int precond, const vcl
     VCLHIP_LAUNCH_W(v2, (bicgc_p_kernel<T, kVecW<T> >), (bicgc_p_kernel<T, 1>), gv, s, n, p, r, Ap, st);
     return check_launch("bicgstab iteration");
   };
-  auto launch_iter = [&](uint32_t) -> int {
+  // Compact mode (unpreconditioned, params->variant 0): four kernels per iteration where every
+  // vector-update workgroup can afford to reduce all partials itself (as CG's compact mode); not with
+  // hipGraph chunks of odd length (the captured parities must alternate across replays).
+  const uint32_t chunk = P->chunk ? P->chunk : 16;
+  const bool compact = !jac && P->variant == 0 &&
+                       static_cast<uint64_t>(gv) * (gv + 3ull * op.grid) <= kCompactPartialReads &&
+                       !(P->use_graph && (chunk & 1u) && !P->monitor);
+  auto launch_iter_compact = [&](uint32_t k) -> int {
+    op.launch(s, p, EpiBicg<T, false, false>{Ap, p, r0, d}, slot_ptr(part, 2), flags, stop_all);
+    VCLHIP_LAUNCH_W(v2, (bicg_update_s_compact_kernel<T, kVecW<T> >), (bicg_update_s_compact_kernel<T, 1>), gv, s,
+                    n, sv, r, Ap, st, part, gv, op.grid);
+    op.launch(s, sv, EpiBicg<T, false, true>{As, sv, r0, d}, slot_ptr(part, 4), flags, stop_all);
+    VCLHIP_LAUNCH_W(v2, (bicg_vec_update_compact_kernel<T, kVecW<T> >), (bicg_vec_update_compact_kernel<T, 1>), gv,
+                    s, n, x, p, sv, r, As, Ap, r0, st, part, k & 1u, gv, op.grid);
+    return check_launch("bicgstab iteration");
+  };
+  auto launch_iter = [&](uint32_t k) -> int {
+    if (compact) return launch_iter_compact(k);
     if (classic) return launch_classic();
     if (jac)
       op.launch(s, p, EpiBicg<T, true, false>{Ap, p, r0, d}, slot_ptr(part, 2), flags, stop_all);
@@ -1495,7 +1644,7 @@ static int bicgstab_solve(const M* Ain, const T* b, T* x, int precond, const vcl
   R->restarts = restarts;
   R->converged = static_cast<int>(fin.converged);
   R->operator_used = used_dia ? VCLHIP_USED_DIA : VCLHIP_USED_AS_GIVEN;
-  R->kernels_per_iter = 0;
+  R->kernels_per_iter = compact ? 4u : (classic ? 8u : 6u);
   return copy_history(P, hist, fin.iter, s);
 }
 

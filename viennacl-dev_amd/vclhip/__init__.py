@@ -387,7 +387,18 @@ def solver_params(tol=1e-8, max_iters=300, abs_tol=0.0, max_iters_before_restart
     return p
 
 
+def _check_vectors(A, *vecs):
+    """The C ABI takes plain device pointers, so a right-hand side or solution of another precision or
+    shorter than the matrix would be read past its end: refuse it on the host."""
+    for v in vecs:
+        if v.dtype != A.dtype:
+            raise ValueError(f"vector of dtype {v.dtype} for a matrix of dtype {A.dtype}")
+        if v.size < A.n:
+            raise ValueError(f"vector of {v.size} elements for a matrix of {A.n} rows")
+
+
 def _solve(kind, A, b, x, precond, params, stream):
+    _check_vectors(A, b, x)
     res = SolverResult()
     if isinstance(A, (CsrMatrix, SellMatrix)) and A.dtype == np.float64:
         mat = "csr" if isinstance(A, CsrMatrix) else "sell"
@@ -414,6 +425,7 @@ def mixed_cg(A, b, x, precond=PRECOND_NONE, values32=None, stream=None, **kw):
     """solve(A, b, mixed_precision_cg_tag): FP32 inner CG, FP64 corrections (mixed_mode 0 = the
     reference's restart schedule, 1 = reliable updates).  A holds FP64 values; values32 is an
     optional DeviceArray with their FP32 copy.  result.restarts counts the FP64 corrections."""
+    _check_vectors(A, b, x)
     params = solver_params(**kw)
     res = SolverResult()
     mat = "csr" if isinstance(A, CsrMatrix) else "sell"
@@ -427,6 +439,7 @@ def gmres(A, b, x, stream=None, **kw):
     SolverResult (iters = Krylov steps, error = the last cycle's estimate, restarts = cycles run).
     The gmres_tag defaults are tol=1e-10, max_iters=300, krylov_dim=20."""
     kw.setdefault("tol", 1e-10)
+    _check_vectors(A, b, x)
     params = solver_params(**kw)
     res = SolverResult()
     prec = "f32" if A.dtype == np.float32 else "f64"
