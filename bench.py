@@ -688,9 +688,9 @@ def profile_workloads(vh, names, steps, warmup):
             out[name] = {"iters": int(res.iters), "operator_used": int(res.operator_used)}
             for a in (b, x, A):
                 a.free()
-        elif name in ("c1_cg", "c2_cg", "c2_bicgstab", "c3_cg_fp64", "c3_mixed"):
+        elif name in ("c1_cg", "c1_bicgstab", "c2_cg", "c2_bicgstab", "c3_cg_fp64", "c3_mixed"):
             # one solve of the configuration as the bench line times it
-            if name == "c1_cg":
+            if name.startswith("c1"):
                 rp, col, val = vh.fdm_laplace_2d(256, 256)
                 kw = dict(tol=1e-8, max_iters=2000)
             elif name.startswith("c2"):
@@ -703,7 +703,7 @@ def profile_workloads(vh, names, steps, warmup):
             n = rp.size - 1
             A = vh.CsrMatrix(rp, col, val)
             b, x = vh.DeviceArray.from_numpy(vh.random_uniform(0, n)), vh.DeviceArray.zeros(n, np.float64)
-            if name == "c2_bicgstab":
+            if name.endswith("bicgstab"):
                 res = vh.bicgstab(A, b, x, stream=stream.handle, **kw)
             elif name == "c3_mixed":
                 res = vh.mixed_cg(A, b, x, stream=stream.handle, mixed_mode=1, **kw)
@@ -765,7 +765,7 @@ def main():
                          "SELL-64 SpMV of the 128^3 Laplacian and the streaming kernels), powerlaw (CSR SpMV of the "
                          "power-law matrix), dia_cg (50 FP64 CG iterations on the 160^3 diffusion operator's DIA "
                          "copy), batched (one FP64 batched CG solve of the 4096 systems), and one solve of a "
-                         "configuration: c1_cg, c2_cg, c2_bicgstab, c3_cg_fp64, c3_mixed")
+                         "configuration: c1_cg, c1_bicgstab, c2_cg, c2_bicgstab, c3_cg_fp64, c3_mixed")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the sharded (row-partitioned) path is not built; run with --gpus 1.",
