@@ -545,11 +545,23 @@ def test_cg_one_kernel_per_iteration_equals_two(dev, vh, oracle, case):
     assert np.linalg.norm(r) / np.linalg.norm(b) < bound
 
 
-def test_cg_fused_mode_only_when_grid_fits_one_round(dev, vh):
-    """One kernel per iteration pays only while its grid is resident at once (two workgroups per CU
-    with the fused kernel's registers): a 392 x 392 Laplacian (608 row blocks of 256 rows, more
-    than twice the 256 CUs) keeps the two-kernel compact mode, with the same results as variant 1."""
-    rp, col, val = vh.fdm_laplace_2d(392, 392)
+FUSED_BOUND_CASES = {
+    # (operator, kernels per iteration expected): grid x (vector + 2 product partials) per iteration
+    "lap2d_350": (lambda vh: vh.fdm_laplace_2d(350, 350), 1),  # 480 x 1080 = 518 K reads
+    "lap2d_392": (lambda vh: vh.fdm_laplace_2d(392, 392), 2),  # 608 x 1368 = 832 K
+    "lap3d_40": (lambda vh: vh.laplace_3d(40), 1),  # 256 x 576 = 147 K
+    "lap3d_50": (lambda vh: vh.laplace_3d(50), 2),  # 496 x 1120 = 555 K
+}
+
+
+@pytest.mark.parametrize("case", list(FUSED_BOUND_CASES))
+def test_cg_fused_mode_only_below_its_partial_read_bound(dev, vh, case):
+    """One kernel per iteration pays while the partial sums that all its workgroups reduce stay few:
+    at most 2^19 reads per iteration (grid x (vector + 2 product partials)), with the grid resident
+    at once.  Above it the solve keeps the two-kernel compact mode; either way the results equal
+    variant 1's bit for bit."""
+    make, want = FUSED_BOUND_CASES[case]
+    rp, col, val = make(vh)
     n = rp.size - 1
     b = vh.random_uniform(9, n)
     A = vh.CsrMatrix(rp, col, val)
@@ -559,8 +571,8 @@ def test_cg_fused_mode_only_when_grid_fits_one_round(dev, vh):
         res = vh.cg(A, db, dx, tol=1e-8, max_iters=3000, variant=variant)
         out.append((dx.to_numpy(), res))
     (x0, r0), (x1, r1) = out
-    assert r0.operator_used == vh.USED_DIA and r0.kernels_per_iter == 2 and r1.kernels_per_iter == 2
-    assert r0.converged and r0.iters == r1.iters and np.array_equal(x0, x1)
+    assert r0.operator_used == vh.USED_DIA and r0.kernels_per_iter == want and r1.kernels_per_iter == 2
+    assert r0.converged and r0.iters == r1.iters and r0.error == r1.error and np.array_equal(x0, x1)
 
 
 BICG_COMPACT_CASES = {

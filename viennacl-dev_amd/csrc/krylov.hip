@@ -39,6 +39,11 @@ constexpr uint32_t kSlots = 8;  // partial-sum slots of kMaxPartials doubles eac
 // (1 Mi: the 512^2 diffusion operator, 256 update workgroups reading 2304 partials each, runs 9%
 // faster than with three kernels; at n = 1 M the reads would cost more than the kernel saved).
 constexpr uint64_t kCompactPartialReads = 1ull << 20;
+// Fused CG (one kernel per iteration): every workgroup of the grid reduces all partials, so the
+// bound is on grid x partials.  Measured (CG on 2-D and 3-D Laplacians, one kernel against two):
+// 8.2 against 10.6 us per iteration at 147 K reads (256^2), 11.0 against 12.0 at 518 K (350^2),
+// even at 555-832 K (50^3, 392^2), 17.6 against 14.6 at 1.4 M (450^2).
+constexpr uint64_t kFusedPartialReads = 1ull << 19;
 
 struct KState {
   double alpha, beta, omega;
@@ -1447,15 +1452,17 @@ static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_so
                       Ap, d, st, part, k & 1u, gv, op.grid);
     return check_launch("cg iteration");
   };
-  // Fused mode: one kernel per iteration (cg_fused_kernel) where compact mode runs on a DIA operator
-  // whose grid is resident in one round (the kernel's registers leave room for two workgroups per
-  // CU; a second round costs more than the launch it saves), unless params->variant is 1.  Its
-  // second copies of r, p and A p are optional memory: without them the solve stays in compact
-  // mode (the same results either way).
+  // Fused mode: one kernel per iteration (cg_fused_kernel) where compact mode runs on a DIA operator,
+  // all workgroups together read at most kFusedPartialReads partial sums per iteration, and the grid
+  // is resident at once (the kernel's registers leave room for three workgroups per CU; a second
+  // round costs more than the launch it saves), unless params->variant is 1.  Its second copies of
+  // r, p and A p are optional memory: without them the solve stays in compact mode (the same results
+  // either way).
   T *r2 = nullptr, *p2 = nullptr, *Ap2 = nullptr;
   const uint32_t g_fused = std::max(gv, op.grid);
   bool fused = compact && op.fmt == OpFormat::Dia && P->variant != 1 &&
-               g_fused <= 2u * static_cast<uint32_t>(device_cu_count());
+               static_cast<uint64_t>(g_fused) * (gv + 2ull * op.grid) <= kFusedPartialReads &&
+               g_fused <= 3u * static_cast<uint32_t>(device_cu_count());
   if (fused) fused = ws.try_alloc(&r2, n) && ws.try_alloc(&p2, n) && ws.try_alloc(&Ap2, n);
   auto launch_iter_fused = [&](uint32_t k) -> int {
     const uint32_t par = k & 1u;
