@@ -629,7 +629,9 @@ typedef struct vclhip_batch_plan vclhip_batch_plan;
  * kernels' partition, vclhip_csr_plan_blocks), which the staged SpMV reads, and that SpMV is the
  * choice where untuned; the batch matrix passed to the solves must then have these row pointers.
  * Without row_ptr every system takes L lanes per row (about one lane per nonzero of an average
- * row).  Allocates device memory, among it an A p workspace of total rows doubles: one solve at a
+ * row).  Allocates device memory, among it an A p workspace of total rows doubles and, at the
+ * first solve or autotune, 2 bytes per matrix entry for the column indices as 16-bit offsets from
+ * their system's first row (refilled from A at every solve; the SpMVs read those): one solve at a
  * time per plan. */
 int vclhip_batch_plan_create(uint32_t num_systems, const uint32_t* sys_ptr_host, const uint32_t* row_ptr_host,
                              vclhip_batch_plan** plan);

@@ -11,6 +11,12 @@
 // (the CSR kernel's scheme: a block's up to 2048 entries read with coalesced loads, the products
 // a_ij p_j put in LDS, the rows summed from there), which keeps many more loads in flight.  The
 // recurrences are the pipelined CG of cg.hpp:128-187, one system at a time.
+//
+// The SpMVs read the column indices as 16-bit offsets from the system's first row (a system has at
+// most 4096 rows), written by one pass over the caller's 32-bit indices at the start of every solve:
+// the matrix stream, re-read in every iteration, shrinks from 12 to 10 bytes per entry in FP64 and
+// from 8 to 6 in FP32, for 6 bytes per entry once.  A column outside the system's block becomes
+// 0xFFFF, which no row of the system reaches (the invalid-system flag, as before).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -31,6 +37,8 @@ struct vclhip_batch_plan {
   uint32_t* blocks_dev = nullptr;     // row blocks of every system (staged SpMV), global rows
   uint32_t* sys_blk_dev = nullptr;    // system s owns blocks sys_blk[s] .. sys_blk[s+1]-1
   double* ap_dev = nullptr;           // A p workspace, total_rows entries
+  uint16_t* col16_dev = nullptr;      // column offsets within each system, col16_cap entries
+  size_t col16_cap = 0;
   uint32_t* counter_dev = nullptr;    // work queue head
   int lanes_log2[16][2];              // per size class (log2 rows) and value type (0 f64, 1 f32); -1 = untuned
                                       // 0..4: 2^l lanes per row, 5: staged over row blocks
@@ -52,7 +60,7 @@ template <typename T>
 struct BatchDev {
   const uint32_t* sys_ptr;
   const uint32_t* row_ptr;
-  const uint32_t* col;
+  const uint16_t* col16;  // column - first row of the system, 0xFFFF outside its block
   const T* val;
   const T* b;
   T* x;
@@ -107,7 +115,7 @@ __device__ __forceinline__ void batch_spmv(const BatchDev<T>& B, uint32_t r0, ui
     const uint32_t a = B.row_ptr[r0 + i], e = B.row_ptr[r0 + i + 1];
     T s = 0;
     for (uint32_t k = a + j; k < e; k += L) {
-      const uint32_t c = B.col[k] - r0;  // unsigned: a column left of the block wraps to a large value
+      const uint32_t c = B.col16[k];
       if (c < n)
         s += B.val[k] * p[c];
       else
@@ -148,7 +156,7 @@ __device__ __forceinline__ void batch_spmv_staged(const BatchDev<T>& B, uint32_t
     if (mode == kModeLongRow) {
       T sum = 0;
       for (uint32_t e = e0 + tid; e < e1; e += kBlock) {
-        const uint32_t c = B.col[e] - rs;
+        const uint32_t c = B.col16[e];
         if (c < n)
           sum += B.val[e] * p[c];
         else
@@ -178,7 +186,7 @@ __device__ __forceinline__ void batch_spmv_staged(const BatchDev<T>& B, uint32_t
 #pragma unroll
       for (int k = 0; k < U; ++k) {
         const uint32_t e = e0 + min(tid + k * kBlock, cnt - 1);
-        c[k] = B.col[e] - rs;
+        c[k] = B.col16[e];
         v[k] = B.val[e];
       }
 #pragma unroll
@@ -378,6 +386,26 @@ __global__ void batch_reset_kernel(uint32_t* counter)
   if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0;
 }
 
+// col16[e] = col[e] - (first row of e's system), 0xFFFF where that is outside the system; entries
+// at or beyond `cap` (the buffer's size) are not written, whatever the row pointers say.
+__global__ __launch_bounds__(kBlock) void batch_local_cols_kernel(const uint32_t* __restrict__ sys_ptr,
+                                                                  const uint32_t* __restrict__ row_ptr,
+                                                                  const uint32_t* __restrict__ col,
+                                                                  uint16_t* __restrict__ col16, uint32_t num_systems,
+                                                                  uint32_t cap)
+{
+  for (uint32_t s = blockIdx.x; s < num_systems; s += gridDim.x) {
+    const uint32_t rs = sys_ptr[s], re = sys_ptr[s + 1];
+    const uint32_t n = re - rs;
+    const uint32_t e0 = row_ptr[rs];
+    const uint32_t e1 = min(row_ptr[re], cap);
+    for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
+      const uint32_t c = col[e] - rs;  // unsigned: a column left of the block wraps to a large value
+      col16[e] = c < n ? static_cast<uint16_t>(c) : static_cast<uint16_t>(0xFFFFu);
+    }
+  }
+}
+
 static int size_class(uint32_t n)
 {
   int c = 0;
@@ -393,6 +421,27 @@ static int persistent_grid(int* grid)
   if (per_cu < 1) per_cu = 1;
   *grid = per_cu * device_cu_count();
   return VCLHIP_SUCCESS;
+}
+
+// Makes room for A's column offsets (outside any capture: it may allocate).
+static int reserve_cols(vclhip_batch_plan* plan, const vclhip_csr* A)
+{
+  const size_t need = std::max<size_t>(A->nnz, 1);
+  if (plan->col16_dev && plan->col16_cap >= need) return VCLHIP_SUCCESS;
+  if (plan->col16_dev) (void)hipFree(plan->col16_dev);
+  plan->col16_dev = nullptr;
+  plan->col16_cap = 0;
+  VCLHIP_HIP(hipMalloc(&plan->col16_dev, need * sizeof(uint16_t)));
+  plan->col16_cap = need;
+  return VCLHIP_SUCCESS;
+}
+
+static int local_cols(const vclhip_batch_plan* plan, const vclhip_csr* A, const uint32_t* sys_ptr, hipStream_t s)
+{
+  const uint32_t grid = std::min<uint32_t>(plan->num_systems, 4096u);
+  hipLaunchKernelGGL(batch_local_cols_kernel, dim3(grid), dim3(kBlock), 0, s, sys_ptr, A->row_ptr, A->col_idx,
+                     plan->col16_dev, plan->num_systems, static_cast<uint32_t>(std::min<size_t>(plan->col16_cap, A->nnz)));
+  return check_launch("batch_local_cols_kernel");
 }
 
 template <typename T>
@@ -423,7 +472,7 @@ static BatchDev<T> make_batch(const vclhip_batch_plan* plan, const vclhip_csr* A
   BatchDev<T> B;
   B.sys_ptr = sys_ptr;
   B.row_ptr = A->row_ptr;
-  B.col = A->col_idx;
+  B.col16 = plan->col16_dev;
   B.val = static_cast<const T*>(A->values);
   B.b = b;
   B.x = x;
@@ -481,6 +530,8 @@ static int autotune(vclhip_batch_plan* plan, const vclhip_csr* A, const uint32_t
   }
   int grid = 0;
   if (!status) status = persistent_grid<T>(&grid);
+  if (!status) status = reserve_cols(plan, A);
+  if (!status) status = local_cols(plan, A, sys_ptr, s);
   vclhip_solver_params P;
   vclhip_solver_params_default(&P);
   P.tol = 0;
@@ -567,14 +618,21 @@ static int batched_solve(vclhip_batch_plan* plan, const vclhip_csr* A, const uin
   int grid = 0;
   VCLHIP_TRY(persistent_grid<T>(&grid));
   const uint32_t g = static_cast<uint32_t>(std::min<int64_t>(grid, plan->num_systems));
+  const uint16_t* const col16_before = plan->col16_dev;
+  VCLHIP_TRY(reserve_cols(plan, A));
+  if (plan->col16_dev != col16_before) drop_graph(plan);
   const BatchDev<T> B = make_batch<T>(plan, A, sys_ptr, b, x, P, iters, err, conv);
-  if (!P->use_graph || !s) return launch_batched(B, g, s);
+  if (!P->use_graph || !s) {
+    VCLHIP_TRY(local_cols(plan, A, sys_ptr, s));
+    return launch_batched(B, g, s);
+  }
   // graph mode: capture once per argument set, replay afterwards
   std::vector<uintptr_t> key = {reinterpret_cast<uintptr_t>(A->row_ptr), reinterpret_cast<uintptr_t>(A->col_idx),
                                 reinterpret_cast<uintptr_t>(A->values), reinterpret_cast<uintptr_t>(sys_ptr),
                                 reinterpret_cast<uintptr_t>(b), reinterpret_cast<uintptr_t>(x),
                                 reinterpret_cast<uintptr_t>(iters), reinterpret_cast<uintptr_t>(err),
-                                reinterpret_cast<uintptr_t>(conv), sizeof(T), P->max_iters};
+                                reinterpret_cast<uintptr_t>(conv), reinterpret_cast<uintptr_t>(plan->col16_dev),
+                                static_cast<uintptr_t>(A->nnz), sizeof(T), P->max_iters};
   uint64_t tb, ab;
   std::memcpy(&tb, &P->tol, 8);
   std::memcpy(&ab, &P->abs_tol, 8);
@@ -583,7 +641,8 @@ static int batched_solve(vclhip_batch_plan* plan, const vclhip_csr* A, const uin
   if (!plan->gexec || plan->gkey != key) {
     drop_graph(plan);
     VCLHIP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const int st = launch_batched(B, g, s);
+    int st = local_cols(plan, A, sys_ptr, s);
+    if (!st) st = launch_batched(B, g, s);
     hipGraph_t graph = nullptr;
     const int st2 = hip_status(hipStreamEndCapture(s, &graph), "hipStreamEndCapture");
     if (st || st2) {
@@ -683,7 +742,7 @@ int vclhip_batch_plan_destroy(vclhip_batch_plan* plan)
     if (l) (void)hipFree(l);
   if (plan->counter_dev) (void)hipFree(plan->counter_dev);
   for (void* q : {static_cast<void*>(plan->blocks_dev), static_cast<void*>(plan->sys_blk_dev),
-                  static_cast<void*>(plan->ap_dev)})
+                  static_cast<void*>(plan->ap_dev), static_cast<void*>(plan->col16_dev)})
     if (q) (void)hipFree(q);
   delete plan;
   return VCLHIP_SUCCESS;
