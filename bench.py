@@ -254,11 +254,21 @@ def powerlaw_bench(vh, stream, steps):
     evict = vh.DeviceArray(EVICT_DOUBLES, np.float64)
     vh.call("vclhip_fill_zero", evict.ptr, C.c_size_t(evict.nbytes), None)
     cold, warm, _ = time_launches(vh, lambda: A.spmv(x, y, stream=stream.handle), stream, steps, 3, evict)
+    # the x gathers alone (the matrix's own column indices, the SpMV's launch shape), cold as well
+    g = (A.num_blocks + 7) // 8 * 8
+    sums = vh.DeviceArray.zeros(g, np.float64)
+    gcold, gwarm, _ = time_launches(vh, lambda: vh.call("vclhip_gather_read_f64", nnz, A.d_col.ptr, x.ptr, n, sums.ptr,
+                                                        g, stream.handle), stream, steps, 3, evict)
+    sums.free()
     evict.free()
     by = csr_bytes(n, nnz)
     out = {"workload": f"power-law nonsymmetric matrix n={n} nnz={nnz}, CSR, tol 1e-10", "n": n, "nnz": nnz,
            "spmv_cold_us": round(float(cold.mean()) * 1e3, 2), "spmv_cold_gbs": round(float(by / (cold.mean() * 1e-3) / 1e9), 1),
-           "spmv_warm_gbs": round(by / (warm * 1e-3) / 1e9, 1), "algorithmic_bytes_per_spmv": by}
+           "spmv_warm_gbs": round(by / (warm * 1e-3) / 1e9, 1), "algorithmic_bytes_per_spmv": by,
+           "gather_floor": {"kernel": "gather_read_kernel", "what": "x[col[e]] summed over every entry, the SpMV's "
+                            "column indices and launch shape, no values, row pointers or y",
+                            "cold_us": round(float(gcold.mean()) * 1e3, 2), "warm_us": round(gwarm * 1e3, 2),
+                            "spmv_over_gathers": round(float(cold.mean() / gcold.mean()), 3)}}
     b = vh.DeviceArray.from_numpy(vh.random_uniform(0, n))
     # without a preconditioner BiCGStab stagnates on this badly scaled matrix; 400 iterations show it
     out["bicgstab_fp64_csr_400"] = timed_solve(vh, "bicgstab", A, b, n, reps=1, tol=1e-10, max_iters=400)

@@ -83,6 +83,13 @@ int vclhip_time_next_launch(void* start_event, void* stop_event);
  * loads and store its sum in sums[0..num_sums).  x must be 16-byte aligned.  Its kernel time over
  * 8 n bytes is the rate at which the device reads HBM; vclhip_time_next_launch applies to it. */
 int vclhip_stream_read_f64(uint64_t n, const double* x_dev, double* sums_dev, uint32_t num_sums, void* stream);
+/* Gather reference (no counterpart in the reference; measurement only): num_sums workgroups (a
+ * multiple of 8, at most 2^20) each sum x[idx[i]] over a contiguous chunk of idx[0..m) and store
+ * the sum in sums[0..num_sums); n is the length of x (an index at or beyond it reads x[0]).
+ * With a CSR matrix's column indices as idx, its kernel time is that of the SpMV's x gathers
+ * alone; vclhip_time_next_launch applies to it. */
+int vclhip_gather_read_f64(uint32_t m, const uint32_t* idx_dev, const double* x_dev, uint32_t n, double* sums_dev,
+                           uint32_t num_sums, void* stream);
 /* Latency references (no counterpart in the reference; measurement only).  vclhip_empty_launch
  * launches a kernel of `workgroups` workgroups that does nothing: back to back, its launches time
  * the floor of a kernel boundary.  vclhip_grid_sync_probe launches `workgroups` workgroups (at most

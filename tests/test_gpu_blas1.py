@@ -165,3 +165,23 @@ def test_latency_probes(dev, vh):
     with pytest.raises(vh.VclHipError) as e:
         vh.call("vclhip_grid_sync_probe", 4097, 1, flags.ptr, None)
     assert e.value.status == vh.ERROR_INVALID_VALUE
+
+
+def test_gather_reference(dev, vh):
+    """vclhip_gather_read_f64 (the power-law SpMV's gather floor in the bench): workgroup w sums
+    x[idx[i]] over the w-th contiguous chunk of idx; an index at or beyond n reads x[0]."""
+    n, m, g = 5000, 123457, 64
+    rng = np.random.default_rng(3)
+    idx = rng.integers(0, n, m).astype(np.uint32)
+    idx[::1000] = n + 7  # out of range: x[0]
+    x = vh.random_uniform(4, n)
+    di, dx = vh.DeviceArray.from_numpy(idx), vh.DeviceArray.from_numpy(x)
+    sums = vh.DeviceArray.zeros(g, np.float64)
+    vh.call("vclhip_gather_read_f64", m, di.ptr, dx.ptr, n, sums.ptr, g, None)
+    got = sums.to_numpy()
+    safe = np.where(idx < n, idx, 0)
+    bounds = [m * w // g for w in range(g + 1)]
+    want = np.array([x[safe[bounds[w]:bounds[w + 1]]].sum() for w in range(g)])
+    scale = np.array([np.abs(x[safe[bounds[w]:bounds[w + 1]]]).sum() for w in range(g)])
+    # sums[w] is chunk w's sum (different summation order: bounded by the absolute sum)
+    assert np.all(np.abs(got - want) <= 1e-12 * scale)

@@ -91,6 +91,40 @@ __global__ __launch_bounds__(kBlock) void stream_read_kernel(uint64_t n, const d
   if (threadIdx.x == 0) sums[lw] = s;
 }
 
+// Gather reference: workgroup lw sums x[idx[i]] over the lw-th contiguous chunk of idx, eight
+// indices per lane loaded before their gathers, and stores one partial sum.  With a matrix's column
+// indices as idx it times the x gathers of a CSR SpMV of that matrix without the values, the row
+// pointers, the row sums or the y stores: the floor that random columns set for the SpMV.
+// Measurement only; nothing on the solver path uses it.
+__global__ __launch_bounds__(kBlock) void gather_read_kernel(uint32_t m, const uint32_t* __restrict__ idx,
+                                                             const double* __restrict__ x, uint32_t n,
+                                                             double* __restrict__ sums)
+{
+  // an index at or beyond n reads x[0] instead
+  const uint32_t lw = xcd_logical_block();
+  const uint64_t G = gridDim.x;
+  const uint32_t b = static_cast<uint32_t>(static_cast<uint64_t>(m) * lw / G);
+  const uint32_t e = static_cast<uint32_t>(static_cast<uint64_t>(m) * (lw + 1) / G);
+  double s = 0;
+  uint32_t i = b + threadIdx.x;
+  for (; i + static_cast<uint32_t>(kStreamU - 1) * kBlock < e; i += static_cast<uint32_t>(kStreamU) * kBlock) {
+    uint32_t c[kStreamU];
+#pragma unroll
+    for (int u = 0; u < kStreamU; ++u) c[u] = idx[i + static_cast<uint32_t>(u) * kBlock];
+    double v[kStreamU];
+#pragma unroll
+    for (int u = 0; u < kStreamU; ++u) v[u] = x[c[u] < n ? c[u] : 0u];
+#pragma unroll
+    for (int u = 0; u < kStreamU; ++u) s += v[u];
+  }
+  for (; i < e; i += kBlock) {
+    const uint32_t c = idx[i];
+    s += x[c < n ? c : 0u];
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) sums[lw] = s;
+}
+
 // Latency references (measurement only; nothing on the solver path uses them).  empty_kernel has
 // no memory work: its launch-to-launch time is the floor of a kernel boundary.  grid_sync_kernel
 // runs `steps` device-wide synchronisations of co-resident workgroups, the price that a persistent
@@ -491,6 +525,17 @@ int vclhip_stream_read_f64(uint64_t n, const double* x, double* sums, uint32_t n
   if (!aligned16({x})) return invalid("x must be 16-byte aligned");
   launch_timed(stream_read_kernel, dim3(num_sums), dim3(kBlock), static_cast<hipStream_t>(stream), n, x, sums);
   return check_launch("stream_read_kernel");
+}
+
+int vclhip_gather_read_f64(uint32_t m, const uint32_t* idx, const double* x, uint32_t n, double* sums,
+                           uint32_t num_sums, void* stream)
+{
+  if (!idx || !x || !sums) return invalid("null pointer");
+  if (num_sums < 8 || num_sums > (1u << 20) || (num_sums & 7u)) return invalid("num_sums: a multiple of 8 in [8, 2^20]");
+  if (n == 0) return invalid("x must hold at least one element");
+  launch_timed(gather_read_kernel, dim3(num_sums), dim3(kBlock), static_cast<hipStream_t>(stream), m, idx, x, n,
+               sums);
+  return check_launch("gather_read_kernel");
 }
 
 int vclhip_empty_launch(uint32_t workgroups, void* stream)

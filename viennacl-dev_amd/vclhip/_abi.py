@@ -104,6 +104,7 @@ _SIGS = {
     "vclhip_memcpy_d2d": (i32, [vp, vp, sz, vp]),
     "vclhip_fill_zero": (i32, [vp, sz, vp]),
     "vclhip_stream_read_f64": (i32, [u64, vp, vp, u32, vp]),
+    "vclhip_gather_read_f64": (i32, [u32, vp, vp, u32, vp, u32, vp]),
     "vclhip_empty_launch": (i32, [u32, vp]),
     "vclhip_grid_sync_probe": (i32, [u32, u32, vp, vp]),
     "vclhip_stream_create": (i32, [P(vp)]),
