@@ -817,7 +817,7 @@ __global__ __launch_bounds__(kBlock) void bicg_update_s_compact_kernel(uint32_t 
   const double ap_r0 = sums[1];
   const double alpha_d = rr / ap_r0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->rr = rr;
+    if (from_part) st->rr = rr;  // otherwise it is the value every workgroup read
     st->ap_r0 = ap_r0;
     st->alpha = alpha_d;
   }
