@@ -547,10 +547,10 @@ def test_cg_one_kernel_per_iteration_equals_two(dev, vh, oracle, case):
 
 FUSED_BOUND_CASES = {
     # (operator, kernels per iteration expected): grid x (vector + 2 product partials) per iteration
-    "lap2d_350": (lambda vh: vh.fdm_laplace_2d(350, 350), 1),  # 480 x 1080 = 518 K reads
-    "lap2d_392": (lambda vh: vh.fdm_laplace_2d(392, 392), 2),  # 608 x 1368 = 832 K
-    "lap3d_40": (lambda vh: vh.laplace_3d(40), 1),  # 256 x 576 = 147 K
-    "lap3d_50": (lambda vh: vh.laplace_3d(50), 2),  # 496 x 1120 = 555 K
+    "lap2d_330": (lambda vh: vh.fdm_laplace_2d(330, 330), 1),  # 432 x (256 + 864) = 484 K reads
+    "lap2d_392": (lambda vh: vh.fdm_laplace_2d(392, 392), 2),  # 608 x 1472 = 895 K
+    "lap3d_40": (lambda vh: vh.laplace_3d(40), 1),  # 256 x 768 = 197 K
+    "lap3d_50": (lambda vh: vh.laplace_3d(50), 2),  # 496 x 1248 = 619 K
 }
 
 

@@ -576,7 +576,7 @@ int gmres_solve(const M* Ain, const T* b, T* x, const vclhip_solver_params* P, v
   VCLHIP_HIP(hipMemcpyAsync(st, &h, sizeof(GState), hipMemcpyHostToDevice, s));
   VCLHIP_HIP(hipStreamSynchronize(s));
 
-  GmLaunch<T> L{n, ld, vec_grid_n(n), aligned16({b, x})};
+  GmLaunch<T> L{n, ld, solver_vec_grid(n), aligned16({b, x})};
   const uint32_t* stop = &st->flags;
   double* pnorm = part + static_cast<size_t>(K) * kMaxPartials;
 

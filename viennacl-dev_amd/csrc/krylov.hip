@@ -40,9 +40,10 @@ constexpr uint32_t kSlots = 8;  // partial-sum slots of kMaxPartials doubles eac
 // faster than with three kernels; at n = 1 M the reads would cost more than the kernel saved).
 constexpr uint64_t kCompactPartialReads = 1ull << 20;
 // Fused CG (one kernel per iteration): every workgroup of the grid reduces all partials, so the
-// bound is on grid x partials.  Measured (CG on 2-D and 3-D Laplacians, one kernel against two):
-// 8.2 against 10.6 us per iteration at 147 K reads (256^2), 11.0 against 12.0 at 518 K (350^2),
-// even at 555-832 K (50^3, 392^2), 17.6 against 14.6 at 1.4 M (450^2).
+// bound is on grid x partials.  Measured (CG on 2-D and 3-D Laplacians, one kernel against two,
+// with vector chunks of 1024 elements): 8.2 against 10.6 us per iteration at 147 K reads (256^2),
+// 11.0 against 12.0 at 518 K (350^2), even at 555-832 K (50^3, 392^2), 17.6 against 14.6 at 1.4 M
+// (450^2).
 constexpr uint64_t kFusedPartialReads = 1ull << 19;
 
 struct KState {
@@ -1393,7 +1394,7 @@ static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_so
   init_state(h, P, hist);
   VCLHIP_TRY(upload_state(st, h, s));
 
-  const uint32_t gv = vec_grid_n(n);
+  const uint32_t gv = solver_vec_grid(n);
   const bool v2 = aligned16({b, x});  // workspace vectors are 256-byte aligned
   // setup: x = 0, r = b, p = M^-1 b, Ap = A p, scalars
   if (jac)
@@ -1551,7 +1552,7 @@ static int bicgstab_solve(const M* Ain, const T* b, T* x, int precond, const vcl
   init_state(h, P, hist);
   VCLHIP_TRY(upload_state(st, h, s));
 
-  const uint32_t gv = vec_grid_n(n);
+  const uint32_t gv = solver_vec_grid(n);
   const bool v2 = aligned16({b, x});
   if (jac)
     VCLHIP_LAUNCH_W(v2, (bicg_init_kernel<T, true, kVecW<T> >), (bicg_init_kernel<T, true, 1>), gv, s, n, b, x, r,
@@ -2328,7 +2329,7 @@ static int mixed_cg_solve(const M* A, const float* values32, const double* b, do
   h.inner_tol = P->inner_tol;
   VCLHIP_TRY(upload_state(st, h, s));
 
-  const uint32_t gv = vec_grid_n(n);
+  const uint32_t gv = solver_vec_grid(n);
   const bool v4 = aligned16({b, x});
   if (jac)
     VCLHIP_LAUNCH_W(v4, (mixed_init_kernel<true, 4>), (mixed_init_kernel<true, 1>), gv, s, n, b, x, x32, r32, p32, d32,

@@ -226,6 +226,17 @@ static int make_op(const vclhip_matrix* A, DevOp<T>& op)
 
 inline uint32_t vec_grid_n(uint32_t n) { return chunk_grid((n + 4 * kBlock - 1) / (4 * kBlock), vec_gmax()); }
 
+// Vector grid of the device-resident solvers: vec_grid_n's chunks of 1024 elements, but where that
+// gives fewer workgroups than CUs, chunks down to 256 elements over up to one workgroup per CU, so
+// that a small system's vector updates use the whole device and not a quarter of it.
+inline uint32_t solver_vec_grid(uint32_t n)
+{
+  uint32_t units = (n + 4 * kBlock - 1) / (4 * kBlock);
+  const uint32_t cus = static_cast<uint32_t>(device_cu_count());
+  if (units < cus) units = std::min((n + kBlock - 1) / kBlock, cus);
+  return chunk_grid(units, vec_gmax());
+}
+
 // ------------------------------------------------------------------ solve stream
 
 // Graph capture needs a real stream.  When the caller passes the null stream and asks for graphs,
