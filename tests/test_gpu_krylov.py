@@ -1,6 +1,6 @@
 """GPU parity of the pipelined operations and the CG / BiCGStab solvers.
 
-Bars (north star): iteration counts within 2% of the reference (golden fixtures / oracle), the same
+Bars: iteration counts within 2% of the reference (golden fixtures / oracle), the same
 stopping quantity below the tag tolerance, and the FP64 true residual ||b - A x|| / ||b|| verified.
 Element-wise updates are bit-exact against the reference (no contraction, same expressions);
 inner products differ only in summation order.

@@ -1,6 +1,6 @@
 """GPU parity of the CSR / SELL SpMV kernels against the oracle and the golden fixtures.
 
-Bar: relative inf-norm error <= 1e-13 for FP64 (north star), bit-exact where a row is reduced by a
+Bar: relative inf-norm error <= 1e-13 for FP64, bit-exact where a row is reduced by a
 single lane (the 5-/7-point stencils: one lane per row, same summation order as the host loop),
 <= 1e-5 relative for FP32.  Full-size (128^3) checks use size-independent properties.
 """
