@@ -21,8 +21,8 @@ import subprocess
 import sys
 import time
 
-# The CPU baseline pins its OpenMP threads as the survey's measurement rules say (close binding,
-# one thread per core); libgomp reads these when the reference build is first loaded.
+# The CPU baseline pins its OpenMP threads (close binding, one thread per core); libgomp reads
+# these when the reference build is first loaded.
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
 

@@ -1,4 +1,7 @@
-"""The benchmark configurations at their full sizes (C1-C5 of the survey), through the C ABI.
+"""The benchmark workloads at their full sizes, through the C ABI: C1 CG on
+generate_fdm_laplace(256, 256); C2 the 128^3 Laplacian (CG, Jacobi-PCG, BiCGStab, Jacobi-BiCGStab,
+CSR and SELL-64) and the 2 M power-law matrix; C3 FP64 and mixed CG on the 160^3 diffusion
+operator; C4 the batched CG of 4096 systems; C5 the 512^2 differentiable solve.
 
 C1 is checked against the reference itself (oracle/_ref, the reference's host backend built from
 its sources; the restated oracle where that build is absent).  At the larger sizes the checks are
