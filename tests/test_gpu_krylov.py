@@ -669,3 +669,57 @@ def test_solvers_refuse_vectors_of_another_precision_or_size(dev, vh):
             fn(A64, b64, short)
     with pytest.raises(ValueError):
         vh.mixed_cg(A64, b32, x32)
+
+
+def test_concurrent_solves_on_two_streams_match_sequential(dev, vh):
+    """The solvers keep their state per call (workspace, scalar record, partial sums on the caller's
+    stream), so solves issued from two host threads on two streams run concurrently and give what
+    they give one after the other, bit for bit: CG and BiCGStab, a small system (one kernel per CG
+    iteration) beside a larger one (three), each thread repeating its solves."""
+    import threading
+
+    systems = [vh.fdm_laplace_2d(200, 200), vh.laplace_3d(48)]
+    mats = [vh.CsrMatrix(rp, col, val) for rp, col, val in systems]
+    sizes = [rp.size - 1 for rp, _, _ in systems]
+    rhs = [vh.random_uniform(11 + i, sizes[i]) for i in (0, 1)]
+    solvers = (vh.cg, vh.bicgstab)
+    reps = 3
+    # every array is made and filled before any solve starts (the solves' streams do not wait for
+    # the null stream that uploads and fills)
+    arrays = {(i, k, r): (vh.DeviceArray.from_numpy(rhs[i]), vh.DeviceArray.zeros(sizes[i], np.float64))
+              for i in (0, 1) for k in range(len(solvers)) for r in range(reps + 1)}
+    vh.call("vclhip_device_synchronize")
+
+    def run(i, r, stream):
+        out = []
+        for k, fn in enumerate(solvers):
+            db, dx = arrays[(i, k, r)]
+            res = fn(mats[i], db, dx, tol=1e-9, max_iters=3000, stream=stream)
+            out.append((int(res.iters), float(res.error), dx.to_numpy(stream)))
+        return out
+
+    want = [run(0, reps, None), run(1, reps, None)]
+    streams = [vh.Stream(), vh.Stream()]
+    got = [[], []]
+    errors = []
+
+    def worker(i):
+        try:
+            for r in range(reps):
+                got[i].append(run(i, r, streams[i].handle))
+        except Exception as e:  # reported below, in the main thread
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in (0, 1)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in threads) and not errors, errors
+    for i in (0, 1):
+        assert len(got[i]) == reps
+        for rep in got[i]:
+            for (it, err, x), (it0, err0, x0) in zip(rep, want[i]):
+                assert it == it0 and err == err0 and np.array_equal(x, x0)
+    for s in streams:
+        s.close()
