@@ -855,6 +855,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic},
         "warm_gbs": round(warm_gbs, 2),
+        "cold_us": {"mean": round(ms * 1e3, 2), "median": round(float(np.median(cold)) * 1e3, 2),
+                    "min": round(float(cold.min()) * 1e3, 2), "max": round(float(cold.max()) * 1e3, 2)},
         "timed_region_wall_s": round(wall, 4),
         "sell64": {"cold_gbs": round(sell_gbs, 2), "cold_ms": round(float(s_cold.mean()), 6),
                    "warm_ms": round(s_warm, 6), "algorithmic_bytes": bytes_sell},
