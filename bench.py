@@ -835,6 +835,19 @@ def main():
                                       args.warmup, evict)
     sell_gbs = bytes_sell / (s_cold.mean() * 1e-3) / 1e9
     log(f"SELL-64 SpMV cold {s_cold.mean() * 1e3:.1f} us ({sell_gbs:.0f} GB/s), warm {s_warm * 1e3:.1f} us")
+    # the FP32 product of the same matrix (values 4 bytes, x and y 4 bytes each)
+    A32 = vh.CsrMatrix(rp, col, val, dtype=np.float32)
+    x32 = vh.DeviceArray.from_numpy(xh.astype(np.float32))
+    y32 = vh.DeviceArray.zeros(n, np.float32)
+    f_cold, f_warm, _ = time_launches(vh, lambda: A32.spmv(x32, y32, stream=stream.handle), stream, args.steps,
+                                      args.warmup, evict)
+    bytes_csr32 = 8 * nnz + 4 * (n + 1) + 8 * n
+    csr32 = {"cold_gbs": round(bytes_csr32 / (f_cold.mean() * 1e-3) / 1e9, 2),
+             "cold_ms": round(float(f_cold.mean()), 6), "warm_ms": round(f_warm, 6),
+             "algorithmic_bytes": bytes_csr32}
+    log("CSR SpMV FP32", csr32)
+    for a in (A32, x32, y32):
+        a.free()
     y_gpu = y.to_numpy()
     ref = stream_reference(vh, stream, 10, evict, bytes_csr, A.num_blocks)
     evict.free()
@@ -860,6 +873,7 @@ def main():
         "timed_region_wall_s": round(wall, 4),
         "sell64": {"cold_gbs": round(sell_gbs, 2), "cold_ms": round(float(s_cold.mean()), 6),
                    "warm_ms": round(s_warm, 6), "algorithmic_bytes": bytes_sell},
+        "csr_fp32": csr32,
         "stream_reference": ref,
     }
 
