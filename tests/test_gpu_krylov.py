@@ -723,3 +723,45 @@ def test_concurrent_solves_on_two_streams_match_sequential(dev, vh):
                 assert it == it0 and err == err0 and np.array_equal(x, x0)
     for s in streams:
         s.close()
+
+
+def tridiag(n):
+    """1-D Laplacian (2 on the diagonal, -1 beside it) in CSR."""
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        for j, v in ((i - 1, -1.0), (i, 2.0), (i + 1, -1.0)):
+            if 0 <= j < n:
+                rows.append(i)
+                cols.append(j)
+                vals.append(v)
+    rp = np.zeros(n + 1, np.uint32)
+    np.add.at(rp, np.array(rows) + 1, 1)
+    return np.cumsum(rp).astype(np.uint32), np.array(cols, np.uint32), np.array(vals)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 63, 64, 65, 257])
+def test_tiny_systems_every_launch_mode(dev, vh, n):
+    """Systems of 1 to 257 rows (partial packs, chunks smaller than a workgroup, grids at their
+    minimum of 8 workgroups) in every launch mode: CG in one and two kernels per iteration and
+    BiCGStab in four and six give the same x bit for bit and solve the system; a NaN in b ends at
+    the iteration cap in every mode.  When s vanishes exactly (a one- or two-row system), the
+    pipelined BiCGStab's omega is 0/0, as in the reference (bicgstab.hpp:193-195); there only the
+    agreement of the modes is checked."""
+    rp, col, val = tridiag(n)
+    A = vh.CsrMatrix(rp, col, val)
+    b = vh.random_uniform(3, n)
+    for kind, modes in (("cg", (0, 1)), ("bicgstab", (0, 1))):
+        exact_breakdown = kind == "bicgstab" and n <= 2
+        xs = []
+        for variant in modes:
+            x, res = solve(vh, kind, A, b, tol=1e-10, max_iters=4 * n + 20, variant=variant)
+            assert res.converged or exact_breakdown, (kind, variant, res.iters)
+            xs.append((x, res.iters))
+        assert np.array_equal(xs[0][0], xs[1][0], equal_nan=True) and xs[0][1] == xs[1][1], kind
+        if not exact_breakdown:
+            assert true_relres(rp, col, val, xs[0][0], b) < 1e-8
+        bn = b.copy()
+        bn[n // 2] = np.nan
+        for variant in modes:
+            _, res = solve(vh, kind, A, bn, tol=1e-10, max_iters=30, variant=variant)
+            assert res.iters == 30 and not res.converged, (kind, variant)
