@@ -689,8 +689,14 @@ def profile_workloads(vh, names, steps, warmup):
             A = vh.CsrMatrix(rp, col, val)
             x, y = vh.DeviceArray.from_numpy(vh.random_uniform(1, n)), vh.DeviceArray.zeros(n, np.float64)
             _, warm, _ = time_launches(vh, lambda: A.spmv(x, y, stream=stream.handle), stream, steps, warmup)
-            out[name] = {"csr_ms": warm, "nnz": int(rp[-1]), "algorithmic_bytes": csr_bytes(n, int(rp[-1]))}
-            for a in (x, y, A):
+            # then the gathers alone (gather_read_kernel over the matrix's column indices)
+            g = (A.num_blocks + 7) // 8 * 8
+            sums = vh.DeviceArray.zeros(g, np.float64)
+            _, gwarm, _ = time_launches(vh, lambda: vh.call("vclhip_gather_read_f64", int(rp[-1]), A.d_col.ptr, x.ptr,
+                                                            n, sums.ptr, g, stream.handle), stream, steps, warmup)
+            out[name] = {"csr_ms": warm, "gather_ms": gwarm, "nnz": int(rp[-1]),
+                         "algorithmic_bytes": csr_bytes(n, int(rp[-1]))}
+            for a in (x, y, A, sums):
                 a.free()
         elif name == "dia_cg":
             g = 160
@@ -777,7 +783,7 @@ def main():
     ap.add_argument("--profile", default=None,
                     help="comma-separated short workloads for rocprofv3 runs, nothing else: spmv (CSR and "
                          "SELL-64 SpMV of the 128^3 Laplacian and the streaming kernels), powerlaw (CSR SpMV of the "
-                         "power-law matrix), dia_cg (50 FP64 CG iterations on the 160^3 diffusion operator's DIA "
+                         "power-law matrix, then its gathers alone), dia_cg (50 FP64 CG iterations on the 160^3 diffusion operator's DIA "
                          "copy), batched (one FP64 batched CG solve of the 4096 systems), and one solve of a "
                          "configuration: c1_cg, c1_bicgstab, c2_cg, c2_bicgstab, c3_cg_fp64, c3_mixed")
     args = ap.parse_args()
