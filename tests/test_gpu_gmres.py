@@ -237,3 +237,25 @@ def test_gmres_convection_diffusion_64(dev, vh, oracle):
     b = vh.random_uniform(1, rp.size - 1)
     for fmt in ("csr", "sell"):
         check_against_oracle(oracle, vh, rp, col, val, b, 1e-8, 600, 20, fmt)
+
+
+@pytest.mark.parametrize("fmt", ["csr", "sell"])
+def test_gmres_absolute_tolerance_vs_oracle(dev, vh, oracle, fmt):
+    """abs_tolerance(): a restart cycle begins with rho_0 = ||b - A x|| and the solve ends there when
+    rho_0 < abs_tol (gmres.hpp:234), or at once when rho_0 <= abs_tol (:227-228).  With a relative
+    tolerance out of reach the absolute one ends the solve at the oracle's step and cycle counts."""
+    rp, col, val = vh.convection_diffusion(24, 20, 12)
+    n = rp.size - 1
+    b = vh.random_uniform(21, n)
+    nb = float(np.linalg.norm(b))
+    H = oracle.CsrHost(rp, col, val)
+    for kd, rel in ((20, 1e-6), (7, 1e-4)):
+        xo, io, eo, _ = oracle.gmres(H, b, 1e-15, 3000, kd, abs_tol=rel * nb)
+        x, res = run_gpu(vh, rp, col, val, b, fmt, tol=1e-15, abs_tol=rel * nb, max_iters=3000, krylov_dim=kd)
+        print(f"[gmres abs] K={kd} abs={rel:g}*||b||: gpu {res.iters} it {res.restarts} cycles | oracle {io} it",
+              flush=True)
+        assert res.converged and iters_close(res.iters, io), (res.iters, io)
+        assert true_relres(rp, col, val, x, b) < rel
+        assert np.linalg.norm(x - xo) <= 10 * rel * np.linalg.norm(xo)
+    x, res = run_gpu(vh, rp, col, val, 1e-3 * b, fmt, tol=1e-15, abs_tol=1e-2 * nb, max_iters=3000, krylov_dim=20)
+    assert res.iters == 0 and not x.any()

@@ -765,3 +765,67 @@ def test_tiny_systems_every_launch_mode(dev, vh, n):
         for variant in modes:
             _, res = solve(vh, kind, A, bn, tol=1e-10, max_iters=30, variant=variant)
             assert res.iters == 30 and not res.converged, (kind, variant)
+
+
+_ABS_ORACLE = {}
+
+
+def abs_tol_oracle(oracle, H, key, b, abs_tol, jacobi_diag):
+    """The oracle's CG, Jacobi-CG and BiCGStab iteration counts at abs_tol (the last as its range
+    over reduction orders), computed once per matrix."""
+    if key not in _ABS_ORACLE:
+        cg_it = oracle.cg_pipelined(H, b, 1e-15, 5000, abs_tol=abs_tol)[1]
+        jac_it = None
+        if jacobi_diag is not None:
+            jac_it = oracle.cg_classic(H, b, 1e-15, 5000, diag=jacobi_diag, abs_tol=abs_tol)[1]
+        env = oracle.iteration_envelope(lambda: oracle.bicgstab_pipelined(H, b, 1e-15, 5000, abs_tol=abs_tol))
+        hist = oracle.bicgstab_pipelined(H, b, 1e-15, 5000, abs_tol=abs_tol, history=True)[3]
+        _ABS_ORACLE[key] = (cg_it, jac_it, env, hist)
+    return _ABS_ORACLE[key]
+
+
+@pytest.mark.parametrize("case", ["fdm_256_fused", "fdm_256_two_kernels", "lap3d_96_dia", "lap3d_96_as_given",
+                                  "lap3d_40_sell"])
+def test_absolute_tolerance_stop_vs_oracle(dev, vh, oracle, case):
+    """abs_tolerance(): CG stops when |<r,r>| < abs_tol^2 (cg.hpp:176, classic form :309) and
+    BiCGStab when ||r|| < abs_tol (bicgstab.hpp:198, :465), whichever of the two tests comes first.
+    With a relative tolerance out of reach (1e-15) the absolute one ends the solve at the oracle's
+    iteration count in every launch mode (one fused kernel, two, three on the 96^3 operator; DIA
+    copy, CSR as given, SELL-64), and ||b|| below abs_tol returns x = 0 at once (cg.hpp:151,
+    bicgstab.hpp:140)."""
+    m, kw = {"fdm_256_fused": (256, {}), "fdm_256_two_kernels": (256, {"variant": 1}),
+             "lap3d_96_dia": (96, {}), "lap3d_96_as_given": (96, {"operator_format": vh.OPERATOR_AS_GIVEN}),
+             "lap3d_40_sell": (40, {})}[case]
+    rp, col, val = vh.fdm_laplace_2d(m, m) if case.startswith("fdm") else vh.laplace_3d(m)
+    n = rp.size - 1
+    b = vh.random_uniform(11, n)
+    sell = case.endswith("sell")
+    if sell:
+        A, H = vh.SellMatrix(rp, col, val, 64), oracle.SellHost(rp, col, val, 64)
+    else:
+        A, H = vh.CsrMatrix(rp, col, val), oracle.CsrHost(rp, col, val)
+    nb = float(np.linalg.norm(b))
+    abs_tol = 1e-5 * nb
+    diag = None if sell else oracle.row_info(rp, col, val, 3)
+    cg_it, jac_it, env, bicg_hist = abs_tol_oracle(oracle, H, (case.split("_")[0], m, sell), b, abs_tol, diag)
+    x, res = solve(vh, "cg", A, b, tol=1e-15, abs_tol=abs_tol, max_iters=5000, **kw)
+    assert res.converged and iters_close(res.iters, cg_it), (res.iters, cg_it)
+    assert true_relres(rp, col, val, x, b) < 1.1e-5
+    if not sell:
+        _, res = solve(vh, "cg", A, b, precond=vh.PRECOND_JACOBI, tol=1e-15, abs_tol=abs_tol, max_iters=5000, **kw)
+        assert res.converged and iters_close(res.iters, jac_it), (res.iters, jac_it)
+    # BiCGStab's residual is jagged at this level: the oracle's own count moves by up to 13% with
+    # the summation order (114-129 on the 96^3 case), and the reference's, whose OpenMP reductions
+    # combine thread partials in arrival order, from run to run (345-411 over four runs of the 256^2
+    # case on 8 threads).  The bar is the oracle's early residual history, the stop at the first
+    # step below abs_tol, and the count within 10% of the oracle's range.
+    hist = np.zeros(5000)
+    x, res = solve(vh, "bicgstab", A, b, tol=1e-15, abs_tol=abs_tol, max_iters=5000, history=hist, **kw)
+    h = hist[: res.iters]
+    assert res.converged and h[-1] < abs_tol and (h[:-1] >= abs_tol).all()  # ||r|| per step
+    assert np.allclose(h[:8], bicg_hist[:8], rtol=1e-6, atol=0)
+    assert 0.9 * env[0] <= res.iters <= 1.1 * env[1], (res.iters, env)
+    assert true_relres(rp, col, val, x, b) < 1.1e-5
+    for kind in ("cg", "bicgstab"):
+        x, res = solve(vh, kind, A, 1e-3 * b, tol=1e-15, abs_tol=1e-2 * nb, max_iters=5000, **kw)
+        assert res.iters == 0 and not x.any(), kind
