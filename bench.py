@@ -178,7 +178,8 @@ def timed_solve(vh, kind, A, b, n, precond=0, reps=2, **kw):
         best = t if best is None else min(best, t)
     rel = true_relres(vh, A, b, x, n)
     out = {"seconds": round(best, 6), "iters": int(res.iters), "error": float(res.error),
-           "true_relres": float(rel), "converged": bool(res.converged)}
+           "true_relres": float(rel), "converged": bool(res.converged),
+           "operator": {0: "as_given", 1: "dia", 2: "dia_coded"}.get(int(res.operator_used), int(res.operator_used))}
     if kind == "gmres":
         out["cycles"] = int(res.restarts)
     else:
@@ -207,7 +208,9 @@ def gmres_bench(vh, grid, reps=2):
     out = {"workload": f"convection-diffusion {grid}^3 (n={n}, nnz={nnz}), GMRES(K) to 1e-8, no preconditioner",
            "n": n, "nnz": nnz}
     spmv = csr_bytes(n, nnz)
-    for name, M, K, kw in (("gmres20_fp64_csr", A, 20, {}), ("gmres20_fp64_csr_graph", A, 20, {"use_graph": True}),
+    for name, M, K, kw in (("gmres20_fp64_csr", A, 20, {}),
+                           ("gmres20_fp64_csr_dia_uncoded", A, 20, {"operator_format": vh.OPERATOR_DIA_UNCODED}),
+                           ("gmres20_fp64_csr_graph", A, 20, {"use_graph": True}),
                            ("gmres20_fp64_sell64", S, 20, {}), ("gmres40_fp64_csr", A, 40, {}),
                            ("gmres10_fp64_csr", A, 10, {})):
         rec = timed_solve(vh, "gmres", M, b, n, reps=reps, tol=1e-8, max_iters=6000, krylov_dim=K, **kw)
@@ -886,14 +889,20 @@ def main():
     if not args.no_solvers:
         b = vh.DeviceArray.from_numpy(vh.random_uniform(0, n))
         sol = {}
-        # a CSR stencil is solved on a DIA copy by default (the same results); "_as_given" keeps CSR
+        # a CSR stencil is solved on a DIA copy by default (the same results), here read through its
+        # value tables (one value per diagonal); "_dia_uncoded" reads the copy's values as stored,
+        # "_as_given" keeps CSR
         sol["cg_fp64_csr"] = timed_solve(vh, "cg", A, b, n, reps=3, tol=1e-10, max_iters=5000)
+        sol["cg_fp64_csr_dia_uncoded"] = timed_solve(vh, "cg", A, b, n, reps=3, tol=1e-10, max_iters=5000,
+                                                     operator_format=vh.OPERATOR_DIA_UNCODED)
         sol["cg_fp64_csr_as_given"] = timed_solve(vh, "cg", A, b, n, reps=3, tol=1e-10, max_iters=5000,
                                                   operator_format=1)
         sol["cg_fp64_csr_graph"] = timed_solve(vh, "cg", A, b, n, tol=1e-10, max_iters=5000, use_graph=True)
         sol["cg_fp64_sell64"] = timed_solve(vh, "cg", S, b, n, tol=1e-10, max_iters=5000)
         sol["jacobi_cg_fp64_csr"] = timed_solve(vh, "cg", A, b, n, precond=1, tol=1e-10, max_iters=5000)
         sol["bicgstab_fp64_csr"] = timed_solve(vh, "bicgstab", A, b, n, tol=1e-10, max_iters=5000)
+        sol["bicgstab_fp64_csr_dia_uncoded"] = timed_solve(vh, "bicgstab", A, b, n, tol=1e-10, max_iters=5000,
+                                                           operator_format=vh.OPERATOR_DIA_UNCODED)
         sol["bicgstab_fp64_csr_as_given"] = timed_solve(vh, "bicgstab", A, b, n, tol=1e-10, max_iters=5000,
                                                         operator_format=1)
         sol["jacobi_bicgstab_fp64_csr"] = timed_solve(vh, "bicgstab", A, b, n, precond=1, tol=1e-10,

@@ -470,16 +470,21 @@ typedef struct vclhip_solver_params {
                                         fused into them are summed per 256-row tile, which is the CSR
                                         plan's grouping when no row has more than 8 entries (then the
                                         whole solve is the same bit for bit; with longer rows it
-                                        agrees to rounding);
-                                        VCLHIP_OPERATOR_AS_GIVEN always uses the CSR matrix; other
-                                        values are rejected */
-  const struct vclhip_dia* dia;      /* optional, with VCLHIP_OPERATOR_AUTO: the detection and DIA
+                                        agrees to rounding).  A copy of 2^18 rows or more whose
+                                        diagonals each hold at most 15 distinct values is read as
+                                        one byte per entry into a table of those values (the same
+                                        values, so the same products bit for bit);
+                                        VCLHIP_OPERATOR_DIA_UNCODED is AUTO with the values read
+                                        as stored; VCLHIP_OPERATOR_AS_GIVEN always uses the CSR
+                                        matrix; other values are rejected */
+  const struct vclhip_dia* dia;      /* optional, with VCLHIP_OPERATOR_AUTO or _DIA_UNCODED: the detection and DIA
                                         copy made once by vclhip_dia_create_* for this CSR matrix,
                                         used instead of detecting and copying in every solve (the
                                         matrix's arrays must be the ones it was made from) */
 } vclhip_solver_params;
 #define VCLHIP_OPERATOR_AUTO 0
 #define VCLHIP_OPERATOR_AS_GIVEN 1
+#define VCLHIP_OPERATOR_DIA_UNCODED 2
 
 #define VCLHIP_INNER_AUTO 0
 #define VCLHIP_INNER_CSR 1
@@ -492,7 +497,8 @@ typedef struct vclhip_solver_result {
   double error;       /* tag.error() */
   uint32_t restarts;  /* mixed CG outer corrections / BiCGStab restarts */
   int converged;
-  int operator_used;  /* the storage the products ran on: VCLHIP_USED_AS_GIVEN or VCLHIP_USED_DIA;
+  int operator_used;  /* the storage the products ran on: VCLHIP_USED_AS_GIVEN, VCLHIP_USED_DIA or
+                         VCLHIP_USED_DIA_CODED (the DIA copy read through its value tables);
                          mixed CG: its FP32 inner copy, VCLHIP_INNER_CSR (the matrix's own format),
                          VCLHIP_INNER_SELL64, VCLHIP_INNER_SELL64_D16 or VCLHIP_INNER_DIA */
   uint32_t kernels_per_iter;  /* CG: kernels launched per iteration after the first (1 in fused mode,
@@ -501,6 +507,7 @@ typedef struct vclhip_solver_result {
 } vclhip_solver_result;
 #define VCLHIP_USED_AS_GIVEN 0
 #define VCLHIP_USED_DIA 1
+#define VCLHIP_USED_DIA_CODED 2
 
 /* Fill params with the defaults of cg_tag() / bicgstab_tag() / mixed_precision_cg_tag(). */
 void vclhip_solver_params_default(vclhip_solver_params* params);

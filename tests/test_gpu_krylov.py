@@ -363,7 +363,7 @@ def test_dia_object_reused_across_solves(dev, vh):
     with pytest.raises(vh.VclHipError):
         vh.cg(B, db, dx, tol=1e-10, max_iters=10, dia=d)
     with pytest.raises(vh.VclHipError):
-        vh.cg(A, db, dx, tol=1e-10, max_iters=10, operator_format=2)
+        vh.cg(A, db, dx, tol=1e-10, max_iters=10, operator_format=3)  # not an operator format
     d.close()
 
 

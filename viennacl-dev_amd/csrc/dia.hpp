@@ -23,6 +23,10 @@ namespace vclhip {
 constexpr uint32_t kDiaMax = 32;    // diagonals of a DIA copy at most
 constexpr uint32_t kDiaHash = 128;  // LDS hash slots per workgroup for the offsets of its rows
 static_assert(kDiaMax <= kShortRow, "a DIA row must be one the CSR kernel sums in one lane");
+static_assert(kDiaMax <= kDiaMaxCoded, "dia_kernel's LDS value tables hold kDiaMaxCoded diagonals");
+constexpr uint32_t kDiaDictSlots = kDiaCodes - 1;   // distinct values per diagonal (code 0 is the zero)
+constexpr uint32_t kDiaCodedMinRows = 1u << 18;      // a value dictionary only where bytes bound the product
+constexpr uint32_t kDiaDictGroups = 256;             // workgroups per diagonal collecting its values
 
 // Marks every offset col - row of the matrix in a bitmap (bit col - row + n).  A workgroup first
 // collects the distinct offsets of its 256 rows in an LDS hash table, so that the global bitmap
@@ -104,6 +108,165 @@ __global__ __launch_bounds__(kBlock) void dia_fill_kernel(const uint32_t* __rest
   }
 }
 
+// ---- value dictionaries: one table of at most kDiaDictSlots distinct values per diagonal.
+// A value is keyed by its bit pattern (widened to 64 bits); key 0 (+0.0) is an absent entry and
+// has code 0.  A set lives in LDS and grows by compare-and-swap: a slot is written once, so a lane
+// that finds another key in a slot moves on to the next, and no key is inserted twice.
+
+template <typename T>
+__device__ __forceinline__ unsigned long long dia_key(T v)
+{
+  if constexpr (sizeof(T) == 8)
+    return static_cast<unsigned long long>(__double_as_longlong(v));
+  else
+    return static_cast<unsigned long long>(__float_as_uint(v));
+}
+
+template <typename T>
+__device__ __forceinline__ T dia_value(unsigned long long k)
+{
+  if constexpr (sizeof(T) == 8)
+    return __longlong_as_double(static_cast<long long>(k));
+  else
+    return __uint_as_float(static_cast<uint32_t>(k));
+}
+
+// Inserts key (nonzero) into slots[0 .. kDiaDictSlots); false when they are full of other keys.
+__device__ __forceinline__ bool dia_set_insert(unsigned long long* slots, unsigned long long key)
+{
+  for (uint32_t j = 0; j < kDiaDictSlots; ++j) {
+    const unsigned long long cur = __hip_atomic_load(&slots[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == key) return true;
+    if (cur == 0ull) {
+      const unsigned long long prev = atomicCAS(&slots[j], 0ull, key);
+      if (prev == 0ull || prev == key) return true;
+    }
+  }
+  return false;
+}
+
+// Pass 1: workgroup (g, d) collects the distinct values of diagonal d over rows g*kBlock + tid,
+// stepping by gridDim.x*kBlock, into sets[(d * gridDim.x + g) * kDiaDictSlots ...]; more than
+// kDiaDictSlots of them raise *overflow.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dia_dict_collect_kernel(const T* __restrict__ val, uint32_t n,
+                                                                  unsigned long long* __restrict__ sets,
+                                                                  uint32_t* __restrict__ overflow)
+{
+  __shared__ unsigned long long slots[kDiaDictSlots];
+  __shared__ uint32_t full;
+  const uint32_t tid = threadIdx.x, d = blockIdx.y;
+  if (tid < kDiaDictSlots) slots[tid] = 0ull;
+  if (tid == 0) full = 0u;
+  __syncthreads();
+  const T* v = val + static_cast<size_t>(d) * n;
+  for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    const uint32_t row = base + tid;
+    if (row < n) {
+      const unsigned long long key = dia_key(v[row]);
+      if (key != 0ull && !dia_set_insert(slots, key)) full = 1u;
+    }
+    __syncthreads();
+    const bool stop_now = full != 0u;  // read by every wave before any wave writes it again
+    __syncthreads();
+    if (stop_now) break;
+  }
+  if (tid < kDiaDictSlots) sets[(static_cast<size_t>(d) * gridDim.x + blockIdx.x) * kDiaDictSlots + tid] = slots[tid];
+  if (tid == 0 && full) *overflow = 1u;
+}
+
+// Pass 2: workgroup d merges the groups' sets of diagonal d into its table (entry 0 the zero of an
+// absent entry, unused entries zero) and keeps the keys for pass 3.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dia_dict_merge_kernel(const unsigned long long* __restrict__ sets,
+                                                                uint32_t groups, T* __restrict__ table,
+                                                                unsigned long long* __restrict__ keys,
+                                                                uint32_t* __restrict__ overflow)
+{
+  __shared__ unsigned long long slots[kDiaDictSlots];
+  __shared__ uint32_t full;
+  const uint32_t tid = threadIdx.x, d = blockIdx.x;
+  if (tid < kDiaDictSlots) slots[tid] = 0ull;
+  if (tid == 0) full = 0u;
+  __syncthreads();
+  const unsigned long long* in = sets + static_cast<size_t>(d) * groups * kDiaDictSlots;
+  for (uint32_t i = tid; i < groups * kDiaDictSlots; i += kBlock) {
+    const unsigned long long key = in[i];
+    if (key != 0ull && !dia_set_insert(slots, key)) full = 1u;
+  }
+  __syncthreads();
+  if (tid < kDiaCodes) {
+    const unsigned long long k = tid == 0 ? 0ull : slots[tid - 1];
+    table[d * kDiaCodes + tid] = dia_value<T>(k);
+    keys[d * kDiaCodes + tid] = k;
+  }
+  if (tid == 0 && full) *overflow = 1u;
+}
+
+// Pass 3: code[d * n + row] = the index of the row's value in its diagonal's table.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dia_encode_kernel(const T* __restrict__ val, uint32_t n, uint32_t nd,
+                                                            const unsigned long long* __restrict__ keys,
+                                                            uint8_t* __restrict__ code)
+{
+  __shared__ unsigned long long sk[kDiaMax * kDiaCodes];
+  for (uint32_t i = threadIdx.x; i < nd * kDiaCodes; i += kBlock) sk[i] = keys[i];
+  __syncthreads();
+  const uint32_t row = blockIdx.x * kBlock + threadIdx.x;
+  if (row >= n) return;
+  for (uint32_t d = 0; d < nd; ++d) {
+    const unsigned long long key = dia_key(val[static_cast<size_t>(d) * n + row]);
+    uint32_t c = 0;
+    if (key != 0ull)
+      for (uint32_t j = 1; j < kDiaCodes; ++j)
+        if (sk[d * kDiaCodes + j] == key) c = j;
+    code[static_cast<size_t>(d) * n + row] = static_cast<uint8_t>(c);
+  }
+}
+
+// Whether DIA copies of kDiaCodedMinRows rows or more get value dictionaries (VCLHIP_DIA_CODED=0
+// turns them off for the whole process).
+bool dia_codes_enabled();
+
+// Bytes a value dictionary adds behind a copy's values (tables first, 256-byte aligned, then codes).
+template <typename T>
+inline size_t dia_code_bytes(uint32_t n, uint32_t nd)
+{
+  return (static_cast<size_t>(nd) * kDiaCodes * sizeof(T) + 255) / 256 * 256 + static_cast<size_t>(nd) * n;
+}
+
+// Builds the value dictionary of `dia` into `extra` (dia_code_bytes<T> bytes); *coded tells
+// whether every diagonal fit.  Temporaries come from `ws`; without them there is no dictionary.
+template <typename T>
+static int build_dia_codes(DiaDev<T>& dia, void* extra, Workspace& ws, uint32_t* flag, hipStream_t s, bool* coded)
+{
+  *coded = false;
+  const uint32_t n = dia.n_rows, nd = dia.nd;
+  const uint32_t groups = std::min<uint32_t>(kDiaDictGroups, (n + kBlock - 1) / kBlock);
+  unsigned long long *sets, *keys;
+  if (!ws.try_alloc(&sets, static_cast<size_t>(nd) * groups * kDiaDictSlots) ||
+      !ws.try_alloc(&keys, static_cast<size_t>(nd) * kDiaCodes))
+    return VCLHIP_SUCCESS;
+  T* table = static_cast<T*>(extra);
+  uint8_t* code = static_cast<uint8_t*>(extra) + (static_cast<size_t>(nd) * kDiaCodes * sizeof(T) + 255) / 256 * 256;
+  VCLHIP_TRY(vclhip_fill_zero(flag, sizeof(uint32_t), s));
+  hipLaunchKernelGGL(dia_dict_collect_kernel<T>, dim3(groups, nd), dim3(kBlock), 0, s, dia.val, n, sets, flag);
+  VCLHIP_TRY(check_launch("dia_dict_collect_kernel"));
+  hipLaunchKernelGGL(dia_dict_merge_kernel<T>, dim3(nd), dim3(kBlock), 0, s, sets, groups, table, keys, flag);
+  VCLHIP_TRY(check_launch("dia_dict_merge_kernel"));
+  uint32_t hover = 0;
+  VCLHIP_HIP(hipMemcpyAsync(&hover, flag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  VCLHIP_HIP(hipStreamSynchronize(s));
+  if (hover) return VCLHIP_SUCCESS;
+  hipLaunchKernelGGL(dia_encode_kernel<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, dia.val, n, nd, keys,
+                     code);
+  VCLHIP_TRY(check_launch("dia_encode_kernel"));
+  dia.code = code;
+  dia.table = table;
+  *coded = true;
+  return VCLHIP_SUCCESS;
+}
+
 // Device memory of at most `bytes` taken for an optional copy: a failure leaves *p null and clears
 // the HIP error, so that the caller keeps the matrix as given.
 struct DiaArrays {
@@ -153,8 +316,12 @@ static int build_dia(const vclhip_csr* A, Workspace& ws, hipStream_t s, double m
   if (nd == 0 || static_cast<double>(nd) * n > max_fill * static_cast<double>(A->nnz)) return VCLHIP_SUCCESS;
   const size_t vbytes = static_cast<size_t>(nd) * n * sizeof(T);
   if (vbytes > max_bytes) return VCLHIP_SUCCESS;
+  // room for a value dictionary behind the values (used when every diagonal's values fit one)
+  const size_t vpad = (vbytes + 255) / 256 * 256;
+  const bool try_codes = n >= kDiaCodedMinRows && dia_codes_enabled() && vpad + dia_code_bytes<T>(n, nd) <= max_bytes;
   DiaArrays arr;
-  if (!alloc_out(nd * sizeof(int32_t), vbytes, &arr)) return VCLHIP_SUCCESS;
+  if (!alloc_out(nd * sizeof(int32_t), try_codes ? vpad + dia_code_bytes<T>(n, nd) : vbytes, &arr))
+    return VCLHIP_SUCCESS;
   T* val = static_cast<T*>(arr.val);
   VCLHIP_HIP(hipMemcpyAsync(arr.off, offs.data(), nd * sizeof(int32_t), hipMemcpyHostToDevice, s));
   VCLHIP_TRY(vclhip_fill_zero(val, vbytes, s));
@@ -169,6 +336,12 @@ static int build_dia(const vclhip_csr* A, Workspace& ws, hipStream_t s, double m
   out->off = arr.off;
   out->n_rows = n;
   out->nd = nd;
+  out->code = nullptr;
+  out->table = nullptr;
+  if (try_codes) {
+    bool coded = false;
+    VCLHIP_TRY(build_dia_codes<T>(*out, static_cast<char*>(arr.val) + vpad, ws, overflow, s, &coded));
+  }
   *built = true;
   return VCLHIP_SUCCESS;
 }
@@ -216,15 +389,18 @@ namespace vclhip {
 // fewer bytes each) unless params->operator_format asks for the matrix as given, or the copy
 // would take more than a quarter of the free device memory.  params->dia, when given, is the
 // detection and copy made once by vclhip_dia_create_* for this matrix; otherwise the copy is built
-// in the solve's workspace.  *used_dia tells whether the solve runs on a DIA copy.
+// in the solve's workspace.  *used_dia is the operator the solve runs on (VCLHIP_USED_*).
 template <typename T>
 static int dia_operator(DevOp<T>& op, Workspace& ws, hipStream_t s, const vclhip_solver_params* P, int* used_dia)
 {
   *used_dia = 0;
   if (!P) return VCLHIP_SUCCESS;
-  if (P->operator_format != VCLHIP_OPERATOR_AUTO && P->operator_format != VCLHIP_OPERATOR_AS_GIVEN)
-    return invalid("operator_format must be VCLHIP_OPERATOR_AUTO or VCLHIP_OPERATOR_AS_GIVEN");
-  if (P->operator_format != VCLHIP_OPERATOR_AUTO || op.fmt != OpFormat::Csr || !op.csr_desc) return VCLHIP_SUCCESS;
+  if (P->operator_format != VCLHIP_OPERATOR_AUTO && P->operator_format != VCLHIP_OPERATOR_AS_GIVEN &&
+      P->operator_format != VCLHIP_OPERATOR_DIA_UNCODED)
+    return invalid("operator_format must be VCLHIP_OPERATOR_AUTO, VCLHIP_OPERATOR_AS_GIVEN or "
+                   "VCLHIP_OPERATOR_DIA_UNCODED");
+  if (P->operator_format == VCLHIP_OPERATOR_AS_GIVEN || op.fmt != OpFormat::Csr || !op.csr_desc)
+    return VCLHIP_SUCCESS;
   DiaDev<T> dia{};
   bool built = false;
   if (P->dia) {
@@ -245,12 +421,16 @@ static int dia_operator(DevOp<T>& op, Workspace& ws, hipStream_t s, const vclhip
                                 &built)));
   }
   if (!built) return VCLHIP_SUCCESS;
+  if (P->operator_format == VCLHIP_OPERATOR_DIA_UNCODED) {  // the values as stored, no dictionary
+    dia.code = nullptr;
+    dia.table = nullptr;
+  }
   op.fmt = OpFormat::Dia;
   op.dia_nt = dia_stream_loads((static_cast<size_t>(dia.nd) + 4) * dia.n_rows * sizeof(T));
   op.sell = false;
   op.dia = dia;
   op.grid = chunk_grid((op.n + kBlock - 1) / kBlock, sell_gmax());
-  *used_dia = 1;
+  *used_dia = dia.code ? VCLHIP_USED_DIA_CODED : VCLHIP_USED_DIA;
   return VCLHIP_SUCCESS;
 }
 

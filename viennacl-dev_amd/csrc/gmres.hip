@@ -684,7 +684,7 @@ int gmres_solve(const M* Ain, const T* b, T* x, const vclhip_solver_params* P, v
   Res->iters = fin.iters;
   Res->error = fin.error;
   Res->restarts = fin.cycles;
-  Res->operator_used = used_dia ? VCLHIP_USED_DIA : VCLHIP_USED_AS_GIVEN;
+  Res->operator_used = used_dia;
   Res->converged = static_cast<int>(fin.converged);
   Res->kernels_per_iter = 0;
   if (hist && fin.iters) {

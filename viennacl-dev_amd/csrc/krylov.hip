@@ -107,7 +107,7 @@ template <typename T, bool Jac, bool Full>
 struct EpiBicg {
   using acc_t = double;
   static constexpr int K = Full ? 3 : 1;
-  static constexpr bool kDiaGrouped = false;  // BiCGStab on a 128^3 DIA operator: 38 ms counted, 41-43 ms grouped
+  static constexpr bool kDiaGrouped = false;  // uncoded 128^3 DIA operator: 38 ms counted, 41-43 ms grouped
   T* __restrict__ out;
   const T* __restrict__ p;
   const T* __restrict__ r0;
@@ -1461,7 +1461,7 @@ static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_so
   // either way).
   T *r2 = nullptr, *p2 = nullptr, *Ap2 = nullptr;
   const uint32_t g_fused = std::max(gv, op.grid);
-  bool fused = compact && op.fmt == OpFormat::Dia && P->variant != 1 &&
+  bool fused = compact && op.fmt == OpFormat::Dia && !op.dia.code && P->variant != 1 &&
                static_cast<uint64_t>(g_fused) * (gv + 2ull * op.grid) <= kFusedPartialReads &&
                g_fused <= 3u * static_cast<uint32_t>(device_cu_count());
   if (fused) fused = ws.try_alloc(&r2, n) && ws.try_alloc(&p2, n) && ws.try_alloc(&Ap2, n);
@@ -1507,7 +1507,7 @@ static int cg_solve(const M* Ain, const T* b, T* x, int precond, const vclhip_so
   if (jac && fin.nrs != 0) R->error = std::sqrt(std::fabs(fin.rr / fin.nrs));
   R->restarts = 0;
   R->converged = static_cast<int>(fin.converged);
-  R->operator_used = used_dia ? VCLHIP_USED_DIA : VCLHIP_USED_AS_GIVEN;
+  R->operator_used = used_dia;
   R->kernels_per_iter = compact ? (fused ? 1u : 2u) : 3u;
   return copy_history(P, hist, fin.iter, s);
 }
@@ -1651,7 +1651,7 @@ static int bicgstab_solve(const M* Ain, const T* b, T* x, int precond, const vcl
   R->error = fin.nrs != 0 ? fin.res / fin.nrs : 0.0;
   R->restarts = restarts;
   R->converged = static_cast<int>(fin.converged);
-  R->operator_used = used_dia ? VCLHIP_USED_DIA : VCLHIP_USED_AS_GIVEN;
+  R->operator_used = used_dia;
   R->kernels_per_iter = compact ? 4u : (classic ? 8u : 6u);
   return copy_history(P, hist, fin.iter, s);
 }

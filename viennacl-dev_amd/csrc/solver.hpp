@@ -74,7 +74,10 @@ struct DevOp {
                            part, kMaxPartials, stop, mask);
         break;
       case OpFormat::Dia:
-        if (dia_nt)
+        if (dia.code)
+          hipLaunchKernelGGL((dia_kernel<T, Epi, false, true>), dim3(grid), dim3(kBlock), 0, s, dia, x, epi, part,
+                             kMaxPartials, stop, mask);
+        else if (dia_nt)
           hipLaunchKernelGGL((dia_kernel<T, Epi, true>), dim3(grid), dim3(kBlock), 0, s, dia, x, epi, part, kMaxPartials,
                              stop, mask);
         else

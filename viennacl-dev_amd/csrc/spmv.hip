@@ -30,6 +30,12 @@ bool dia_stream_loads(size_t working_set)
   return working_set > (size_t(256) << 20);
 }
 
+bool dia_codes_enabled()
+{
+  static const uint32_t on = env_u32("VCLHIP_DIA_CODED", 1, 0, 1);
+  return on == 1;
+}
+
 uint32_t csr_gmax()
 {
   static const uint32_t g = env_u32("VCLHIP_CSR_GMAX", 2048, 8, kMaxPartials);

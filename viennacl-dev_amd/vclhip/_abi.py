@@ -33,8 +33,10 @@ PRECOND_JACOBI = 1
 
 OPERATOR_AUTO = 0
 OPERATOR_AS_GIVEN = 1
+OPERATOR_DIA_UNCODED = 2
 USED_AS_GIVEN = 0
 USED_DIA = 1
+USED_DIA_CODED = 2
 INNER_AUTO, INNER_CSR, INNER_SELL64, INNER_SELL64_D16, INNER_DIA = 0, 1, 2, 3, 4
 
 
