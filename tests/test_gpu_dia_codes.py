@@ -108,3 +108,19 @@ def test_kept_copy_is_coded(dev, vh):
     d.close()
     assert r0.operator_used == r1.operator_used == vh.USED_DIA_CODED and r2.operator_used == vh.USED_DIA
     assert r0.iters == r1.iters == r2.iters and np.array_equal(x0, x1) and np.array_equal(x0, x2)
+
+
+def test_coded_27_point_stencil(dev, vh):
+    """The 27-point stencil on 64^3 (2^18 rows, 27 diagonals of values -1 and 26.5: three full
+    groups of eight diagonals in the product and a partial fourth, 27 value tables) runs coded and
+    equals the uncoded copy bit for bit."""
+    from test_gpu_krylov import stencil_27
+    rp, col, val = stencil_27(64, 64, 64)
+    n = rp.size - 1
+    A = vh.CsrMatrix(rp, col, val)
+    b = vh.random_uniform(29, n)
+    for kind in ("cg", "bicgstab"):
+        x, r, h = run(vh, kind, A, b)
+        xu, ru, hu = run(vh, kind, A, b, operator_format=vh.OPERATOR_DIA_UNCODED)
+        assert r.operator_used == vh.USED_DIA_CODED and ru.operator_used == vh.USED_DIA, kind
+        assert r.converged and r.iters == ru.iters and np.array_equal(x, xu) and np.array_equal(h, hu), kind
