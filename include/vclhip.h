@@ -491,6 +491,8 @@ typedef struct vclhip_solver_params {
 #define VCLHIP_INNER_SELL64 2
 #define VCLHIP_INNER_SELL64_D16 3
 #define VCLHIP_INNER_DIA 4
+#define VCLHIP_INNER_DIA_CODED 5 /* reported in result.operator_used: the DIA inner copy read through
+                                    value tables (2^18 rows or more, at most 15 values a diagonal) */
 
 typedef struct vclhip_solver_result {
   uint32_t iters;     /* tag.iters() */
@@ -500,7 +502,8 @@ typedef struct vclhip_solver_result {
   int operator_used;  /* the storage the products ran on: VCLHIP_USED_AS_GIVEN, VCLHIP_USED_DIA or
                          VCLHIP_USED_DIA_CODED (the DIA copy read through its value tables);
                          mixed CG: its FP32 inner copy, VCLHIP_INNER_CSR (the matrix's own format),
-                         VCLHIP_INNER_SELL64, VCLHIP_INNER_SELL64_D16 or VCLHIP_INNER_DIA */
+                         VCLHIP_INNER_SELL64, VCLHIP_INNER_SELL64_D16, VCLHIP_INNER_DIA or
+                         VCLHIP_INNER_DIA_CODED */
   uint32_t kernels_per_iter;  /* CG: kernels launched per iteration after the first (1 in fused mode,
                                  2 in compact mode, else 3); BiCGStab: 4 in compact mode, 6 pipelined,
                                  8 with the reference's Jacobi recurrences; 0 from the other solvers */

@@ -134,6 +134,19 @@ def test_dia_inner_copy_equals_sell(dev, vh, jacobi):
         assert rd.converged and true_relres(rp, col, val, xd, b) < 1e-10
 
 
+def test_coded_dia_inner_copy_equals_sell(dev, vh):
+    """The 72^3 Laplacian (373,248 rows, one value per diagonal): the DIA inner copy is read through
+    its value tables (reported as VCLHIP_INNER_DIA_CODED) and gives the SELL-64 inner copy's
+    iterations, corrections and x bit for bit."""
+    rp, col, val = vh.laplace_3d(72)
+    b = vh.random_uniform(31, rp.size - 1)
+    xs, rs = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=3000, mixed_mode=1, inner_format=3)
+    xd, rd = run_mixed(vh, rp, col, val, b, tol=1e-10, max_iters=3000, mixed_mode=1)
+    assert rd.operator_used == vh.INNER_DIA_CODED and rs.operator_used == vh.INNER_SELL64_D16
+    assert rd.converged and rd.iters == rs.iters and rd.restarts == rs.restarts and rd.error == rs.error
+    assert np.array_equal(xd, xs)
+
+
 def test_dia_inner_copy_refuses_what_it_cannot_hold(dev, vh):
     """More than 32 diagonals (a random graph, a band of 33) or more than 25% zeros: DIA, forced,
     is refused; the automatic choice falls back to SELL-64 or CSR and solves."""

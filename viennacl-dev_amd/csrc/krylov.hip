@@ -2408,7 +2408,7 @@ static int mixed_cg_solve(const M* A, const float* values32, const double* b, do
   R->restarts = fin.updates;
   R->converged = static_cast<int>(fin.converged);
   R->kernels_per_iter = 0;
-  R->operator_used = inner_is_dia    ? VCLHIP_INNER_DIA
+  R->operator_used = inner_is_dia    ? (inner_dia.code ? VCLHIP_INNER_DIA_CODED : VCLHIP_INNER_DIA)
                      : inner_is_sell ? (inner_dcol ? VCLHIP_INNER_SELL64_D16 : VCLHIP_INNER_SELL64)
                                      : VCLHIP_INNER_CSR;
   return copy_history(P, hist, fin.iter, s);

@@ -11,7 +11,7 @@ import numpy as np
 from . import _abi
 from ._abi import (Csr, Sell, Coo, Ell, Hyb, Matrix, SolverParams, SolverResult, MONITOR, VclHipError, call, lib,
                    PRECOND_NONE, PRECOND_JACOBI, LIB_PATH, declared_functions, OPERATOR_AUTO, OPERATOR_AS_GIVEN,
-                   OPERATOR_DIA_UNCODED, USED_AS_GIVEN, USED_DIA, USED_DIA_CODED, INNER_AUTO, INNER_CSR, INNER_SELL64, INNER_SELL64_D16, INNER_DIA,
+                   OPERATOR_DIA_UNCODED, USED_AS_GIVEN, USED_DIA, USED_DIA_CODED, INNER_AUTO, INNER_CSR, INNER_SELL64, INNER_SELL64_D16, INNER_DIA, INNER_DIA_CODED,
                    FORMAT_CSR, FORMAT_SELL, FORMAT_COO, FORMAT_ELL, FORMAT_HYB, ERROR_INVALID_VALUE,
                    ERROR_ZERO_DIAGONAL, ERROR_NOT_SUPPORTED, ERROR_NO_DEVICE, ERROR_TOO_LARGE)
 

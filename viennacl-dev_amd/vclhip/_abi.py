@@ -38,6 +38,7 @@ USED_AS_GIVEN = 0
 USED_DIA = 1
 USED_DIA_CODED = 2
 INNER_AUTO, INNER_CSR, INNER_SELL64, INNER_SELL64_D16, INNER_DIA = 0, 1, 2, 3, 4
+INNER_DIA_CODED = 5  # reported only
 
 
 class Csr(C.Structure):
