@@ -318,10 +318,13 @@ static int build_dia(const vclhip_csr* A, Workspace& ws, hipStream_t s, double m
   if (vbytes > max_bytes) return VCLHIP_SUCCESS;
   // room for a value dictionary behind the values (used when every diagonal's values fit one)
   const size_t vpad = (vbytes + 255) / 256 * 256;
-  const bool try_codes = n >= kDiaCodedMinRows && dia_codes_enabled() && vpad + dia_code_bytes<T>(n, nd) <= max_bytes;
+  bool try_codes = n >= kDiaCodedMinRows && dia_codes_enabled() && vpad + dia_code_bytes<T>(n, nd) <= max_bytes;
   DiaArrays arr;
-  if (!alloc_out(nd * sizeof(int32_t), try_codes ? vpad + dia_code_bytes<T>(n, nd) : vbytes, &arr))
-    return VCLHIP_SUCCESS;
+  if (!alloc_out(nd * sizeof(int32_t), try_codes ? vpad + dia_code_bytes<T>(n, nd) : vbytes, &arr)) {
+    // without room for a dictionary the values alone still make the copy
+    if (!try_codes || !alloc_out(nd * sizeof(int32_t), vbytes, &arr)) return VCLHIP_SUCCESS;
+    try_codes = false;
+  }
   T* val = static_cast<T*>(arr.val);
   VCLHIP_HIP(hipMemcpyAsync(arr.off, offs.data(), nd * sizeof(int32_t), hipMemcpyHostToDevice, s));
   VCLHIP_TRY(vclhip_fill_zero(val, vbytes, s));
