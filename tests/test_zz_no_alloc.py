@@ -65,3 +65,17 @@ def test_noalloc_cg_refuses_what_it_cannot_hold(dev, vh):
     assert e.value.status == vh.ERROR_INVALID_VALUE
     # the refusals left nothing behind: a solve right after them still equals the allocating one
     same_solve(vh, A, vh.random_uniform(4, 400), tol=1e-10, max_iters=500)
+
+
+def test_noalloc_cg_on_a_coded_dia_copy(dev, vh):
+    """2^18 rows (the 512 x 512 Poisson matrix): the DIA copy, its value dictionary and the
+    dictionary's temporaries all come from the scratch array, and the solve equals the allocating
+    one bit for bit on the coded copy; a smaller call after it still does."""
+    rp, col, val = vh.fdm_laplace_2d(512, 512)
+    A = vh.CsrMatrix(rp, col, val)
+    r = same_solve(vh, A, vh.random_uniform(8, rp.size - 1), tol=1e-8, max_iters=3000)
+    assert r.converged and r.operator_used == vh.USED_DIA_CODED
+    rp, col, val = vh.fdm_laplace_2d(100, 90)
+    A = vh.CsrMatrix(rp, col, val)
+    r = same_solve(vh, A, vh.random_uniform(9, rp.size - 1), tol=1e-8, max_iters=3000)
+    assert r.converged and r.operator_used == vh.USED_DIA
